@@ -1,0 +1,638 @@
+// FRI compression of a flat FP64 vector on the GPU (gfx950).
+//
+// Replaces fri_compression / get_d_largest / one_norm / argsort /
+// make_probability_vector / sample_systematic (src/fri_utils.cpp:11-36,
+// 45-58, 129-171, 367-522) and the "largest" path of the SparseTensor4d
+// constructor (include/sparse_tensor.hpp:46-56).
+//
+// Canonical semantics (identical, bit for bit, to the CPU oracle's canonical
+// mode):
+//   sigma   = indices by |x| descending, ties by ascending index
+//   R0      = round(sum FX(|x_i|))                     exact fixed point
+//   D       = longest prefix of sigma (k < m) with
+//             (m-k)*y_k >= R_k - y_k && R_k > 1e-14,    R_k = round(Ftot - sum_{l<k} FX(y_l))
+//   p_j     = (j in D) ? 0 : |x_j| / R_|D| * n'         n' = m - |D|
+//   P_j     = round(2^-96 * sum_{l<j} floor(p_l * 2^96))
+//   S_k     = j with P_j <= fl(k + u) < P_{j+1}, or 0 if no such j
+//   output  = [sigma_0..sigma_{|D|-1}, S_0..S_{n'-1}],
+//             values [x_D, x_S / p_S]
+//
+// Pipeline (one stream, two host syncs):
+//   A  histogram of the top 12 bits of |x| (+ non-finite detection, max exponent)
+//   sel choose the top-12-bit bucket b1 holding the m-th largest
+//   B  exact FX 1-norm + per-tile count of candidates (bucket >= b1)
+//   C  order-preserving compaction of the candidates (tile scan + scatter)
+//   sort stable LSD radix sort of the candidates by |x| descending
+//   D  exact prefix of FX over the sorted candidates, parallel test, |D| by atomicMin
+//   S  exact prefix of FXP(p) over all N, element-parallel systematic emission
+#include "common.h"
+#include "fricc_internal.h"
+
+namespace fricc {
+
+constexpr int NT = 256;
+constexpr int ITEMS = 16;
+constexpr int TILE = NT * ITEMS;  // elements per tile, striped rounds of NT
+constexpr uint64_t MAXKEY = 0x7fffffffffffffffull;
+
+struct CompScalars {
+  uint32_t ebmax, nonfinite, b1, pflag;
+  uint64_t n_cand;
+  uint64_t limbs[4];
+  unsigned long long K;
+  uint64_t n_sampled;
+  double R0, RK, Ptot;
+  uint64_t thr_key, thr_idx;
+  uint64_t k_notfound;
+  uint64_t n_notfound;
+};
+
+// ---------------------------------------------------------------- pass A
+__global__ void k_hist_top(const double* __restrict__ x, int64_t n, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[4096];
+  for (int i = threadIdx.x; i < 4096; i += NT) h[i] = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
+    uint64_t yb = dbits(x[i]) & MAXKEY;
+    atomicAdd(&h[yb >> 51], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += NT)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Single block: bucket b1 with count(>b1) < m <= count(>=b1).
+__global__ void k_select_top(const uint32_t* __restrict__ hist, int64_t m, CompScalars* sc) {
+  __shared__ uint32_t sm[NT / WAVE + 1];
+  __shared__ int s_top;
+  if (threadIdx.x == 0) s_top = -1;
+  __syncthreads();
+  // thread t owns buckets [4095 - 16t - 15, 4095 - 16t] (descending)
+  uint32_t c[16];
+  uint32_t loc = 0;
+  for (int q = 0; q < 16; q++) {
+    int b = 4095 - (threadIdx.x * 16 + q);
+    c[q] = hist[b];
+    loc += c[q];
+  }
+  for (int q = 0; q < 16; q++)
+    if (c[q]) {
+      atomicMax(&s_top, 4095 - (int)(threadIdx.x * 16 + q));
+      break;
+    }
+  uint32_t tot;
+  uint32_t above = block_excl_scan_u32<NT>(loc, &tot, sm);  // count strictly above my range
+  if ((int64_t)above < m && (int64_t)(above + loc) >= m) {
+    uint64_t run = above;
+    for (int q = 0; q < 16; q++) {
+      if ((int64_t)(run + c[q]) >= m) {
+        sc->b1 = 4095 - (threadIdx.x * 16 + q);
+        sc->n_cand = run + c[q];
+        break;
+      }
+      run += c[q];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int top = s_top < 0 ? 0 : s_top;
+    sc->ebmax = (uint32_t)top >> 1;
+    sc->nonfinite = (top >= 0xffe) ? 1u : 0u;
+  }
+}
+
+// ---------------------------------------------------------------- pass B
+__global__ void k_norm_count(const double* __restrict__ x, int64_t n, CompScalars* sc,
+                             uint32_t* __restrict__ tilecnt) {
+  __shared__ uint64_t sl[4][NT / WAVE];
+  __shared__ uint32_t sm[NT / WAVE + 1];
+  const uint32_t ebmax = sc->ebmax, b1 = sc->b1;
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  uint64_t L[4] = {0, 0, 0, 0};
+  uint32_t cnt = 0;
+#pragma unroll 4
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t i = base + (int64_t)r * NT + threadIdx.x;
+    if (i < n) {
+      uint64_t yb = dbits(x[i]) & MAXKEY;
+      limbs_add(L, fx_of(yb, ebmax));
+      cnt += (uint32_t)((yb >> 51) >= b1);
+    }
+  }
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  for (int q = 0; q < 4; q++) {
+    uint64_t v = L[q];
+    for (int d = WAVE / 2; d > 0; d >>= 1) v += (uint64_t)__shfl_down((unsigned long long)v, d, WAVE);
+    if (lane == 0) sl[q][w] = v;
+  }
+  uint32_t tot;
+  block_excl_scan_u32<NT>(cnt, &tot, sm);
+  if (threadIdx.x < 4) {
+    uint64_t v = 0;
+    for (int k = 0; k < NT / WAVE; k++) v += sl[threadIdx.x][k];
+    atomicAdd((unsigned long long*)&sc->limbs[threadIdx.x], (unsigned long long)v);
+  }
+  if (threadIdx.x == 0) tilecnt[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------- pass C
+// Order-preserving compaction of candidates: key = MAXKEY - |x| bits (ascending
+// key == descending |x|), value = index.
+__global__ void k_compact(const double* __restrict__ x, int64_t n, const CompScalars* __restrict__ sc,
+                          const uint32_t* __restrict__ tileoff, uint64_t* __restrict__ keys,
+                          uint32_t* __restrict__ vals) {
+  __shared__ uint32_t wc[NT / WAVE];
+  __shared__ uint32_t s_run;
+  const uint32_t b1 = sc->b1;
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  if (threadIdx.x == 0) s_run = tileoff[blockIdx.x];
+  __syncthreads();
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t i = base + (int64_t)r * NT + threadIdx.x;
+    uint64_t yb = 0;
+    bool pred = false;
+    if (i < n) {
+      yb = dbits(x[i]) & MAXKEY;
+      pred = (yb >> 51) >= b1;
+    }
+    uint64_t bal = __ballot(pred);
+    uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wc[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = s_run;
+    for (int k = 0; k < w; k++) off += wc[k];
+    if (pred) {
+      keys[off + rank] = MAXKEY - yb;
+      vals[off + rank] = (uint32_t)i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int k = 0; k < NT / WAVE; k++) t += wc[k];
+      s_run += t;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- radix sort
+// Global histograms of all 8 byte digits (to skip constant digits).
+__global__ void k_sort_hist_all(const uint64_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[8][256];
+  for (int i = threadIdx.x; i < 8 * 256; i += NT) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
+    uint64_t k = keys[i];
+#pragma unroll
+    for (int d = 0; d < 8; d++) atomicAdd(&h[d][(k >> (8 * d)) & 255], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * 256; i += NT)
+    if ((&h[0][0])[i]) atomicAdd(&hist[i], (&h[0][0])[i]);
+}
+
+__global__ void k_sort_count(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                             uint32_t* __restrict__ blkhist, int64_t nblk) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t i = base + (int64_t)r * NT + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  blkhist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: tile order = index order (rounds of NT, wave order, lane order).
+__global__ void k_sort_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
+                               int shift, const uint32_t* __restrict__ blkoff, int64_t nblk) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[NT / WAVE][256];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  run[threadIdx.x] = blkoff[(int64_t)threadIdx.x * nblk + blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int r = 0; r < ITEMS; r++) {
+    for (int q = threadIdx.x; q < (NT / WAVE) * 256; q += NT) (&wcnt[0][0])[q] = 0;
+    __syncthreads();
+    int64_t i = base + (int64_t)r * NT + threadIdx.x;
+    bool valid = i < n;
+    uint64_t k = valid ? kin[i] : 0;
+    uint32_t v = valid ? vin[i] : 0;
+    uint32_t d = (uint32_t)((k >> shift) & 255);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      bool bit = (d >> b) & 1;
+      uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = run[d] + rank;
+      for (int q = 0; q < w; q++) off += wcnt[q][d];
+      kout[off] = k;
+      vout[off] = v;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int q = 0; q < NT / WAVE; q++) add += wcnt[q][threadIdx.x];
+    run[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- D test
+__device__ inline double key_to_y(uint64_t key) {
+  return __longlong_as_double((long long)(MAXKEY - key));
+}
+
+__global__ void k_dfx_tilesums(const uint64_t* __restrict__ keys, int64_t m, const CompScalars* __restrict__ sc,
+                               u128* __restrict__ tsum) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  const uint32_t ebmax = sc->ebmax;
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  u128 s = 0;
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t k = base + r;
+    if (k < m) s += fx_of(MAXKEY - keys[k], ebmax);
+  }
+  u128 tot;
+  block_excl_scan_u128<NT>(s, &tot, sm);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+// Single block, exclusive scan of u128 tile sums in place.
+__global__ void k_scan_u128_sums(u128* sums, int64_t nt, u128* total) {
+  __shared__ u128 sm[1024 / WAVE + 1];
+  const int64_t chunk = (nt + 1023) / 1024;
+  const int64_t lo = threadIdx.x * chunk, hi = lo + chunk < nt ? lo + chunk : nt;
+  u128 s = 0;
+  for (int64_t i = lo; i < hi; i++) s += sums[i];
+  u128 tot;
+  u128 ex = block_excl_scan_u128<1024>(s, &tot, sm);
+  for (int64_t i = lo; i < hi; i++) {
+    u128 t = sums[i];
+    sums[i] = ex;
+    ex += t;
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
+__global__ void k_dtest(const uint64_t* __restrict__ keys, int64_t m, CompScalars* sc,
+                        const u128* __restrict__ tsum) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  const uint32_t ebmax = sc->ebmax;
+  const int G = fx_grid(ebmax);
+  const u128 Ftot = limbs_value(sc->limbs);
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  u128 f[ITEMS];
+  u128 s = 0;
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t k = base + r;
+    f[r] = k < m ? fx_of(MAXKEY - keys[k], ebmax) : (u128)0;
+    s += f[r];
+  }
+  u128 tot;
+  u128 pre = block_excl_scan_u128<NT>(s, &tot, sm) + tsum[blockIdx.x];
+  unsigned long long first_fail = (unsigned long long)m;
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t k = base + r;
+    if (k < m) {
+      double Rk = u128_to_double(Ftot - pre, G);
+      double y = key_to_y(keys[k]);
+      bool pass = ((double)(m - k) * y >= Rk - y) && (Rk > 1e-14);
+      if (!pass && (unsigned long long)k < first_fail) first_fail = (unsigned long long)k;
+    }
+    pre += f[r];
+  }
+  if (first_fail < (unsigned long long)m) atomicMin(&sc->K, first_fail);
+}
+
+// Single block: R_K, n', D threshold.
+__global__ void k_dfinal(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t m,
+                         CompScalars* sc, const u128* __restrict__ tsum) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  const uint32_t ebmax = sc->ebmax;
+  const int G = fx_grid(ebmax);
+  const u128 Ftot = limbs_value(sc->limbs);
+  const int64_t K = (int64_t)sc->K;
+  // prefix of FX over sigma_0..sigma_{K-1}: tile prefix + partial sum inside
+  // the tile holding position K (the last tile when K == m)
+  const int64_t ntm = (m + TILE - 1) / TILE;
+  const int64_t t = (K / TILE < ntm) ? K / TILE : ntm - 1;
+  const int64_t base = t * TILE + (int64_t)threadIdx.x * ITEMS;
+  u128 s = 0;
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t k = base + r;
+    if (k < K && k < m) s += fx_of(MAXKEY - keys[k], ebmax);
+  }
+  u128 tot;
+  block_excl_scan_u128<NT>(s, &tot, sm);
+  if (threadIdx.x == 0) {
+    u128 pre = tsum[t] + tot;
+    sc->R0 = u128_to_double(Ftot, G);
+    sc->RK = u128_to_double(Ftot - pre, G);
+    sc->n_sampled = (uint64_t)(m - K);
+    if (K > 0) {
+      sc->thr_key = keys[K - 1];
+      sc->thr_idx = vals[K - 1];
+    } else {
+      sc->thr_key = 0;
+      sc->thr_idx = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- sampling
+__device__ inline double prob_of(double xv, uint64_t j, const CompScalars& c, double nsd) {
+  uint64_t yb = dbits(xv) & MAXKEY;
+  uint64_t key = MAXKEY - yb;
+  bool inD = c.K > 0 && (key < c.thr_key || (key == c.thr_key && j <= c.thr_idx));
+  return inD ? 0.0 : (fabs(xv) / c.RK) * nsd;
+}
+
+// pin != nullptr: sample from the given probabilities (sample_systematic,
+// src/fri_utils.cpp:129-159) instead of deriving them from x and D.
+__global__ void k_samp_tilesums(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
+                                CompScalars* sc, u128* __restrict__ tsum) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  const CompScalars c = *sc;
+  if (c.n_sampled == 0) return;
+  const double nsd = (double)c.n_sampled;
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  u128 s = 0;
+  uint32_t bad = 0;
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t j = base + r;
+    if (j < n) {
+      double p = pin ? pin[j] : prob_of(x[j], (uint64_t)j, c, nsd);
+      bad |= !(p <= 1.0);  // also catches NaN (remaining norm 0, quirk A.6)
+      s += fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
+    }
+  }
+  if (bad) atomicOr(&sc->pflag, 1u);
+  u128 tot;
+  block_excl_scan_u128<NT>(s, &tot, sm);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__device__ inline int64_t first_k_at_least(double P, double u) {
+  // smallest k >= 0 with fl(k + u) >= P (fl(k+u) is monotone in k); interval
+  // ends are finite and below 2^32 by construction
+  if (!(P < 4294967296.0)) return (int64_t)4294967296;
+  double g = floor(P - u) - 2.0;
+  int64_t k = g > 0.0 ? (int64_t)g : 0;
+  while ((double)k + u < P) k++;
+  return k;
+}
+
+__global__ void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
+                            const CompScalars* __restrict__ sc, const u128* __restrict__ tsum, double u,
+                            uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  const CompScalars c = *sc;
+  if (c.n_sampled == 0) return;
+  const double nsd = (double)c.n_sampled;
+  const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  double xv[ITEMS], p[ITEMS];
+  u128 f[ITEMS];
+  u128 s = 0;
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t j = base + r;
+    xv[r] = (j < n && x) ? x[j] : 0.0;
+    p[r] = j < n ? (pin ? pin[j] : prob_of(xv[r], (uint64_t)j, c, nsd)) : 0.0;
+    f[r] = fx_of(dbits(p[r] <= 1.0 ? p[r] : 0.0), 1023u);
+    s += f[r];
+  }
+  u128 tot;
+  u128 pre = block_excl_scan_u128<NT>(s, &tot, sm) + tsum[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    int64_t j = base + r;
+    u128 nxt = pre + f[r];
+    if (j < n && f[r] != 0) {
+      double Plo = u128_to_double(pre, -96), Phi = u128_to_double(nxt, -96);
+      if (Phi > Plo) {
+        int64_t klo = first_k_at_least(Plo, u), khi = first_k_at_least(Phi, u);
+        if (khi > ns) khi = ns;
+        for (int64_t k = klo; k < khi; k++) {
+          out_idx[K + k] = (uint32_t)j;
+          if (out_val) out_val[K + k] = xv[r] / p[r];
+        }
+      }
+    }
+    pre = nxt;
+  }
+}
+
+// Single thread: total interval end and the first not-found position.
+__global__ void k_samp_final(CompScalars* sc, const u128* __restrict__ total, double u) {
+  if (sc->n_sampled == 0) {
+    sc->Ptot = 0.0;
+    sc->k_notfound = 0;
+    sc->n_notfound = 0;
+    return;
+  }
+  double Ptot = u128_to_double(*total, -96);
+  int64_t kn = first_k_at_least(Ptot, u);
+  int64_t ns = (int64_t)sc->n_sampled;
+  sc->Ptot = Ptot;
+  sc->k_notfound = (uint64_t)(kn < ns ? kn : ns);
+  sc->n_notfound = (uint64_t)(kn < ns ? ns - kn : 0);
+}
+
+// Positions past the last interval keep index 0 (src/fri_utils.cpp:132).
+__global__ void k_samp_notfound(const double* __restrict__ x, const CompScalars* __restrict__ sc,
+                                uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
+  const CompScalars c = *sc;
+  const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
+  const int64_t k = (int64_t)c.k_notfound + (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (k >= ns) return;
+  out_idx[K + k] = 0;
+  if (out_val) out_val[K + k] = x[0] / prob_of(x[0], 0, c, (double)c.n_sampled);
+}
+
+// D part of the output (or all of it for "largest").
+__global__ void k_emit_sorted(const double* __restrict__ x, const uint32_t* __restrict__ vals,
+                              const CompScalars* __restrict__ sc, int64_t cnt_if_largest,
+                              uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
+  const int64_t cnt = cnt_if_largest >= 0 ? cnt_if_largest : (int64_t)sc->K;
+  const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (k >= cnt) return;
+  uint32_t j = vals[k];
+  out_idx[k] = j;
+  out_val[k] = x[j];
+}
+
+__global__ void k_init_scalars(CompScalars* sc, int64_t m) {
+  CompScalars z = {};
+  z.K = (unsigned long long)m;
+  *sc = z;
+}
+
+// ---------------------------------------------------------------- driver
+enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
+       WS_BH = 8, WS_TS = 9, WS_TOT = 10 };
+
+int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
+             double* out_val, CompressInfo* info, hipStream_t s) {
+  memset(info, 0, sizeof(*info));
+  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "compress: vector length must be in [1, 2^31)");
+  FRICC_CHECK(m > 0 && m <= n, "compress: need 0 < m <= n (fricc/friccsd.py:68-72)");
+  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC, "compress: unknown compression method");
+  FRICC_CHECK(u >= 0.0 && u <= 1.0, "compress: uniform must lie in [0, 1]");
+  void *p_sc, *p_hist, *p_tcnt, *p_k0, *p_v0, *p_k1, *p_v1, *p_sh, *p_tot;
+  const int64_t nt = ceil_div(n, TILE);
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_HIST, 4096 * 4 + 8 * 256 * 4, &p_hist));
+  FRICC_TRY(ws_get(WS_TCNT, (size_t)(nt + 1) * 4, &p_tcnt));
+  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  CompScalars* sc = (CompScalars*)p_sc;
+  uint32_t* hist = (uint32_t*)p_hist;
+  uint32_t* hist8 = hist + 4096;
+  uint32_t* tcnt = (uint32_t*)p_tcnt;
+
+  k_init_scalars<<<1, 1, 0, s>>>(sc, m);
+  FRICC_HIP(hipMemsetAsync(hist, 0, 4096 * 4 + 8 * 256 * 4, s));
+  const int gA = (int)std::min<int64_t>(2048, ceil_div(n, NT * 8));
+  k_hist_top<<<gA, NT, 0, s>>>(x, n, hist);
+  k_select_top<<<1, NT, 0, s>>>(hist, m, sc);
+  k_norm_count<<<nt, NT, 0, s>>>(x, n, sc, tcnt);
+  FRICC_HIP(hipGetLastError());
+  CompScalars h;
+  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  if (h.nonfinite) {
+    set_error("compress: non-finite value in the input vector");
+    info->nonfinite = 1;
+    return ENONFINITE;
+  }
+  const int64_t nc = (int64_t)h.n_cand;
+  FRICC_TRY(scan_u32(tcnt, tcnt, nt, nullptr, s));
+  FRICC_TRY(ws_get(WS_K0, (size_t)nc * 8, &p_k0));
+  FRICC_TRY(ws_get(WS_V0, (size_t)nc * 4, &p_v0));
+  FRICC_TRY(ws_get(WS_K1, (size_t)nc * 8, &p_k1));
+  FRICC_TRY(ws_get(WS_V1, (size_t)nc * 4, &p_v1));
+  uint64_t* kA = (uint64_t*)p_k0;
+  uint32_t* vA = (uint32_t*)p_v0;
+  uint64_t* kB = (uint64_t*)p_k1;
+  uint32_t* vB = (uint32_t*)p_v1;
+  k_compact<<<nt, NT, 0, s>>>(x, n, sc, tcnt, kA, vA);
+
+  // --- sort candidates (stable LSD, skipping constant digits)
+  const int gH = (int)std::min<int64_t>(1024, ceil_div(nc, NT * 8));
+  k_sort_hist_all<<<gH, NT, 0, s>>>(kA, nc, hist8);
+  uint32_t h8[8 * 256];
+  FRICC_HIP(hipMemcpyAsync(h8, hist8, sizeof(h8), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  const int64_t nblk = ceil_div(nc, TILE);
+  FRICC_TRY(ws_get(WS_SH, (size_t)nblk * 256 * 4, &p_sh));
+  uint32_t* bh = (uint32_t*)p_sh;
+  for (int d = 0; d < 8; d++) {
+    bool trivial = false;
+    for (int b = 0; b < 256; b++)
+      if ((int64_t)h8[d * 256 + b] == nc) trivial = true;
+    if (trivial) continue;
+    k_sort_count<<<nblk, NT, 0, s>>>(kA, nc, 8 * d, bh, nblk);
+    FRICC_TRY(scan_u32(bh, bh, nblk * 256, nullptr, s));
+    k_sort_scatter<<<nblk, NT, 0, s>>>(kA, vA, kB, vB, nc, 8 * d, bh, nblk);
+    std::swap(kA, kB);
+    std::swap(vA, vB);
+  }
+  FRICC_HIP(hipGetLastError());
+
+  if (method == COMP_LARGEST) {
+    k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, m, out_idx, out_val);
+    FRICC_HIP(hipGetLastError());
+    info->n_D = m;
+    info->n_sampled = 0;
+    return OK;
+  }
+
+  // --- D test over the first m sorted candidates
+  const int64_t ntm = ceil_div(m, TILE);
+  void* p_ts;
+  FRICC_TRY(ws_get(WS_TS, (size_t)(std::max(ntm, nt) + 1) * sizeof(u128), &p_ts));
+  u128* ts = (u128*)p_ts;
+  k_dfx_tilesums<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
+  k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, ntm, nullptr);
+  k_dtest<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
+  k_dfinal<<<1, NT, 0, s>>>(kA, vA, m, sc, ts);
+  k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, -1, out_idx, out_val);
+  // --- systematic sampling of the remainder
+  k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts);
+  k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
+  k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, u, out_idx, out_val);
+  k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot, u);
+  FRICC_HIP(hipGetLastError());
+  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  if (h.n_notfound > 0)
+    k_samp_notfound<<<ceil_div((int64_t)h.n_notfound, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
+  FRICC_HIP(hipGetLastError());
+  info->n_D = (int64_t)h.K;
+  info->n_sampled = (int64_t)h.n_sampled;
+  info->n_notfound = (int64_t)h.n_notfound;
+  info->norm0 = h.R0;
+  info->remaining_norm = h.RK;
+  info->p_total = h.Ptot;
+  info->n_candidates = nc;
+  if (h.pflag) {
+    set_error("compress: a sampling probability exceeds 1 (reference exits, src/fri_utils.cpp:383-395)");
+    return EPROB;
+  }
+  return OK;
+}
+
+// Systematic sampling of n_sample indices from given probabilities p[n]
+// (each in [0,1]) with the canonical interval ends.
+int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
+                      int64_t* n_notfound, hipStream_t s) {
+  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "sample_systematic: length must be in [1, 2^31)");
+  FRICC_CHECK(n_sample >= 0, "sample_systematic: negative sample count");
+  FRICC_CHECK(u >= 0.0 && u <= 1.0, "sample_systematic: uniform must lie in [0, 1]");
+  if (n_notfound) *n_notfound = 0;
+  if (n_sample == 0) return OK;
+  void *p_sc, *p_ts, *p_tot;
+  const int64_t nt = ceil_div(n, TILE);
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
+  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  CompScalars* sc = (CompScalars*)p_sc;
+  u128* ts = (u128*)p_ts;
+  CompScalars h = {};
+  h.K = 0;
+  h.n_sampled = (uint64_t)n_sample;
+  FRICC_HIP(hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, s));
+  k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts);
+  k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
+  k_samp_emit<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, u, out_idx, nullptr);
+  k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot, u);
+  FRICC_HIP(hipGetLastError());
+  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  if (h.n_notfound > 0)
+    k_samp_notfound<<<ceil_div((int64_t)h.n_notfound, NT), NT, 0, s>>>(p, sc, out_idx, nullptr);
+  FRICC_HIP(hipGetLastError());
+  if (n_notfound) *n_notfound = (int64_t)h.n_notfound;
+  if (h.pflag) {
+    set_error("sample_systematic: a probability exceeds 1");
+    return EPROB;
+  }
+  return OK;
+}
+
+}  // namespace fricc

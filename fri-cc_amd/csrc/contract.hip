@@ -1,0 +1,186 @@
+// Gathered dense-tile contraction: OUT[x, y] += alpha * sum_k A[rowlist[k], x] * B[k, y].
+//
+// Replaces the 11 loop nests of src/sparse_tensor.cpp:11-346.  The reference
+// loops over output blocks and rescans all m entries per block; here the
+// sparse operand is a View (entries gathered per output-column tile into a
+// dense [rows x TY] block) and the dense operand A is read row-by-row for the
+// gathered rows only, so each tile is one MFMA GEMM with a gathered K axis.
+//
+// Tiling (f64 / f32, v_mfma_*_16x16x4): a 256-thread workgroup owns a
+// 128 (x) x 64 (y) output tile; wave w owns x rows [32w, 32w+32) and all 64
+// columns: 2 x 4 MFMA 16x16 blocks.  Per K step (4 gathered rows) a lane
+// loads one A value per 16-row x block (16 consecutive x of one gathered row:
+// 128 B segments) and one B value per 16-column block.
+#include "fricc_internal.h"
+
+namespace fricc {
+
+typedef __attribute__((ext_vector_type(4))) double d4;
+typedef __attribute__((ext_vector_type(4))) float f4;
+
+struct OutDev {
+  double* out;
+  uint32_t Xl, Yl;
+  int64_t oxh, oxl, oyh, oyl;
+};
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+  typedef d4 type;
+  static __device__ inline d4 mfma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // D register i of lane group g holds row g + 4 i (measured on gfx950)
+  static __device__ inline int row(int g, int i) { return g + 4 * i; }
+};
+template <>
+struct Acc<float> {
+  typedef f4 type;
+  static __device__ inline f4 mfma(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // D register i of lane group g holds row 4 g + i
+  static __device__ inline int row(int g, int i) { return 4 * g + i; }
+};
+
+constexpr int GT_X = 128;
+constexpr int GT_Y = 64;
+
+// Blocks are numbered so that consecutive ids on one XCD (ids are dealt
+// round-robin over the 8 XCDs) walk the x tiles of one column tile, which
+// share the same B block and gathered A rows in that XCD's L2.
+// order 0: x tiles fastest (B block reused from L2); order 1: column tiles
+// fastest (the gathered A rows of one x range reused across column tiles).
+__device__ inline void xcd_remap(int64_t nbx, int64_t nty, int order, int64_t& bx, int64_t& ty) {
+  const int64_t nb = nbx * nty;
+  int64_t id = blockIdx.x;
+  const int64_t per = nb / 8;
+  if (per * 8 == nb) id = (id % 8) * per + id / 8;
+  if (order == 0) {
+    ty = id / nbx;
+    bx = id % nbx;
+  } else {
+    bx = id / nty;
+    ty = id % nty;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t X, const uint32_t* __restrict__ tile_off,
+                                               const uint32_t* __restrict__ rowlist, const T* __restrict__ B,
+                                               int64_t nbx, int64_t ntiles, int64_t Y, OutDev om, double alpha, int order) {
+  typedef typename Acc<T>::type AT;
+  int64_t bx, t;
+  xcd_remap(nbx, ntiles, order, bx, t);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t xw = bx * GT_X + 32 * w;
+  const uint32_t k_beg = tile_off[t], k_end = tile_off[t + 1];
+  AT acc[2][4];
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[s][q] = AT{0, 0, 0, 0};
+  int64_t xa0 = xw + lr, xa1 = xw + 16 + lr;
+  if (xa0 >= X) xa0 = X - 1;
+  if (xa1 >= X) xa1 = X - 1;
+  for (uint32_t k0 = k_beg; k0 < k_end; k0 += 4) {
+    const uint32_t kr = k0 + lk;
+    const int64_t row = rowlist[kr];
+    const T* ar = A + row * X;
+    T a0 = ar[xa0], a1 = ar[xa1];
+    const T* br = B + (int64_t)kr * GT_Y + lr;
+    T b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[q] = br[16 * q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      acc[0][q] = Acc<T>::mfma(a0, b[q], acc[0][q]);
+      acc[1][q] = Acc<T>::mfma(a1, b[q], acc[1][q]);
+    }
+  }
+  // D layout: lane holds column lane%16 of each 16x16 block, rows Acc<T>::row(lane/16, i)
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int64_t y = t * GT_Y + 16 * q + lr;
+    if (y >= Y) continue;
+    const uint32_t yy = (uint32_t)y;
+    const int64_t oy = (int64_t)(yy / om.Yl) * om.oyh + (int64_t)(yy % om.Yl) * om.oyl;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int64_t x = xw + 16 * s + Acc<T>::row(lk, i);
+        if (x < X) {
+          const uint32_t xx = (uint32_t)x;
+          double* o = om.out + (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy;
+          *o += alpha * (double)acc[s][q][i];
+        }
+      }
+    }
+  }
+}
+
+int g_order = 1;
+
+int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const OutMap& m, double alpha,
+                  hipStream_t s) {
+  FRICC_CHECK(v->TY == GT_Y, "gathered_gemm: view tile width must be 64");
+  FRICC_CHECK(v->dtype == adtype, "gathered_gemm: operand dtypes differ");
+  FRICC_CHECK(X > 0 && X < ((int64_t)1 << 31), "gathered_gemm: bad X");
+  if (v->ntiles == 0) return OK;
+  OutDev om;
+  om.out = m.out;
+  om.Xl = (uint32_t)m.Xl;
+  om.Yl = (uint32_t)v->Yl;
+  om.oxh = m.oxh;
+  om.oxl = m.oxl;
+  om.oyh = m.oyh;
+  om.oyl = m.oyl;
+  const int64_t nbx = ceil_div(X, GT_X);
+  const int64_t nb = nbx * v->ntiles;
+  if (adtype == DT_F64) {
+    k_ggemm<double><<<nb, 256, 0, s>>>((const double*)A, X, v->tile_off, v->rowlist, (const double*)v->B, nbx,
+                                       v->ntiles, v->Y, om, alpha, g_order);
+  } else if (adtype == DT_F32) {
+    k_ggemm<float><<<nb, 256, 0, s>>>((const float*)A, X, v->tile_off, v->rowlist, (const float*)v->B, nbx,
+                                      v->ntiles, v->Y, om, alpha, g_order);
+  } else {
+    FRICC_CHECK(false, "gathered_gemm: unsupported dtype");
+  }
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_permute(const double* __restrict__ W, int64_t Rl, int64_t Xh, int64_t Xl, int64_t s0, int64_t s1,
+                          int64_t s2, int64_t s3, T* __restrict__ A) {
+  const int64_t r = blockIdx.x;
+  const int64_t rh = r / Rl, rl = r % Rl;
+  const int64_t X = Xh * Xl;
+  const double* wr = W + rh * s0 + rl * s1;
+  T* ar = A + r * X;
+  for (int64_t x = threadIdx.x; x < X; x += blockDim.x) {
+    const int64_t xh = x / Xl, xl = x % Xl;
+    ar[x] = (T)wr[xh * s2 + xl * s3];
+  }
+}
+
+int permute_operand(const double* W, int64_t Rh, int64_t Rl, int64_t Xh, int64_t Xl, const int64_t st[4], void* A,
+                    int dtype, hipStream_t s) {
+  const int64_t R = Rh * Rl;
+  if (R == 0 || Xh * Xl == 0) return OK;
+  if (dtype == DT_F64)
+    k_permute<double><<<R, 256, 0, s>>>(W, Rl, Xh, Xl, st[0], st[1], st[2], st[3], (double*)A);
+  else if (dtype == DT_F32)
+    k_permute<float><<<R, 256, 0, s>>>(W, Rl, Xh, Xl, st[0], st[1], st[2], st[3], (float*)A);
+  else
+    FRICC_CHECK(false, "permute_operand: unsupported dtype");
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace fricc

@@ -1,0 +1,72 @@
+// Internal (C++) interfaces shared by the HIP translation units.
+#pragma once
+#include "common.h"
+
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+namespace fricc {
+
+enum { COMP_LARGEST = 0, COMP_FRI_SYSTEMATIC = 1 };
+enum { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
+
+struct CompressInfo {
+  int64_t n_D, n_sampled, n_notfound, n_candidates;
+  double norm0, remaining_norm, p_total;
+  int32_t nonfinite, pad;
+};
+
+int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
+             double* out_val, CompressInfo* info, hipStream_t s);
+int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
+                      int64_t* n_notfound, hipStream_t s);
+
+// A gathered view of a sparse 4-index tensor as a matrix Tm[r, y]:
+// rows r = (i_rh, i_rl), columns y = (i_yh, i_yl) (positions into the tensor's
+// four indices).  Columns are cut into tiles of TY; for each tile the rows
+// that hold any entry are gathered and stored densely ([rows x TY]).
+struct View {
+  int pos[4];          // rh, rl, yh, yl
+  int64_t Rh, Rl, Yh, Yl, R, Y;
+  int TY, kstep, dtype;
+  int64_t ntiles = 0, total_rows = 0, nnz = 0;
+  uint32_t* tile_off = nullptr;  // [ntiles + 1] first gathered row of each tile
+  uint32_t* rowlist = nullptr;   // [total_rows] r of each gathered row
+  void* B = nullptr;             // [total_rows x TY] values (dtype)
+  size_t cap_rows = 0;
+  uint64_t gen = 0;
+  ~View();
+};
+
+struct Sparse {
+  int64_t dims[4] = {0, 0, 0, 0};
+  int64_t nnz = 0, cap = 0;
+  uint32_t* idx = nullptr;   // flat index ((i*d1+j)*d2+a)*d3+b
+  double* val = nullptr;
+  CompressInfo info{};
+  int device = 0;
+  uint64_t gen = 1;  // bumped whenever the entries change
+  std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<View>> views;
+  ~Sparse();
+};
+
+int sparse_reserve(Sparse* sp, int64_t nnz);
+int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, View** out);
+
+// OUT[x, y] += alpha * sum_k A[rowlist[k], x] * B[k, y]   (per column tile)
+// A: row-major [R x X] in `adtype` (F64 or F32 here; BF16 uses its own path).
+// Output element (x, y) lives at out + (x/Xl)*oxh + (x%Xl)*oxl + (y/Yl)*oyh + (y%Yl)*oyl.
+struct OutMap {
+  double* out;
+  int64_t Xh, Xl, oxh, oxl, oyh, oyl;
+};
+int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const OutMap& om, double alpha,
+                  hipStream_t s);
+
+// A[r, x] = W[rh*s0 + rl*s1 + xh*s2 + xl*s3] (r = rh*Rl + rl, x = xh*Xl + xl) in dtype.
+int permute_operand(const double* W, int64_t Rh, int64_t Rl, int64_t Xh, int64_t Xl, const int64_t s[4],
+                    void* A, int dtype, hipStream_t s_);
+
+}  // namespace fricc

@@ -1,0 +1,116 @@
+// Error state, device workspace arena and a generic exclusive u32 scan.
+#include "common.h"
+
+#include <mutex>
+
+namespace fricc {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+static Workspace g_ws[16];
+static std::mutex g_ws_mu;
+
+// Grow-only per-device arena slot.  Contents are not preserved on growth.
+int ws_get(int slot, size_t bytes, void** out) {
+  int dev = 0;
+  FRICC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 16 || slot < 0 || slot >= 64) {
+    set_error("workspace slot/device out of range");
+    return EINVAL_;
+  }
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& w = g_ws[dev];
+  if (bytes == 0) bytes = 256;
+  if (w.cap[slot] < bytes) {
+    if (w.ptr[slot]) {
+      FRICC_HIP(hipDeviceSynchronize());
+      FRICC_HIP(hipFree(w.ptr[slot]));
+      w.ptr[slot] = nullptr;
+      w.cap[slot] = 0;
+    }
+    size_t b = bytes + bytes / 8 + 256;
+    hipError_t e = hipMalloc(&w.ptr[slot], b);
+    if (e != hipSuccess) {
+      set_error(std::string("workspace hipMalloc failed: ") + hipGetErrorString(e));
+      return ENOMEM_;
+    }
+    w.cap[slot] = b;
+  }
+  *out = w.ptr[slot];
+  return OK;
+}
+
+// --- exclusive scan of u32 ----------------------------------------------------
+constexpr int SCAN_NT = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_NT * SCAN_ITEMS;
+
+__global__ void k_scan_tile_sums(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sm[SCAN_NT / WAVE + 1];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (int r = 0; r < SCAN_ITEMS; r++) {
+    int64_t i = base + (int64_t)r * SCAN_NT + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  block_excl_scan_u32<SCAN_NT>(s, &tot, sm);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Single block: exclusive scan of the tile sums in place; writes the total.
+__global__ void k_scan_sums(uint32_t* sums, int64_t nt, uint32_t* total) {
+  __shared__ uint32_t sm[1024 / WAVE + 1];
+  const int64_t chunk = (nt + 1023) / 1024;
+  const int64_t lo = threadIdx.x * chunk, hi = lo + chunk < nt ? lo + chunk : nt;
+  uint32_t s = 0;
+  for (int64_t i = lo; i < hi; i++) s += sums[i];
+  uint32_t tot;
+  uint32_t ex = block_excl_scan_u32<1024>(s, &tot, sm);
+  for (int64_t i = lo; i < hi; i++) {
+    uint32_t t = sums[i];
+    sums[i] = ex;
+    ex += t;
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
+__global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
+                             const uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sm[SCAN_NT / WAVE + 1];
+  // blocked arrangement per thread: SCAN_ITEMS consecutive elements
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < SCAN_ITEMS; r++) {
+    int64_t i = base + r;
+    v[r] = i < n ? in[i] : 0u;
+    s += v[r];
+  }
+  uint32_t tot;
+  uint32_t ex = block_excl_scan_u32<SCAN_NT>(s, &tot, sm) + sums[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < SCAN_ITEMS; r++) {
+    int64_t i = base + r;
+    if (i < n) out[i] = ex;
+    ex += v[r];
+  }
+}
+
+int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, hipStream_t s) {
+  if (n <= 0) return OK;
+  const int64_t nt = ceil_div(n, SCAN_TILE);
+  void* p = nullptr;
+  FRICC_TRY(ws_get(63, (size_t)nt * 4, &p));
+  uint32_t* sums = (uint32_t*)p;
+  k_scan_tile_sums<<<nt, SCAN_NT, 0, s>>>(in, n, sums);
+  k_scan_sums<<<1, 1024, 0, s>>>(sums, nt, total_dev);
+  k_scan_tiles<<<nt, SCAN_NT, 0, s>>>(in, out, n, sums);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace fricc

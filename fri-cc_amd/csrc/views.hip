@@ -1,0 +1,218 @@
+// Device-resident SparseTensor4d storage and its gathered views.
+//
+// Storage replaces the AoS vector<array<size_t,4>> + vector<double> of
+// include/sparse_tensor.hpp:15-18 (40 B/entry) with SoA uint32 flat index +
+// FP64 value (12 B/entry) in compression order.  Instead of rescanning every
+// entry per output block (src/sparse_tensor.cpp, e.g. :63-77), each
+// contraction reads a view that groups the entries by output-column tile and
+// stores, per tile, the dense block of the rows that hold any entry.
+#include "fricc_internal.h"
+
+namespace fricc {
+
+constexpr int VNT = 256;
+
+View::~View() {
+  if (tile_off) (void)hipFree(tile_off);
+  if (rowlist) (void)hipFree(rowlist);
+  if (B) (void)hipFree(B);
+}
+
+Sparse::~Sparse() {
+  views.clear();
+  if (idx) (void)hipFree(idx);
+  if (val) (void)hipFree(val);
+}
+
+int sparse_reserve(Sparse* sp, int64_t nnz) {
+  if (sp->cap >= nnz && sp->idx) return OK;
+  if (sp->idx) FRICC_HIP(hipFree(sp->idx));
+  if (sp->val) FRICC_HIP(hipFree(sp->val));
+  sp->idx = nullptr;
+  sp->val = nullptr;
+  FRICC_HIP(hipMalloc(&sp->idx, (size_t)std::max<int64_t>(nnz, 1) * 4));
+  FRICC_HIP(hipMalloc(&sp->val, (size_t)std::max<int64_t>(nnz, 1) * 8));
+  sp->cap = nnz;
+  return OK;
+}
+
+struct VGeom {
+  uint32_t d1, d2, d3;
+  int rh, rl, yh, yl;
+  uint32_t Rl, Yl, TY;
+  uint32_t R;
+};
+
+__device__ inline void view_rc(uint32_t f, const VGeom& g, uint32_t& r, uint32_t& y) {
+  uint32_t i[4];
+  i[3] = f % g.d3;
+  f /= g.d3;
+  i[2] = f % g.d2;
+  f /= g.d2;
+  i[1] = f % g.d1;
+  i[0] = f / g.d1;
+  r = i[g.rh] * g.Rl + i[g.rl];
+  y = i[g.yh] * g.Yl + i[g.yl];
+}
+
+__global__ void k_mark(const uint32_t* __restrict__ idx, int64_t nnz, VGeom g, uint32_t* __restrict__ slot) {
+  int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
+  if (e >= nnz) return;
+  uint32_t r, y;
+  view_rc(idx[e], g, r, y);
+  slot[(int64_t)(y / g.TY) * g.R + r] = 1u;
+}
+
+__global__ void k_tile_count(const uint32_t* __restrict__ slot, uint32_t R, int kstep, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t sm[VNT / WAVE + 1];
+  const uint32_t* row = slot + (int64_t)blockIdx.x * R;
+  uint32_t c = 0;
+  for (uint32_t r = threadIdx.x; r < R; r += VNT) c += row[r];
+  uint32_t tot;
+  block_excl_scan_u32<VNT>(c, &tot, sm);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = (tot + kstep - 1) / kstep * kstep;
+}
+
+__global__ void k_tile_slots(uint32_t* __restrict__ slot, uint32_t R, const uint32_t* __restrict__ tile_off,
+                             uint32_t* __restrict__ rowlist) {
+  __shared__ uint32_t sm[VNT / WAVE + 1];
+  uint32_t* row = slot + (int64_t)blockIdx.x * R;
+  const uint32_t base = tile_off[blockIdx.x], end = tile_off[blockIdx.x + 1];
+  uint32_t run = base;
+  constexpr int IT = 16;
+  for (uint32_t c0 = 0; c0 < R; c0 += VNT * IT) {
+    uint32_t f[IT];
+    uint32_t s = 0;
+    const uint32_t r0 = c0 + threadIdx.x * IT;
+#pragma unroll
+    for (int q = 0; q < IT; q++) {
+      f[q] = (r0 + q < R) ? row[r0 + q] : 0u;
+      s += f[q];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan_u32<VNT>(s, &tot, sm) + run;
+#pragma unroll
+    for (int q = 0; q < IT; q++) {
+      if (f[q]) {
+        row[r0 + q] = ex;
+        rowlist[ex] = r0 + q;
+        ex++;
+      }
+    }
+    run += tot;
+  }
+  // padding rows (zero values) point at row 0
+  for (uint32_t k = run + threadIdx.x; k < end; k += VNT) rowlist[k] = 0u;
+}
+
+template <typename T>
+__global__ void k_fill(const uint32_t* __restrict__ idx, const double* __restrict__ val, int64_t nnz, VGeom g,
+                       const uint32_t* __restrict__ slot, T* __restrict__ B) {
+  int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
+  if (e >= nnz) return;
+  uint32_t r, y;
+  view_rc(idx[e], g, r, y);
+  uint32_t t = y / g.TY;
+  uint32_t k = slot[(int64_t)t * g.R + r];
+  atomicAdd(&B[(int64_t)k * g.TY + (y % g.TY)], (T)val[e]);
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ in, __hip_bfloat16* __restrict__ out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x;
+  if (i < n) out[i] = __float2bfloat16(in[i]);
+}
+
+static int build_view(Sparse* sp, View* v, hipStream_t s) {
+  const int64_t* d = sp->dims;
+  v->Rh = d[v->pos[0]];
+  v->Rl = d[v->pos[1]];
+  v->Yh = d[v->pos[2]];
+  v->Yl = d[v->pos[3]];
+  v->R = v->Rh * v->Rl;
+  v->Y = v->Yh * v->Yl;
+  v->ntiles = ceil_div(v->Y, v->TY);
+  v->nnz = sp->nnz;
+  VGeom g;
+  g.d1 = (uint32_t)d[1];
+  g.d2 = (uint32_t)d[2];
+  g.d3 = (uint32_t)d[3];
+  g.rh = v->pos[0];
+  g.rl = v->pos[1];
+  g.yh = v->pos[2];
+  g.yl = v->pos[3];
+  g.Rl = (uint32_t)v->Rl;
+  g.Yl = (uint32_t)v->Yl;
+  g.TY = (uint32_t)v->TY;
+  g.R = (uint32_t)v->R;
+  void *p_slot, *p_cnt;
+  const size_t nslot = (size_t)v->ntiles * v->R;
+  FRICC_TRY(ws_get(20, nslot * 4, &p_slot));
+  FRICC_TRY(ws_get(21, (size_t)(v->ntiles + 1) * 4, &p_cnt));
+  uint32_t* slot = (uint32_t*)p_slot;
+  uint32_t* cnt = (uint32_t*)p_cnt;
+  if (!v->tile_off) FRICC_HIP(hipMalloc(&v->tile_off, (size_t)(v->ntiles + 1) * 4));
+  FRICC_HIP(hipMemsetAsync(slot, 0, nslot * 4, s));
+  const int64_t nnz = sp->nnz;
+  if (nnz > 0) k_mark<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, nnz, g, slot);
+  k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
+  FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));
+  uint32_t total = 0;
+  FRICC_HIP(hipMemcpyAsync(&total, v->tile_off + v->ntiles, 4, hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  v->total_rows = total;
+  const size_t esz = v->dtype == DT_F64 ? 8 : (v->dtype == DT_F32 ? 4 : 2);
+  if (v->cap_rows < (size_t)total || !v->rowlist) {
+    if (v->rowlist) FRICC_HIP(hipFree(v->rowlist));
+    if (v->B) FRICC_HIP(hipFree(v->B));
+    v->rowlist = nullptr;
+    v->B = nullptr;
+    size_t cap = (size_t)total + (size_t)total / 8 + 64;
+    FRICC_HIP(hipMalloc(&v->rowlist, cap * 4));
+    FRICC_HIP(hipMalloc(&v->B, cap * (size_t)v->TY * esz));
+    v->cap_rows = cap;
+  }
+  k_tile_slots<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->tile_off, v->rowlist);
+  const size_t nB = (size_t)total * v->TY;
+  if (v->dtype == DT_F64) {
+    FRICC_HIP(hipMemsetAsync(v->B, 0, nB * 8, s));
+    if (nnz > 0) k_fill<double><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (double*)v->B);
+  } else if (v->dtype == DT_F32) {
+    FRICC_HIP(hipMemsetAsync(v->B, 0, nB * 4, s));
+    if (nnz > 0) k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)v->B);
+  } else {
+    void* p_tmp;
+    FRICC_TRY(ws_get(22, nB * 4, &p_tmp));
+    FRICC_HIP(hipMemsetAsync(p_tmp, 0, nB * 4, s));
+    if (nnz > 0) k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)p_tmp);
+    if (nB > 0) k_f32_to_bf16<<<ceil_div((int64_t)nB, VNT), VNT, 0, s>>>((float*)p_tmp, (__hip_bfloat16*)v->B, nB);
+  }
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, View** out) {
+  auto key = std::make_tuple(pos[0], pos[1], pos[2], pos[3], TY, dtype);
+  auto it = sp->views.find(key);
+  View* v = nullptr;
+  if (it != sp->views.end()) {
+    v = it->second.get();
+    if (v->gen == sp->gen) {
+      *out = v;
+      return OK;
+    }
+  } else {
+    std::unique_ptr<View> nv(new View());
+    for (int q = 0; q < 4; q++) nv->pos[q] = pos[q];
+    nv->TY = TY;
+    nv->kstep = dtype == DT_BF16 ? 32 : 4;
+    nv->dtype = dtype;
+    v = nv.get();
+    sp->views[key] = std::move(nv);
+  }
+  FRICC_TRY(build_view(sp, v, s));
+  v->gen = sp->gen;
+  *out = v;
+  return OK;
+}
+
+}  // namespace fricc

@@ -1,0 +1,183 @@
+"""Synthetic density-fitted molecular systems and GPU integral containers.
+
+The reference takes integrals from PySCF (``self.ao2mo``, fricc/friccsd.py:503-504),
+which is not available here.  The benchmark systems are synthetic:
+
+* orbital energies: occupied ~ U[-1.5,-0.3], virtual ~ U[0.1,3.0], each
+  sorted (``numpy.random.default_rng(seed)``, drawn in that order);
+* factors L[Q,p,q] = N(0,1) * exp(-|p-q|/10) / sqrt(naux), then
+  L <- (L + L^T_pq)/2;
+* (pq|rs) = sum_Q L[Q,p,q] L[Q,r,s], built on the GPU as a sequential fma chain
+  over Q (bitwise 8-fold symmetric; libfricc_hip ``fricc_eri_build``);
+* Fock = diag(orbital energies), so f_ov = 0 and the MP2 guess has t1 = 0.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+
+CONFIGS = {
+    # name: (nocc, nvir, naux, seed, m_keep, iterations)
+    "c2": dict(nocc=20, nvir=180, naux=600, seed=1, m_keep=1_000_000, iterations=30),
+    "c3": dict(nocc=32, nvir=256, naux=864, seed=1, m_keep=4_000_000, iterations=40),
+}
+
+
+def synthetic_factors(nocc, nvir, naux, seed):
+    """(L[naux,nmo,nmo], mo_energy[nmo]) of the synthetic recipe."""
+    nmo = nocc + nvir
+    rng = np.random.default_rng(seed)
+    eo = np.sort(rng.uniform(-1.5, -0.3, nocc))
+    ev = np.sort(rng.uniform(0.1, 3.0, nvir))
+    p = np.arange(nmo)
+    decay = np.exp(-np.abs(p[:, None] - p[None, :]) / 10.0) / np.sqrt(naux)
+    L = rng.standard_normal((naux, nmo, nmo)) * decay
+    L = 0.5 * (L + L.transpose(0, 2, 1))
+    return np.ascontiguousarray(L), np.concatenate([eo, ev])
+
+
+class SyntheticSystem:
+    """A synthetic closed-shell system (nocc doubly occupied orbitals)."""
+
+    def __init__(self, nocc, nvir, naux, seed):
+        self.nocc, self.nvir, self.naux, self.seed = int(nocc), int(nvir), int(naux), int(seed)
+        self.nmo = self.nocc + self.nvir
+        self.L, self.mo_energy = synthetic_factors(self.nocc, self.nvir, self.naux, self.seed)
+        self.fock = np.diag(self.mo_energy)
+        self.e_hf = 0.0
+
+    @classmethod
+    def from_config(cls, name):
+        c = CONFIGS[name]
+        return cls(c["nocc"], c["nvir"], c["naux"], c["seed"])
+
+
+def eri_block(Ld, naux, nmo, ranges, out, strides, stream=None):
+    """(pq|rs) for the four [lo, hi) ranges into ``out`` at the given element strides."""
+    import torch
+
+    assert out.dtype == torch.float64 and out.is_cuda
+    rng = _capi.I64x8(*[int(v) for r in ranges for v in r])
+    _capi.check(_capi.lib().fricc_eri_build(ctypes.c_void_p(Ld.data_ptr()), int(naux), int(nmo), rng,
+                                            _capi.i64x4(strides), ctypes.c_void_p(out.data_ptr()),
+                                            _capi.stream_ptr(stream)))
+    return out
+
+
+def _cstrides(shape):
+    s = [1, 1, 1, 1]
+    for i in (2, 1, 0):
+        s[i] = s[i + 1] * shape[i + 1]
+    return s
+
+
+class DeviceERIs:
+    """GPU-resident chemists'-notation integrals with the attributes
+    ``ccsd._ChemistsERIs`` exposes to update_amps (fricc/friccsd.py:80-86,
+    164-187, 310-330): fock, mo_energy, oooo, ovoo, ovov, oovv, ovvo,
+    get_ovvv(), plus ``ladder`` = W[c,d,a,b] = (ac|bd), the 2323 operand."""
+
+    def __init__(self):
+        self._prep = {}
+
+    @classmethod
+    def from_factors(cls, L, mo_energy, nocc, fock=None, build_ladder=True):
+        import torch
+
+        e = cls()
+        L = np.ascontiguousarray(L, dtype=np.float64)
+        naux, nmo, _ = L.shape
+        o, v = int(nocc), nmo - int(nocc)
+        e.nocc, e.nvir, e.nmo, e.naux = o, v, nmo, naux
+        dev = torch.device("cuda")
+        e.mo_energy = torch.as_tensor(np.asarray(mo_energy, dtype=np.float64), device=dev)
+        e.fock = torch.as_tensor(np.diag(mo_energy) if fock is None else np.asarray(fock, np.float64), device=dev)
+        Ld = torch.as_tensor(L, device=dev)
+        O, V = (0, o), (o, nmo)
+
+        def blk(r0, r1, r2, r3):
+            shape = [r[1] - r[0] for r in (r0, r1, r2, r3)]
+            t = torch.empty(shape, dtype=torch.float64, device=dev)
+            return eri_block(Ld, naux, nmo, (r0, r1, r2, r3), t, _cstrides(shape))
+
+        e.oooo = blk(O, O, O, O)
+        e.ovoo = blk(O, V, O, O)
+        e.ovov = blk(O, V, O, V)
+        e.oovv = blk(O, O, V, V)
+        e.ovvo = blk(O, V, V, O)
+        e.ovvv = blk(O, V, V, V)
+        if build_ladder:
+            # W[c,d,a,b] = (ac|bd): p=a, q=c, r=b, s=d
+            W = torch.empty((v, v, v, v), dtype=torch.float64, device=dev)
+            e.ladder = eri_block(Ld, naux, nmo, (V, V, V, V), W, [v, v ** 3, 1, v * v])
+        else:
+            e.ladder = None
+        e.L = Ld
+        return e
+
+    @classmethod
+    def from_blocks(cls, fock, mo_energy, nocc, oooo, ovoo, ovov, oovv, ovvo, ovvv, vvvv):
+        """From host chemists' blocks (e.g. a PySCF ``_ChemistsERIs`` with vvvv
+        unpacked to (v,v,v,v))."""
+        import torch
+
+        e = cls()
+        dev = torch.device("cuda")
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+        e.nocc = int(nocc)
+        e.mo_energy, e.fock = t(mo_energy), t(fock)
+        e.oooo, e.ovoo, e.ovov, e.oovv, e.ovvo, e.ovvv = t(oooo), t(ovoo), t(ovov), t(oovv), t(ovvo), t(ovvv)
+        e.nvir = e.ovov.shape[1]
+        e.nmo = e.nocc + e.nvir
+        # W[c,d,a,b] = (ac|bd) = vvvv[a,c,b,d]
+        e.ladder = t(vvvv).permute(1, 3, 0, 2).contiguous()
+        e.L = None
+        return e
+
+    def get_ovvv(self):
+        return self.ovvv
+
+    def prepared(self, dtype):
+        """Static operands in the layouts the update reads, cached per dtype."""
+        import torch
+
+        key = str(dtype)
+        if key in self._prep:
+            return self._prep[key]
+        o, v = self.nocc, self.nvir
+        c = lambda a: a.to(dtype).contiguous()
+        ovov, ovvv = self.ovov, self.ovvv
+        p = {}
+        # sparse-contraction operands A[r][x] (kernel layout, see fricc_contract_prepared)
+        p["A1323"] = c(ovov.permute(1, 3, 0, 2))          # [c,d,k,l] = (kc|ld)
+        p["A0112"] = c(ovov)                              # [l,d,k,c] = (ld|kc)
+        p["A0313"] = c(ovov.permute(0, 3, 2, 1))          # [l,d,k,c] = (lc|kd)
+        p["Gx"] = c(ovvv.permute(3, 1, 0, 2))             # [c,d,k,a] = (kd|ac)
+        p["Gy"] = c(ovvv.permute(1, 3, 0, 2))             # [c,d,k,b] = (kc|bd)
+        # dense-remainder operands
+        P1 = ovvv.permute(0, 1, 3, 2)                     # [k,c,d,a] = (kc|ad)
+        P2 = ovvv.permute(0, 3, 1, 2)                     # [k,c,d,a] = (kd|ac)
+        p["P1"] = c(P1)
+        p["P2"] = c(P2)
+        p["Z"] = c(2 * P2 - P1)
+        p["Qv"] = c(ovvv.permute(0, 2, 1, 3))             # [i,c,a,b] = (ia|cb)
+        p["Lov"] = c(2 * ovov.permute(0, 2, 3, 1) - ovov.permute(0, 2, 1, 3))   # [k,l,d,c]
+        p["Lov2"] = c(2 * ovov.permute(0, 2, 1, 3) - ovov.permute(0, 2, 3, 1))  # [k,l,c,d]
+        p["Lovov"] = c(2 * ovov.permute(0, 2, 1, 3) - ovov.permute(0, 2, 3, 1))  # [i,j,a,b]
+        p["ovovT"] = c(ovov.permute(0, 2, 1, 3))           # [i,j,a,b] = (ia|jb)
+        p["ovvo_p"] = c(self.ovvo.permute(0, 1, 3, 2))     # [k,c,i,a] = (kc|ai)
+        p["oovv_p"] = c(self.oovv.permute(0, 3, 1, 2))     # [k,c,i,a] = (ki|ac)
+        p["ovov"] = c(ovov)
+        p["ovoo"] = c(self.ovoo)
+        p["oovv"] = c(self.oovv)
+        p["ovvo"] = c(self.ovvo)
+        p["oooo"] = c(self.oooo)
+        if self.ladder is not None:
+            if dtype == torch.float64:
+                p["ladder"] = self.ladder
+            # sum over (c,d) of the bare part of Wvvvv (ac|bd)
+            p["s_ab"] = self.ladder.reshape(v * v, v * v).sum(0).reshape(v, v).to(dtype)
+        p["g_ka"] = ovvv.sum(dim=(1, 3)).to(dtype)         # sum_cd (kd|ac) = sum_cd (kc|ad)
+        self._prep[key] = p
+        return p

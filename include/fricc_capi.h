@@ -1,0 +1,145 @@
+/*
+ * fricc_capi.h -- C ABI of libfricc_hip.so, the MI355X (gfx950) FRI-CCSD hot path.
+ *
+ * This is the drop-in boundary that replaces the reference's pybind11 module
+ * `py_rccsd` (src/py_rccsd.cpp:16-55).  Plain C types only: device pointers
+ * are raw `double*` / `uint32_t*`, shapes and strides are `int64_t[4]` in
+ * elements, streams are `void*` (a hipStream_t; NULL = default stream).
+ * Every function returns an int status (FRICC_OK == 0); on failure
+ * fricc_last_error() describes it.  No function ever calls exit(): the cases
+ * where the reference terminates the process (bad compression/term string,
+ * p > 1, out-of-range index) return an error status instead.
+ *
+ * Threading: calls on one device are serialised by the caller; the library
+ * keeps a per-device workspace arena.
+ */
+#ifndef FRICC_CAPI_H
+#define FRICC_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRICC_ABI_VERSION 1
+
+enum fricc_status {
+  FRICC_OK = 0,
+  FRICC_EINVAL = 1,     /* bad argument (reference: exit or pybind TypeError)   */
+  FRICC_EDEVICE = 2,    /* HIP runtime error                                     */
+  FRICC_ENOMEM = 3,     /* device allocation failed                              */
+  FRICC_EPROB = 4,      /* a sampling probability exceeds 1 (src/fri_utils.cpp:383-395) */
+  FRICC_ENONFINITE = 5  /* non-finite value in the vector to compress            */
+};
+
+enum fricc_compression { FRICC_LARGEST = 0, FRICC_FRI_SYSTEMATIC = 1 };
+enum fricc_precision { FRICC_FP64 = 0, FRICC_FP32 = 1, FRICC_BF16 = 2 };
+
+typedef struct fricc_sparse fricc_sparse; /* device-resident SparseTensor4d */
+
+typedef struct fricc_compress_info {
+  int64_t n_D;            /* exactly kept entries |D|                         */
+  int64_t n_sampled;      /* n' = m - |D|                                      */
+  int64_t n_notfound;     /* samples past the last interval (index 0)          */
+  int64_t n_candidates;   /* elements that entered the top-m sort              */
+  double norm0;           /* 1-norm of x                                       */
+  double remaining_norm;  /* 1-norm left after D (denominator of p)            */
+  double p_total;         /* last interval end                                 */
+  int32_t nonfinite;
+  int32_t pad;
+} fricc_compress_info;
+
+int fricc_abi_version(void);
+const char* fricc_last_error(void);
+
+/* FRI compression of a device vector x[n] to m entries.
+ * Replaces fri_compression(x, m, "systematic") (src/fri_utils.cpp:439-522,
+ * include/fri_utils.hpp:30-32) and, for compression == FRICC_LARGEST, the
+ * argsort-top-m path of the SparseTensor4d ctor (include/sparse_tensor.hpp:46-56).
+ * `u` is the one uniform the reference draws per call as rand()/RAND_MAX
+ * (src/fri_utils.cpp:47); it is ignored for FRICC_LARGEST.
+ * out_idx/out_val: device arrays of m; flat indices, D first (|x| descending,
+ * ties by index), then the samples in ascending order.  n < 2^31. */
+int fricc_compress(const double* x, int64_t n, int64_t m, int compression, double u, uint32_t* out_idx,
+                   double* out_val, fricc_compress_info* info, void* stream);
+
+/* sample_systematic(n_sample, probs) (src/fri_utils.cpp:129-159) on device
+ * probabilities p[n] (each in [0,1]): S_k = j with P_j <= fl(k+u) < P_{j+1},
+ * P the canonical (exact, once-rounded) interval ends; k past the last
+ * interval keep index 0 (src/fri_utils.cpp:132).  out_idx: device array of
+ * n_sample.  *n_notfound (host) receives the number of such k. */
+int fricc_sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
+                            int64_t* n_notfound, void* stream);
+
+/* SparseTensor4d(dims, nnz) -- empty tensor (include/sparse_tensor.hpp:27-31). */
+int fricc_sparse_create(const int64_t dims[4], fricc_sparse** out);
+int fricc_sparse_destroy(fricc_sparse* sp);
+
+/* SparseTensor4d(tensor_arr, dims, m, compression, "systematic")
+ * (include/sparse_tensor.hpp:33-77): compress the device tensor x (C order,
+ * prod(dims) doubles) into sp, reusing its storage. */
+int fricc_sparse_compress(fricc_sparse* sp, const double* x, int64_t m, int compression, double u, void* stream);
+
+/* set_element for all entries at once (include/sparse_tensor.hpp:127-136):
+ * host arrays of nnz flat indices and values; indices must be < prod(dims). */
+int fricc_sparse_set(fricc_sparse* sp, const int64_t* idx, const double* val, int64_t nnz, void* stream);
+
+/* dimensions / size / compression statistics */
+int fricc_sparse_info(const fricc_sparse* sp, int64_t dims[4], int64_t* nnz, fricc_compress_info* info);
+
+/* get_element for all entries (include/sparse_tensor.hpp:138-143): copies the
+ * flat indices and values to host arrays of nnz. */
+int fricc_sparse_get(const fricc_sparse* sp, int64_t* idx, double* val, void* stream);
+
+/* device pointers of the entries (valid until the next compress/set/destroy) */
+int fricc_sparse_device(const fricc_sparse* sp, const uint32_t** idx, const double** val);
+
+/* contract_DTSpT(W, T, ouput, term) (src/sparse_tensor.cpp:349-382,
+ * src/py_rccsd.cpp:50-51): out += term(W, T), accumulated in place.
+ * term in {"0101","2323","1302","1202","1303","1203","1323","0112","0113",
+ * "0313","0312"}; W and out are device FP64 4-D arrays with arbitrary strides
+ * (in elements).  precision: FRICC_FP64, FRICC_FP32 (FP32 operands, FP32
+ * accumulate) or FRICC_BF16 (BF16 operands on MFMA, FP32 accumulate). */
+int fricc_contract(const char* term, const double* W, const int64_t W_shape[4], const int64_t W_strides[4],
+                   fricc_sparse* T, double* out, const int64_t out_shape[4], const int64_t out_strides[4],
+                   int precision, void* stream);
+
+/* Same contraction with the dense operand already laid out as the kernel
+ * reads it: A[r][x] row-major, r = the contracted index pair in T order,
+ * x = W's free index pair in output order, element type per precision
+ * (double / float / bf16).  Used by the device-resident update, which builds
+ * its intermediates directly in this layout.  No counterpart in the reference. */
+int fricc_contract_prepared(const char* term, const void* A, const int64_t W_shape[4], fricc_sparse* T,
+                            double* out, const int64_t out_shape[4], const int64_t out_strides[4], int precision,
+                            void* stream);
+
+/* General form of the two calls above for any contraction of a dense 4-index
+ * operand with T over two indices: spec "wwww,tttt->oooo" (one letter per
+ * axis, exactly two letters shared by W and T), out += alpha * einsum(spec).
+ * prepared != 0: W_or_A is the kernel layout A[r][x] (see above) in the
+ * precision's element type; otherwise an FP64 array with W_strides.
+ * Used for the t1-dressing of the ladder (cc_Wvvvv, fricc/friccsd.py:325-331)
+ * applied to the sparse T instead of materialising W. */
+int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int prepared, const int64_t W_shape[4],
+                        const int64_t W_strides[4], fricc_sparse* T, double* out, const int64_t out_shape[4],
+                        const int64_t out_strides[4], int precision, void* stream);
+
+/* (pq|rs) = sum_Q L[Q,p,q] L[Q,r,s] (sequential fma chain over Q) for
+ * p in [rng[0],rng[1]), q in [rng[2],rng[3]), r in [rng[4],rng[5]),
+ * s in [rng[6],rng[7]); written at out[(p-rng[0])*st[0] + (q-rng[2])*st[1] +
+ * (r-rng[4])*st[2] + (s-rng[6])*st[3]].  L: device [naux][nmo][nmo].
+ * Integral provider for synthetic density-fitted systems; the reference takes
+ * PySCF integrals instead (fricc/friccsd.py:503-504). */
+int fricc_eri_build(const double* L, int64_t naux, int64_t nmo, const int64_t rng[8], const int64_t strides[4],
+                    double* out, void* stream);
+
+/* Block ordering of the gathered contraction GEMM: 0 = x tiles fastest,
+ * 1 = column tiles fastest (default).  Tuning knob; no counterpart in the
+ * reference. */
+int fricc_set_gemm_order(int order);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FRICC_CAPI_H */

@@ -1,0 +1,422 @@
+// CPU oracle for the FRI-CCSD hot path.  TEST INFRASTRUCTURE ONLY.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library, and only as the checker (or as the timed CPU baseline).
+// The product path (fri-cc_amd/) never links or calls it.
+//
+// It restates, by semantics, the reference's native hot path:
+//   * FRI compression            src/fri_utils.cpp:11-36, 45-58, 129-159,
+//                                164-171, 367-398, 400-437, 439-522
+//   * SparseTensor4d             include/sparse_tensor.hpp:33-77, 102-143
+//   * the 11 contraction kernels src/sparse_tensor.cpp:11-382
+//
+// Two compression modes:
+//   CANONICAL (mode 0) - the definition the GPU implements bit-for-bit:
+//     every sum that feeds a decision is an exact fixed-point sum, rounded
+//     once to FP64 (round-to-nearest-even), so its value does not depend on
+//     summation order or thread count.
+//       - 1-norm R0: sum of FX(|x_i|) on the grid 2^G, G = max(eb_max,1)-1119
+//         (eb_max = biased exponent of max|x|; elements are truncated at
+//         2^-96 relative to the largest binade).
+//       - D loop: R_k = round(Ftot - sum_{l<k} FX(y_sigma_l)); the test
+//         (m-k)*y >= R_k - y && R_k > 1e-14 is evaluated in FP64 exactly as
+//         src/fri_utils.cpp:415 does.
+//       - systematic intervals: P_j = round(sum_{l<j} FXP(p_l) * 2^-96),
+//         FXP(p) = floor(p * 2^96).
+//     Ties in |x| are ordered by ascending flat index (what a stable merge
+//     sort of (x_i, i) pairs built in index order yields, src/fri_utils.cpp:13-25).
+//   FAITHFUL (mode 1) - the reference's own order of operations: sequential
+//     one_norm, sequential subtraction in the D loop (src/fri_utils.cpp:411-421),
+//     std::partial_sum intervals (src/fri_utils.cpp:142-143).
+// Both modes then run the reference's forward scan over the intervals
+// (src/fri_utils.cpp:147-157) literally, including the "not found -> index 0"
+// behaviour of the zero-initialised S vector (src/fri_utils.cpp:132).
+//
+// Parity status of each function is recorded in the header comment of the
+// function and pinned by tests/test_oracle_*.py.
+
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include <math.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+#include <string>
+
+typedef unsigned __int128 u128;
+
+extern "C" {
+
+struct orc_info {
+  int64_t n_D;            // |D|, exactly kept entries
+  int64_t n_sampled;      // n' = m - |D|
+  int64_t n_notfound;     // samples that fell outside every interval (index 0)
+  double norm0;           // 1-norm before the D loop
+  double remaining_norm;  // R after the D loop (denominator of p)
+  double p_total;         // last interval end P_N
+  int32_t status;         // 0 ok, 1 p>1 (reference exits), 2 bad args, 3 non-finite
+  int32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Fixed-point helpers (canonical mode).  Pinned by tests/test_oracle_fri.py
+// against Python's exact rational arithmetic (math.fsum / fractions).
+
+static inline uint64_t dbits(double v) { uint64_t b; memcpy(&b, &v, 8); return b; }
+
+// FX(y) on the grid 2^G with G = max(eb_max,1) - 1119: exact for every value
+// within 2^44 of the top binade, truncated below.
+static inline u128 fx_of(uint64_t ybits, uint32_t eb_max) {
+  uint32_t eb = (uint32_t)((ybits >> 52) & 0x7ff);
+  uint64_t M = (ybits & ((1ull << 52) - 1)) | (eb ? (1ull << 52) : 0ull);
+  int s = (int)std::max(eb, 1u) - (int)std::max(eb_max, 1u) + 44;
+  if (eb == 0x7ffu) return 0;
+  if (s > 74) s = 74;
+  if (s >= 0) return ((u128)M) << s;
+  int r = -s;
+  return r >= 64 ? (u128)0 : (u128)(M >> r);
+}
+static inline int fx_grid(uint32_t eb_max) { return (int)std::max(eb_max, 1u) - 1119; }
+
+// Round-to-nearest-even conversion of v * 2^G to double.
+static inline double u128_to_double(u128 v, int G) {
+  if (v == 0) return 0.0;
+  uint64_t hi = (uint64_t)(v >> 64), lo = (uint64_t)v;
+  int nb = hi ? 128 - __builtin_clzll(hi) : 64 - __builtin_clzll(lo);
+  uint64_t mant;
+  int e = 0;
+  if (nb <= 53) {
+    mant = lo;
+  } else {
+    int sh = nb - 53;
+    mant = (uint64_t)(v >> sh);
+    u128 rem = v & ((((u128)1) << sh) - 1);
+    u128 half = ((u128)1) << (sh - 1);
+    if (rem > half || (rem == half && (mant & 1ull))) {
+      mant++;
+      if (mant == (1ull << 53)) { mant >>= 1; sh++; }
+    }
+    e = sh;
+  }
+  return ldexp((double)mant, e + G);
+}
+
+double orc_norm_canonical(const double* x, int64_t n) {
+  uint32_t ebm = 0;
+  for (int64_t i = 0; i < n; i++) ebm = std::max(ebm, (uint32_t)((dbits(x[i]) >> 52) & 0x7ff));
+  u128 F = 0;
+  for (int64_t i = 0; i < n; i++) F += fx_of(dbits(x[i]) & 0x7fffffffffffffffull, ebm);
+  return u128_to_double(F, fx_grid(ebm));
+}
+
+// P[0..n] canonical interval ends for probabilities p (all in [0,1]).
+void orc_prefix_canonical(const double* p, int64_t n, double* P) {
+  u128 F = 0;
+  P[0] = 0.0;
+  for (int64_t j = 0; j < n; j++) {
+    F += fx_of(dbits(p[j]), 1023u);
+    P[j + 1] = u128_to_double(F, -96);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sigma: indices sorted by |x| descending, ties by ascending index.
+static void sigma_sort(const double* x, int64_t n, std::vector<int64_t>& sig, int64_t keep) {
+  sig.resize(n);
+  std::iota(sig.begin(), sig.end(), (int64_t)0);
+  auto cmp = [x](int64_t a, int64_t b) {
+    double fa = fabs(x[a]), fb = fabs(x[b]);
+    if (fa != fb) return fa > fb;
+    return a < b;
+  };
+  if (keep < n) {
+    std::partial_sort(sig.begin(), sig.begin() + keep, sig.end(), cmp);
+    sig.resize(keep);
+  } else {
+    std::sort(sig.begin(), sig.end(), cmp);
+  }
+}
+
+// Reference forward scan of sample_systematic (src/fri_utils.cpp:129-159)
+// over the given interval ends P[0..n] (uk_i = i*1.0 + u, lin_space_add_const
+// src/fri_utils.cpp:45-58 with step (n'-0)/n' == 1.0).
+static int64_t systematic_scan(int64_t n_sample, const double* P, int64_t n, double u,
+                               std::vector<int64_t>& S) {
+  S.assign(n_sample, 0);
+  int64_t last = 0, notfound = 0;
+  for (int64_t i = 0; i < n_sample; i++) {
+    double uk = (double)i * 1.0 + u;
+    bool found = false;
+    for (int64_t j = last; j < n; j++) {
+      if (uk >= P[j] && uk < P[j + 1]) { S[i] = j; last = j; found = true; break; }
+    }
+    if (!found) notfound++;
+  }
+  return notfound;
+}
+
+// FRI / largest compression.  method 0 = "largest" (include/sparse_tensor.hpp:46-56),
+// method 1 = "fri" + "systematic" (include/sparse_tensor.hpp:58-67).
+int orc_compress(const double* x, int64_t n, int64_t m, int method, int mode, double u,
+                 int64_t* out_idx, double* out_val, orc_info* info) {
+  memset(info, 0, sizeof(*info));
+  if (n <= 0 || m <= 0 || m > n) { info->status = 2; return 2; }
+  for (int64_t i = 0; i < n; i++)
+    if (!std::isfinite(x[i])) { info->status = 3; return 3; }
+  std::vector<int64_t> sig;
+  if (method == 0) {
+    sigma_sort(x, n, sig, m);
+    for (int64_t k = 0; k < m; k++) { out_idx[k] = sig[k]; out_val[k] = x[sig[k]]; }
+    info->n_D = m;
+    return 0;
+  }
+  // get_d_largest (src/fri_utils.cpp:400-437)
+  sigma_sort(x, n, sig, m);
+  int64_t K = m;
+  double R = 0.0;
+  if (mode == 0) {
+    uint32_t ebm = 0;
+    for (int64_t i = 0; i < n; i++) ebm = std::max(ebm, (uint32_t)((dbits(x[i]) >> 52) & 0x7ff));
+    const int G = fx_grid(ebm);
+    u128 Ftot = 0;
+    for (int64_t i = 0; i < n; i++) Ftot += fx_of(dbits(x[i]) & 0x7fffffffffffffffull, ebm);
+    info->norm0 = u128_to_double(Ftot, G);
+    u128 Fpre = 0;
+    for (int64_t k = 0; k < m; k++) {
+      double Rk = u128_to_double(Ftot - Fpre, G);
+      double y = fabs(x[sig[k]]);
+      if ((double)(m - k) * y >= Rk - y && Rk > 1e-14) {
+        Fpre += fx_of(dbits(y), ebm);
+      } else { K = k; break; }
+    }
+    R = u128_to_double(Ftot - Fpre, G);
+  } else {
+    double r = 0.0;
+    for (int64_t i = 0; i < n; i++) r += fabs(x[i]);
+    info->norm0 = r;
+    for (int64_t k = 0; k < m; k++) {
+      double y = fabs(x[sig[k]]);
+      if ((double)(m - k) * y >= r - y && r > 1e-14) { r -= y; }
+      else { K = k; break; }
+    }
+    R = r;
+  }
+  info->n_D = K;
+  info->remaining_norm = R;
+  const int64_t ns = m - K;
+  info->n_sampled = ns;
+  // make_probability_vector (src/fri_utils.cpp:367-398)
+  std::vector<double> p(n);
+  const double nsd = (double)ns;
+  for (int64_t i = 0; i < n; i++) p[i] = fabs(x[i]) / R * nsd;
+  for (int64_t k = 0; k < K; k++) p[sig[k]] = 0.0;
+  // p > 1 makes the reference exit (src/fri_utils.cpp:383-395); a NaN p
+  // (remaining norm 0 when m exceeds the nonzeros, see the guard at :415)
+  // makes it emit index 0 with a non-finite value.  Both are reported.
+  for (int64_t i = 0; i < n; i++)
+    if (!(p[i] <= 1.0)) { info->status = 1; return 1; }
+  std::vector<double> P(n + 1);
+  if (mode == 0) {
+    orc_prefix_canonical(p.data(), n, P.data());
+  } else {
+    P[0] = 0.0;
+    std::partial_sum(p.begin(), p.end(), P.begin() + 1);
+  }
+  info->p_total = P[n];
+  std::vector<int64_t> S;
+  info->n_notfound = systematic_scan(ns, P.data(), n, u, S);
+  // wrap-up (src/fri_utils.cpp:494-504)
+  for (int64_t k = 0; k < K; k++) { out_idx[k] = sig[k]; out_val[k] = x[sig[k]]; }
+  for (int64_t i = 0; i < ns; i++) {
+    int64_t j = S[i];
+    out_idx[K + i] = j;
+    out_val[K + i] = x[j] / p[j];
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The 11 contraction kernels, in the loop structure of src/sparse_tensor.cpp
+// (output-block parallel loops that rescan every sparse entry).  Used as the
+// timed CPU baseline and as a restatement checked against np.einsum by
+// tests/test_oracle_contract.py (the reference's own test identities,
+// fricc/tests/test_sparse_tensors.py:48-269).
+//
+// idx4: nnz x 4 unpacked indices (SparseTensor4d::unpack_idx,
+// include/sparse_tensor.hpp:108-125), val: nnz values.  Strides in elements.
+// block_lo/block_hi restrict the outer collapsed loop to a sub-range of
+// output blocks (used by the bounded CPU-baseline sample); pass 0/-1 for all.
+
+#define W4(a, b, c, d) W[(a) * ws[0] + (b) * ws[1] + (c) * ws[2] + (d) * ws[3]]
+#define O4(a, b, c, d) out[(a) * os[0] + (b) * os[1] + (c) * os[2] + (d) * os[3]]
+
+int orc_contract_ref(const char* term, const double* W, const int64_t* wshape, const int64_t* ws,
+                     const int64_t* idx4, const double* val, int64_t nnz, const int64_t* tdims,
+                     double* out, const int64_t* os, int64_t block_lo, int64_t block_hi) {
+  const int64_t no = tdims[0], nv = tdims[2];
+  std::string t(term);
+  auto run = [&](int64_t d0, int64_t d1, auto body) {
+    int64_t nb = d0 * d1;
+    int64_t lo = std::max<int64_t>(0, block_lo), hi = block_hi < 0 ? nb : std::min(nb, block_hi);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t blk = lo; blk < hi; blk++) body(blk / d1, blk % d1);
+  };
+  if (t == "0101") {  // src/sparse_tensor.cpp:12-37
+    run(nv, nv, [&](int64_t a, int64_t b) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[2] == a && id[3] == b) {
+          int64_t k = id[0], l = id[1];
+          for (int64_t i = 0; i < no; i++)
+            for (int64_t j = 0; j < no; j++) O4(i, j, a, b) += W4(k, l, i, j) * val[s];
+        }
+      }
+    });
+  } else if (t == "2323") {  // :40-86
+    const int64_t da = wshape[0], db = wshape[1];
+    run(tdims[0], tdims[1], [&](int64_t i, int64_t j) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[0] == i && id[1] == j) {
+          int64_t c = id[2], d = id[3];
+          for (int64_t a = 0; a < da; a++)
+            for (int64_t b = 0; b < db; b++) O4(i, j, a, b) += W4(a, b, c, d) * val[s];
+        }
+      }
+    });
+  } else if (t == "1302") {  // :89-115
+    run(nv, no, [&](int64_t b, int64_t j) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[3] == b && id[1] == j) {
+          int64_t k = id[0], c = id[2];
+          for (int64_t i = 0; i < no; i++)
+            for (int64_t a = 0; a < nv; a++) O4(i, j, a, b) += 2 * W4(a, k, i, c) * val[s];
+        }
+      }
+    });
+  } else if (t == "1202") {  // :118-144
+    run(nv, no, [&](int64_t b, int64_t j) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[3] == b && id[1] == j) {
+          int64_t k = id[0], c = id[2];
+          for (int64_t i = 0; i < no; i++)
+            for (int64_t a = 0; a < nv; a++) O4(i, j, a, b) -= W4(a, k, c, i) * val[s];
+        }
+      }
+    });
+  } else if (t == "1303") {  // :147-173
+    run(nv, no, [&](int64_t b, int64_t j) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[2] == b && id[1] == j) {
+          int64_t k = id[0], c = id[3];
+          for (int64_t i = 0; i < no; i++)
+            for (int64_t a = 0; a < nv; a++) O4(i, j, a, b) += W4(a, k, i, c) * val[s];
+        }
+      }
+    });
+  } else if (t == "1203") {  // :176-202
+    run(nv, no, [&](int64_t a, int64_t j) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[2] == a && id[1] == j) {
+          int64_t k = id[0], c = id[3];
+          for (int64_t i = 0; i < no; i++)
+            for (int64_t b = 0; b < nv; b++) O4(i, j, a, b) += W4(b, k, c, i) * val[s];
+        }
+      }
+    });
+  } else if (t == "1323") {  // :205-230
+    run(no, no, [&](int64_t i, int64_t j) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[0] == i && id[1] == j) {
+          int64_t c = id[2], d = id[3];
+          for (int64_t k = 0; k < no; k++)
+            for (int64_t l = 0; l < no; l++) O4(k, l, i, j) += W4(k, c, l, d) * val[s];
+        }
+      }
+    });
+  } else if (t == "0112") {  // :233-259
+    run(nv, no, [&](int64_t a, int64_t i) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[3] == a && id[0] == i) {
+          int64_t l = id[1], d = id[2];
+          for (int64_t k = 0; k < no; k++)
+            for (int64_t c = 0; c < nv; c++) O4(a, k, i, c) -= 0.5 * W4(l, d, k, c) * val[s];
+        }
+      }
+    });
+  } else if (t == "0313") {  // :262-289
+    run(nv, no, [&](int64_t a, int64_t i) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[2] == a && id[0] == i) {
+          int64_t l = id[1], d = id[3];
+          for (int64_t k = 0; k < no; k++)
+            for (int64_t c = 0; c < nv; c++) O4(a, k, i, c) -= 0.5 * W4(l, c, k, d) * val[s];
+        }
+      }
+    });
+  } else if (t == "0113") {  // :292-317
+    run(nv, no, [&](int64_t a, int64_t i) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[2] == a && id[0] == i) {
+          int64_t l = id[1], d = id[3];
+          for (int64_t k = 0; k < no; k++)
+            for (int64_t c = 0; c < nv; c++) O4(a, k, i, c) += W4(l, d, k, c) * val[s];
+        }
+      }
+    });
+  } else if (t == "0312") {  // :320-346
+    run(nv, no, [&](int64_t a, int64_t i) {
+      for (int64_t s = 0; s < nnz; s++) {
+        const int64_t* id = idx4 + 4 * s;
+        if (id[3] == a && id[0] == i) {
+          int64_t l = id[1], d = id[2];
+          for (int64_t k = 0; k < no; k++)
+            for (int64_t c = 0; c < nv; c++) O4(a, k, c, i) -= 0.5 * W4(l, c, k, d) * val[s];
+        }
+      }
+    });
+  } else {
+    return 2;  // the reference exits (src/sparse_tensor.cpp:378-381)
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Two-electron integrals (pq|rs) = sum_Q L[Q,p,q] L[Q,r,s] as a sequential
+// fused-multiply-add chain over Q = 0..naux-1 starting from 0.  Because
+// fma(a,b,c) == fma(b,a,c) and L is bitwise symmetric in (p,q), every member
+// of an 8-fold orbit gets bitwise the same value.  The GPU builder uses the
+// same chain, so the integrals are bit-identical between the two.
+// Writes out[(p-p0), (q-q0), (r-r0), (s-s0)] row-major.
+void orc_eri_block(const double* L, int64_t naux, int64_t nmo, int64_t p0, int64_t p1,
+                   int64_t q0, int64_t q1, int64_t r0, int64_t r1, int64_t s0, int64_t s1,
+                   double* out) {
+  const int64_t np = p1 - p0, nq = q1 - q0, nr = r1 - r0, ns = s1 - s0;
+  const int64_t n2 = nmo * nmo;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int64_t p = 0; p < np; p++)
+    for (int64_t q = 0; q < nq; q++)
+      for (int64_t r = 0; r < nr; r++)
+        for (int64_t s = 0; s < ns; s++) {
+          const double* a = L + (p + p0) * nmo + (q + q0);
+          const double* b = L + (r + r0) * nmo + (s + s0);
+          double acc = 0.0;
+          for (int64_t Q = 0; Q < naux; Q++) acc = fma(a[Q * n2], b[Q * n2], acc);
+          out[((p * nq + q) * nr + r) * ns + s] = acc;
+        }
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+"""Device-resident FRICCSD.update_amps vs the oracle's NumPy restatement of
+fricc/friccsd.py:29-301 on the same integrals (bit-identical fma-chain ERIs),
+the same MP2 guess (bit-identical) and the same uniforms.
+
+Bar: sampled index sets identical; t1/t2 within 1e-10 relative (FP64 plain);
+"mixed" precision within its stated tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    from fricc.friccsd import FRICCSD
+    from fricc.systems import SyntheticSystem
+    from oracle import rccsd
+
+    assert torch.cuda.is_available()
+    return torch, FRICCSD, SyntheticSystem, rccsd
+
+
+def _pair(env, o, v, naux, seed, settings, us):
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    sysm = SyntheticSystem(o, v, naux, seed)
+    cc = FRICCSD(sysm, settings, uniforms=iter(list(us)))
+    cc.level_shift = settings.get("level_shift", 0.0)
+    eris = cc.ao2mo()
+    oeris, _ = rccsd.synthetic_eris(o, v, naux, seed)
+    s = rccsd.default_settings(**{k: settings[k] for k in settings if k in ("m_keep", "compression",
+                                                                            "compressed_contractions",
+                                                                            "vvvv_t1t1", "level_shift")})
+    return cc, eris, oeris, s
+
+
+def _perturbed(t1, t2, seed):
+    rng = np.random.default_rng(seed)
+    t1 = t1 + 0.02 * rng.standard_normal(t1.shape)
+    t2 = t2 + 0.005 * rng.standard_normal(t2.shape)
+    return t1, t2
+
+
+@pytest.mark.parametrize("ccs", [["O^2V^4", "O^3V^3", "O^4V^2"], ["O^2V^4"], ["O^3V^3"], ["O^4V^2"], []])
+@pytest.mark.parametrize("comp", ["fri", "largest"])
+def test_update_amps_fp64_matches_oracle(env, ccs, comp):
+    torch = env[0]
+    o, v, naux = 4, 13, 30
+    settings = {"m_keep": 900, "compression": comp, "sampling_method": "systematic",
+                "compressed_contractions": ccs}
+    us = [0.31, 0.77]
+    cc, eris, oeris, s = _pair(env, o, v, naux, 5, settings, us)
+    _, t1, t2 = env[3].init_amps(oeris)
+    gt1, gt2 = cc.get_init_guess(eris)
+    assert np.array_equal(gt2.cpu().numpy(), t2), "MP2 guess must be bit-identical"
+    t1, t2 = _perturbed(t1, t2, 1)  # nonzero t1 exercises every t1 term
+    rt1, rt2, (ts, ts_alt) = env[3].update_amps(t1, t2, oeris, s, iter(us), return_sparse=True)
+    dt1, dt2 = cc.update_amps(torch.tensor(t1, device="cuda"), torch.tensor(t2, device="cuda"), eris)
+    if ccs:
+        assert np.array_equal(cc._slots[0].indices(), ts.idx)
+    if ts_alt is not None:
+        assert np.array_equal(cc._slots[1].indices(), ts_alt.idx)
+    e2 = np.abs(dt2.cpu().numpy() - rt2).max() / np.abs(rt2).max()
+    e1 = np.abs(dt1.cpu().numpy() - rt1).max() / np.abs(rt1).max()
+    assert e2 < 1e-10 and e1 < 1e-10, (e1, e2)
+
+
+def test_update_amps_rccsd_t1t1_mode(env):
+    torch = env[0]
+    settings = {"m_keep": 900, "compression": "largest", "vvvv_t1t1": "rccsd"}
+    cc, eris, oeris, s = _pair(env, 3, 10, 25, 2, settings, [])
+    _, t1, t2 = env[3].init_amps(oeris)
+    t1, t2 = _perturbed(t1, t2, 2)
+    rt1, rt2 = env[3].update_amps(t1, t2, oeris, s, iter([]))
+    dt1, dt2 = cc.update_amps(torch.tensor(t1, device="cuda"), torch.tensor(t2, device="cuda"), eris)
+    assert np.abs(dt2.cpu().numpy() - rt2).max() / np.abs(rt2).max() < 1e-10
+
+
+def test_update_amps_mixed_precision(env):
+    torch = env[0]
+    o, v, naux = 6, 24, 40
+    us = [0.42, 0.58]
+    base = {"m_keep": 6000, "compression": "fri", "sampling_method": "systematic"}
+    cc64, eris, oeris, s = _pair(env, o, v, naux, 3, base, us)
+    ccm, erism, _, _ = _pair(env, o, v, naux, 3, dict(base, precision="mixed"), us)
+    t1, t2 = cc64.get_init_guess(eris)
+    a1, a2 = cc64.update_amps(t1, t2, eris)
+    b1, b2 = ccm.update_amps(t1, t2, erism)
+    rel = (b2 - a2).abs().max().item() / a2.abs().max().item()
+    assert rel < 5e-6, rel  # FP32 operands and accumulation
+    assert np.array_equal(cc64._slots[0].indices(), ccm._slots[0].indices())
+
+
+def test_kernel_trajectory_matches_oracle(env):
+    """Five FRI-CCSD iterations (level shift 2.0): energies agree with the oracle
+    loop on the same uniforms.  Compressions after the first act on GPU and CPU
+    t2 that differ in the last bits, so index sets may differ at a rounding
+    boundary; energies are compared to 1e-9 relative."""
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    o, v, naux = 4, 14, 40
+    us = list(np.random.default_rng(9).random(20))
+    settings = {"m_keep": 1500, "compression": "fri", "sampling_method": "systematic", "level_shift": 2.0}
+    cc, eris, oeris, s = _pair(env, o, v, naux, 8, settings, us)
+    cc.max_cycle = 5
+    cc.conv_tol = 0
+    cc.conv_tol_normt = 0
+    cc.kernel()
+    _, _, _, _, en = rccsd.kernel(oeris, s, iter(us), max_cycle=5, tol=0, tolnormt=0)
+    assert len(cc.energies) == 5
+    np.testing.assert_allclose(cc.energies, en, rtol=1e-9)
+
+
+def test_nan_stops_the_loop(env):
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    cc = FRICCSD(SyntheticSystem(3, 8, 20, 1), {"m_keep": 300, "compressed_contractions": []},
+                 uniforms=iter(np.random.default_rng(0).random(100)))
+    eris = cc.ao2mo()
+    t1, t2 = cc.get_init_guess(eris)
+    t2 = t2.clone()
+    t2[0, 0, 0, 0] = float("nan")
+    cc.max_cycle = 3
+    with pytest.raises(Exception):
+        # compression rejects a non-finite amplitude vector (the reference would
+        # propagate NaN into the energy and stop at fricc/friccsd.py:444-447)
+        cc.kernel(t1=t1, t2=t2)
+
+
+def test_bad_settings(env):
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    sysm = SyntheticSystem(2, 4, 8, 1)
+    with pytest.raises(ValueError):
+        FRICCSD(sysm, {"compressed_contractions": ["O^5V^5"]})
+    cc = FRICCSD(sysm, {"m_keep": 0})
+    eris = cc.ao2mo()
+    t1, t2 = cc.get_init_guess(eris)
+    with pytest.raises(ValueError):
+        cc.update_amps(t1, t2, eris)
+    cc = FRICCSD(sysm, {"m_keep": 10 ** 9})
+    with pytest.raises(ValueError):
+        cc.update_amps(t1, t2, eris)
