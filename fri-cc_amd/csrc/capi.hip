@@ -289,6 +289,23 @@ int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int 
   return run_contract(p, A, dt, T, out, out_strides, s);
 }
 
+int fricc_sparse_prepare(fricc_sparse* T, const char* spec, const int64_t W_shape[4], const int64_t out_shape[4],
+                         int precision, void* stream) {
+  FRICC_CHECK(T != nullptr, "sparse_prepare: null tensor");
+  Plan p;
+  if (spec && strlen(spec) == 4) {
+    FRICC_TRY(make_plan(spec, W_shape, T->dims, out_shape, &p));
+  } else {
+    char w[5], t[5], o[5];
+    FRICC_TRY(parse_spec(spec, w, t, o));
+    FRICC_TRY(make_plan_spec(w, t, o, 1.0, W_shape, T->dims, out_shape, &p));
+  }
+  int dt;
+  FRICC_TRY(dtype_of(precision, &dt));
+  View* v = nullptr;
+  return get_view(T, p.pos, 64, dt, (hipStream_t)stream, &v);
+}
+
 // Block ordering of the gathered GEMM (0: x tiles fastest, 1: column tiles
 // fastest).  Tuning knob; no counterpart in the reference.
 int fricc_set_gemm_order(int order) {

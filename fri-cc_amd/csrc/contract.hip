@@ -45,7 +45,7 @@ struct Acc<float> {
   static __device__ inline int row(int g, int i) { return 4 * g + i; }
 };
 
-constexpr int GT_X = 128;
+constexpr int GT_X = 256;
 constexpr int GT_Y = 64;
 
 // Blocks are numbered so that consecutive ids on one XCD (ids are dealt
@@ -67,6 +67,42 @@ __device__ inline void xcd_remap(int64_t nbx, int64_t nty, int order, int64_t& b
   }
 }
 
+// Each wave owns a 64 (x) x 64 (y) tile = 4 x 4 MFMA blocks; the 4 waves of a
+// workgroup split 256 consecutive x.  K advances in chunks of 16 gathered rows
+// (4 MFMA K steps); the loads of chunk c+1 are issued into a second register
+// set before the MFMAs of chunk c, so memory latency overlaps the matrix pipe.
+// K steps (of 4 gathered rows) per pipeline stage: FP64 fragments are twice
+// as wide in registers, so they stage half as many rows.
+template <typename T>
+struct Stage {
+  static constexpr int KS = 4;
+};
+template <>
+struct Stage<double> {
+  static constexpr int KS = 2;
+};
+
+template <typename T>
+struct Frag {
+  T a[Stage<T>::KS][4];  // [k step][x block]
+  T b[Stage<T>::KS][4];  // [k step][y block]
+};
+
+template <typename T>
+__device__ inline void load_chunk(Frag<T>& f, const T* __restrict__ A, int64_t X, const uint32_t* __restrict__ rowlist,
+                                  const T* __restrict__ B, uint32_t kc, const int64_t xa[4], int lr, int lk) {
+#pragma unroll
+  for (int j = 0; j < Stage<T>::KS; j++) {
+    const uint32_t kr = kc + 4 * j + lk;
+    const T* ar = A + (int64_t)rowlist[kr] * X;
+#pragma unroll
+    for (int s = 0; s < 4; s++) f.a[j][s] = ar[xa[s]];
+    const T* br = B + (int64_t)kr * GT_Y + lr;
+#pragma unroll
+    for (int q = 0; q < 4; q++) f.b[j][q] = br[16 * q];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t X, const uint32_t* __restrict__ tile_off,
                                                const uint32_t* __restrict__ rowlist, const T* __restrict__ B,
@@ -76,29 +112,45 @@ __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t 
   xcd_remap(nbx, ntiles, order, bx, t);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t xw = bx * GT_X + 32 * w;
+  const int64_t xw = bx * GT_X + 64 * w;
   const uint32_t k_beg = tile_off[t], k_end = tile_off[t + 1];
-  AT acc[2][4];
+  AT acc[4][4];
 #pragma unroll
-  for (int s = 0; s < 2; s++)
+  for (int s = 0; s < 4; s++)
 #pragma unroll
     for (int q = 0; q < 4; q++) acc[s][q] = AT{0, 0, 0, 0};
-  int64_t xa0 = xw + lr, xa1 = xw + 16 + lr;
-  if (xa0 >= X) xa0 = X - 1;
-  if (xa1 >= X) xa1 = X - 1;
-  for (uint32_t k0 = k_beg; k0 < k_end; k0 += 4) {
-    const uint32_t kr = k0 + lk;
-    const int64_t row = rowlist[kr];
-    const T* ar = A + row * X;
-    T a0 = ar[xa0], a1 = ar[xa1];
-    const T* br = B + (int64_t)kr * GT_Y + lr;
-    T b[4];
+  int64_t xa[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) b[q] = br[16 * q];
+  for (int s = 0; s < 4; s++) {
+    xa[s] = xw + 16 * s + lr;
+    if (xa[s] >= X) xa[s] = X - 1;
+  }
+  constexpr uint32_t KCH = 4 * Stage<T>::KS;  // rows per stage; views pad tiles to 16 rows
+  if (k_beg < k_end) {
+    Frag<T> f0, f1;
+    load_chunk(f0, A, X, rowlist, B, k_beg, xa, lr, lk);
+    uint32_t kc = k_beg;
+    while (true) {
+      const bool more = kc + KCH < k_end;
+      if (more) load_chunk(f1, A, X, rowlist, B, kc + KCH, xa, lr, lk);
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      acc[0][q] = Acc<T>::mfma(a0, b[q], acc[0][q]);
-      acc[1][q] = Acc<T>::mfma(a1, b[q], acc[1][q]);
+      for (int j = 0; j < Stage<T>::KS; j++)
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) acc[s][q] = Acc<T>::mfma(f0.a[j][s], f0.b[j][q], acc[s][q]);
+      if (!more) break;
+      kc += KCH;
+      const bool more2 = kc + KCH < k_end;
+      if (more2) load_chunk(f0, A, X, rowlist, B, kc + KCH, xa, lr, lk);
+#pragma unroll
+      for (int j = 0; j < Stage<T>::KS; j++)
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) acc[s][q] = Acc<T>::mfma(f1.a[j][s], f1.b[j][q], acc[s][q]);
+      if (!more2) break;
+      kc += KCH;
     }
   }
   // D layout: lane holds column lane%16 of each 16x16 block, rows Acc<T>::row(lane/16, i)
@@ -109,7 +161,7 @@ __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t 
     const uint32_t yy = (uint32_t)y;
     const int64_t oy = (int64_t)(yy / om.Yl) * om.oyh + (int64_t)(yy % om.Yl) * om.oyl;
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
+    for (int s = 0; s < 4; s++) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int64_t x = xw + 16 * s + Acc<T>::row(lk, i);

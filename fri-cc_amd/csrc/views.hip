@@ -204,7 +204,7 @@ int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, Vie
     std::unique_ptr<View> nv(new View());
     for (int q = 0; q < 4; q++) nv->pos[q] = pos[q];
     nv->TY = TY;
-    nv->kstep = dtype == DT_BF16 ? 32 : 4;
+    nv->kstep = dtype == DT_BF16 ? 32 : 16;
     nv->dtype = dtype;
     v = nv.get();
     sp->views[key] = std::move(nv);

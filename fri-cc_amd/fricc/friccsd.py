@@ -75,6 +75,75 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+class SectionTimer:
+    """HIP-event timing of update_amps sections on the current stream
+    (enabled with ``cc.timer = SectionTimer()``)."""
+
+    def __init__(self):
+        self.events = {}
+
+    def start(self, name):
+        import torch
+
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.events.setdefault(name, []).append([e, None])
+
+    def stop(self, name):
+        import torch
+
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.events[name][-1][1] = e
+
+    def ms(self):
+        """{section: [ms per call]} (synchronizes)."""
+        import torch
+
+        torch.cuda.synchronize()
+        return {k: [a.elapsed_time(b) for a, b in v if b is not None] for k, v in self.events.items()}
+
+
+class _NoTimer:
+    def start(self, name):
+        pass
+
+    def stop(self, name):
+        pass
+
+
+def _prepare(T, spec, wshape, oshape, precision):
+    _capi.check(_capi.lib().fricc_sparse_prepare(T.handle(), spec.encode(), _capi.i64x4(wshape),
+                                                 _capi.i64x4(oshape), _capi.PRECISIONS[precision],
+                                                 _capi.stream_ptr()))
+
+
+def _mm_splitk(A, B, chunk=8192):
+    """A (M x K) @ B (K x N), both contiguous, for large K and small M*N: K is cut
+    into S chunks run as one batched GEMM, then summed (a plain GEMM of this
+    shape leaves most of the GPU idle)."""
+    torch = _torch()
+    M, K = A.shape
+    N = B.shape[1]
+    S = max(1, K // chunk)
+    while K % S:
+        S -= 1
+    Kc = K // S
+    return torch.bmm(A.reshape(M, S, Kc).transpose(0, 1), B.reshape(S, Kc, N)).sum(0)
+
+
+def _mm_splitk_nt(A, B, chunk=8192):
+    """A (M x K) @ B^T for B (N x K), both contiguous row-major."""
+    torch = _torch()
+    M, K = A.shape
+    N = B.shape[0]
+    S = max(1, K // chunk)
+    while K % S:
+        S -= 1
+    Kc = K // S
+    return torch.bmm(A.reshape(M, S, Kc).transpose(0, 1), B.reshape(N, S, Kc).permute(1, 2, 0)).sum(0)
+
+
 def _tr(t):
     """tmp.transpose(1,0,3,2)"""
     return t.permute(1, 0, 3, 2)
@@ -92,6 +161,7 @@ def update_amps(cc, t1, t2, eris):
     ccs = st["compressed_contractions"]
     sp = cc.sparse_precision
     timings = cc._timings = {}
+    tm = cc.timer if getattr(cc, "timer", None) is not None else _NoTimer()
     nocc, nvir = t1.shape
     o, v = nocc, nvir
     f64 = torch.float64
@@ -112,6 +182,7 @@ def update_amps(cc, t1, t2, eris):
 
     # ---- compression (fricc/friccsd.py:99-138)
     t0 = time.perf_counter()
+    tm.start("compress")
     comp = st["compression"]
     draw = (lambda: next(cc.uniforms)) if comp == "fri" else (lambda: 0.0)
     ts = cc._slot(0, t2.shape).recompress(t2, int(m_keep), comp, u=draw())
@@ -126,13 +197,15 @@ def update_amps(cc, t1, t2, eris):
             else:
                 cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
             ts_alt = cc._slot(1, t2.shape).recompress(cc._t2_avg, int(m_keep), comp, u=draw())
+    tm.stop("compress")
     timings["compress"] = time.perf_counter() - t0
 
     # ---- Foo, Fvv, Fov (imd.cc_Foo / cc_Fvv / cc_Fov)
     ovov = P["ovov"]
     Fov_ = 2 * torch.einsum("kcld,ld->kc", ovov, t1c) - torch.einsum("kdlc,ld->kc", ovov, t1c)
-    Fki = P["Lov2"].reshape(o, -1) @ t2c.reshape(o, -1).T + Fov_ @ t1c.T
-    Fac = -(t2c.permute(2, 0, 1, 3).reshape(v, -1) @ P["Lov"].reshape(-1, v)) - t1c.T @ Fov_
+    # Fki = sum_lcd Lov2[k,(lcd)] t2[i,(lcd)];  Fac = -sum_kl t2[kl] @ Lov[kl]
+    Fki = _mm_splitk_nt(P["Lov2"].reshape(o, -1), t2c.reshape(o, -1)) + Fov_ @ t1c.T
+    Fac = -torch.bmm(t2c.reshape(o * o, v, v), P["Lov"].reshape(o * o, v, v)).sum(0) - t1c.T @ Fov_
     Foo_u = C(Fki) + foo
     Fvv_u = C(Fac) + fvv
     Fov = C(Fov_) + fov
@@ -150,12 +223,14 @@ def update_amps(cc, t1, t2, eris):
     t1new += fov
     t1new += 2 * torch.einsum("kcai,kc->ia", eris.ovvo, t1) - torch.einsum("kiac,kc->ia", eris.oovv, t1)
     # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] @ Z[(kcd),a]
-    t1new += C(t2c.reshape(o, -1) @ P["Z"].reshape(-1, v))
+    t1new += C(_mm_splitk(t2c.reshape(o, -1), P["Z"].reshape(-1, v)))
     # sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac)
     LvvT1 = C(torch.matmul(t1c.reshape(o, 1, 1, v), P["Z"]).sum(0).reshape(v, v).T)
     t1new += t1 @ LvvT1.T
     ovoo = eris.ovoo
-    t1new += -2 * torch.einsum("lcki,klac->ia", ovoo, t2) + torch.einsum("kcli,klac->ia", ovoo, t2)
+    # -2 lcki,klac + kcli,klac  ==  sum_kl Aoo[kl] (i x c) @ t2[kl]^T (c x a)
+    Aoo = (-2 * ovoo.permute(2, 0, 3, 1) + ovoo.permute(0, 2, 3, 1)).reshape(o * o, o, v)
+    t1new += torch.bmm(Aoo, t2.reshape(o * o, v, v).transpose(1, 2)).sum(0)
     t1new += -2 * torch.einsum("lcki,lc,ka->ia", ovoo, t1, t1) + torch.einsum("kcli,lc,ka->ia", ovoo, t1, t1)
 
     # ---- T2 equation, t1-driven part (fricc/friccsd.py:178-187)
@@ -234,7 +309,10 @@ def update_amps(cc, t1, t2, eris):
     X = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)
     Y = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)
     if "O^2V^4" in ccs:
+        _prepare(ts, "2323", (v, v, v, v), (o, o, v, v), sp)
+        tm.start("ladder")
         _contract("2323", cc._ladder_operand(eris), (v, v, v, v), ts, t2new, sp)
+        tm.stop("ladder")
         _contract_spec("kdac,ijcd->ijka", 1.0, P["Gx"], (o, v, v, v), ts, X, sp)
         _contract_spec("kcbd,ijcd->ijkb", 1.0, P["Gy"], (o, v, v, v), ts, Y, sp)
     else:
@@ -253,6 +331,9 @@ def update_amps(cc, t1, t2, eris):
     # ---- O^3V^3 (fricc/friccsd.py:243-279)
     t0 = time.perf_counter()
     if "O^3V^3" in ccs:
+        _prepare(ts, "1302", (v, o, o, v), (o, o, v, v), sp)
+        _prepare(ts, "1303", (v, o, o, v), (o, o, v, v), sp)
+        tm.start("O3V3")
         Av = A_voov.to(cdt) if sp != "fp64" else A_voov
         Avo = A_vovo.to(cdt) if sp != "fp64" else A_vovo
         tmp = torch.zeros_like(t2new)
@@ -265,6 +346,7 @@ def update_amps(cc, t1, t2, eris):
         tmp = torch.zeros_like(t2new)
         _contract("1203", Avo, (v, o, v, o), ts, tmp, sp)
         t2new -= tmp + _tr(tmp)
+        tm.stop("O3V3")
     else:
         tmp = 2 * torch.einsum("akic,kjcb->ijab", Wvoov, t2)
         tmp -= torch.einsum("akci,kjcb->ijab", Wvovo, t2)
@@ -303,7 +385,7 @@ def init_amps(eris):
     eia = mo_e[:o, None] - mo_e[None, o:]
     t1 = eris.fock[:o, o:] / eia
     t2 = (eris.prepared(torch.float64)["ovovT"] / (eia[:, None, :, None] + eia[None, :, None, :])).contiguous()
-    emp2 = 2 * torch.einsum("ijab,iajb", t2, eris.ovov) - torch.einsum("jiab,iajb", t2, eris.ovov)
+    emp2 = (t2 * eris.prepared(torch.float64)["Lovov"]).sum()
     return float(emp2.item()), t1, t2
 
 
@@ -409,6 +491,7 @@ class FRICCSD:
         self._eris = None
         self.uniforms = uniforms if uniforms is not None else GlibcUniforms(settings.get("seed"))
         self.vvvv_t1t1 = settings["vvvv_t1t1"]
+        self.timer = None
         torch = _torch()
         mixed = settings["precision"] == "mixed"
         self.sparse_precision = "fp32" if mixed else "fp64"

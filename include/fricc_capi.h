@@ -134,6 +134,13 @@ int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int 
 int fricc_eri_build(const double* L, int64_t naux, int64_t nmo, const int64_t rng[8], const int64_t strides[4],
                     double* out, void* stream);
 
+/* Build (or reuse) the gathered view of T that a term or spec needs, so a
+ * following contraction runs its GEMM only (timing, graph capture).  The view
+ * is rebuilt automatically when T's entries change.  No counterpart in the
+ * reference (which rescans T inside every kernel, src/sparse_tensor.cpp:66-68). */
+int fricc_sparse_prepare(fricc_sparse* T, const char* term_or_spec, const int64_t W_shape[4],
+                         const int64_t out_shape[4], int precision, void* stream);
+
 /* Block ordering of the gathered contraction GEMM: 0 = x tiles fastest,
  * 1 = column tiles fastest (default).  Tuning knob; no counterpart in the
  * reference. */
