@@ -68,9 +68,36 @@ def run_mode(eris, cfg, mode, steps, warmup, seed):
     dt = time.perf_counter() - t0
     sec = cc.timer.ms()
     info = cc.sparse_info[0]
-    return {"it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "p50_ms": 1e3 * _pct(lat, 50),
+    cc.final_amplitudes = (t1, t2)
+    return {"state": (t1, t2), "it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "p50_ms": 1e3 * _pct(lat, 50),
             "p99_ms": 1e3 * _pct(lat, 99), "sections_ms": {k: sum(v) / len(v) for k, v in sec.items() if v},
             "energies": energies, "n_D": info["n_D"], "n_sampled": info["n_sampled"]}
+
+
+def precision_error(eris, cfg, mode, t1, t2, seed):
+    """One step from the same (t1, t2) with the same uniforms in FP64 and in
+    ``mode``: both compress the same T2 (compression is FP64 in every mode),
+    so the difference is the arithmetic alone."""
+    import torch
+
+    from fricc.friccsd import FRICCSD
+    from fricc.uniforms import FixedUniforms
+
+    us = [0.3141592653589793, 0.2718281828459045]
+    out = {}
+    settings = {"m_keep": cfg["m_keep"], "compression": "fri", "sampling_method": "systematic"}
+    c64 = FRICCSD(eris, dict(settings, precision="fp64"), uniforms=FixedUniforms(us))
+    cm = FRICCSD(eris, dict(settings, precision=mode), uniforms=FixedUniforms(us))
+    c64.level_shift = cm.level_shift = 2.0
+    a1, a2 = c64.update_amps(t1, t2, eris)
+    b1, b2 = cm.update_amps(t1, t2, eris)
+    out["t2_max_rel"] = (b2 - a2).abs().max().item() / a2.abs().max().item()
+    out["t1_max_rel"] = (b1 - a1).abs().max().item() / max(a1.abs().max().item(), 1e-300)
+    out["dE_corr"] = abs(c64.energy(a1, a2, eris) - cm.energy(b1, b2, eris))
+    out["E_corr_fp64"] = c64.energy(a1, a2, eris)
+    del a1, a2, b1, b2
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -79,7 +106,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--modes", default="mixed,fp64")
+    ap.add_argument("--modes", default="mixed,fp32,fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     args = ap.parse_args()
@@ -104,13 +131,20 @@ def main():
     res = {}
     for mode in args.modes.split(","):
         res[mode] = run_mode(eris, cfg, mode, args.steps, args.warmup, args.seed)
-        print(f"# {mode}: {json.dumps({k: v for k, v in res[mode].items() if k != 'energies'})}", file=sys.stderr,
-              flush=True)
+        print(f"# {mode}: {json.dumps({k: v for k, v in res[mode].items() if k not in ('energies', 'state')})}",
+              file=sys.stderr, flush=True)
+    errs = {}
+    if "fp64" in res:
+        t1s, t2s = res[list(res)[0]]["state"]
+        for mode in res:
+            if mode != "fp64":
+                errs[mode] = precision_error(eris, cfg, mode, t1s, t2s, args.seed)
+                print(f"# error {mode}: {errs[mode]}", file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in res else list(res)[0]
     r = res[head]
     ladder_flops = 2.0 * m * v * v
     lad_ms = r["sections_ms"].get("ladder")
-    prec = "fp32" if head == "mixed" else "fp64"
+    prec = {"mixed": "bf16", "fp32": "fp32", "fp64": "fp64"}[head]
     achieved = ladder_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None
     out = {
         "metric": "FRI-CCSD iter/s at o=32,v=256,m=4e6 (fp32/bf16 vs fp64); p50/p99 solve latency",
@@ -123,7 +157,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if head == "mixed" else "fp64",
+        "dtype": {"mixed": "bf16/fp32", "fp32": "fp32", "fp64": "fp64"}[head],
         "data": "synthetic",
         "config": {"workload": f"FRI-CCSD iteration, synthetic DF system nocc={o} nvir={v} naux={cfg['naux']} "
                                f"seed={cfg['seed']}, m_keep={m}, systematic FRI, level_shift=2.0",
@@ -138,13 +172,12 @@ def main():
         "n_D": r["n_D"],
         "n_sampled": r["n_sampled"],
     }
+    out["modes"] = {m_: {"value": res[m_]["it_per_s"], "ms_per_step": res[m_]["ms_per_step"],
+                         "p50_ms": res[m_]["p50_ms"], "p99_ms": res[m_]["p99_ms"],
+                         "sections_ms": res[m_]["sections_ms"]} for m_ in res}
     if "fp64" in res and head != "fp64":
-        out["plain_fp64"] = {"value": res["fp64"]["it_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
-                             "sections_ms": res["fp64"]["sections_ms"]}
         out["speedup_vs_fp64"] = r["it_per_s"] / res["fp64"]["it_per_s"]
-        e64 = res["fp64"]["energies"]
-        em = r["energies"]
-        out["energy_diff_vs_fp64"] = [abs(a - b) for a, b in zip(em, e64)]
+    out["precision_error_vs_fp64"] = errs
     out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 

@@ -118,9 +118,10 @@ static int dtype_of(int precision, int* dt) {
   return OK;
 }
 
+static int g_npass = 3;
+
 static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, double* out, const int64_t os[4],
                         hipStream_t s) {
-  FRICC_CHECK(dt != DT_BF16, "contract: BF16 operands are only supported through the prepared ladder path");
   View* v = nullptr;
   FRICC_TRY(get_view(T, p.pos, 64, dt, s, &v));
   OutMap om;
@@ -131,7 +132,23 @@ static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, double*
   om.oxl = os[p.opos[1]];
   om.oyh = os[p.opos[2]];
   om.oyl = os[p.opos[3]];
+  if (dt == DT_BF16) return gathered_gemm_bf16(v, (const uint16_t*)A, p.Rh * p.Rl, p.Xh * p.Xl, om, p.alpha, g_npass, s);
   return gathered_gemm(v, A, dt, p.Xh * p.Xl, om, p.alpha, s);
+}
+
+// FP64/FP32 (or a permuted FP64 W) -> the BF16 [hi | lo] operand in workspace slot 31
+static int bf16_operand_from_w(const Plan& p, const double* W, const int64_t W_strides[4], hipStream_t s,
+                               const uint16_t** out) {
+  const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
+  void* A = nullptr;
+  void* A2 = nullptr;
+  FRICC_TRY(ws_get(30, (size_t)R * X * 4, &A));
+  FRICC_TRY(ws_get(31, (size_t)((R + 7) / 8) * X * 8 * 2 * 2, &A2));
+  const int64_t st[4] = {W_strides[p.wpos[0]], W_strides[p.wpos[1]], W_strides[p.wpos[2]], W_strides[p.wpos[3]]};
+  FRICC_TRY(permute_operand(W, p.Rh, p.Rl, p.Xh, p.Xl, st, A, DT_F32, s));
+  FRICC_TRY(split_bf16(A, DT_F32, R, X, (uint16_t*)A2, s));
+  *out = (const uint16_t*)A2;
+  return OK;
 }
 
 }  // namespace fricc
@@ -248,8 +265,12 @@ int fricc_contract(const char* term, const double* W, const int64_t W_shape[4], 
   FRICC_TRY(make_plan(term, W_shape, T->dims, out_shape, &p));
   int dt;
   FRICC_TRY(dtype_of(precision, &dt));
-  FRICC_CHECK(dt != DT_BF16, "contract: BF16 is only available through fricc_contract_prepared");
   hipStream_t s = (hipStream_t)stream;
+  if (dt == DT_BF16) {
+    const uint16_t* A2 = nullptr;
+    FRICC_TRY(bf16_operand_from_w(p, W, W_strides, s, &A2));
+    return run_contract(p, A2, dt, T, out, out_strides, s);
+  }
   const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
   void* A = nullptr;
   FRICC_TRY(ws_get(30, (size_t)R * X * (dt == DT_F64 ? 8 : 4), &A));
@@ -280,7 +301,11 @@ int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int 
   FRICC_TRY(dtype_of(precision, &dt));
   hipStream_t s = (hipStream_t)stream;
   if (prepared) return run_contract(p, W_or_A, dt, T, out, out_strides, s);
-  FRICC_CHECK(dt != DT_BF16, "contract_spec: BF16 needs a prepared operand");
+  if (dt == DT_BF16) {
+    const uint16_t* A2 = nullptr;
+    FRICC_TRY(bf16_operand_from_w(p, (const double*)W_or_A, W_strides, s, &A2));
+    return run_contract(p, A2, dt, T, out, out_strides, s);
+  }
   const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
   void* A = nullptr;
   FRICC_TRY(ws_get(30, (size_t)R * X * (dt == DT_F64 ? 8 : 4), &A));
@@ -304,6 +329,18 @@ int fricc_sparse_prepare(fricc_sparse* T, const char* spec, const int64_t W_shap
   FRICC_TRY(dtype_of(precision, &dt));
   View* v = nullptr;
   return get_view(T, p.pos, 64, dt, (hipStream_t)stream, &v);
+}
+
+int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream) {
+  int dt;
+  FRICC_TRY(dtype_of(src_precision, &dt));
+  return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream);
+}
+
+int fricc_set_bf16_passes(int npass) {
+  FRICC_CHECK(npass == 1 || npass == 3, "set_bf16_passes: 1 or 3");
+  g_npass = npass;
+  return OK;
 }
 
 // Block ordering of the gathered GEMM (0: x tiles fastest, 1: column tiles

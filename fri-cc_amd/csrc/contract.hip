@@ -207,6 +207,185 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
 }
 
 // ---------------------------------------------------------------------------
+// BF16 gathered contraction (v_mfma_f32_16x16x32_bf16, FP32 accumulate).
+// A and B come as hi/lo bf16 planes; the 8 rows of a group are contiguous
+// (16 bytes) so each lane loads its 8 K values of one x (or y) at once.
+// npass 3: hi*hi + hi*lo + lo*hi (about 16 significant bits per product).
+typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
+
+__device__ inline bf8 as_bf8(uint4 u) {
+  bf8 r;
+  __builtin_memcpy(&r, &u, 16);
+  return r;
+}
+
+template <int NPASS>
+struct FragB16 {
+  uint4 ah[4], al[4], bh[4], bl[4];
+};
+
+template <int NPASS>
+__device__ inline void load_b16(FragB16<NPASS>& f, const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
+                                int64_t X, const uint32_t* __restrict__ glist, const uint4* __restrict__ Bhi,
+                                const uint4* __restrict__ Blo, uint32_t kg, const int64_t xa[4], int lr) {
+  const int64_t ga = (int64_t)glist[kg] * X;
+  const int64_t gb = (int64_t)kg * GT_Y + lr;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    f.ah[s] = Ahi[ga + xa[s]];
+    if (NPASS > 1) f.al[s] = Alo[ga + xa[s]];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    f.bh[q] = Bhi[gb + 16 * q];
+    if (NPASS > 1) f.bl[q] = Blo[gb + 16 * q];
+  }
+}
+
+template <int NPASS>
+__device__ inline void mma_b16(f4 (&acc)[4][4], const FragB16<NPASS>& f) {
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(f.ah[s]), as_bf8(f.bh[q]), acc[s][q], 0, 0, 0);
+      if (NPASS > 1) {
+        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(f.ah[s]), as_bf8(f.bl[q]), acc[s][q], 0, 0, 0);
+        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(f.al[s]), as_bf8(f.bh[q]), acc[s][q], 0, 0, 0);
+      }
+    }
+}
+
+template <int NPASS>
+__global__ __launch_bounds__(256) void k_ggemm_bf16(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
+                                                    int64_t X, const uint32_t* __restrict__ tile_off,
+                                                    const uint32_t* __restrict__ glist, const uint4* __restrict__ Bhi,
+                                                    const uint4* __restrict__ Blo, int64_t nbx, int64_t ntiles,
+                                                    int64_t Y, OutDev om, double alpha, int order) {
+  int64_t bx, t;
+  xcd_remap(nbx, ntiles, order, bx, t);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t xw = bx * GT_X + 64 * w;
+  const uint32_t g_beg = tile_off[t], g_end = tile_off[t + 1];  // groups; multiple of 4 per tile
+  f4 acc[4][4];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[s][q] = f4{0, 0, 0, 0};
+  int64_t xa[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    xa[s] = xw + 16 * s + lr;
+    if (xa[s] >= X) xa[s] = X - 1;
+  }
+  if (g_beg < g_end) {
+    FragB16<NPASS> f0, f1;
+    load_b16<NPASS>(f0, Ahi, Alo, X, glist, Bhi, Blo, g_beg + lk, xa, lr);
+    uint32_t kg = g_beg;
+    while (true) {
+      const bool more = kg + 4 < g_end;
+      if (more) load_b16<NPASS>(f1, Ahi, Alo, X, glist, Bhi, Blo, kg + 4 + lk, xa, lr);
+      mma_b16<NPASS>(acc, f0);
+      if (!more) break;
+      kg += 4;
+      const bool more2 = kg + 4 < g_end;
+      if (more2) load_b16<NPASS>(f0, Ahi, Alo, X, glist, Bhi, Blo, kg + 4 + lk, xa, lr);
+      mma_b16<NPASS>(acc, f1);
+      if (!more2) break;
+      kg += 4;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int64_t y = t * GT_Y + 16 * q + lr;
+    if (y >= Y) continue;
+    const uint32_t yy = (uint32_t)y;
+    const int64_t oy = (int64_t)(yy / om.Yl) * om.oyh + (int64_t)(yy % om.Yl) * om.oyl;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int64_t x = xw + 16 * s + 4 * lk + i;
+        if (x < X) {
+          const uint32_t xx = (uint32_t)x;
+          double* o = om.out + (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy;
+          *o += alpha * (double)acc[s][q][i];
+        }
+      }
+    }
+  }
+}
+
+int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& m, double alpha,
+                       int npass, hipStream_t s) {
+  FRICC_CHECK(v->TY == GT_Y && v->dtype == DT_BF16, "gathered_gemm_bf16: needs a BF16 view with tile width 64");
+  FRICC_CHECK(X > 0 && X < ((int64_t)1 << 31), "gathered_gemm_bf16: bad X");
+  FRICC_CHECK(npass == 1 || npass == 3, "gathered_gemm_bf16: npass must be 1 or 3");
+  if (v->ntiles == 0) return OK;
+  OutDev om;
+  om.out = m.out;
+  om.Xl = (uint32_t)m.Xl;
+  om.Yl = (uint32_t)v->Yl;
+  om.oxh = m.oxh;
+  om.oxl = m.oxl;
+  om.oyh = m.oyh;
+  om.oyl = m.oyl;
+  const int64_t Rg = (R + 7) / 8;
+  const uint4* Ahi = (const uint4*)A2;
+  const uint4* Alo = Ahi + Rg * X;
+  const int64_t nbx = ceil_div(X, GT_X);
+  const int64_t nb = nbx * v->ntiles;
+  if (npass == 3)
+    k_ggemm_bf16<3><<<nb, 256, 0, s>>>(Ahi, Alo, X, v->tile_off, v->rowlist, (const uint4*)v->B, (const uint4*)v->Blo,
+                                       nbx, v->ntiles, v->Y, om, alpha, g_order);
+  else
+    k_ggemm_bf16<1><<<nb, 256, 0, s>>>(Ahi, Alo, X, v->tile_off, v->rowlist, (const uint4*)v->B, (const uint4*)v->Blo,
+                                       nbx, v->ntiles, v->Y, om, alpha, g_order);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+// A[R][X] (f64 or f32) -> hi/lo planes [(R+7)/8][X][8]; rows past R are zero.
+template <typename T>
+__global__ void k_split_bf16(const T* __restrict__ A, int64_t R, int64_t X, __hip_bfloat16* __restrict__ hi,
+                             __hip_bfloat16* __restrict__ lo) {
+  const int64_t gx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t Rg = (R + 7) / 8;
+  if (gx >= Rg * X) return;
+  const int64_t g = gx / X, x = gx % X;
+  __hip_bfloat16 h[8], l[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int64_t r = g * 8 + j;
+    const float v = r < R ? (float)A[r * X + x] : 0.0f;
+    h[j] = __float2bfloat16(v);
+    l[j] = __float2bfloat16(v - __bfloat162float(h[j]));
+  }
+  uint4 uh, ul;
+  __builtin_memcpy(&uh, h, 16);
+  __builtin_memcpy(&ul, l, 16);
+  ((uint4*)hi)[gx] = uh;
+  ((uint4*)lo)[gx] = ul;
+}
+
+int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s) {
+  const int64_t Rg = (R + 7) / 8;
+  const int64_t n = Rg * X;
+  if (n == 0) return OK;
+  __hip_bfloat16* hi = (__hip_bfloat16*)A2;
+  __hip_bfloat16* lo = hi + n * 8;
+  if (src_dtype == DT_F64)
+    k_split_bf16<double><<<ceil_div(n, 256), 256, 0, s>>>((const double*)A, R, X, hi, lo);
+  else if (src_dtype == DT_F32)
+    k_split_bf16<float><<<ceil_div(n, 256), 256, 0, s>>>((const float*)A, R, X, hi, lo);
+  else
+    FRICC_CHECK(false, "split_bf16: source must be FP64 or FP32");
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_permute(const double* __restrict__ W, int64_t Rl, int64_t Xh, int64_t Xl, int64_t s0, int64_t s1,
                           int64_t s2, int64_t s3, T* __restrict__ A) {

@@ -16,6 +16,7 @@ View::~View() {
   if (tile_off) (void)hipFree(tile_off);
   if (rowlist) (void)hipFree(rowlist);
   if (B) (void)hipFree(B);
+  if (Blo) (void)hipFree(Blo);
 }
 
 Sparse::~Sparse() {
@@ -40,7 +41,8 @@ struct VGeom {
   uint32_t d1, d2, d3;
   int rh, rl, yh, yl;
   uint32_t Rl, Yl, TY;
-  uint32_t R;
+  uint32_t R;       // rows of the slot table (groups for BF16 views)
+  uint32_t gshift;  // log2 of the row group (3 for BF16 views, else 0)
 };
 
 __device__ inline void view_rc(uint32_t f, const VGeom& g, uint32_t& r, uint32_t& y) {
@@ -60,7 +62,7 @@ __global__ void k_mark(const uint32_t* __restrict__ idx, int64_t nnz, VGeom g, u
   if (e >= nnz) return;
   uint32_t r, y;
   view_rc(idx[e], g, r, y);
-  slot[(int64_t)(y / g.TY) * g.R + r] = 1u;
+  slot[(int64_t)(y / g.TY) * g.R + (r >> g.gshift)] = 1u;
 }
 
 __global__ void k_tile_count(const uint32_t* __restrict__ slot, uint32_t R, int kstep, uint32_t* __restrict__ cnt) {
@@ -117,9 +119,28 @@ __global__ void k_fill(const uint32_t* __restrict__ idx, const double* __restric
   atomicAdd(&B[(int64_t)k * g.TY + (y % g.TY)], (T)val[e]);
 }
 
-__global__ void k_f32_to_bf16(const float* __restrict__ in, __hip_bfloat16* __restrict__ out, int64_t n) {
+// BF16 view: accumulate into an FP32 [group][TY][8] buffer
+__global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* __restrict__ val, int64_t nnz, VGeom g,
+                               const uint32_t* __restrict__ slot, float* __restrict__ B) {
+  int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
+  if (e >= nnz) return;
+  uint32_t r, y;
+  view_rc(idx[e], g, r, y);
+  uint32_t t = y / g.TY;
+  uint32_t k = slot[(int64_t)t * g.R + (r >> 3)];
+  atomicAdd(&B[((int64_t)k * g.TY + (y % g.TY)) * 8 + (r & 7)], (float)val[e]);
+}
+
+// f32 -> bf16 hi (round to nearest) and lo = bf16(v - hi)
+__global__ void k_split_f32(const float* __restrict__ in, __hip_bfloat16* __restrict__ hi, __hip_bfloat16* __restrict__ lo,
+                            int64_t n) {
   int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x;
-  if (i < n) out[i] = __float2bfloat16(in[i]);
+  if (i < n) {
+    float v = in[i];
+    __hip_bfloat16 h = __float2bfloat16(v);
+    hi[i] = h;
+    lo[i] = __float2bfloat16(v - __bfloat162float(h));
+  }
 }
 
 static int build_view(Sparse* sp, View* v, hipStream_t s) {
@@ -143,9 +164,11 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   g.Rl = (uint32_t)v->Rl;
   g.Yl = (uint32_t)v->Yl;
   g.TY = (uint32_t)v->TY;
-  g.R = (uint32_t)v->R;
+  const bool grouped = v->dtype == DT_BF16;
+  g.gshift = grouped ? 3u : 0u;
+  g.R = (uint32_t)(grouped ? (v->R + 7) / 8 : v->R);
   void *p_slot, *p_cnt;
-  const size_t nslot = (size_t)v->ntiles * v->R;
+  const size_t nslot = (size_t)v->ntiles * g.R;
   FRICC_TRY(ws_get(20, nslot * 4, &p_slot));
   FRICC_TRY(ws_get(21, (size_t)(v->ntiles + 1) * 4, &p_cnt));
   uint32_t* slot = (uint32_t*)p_slot;
@@ -160,15 +183,19 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   FRICC_HIP(hipMemcpyAsync(&total, v->tile_off + v->ntiles, 4, hipMemcpyDeviceToHost, s));
   FRICC_HIP(hipStreamSynchronize(s));
   v->total_rows = total;
-  const size_t esz = v->dtype == DT_F64 ? 8 : (v->dtype == DT_F32 ? 4 : 2);
+  // bytes per gathered row (or group of 8 rows) of the B block
+  const size_t rowbytes = grouped ? (size_t)v->TY * 8 * 2 : (size_t)v->TY * (v->dtype == DT_F64 ? 8 : 4);
   if (v->cap_rows < (size_t)total || !v->rowlist) {
     if (v->rowlist) FRICC_HIP(hipFree(v->rowlist));
     if (v->B) FRICC_HIP(hipFree(v->B));
+    if (v->Blo) FRICC_HIP(hipFree(v->Blo));
     v->rowlist = nullptr;
     v->B = nullptr;
+    v->Blo = nullptr;
     size_t cap = (size_t)total + (size_t)total / 8 + 64;
     FRICC_HIP(hipMalloc(&v->rowlist, cap * 4));
-    FRICC_HIP(hipMalloc(&v->B, cap * (size_t)v->TY * esz));
+    FRICC_HIP(hipMalloc(&v->B, cap * rowbytes));
+    if (grouped) FRICC_HIP(hipMalloc(&v->Blo, cap * rowbytes));
     v->cap_rows = cap;
   }
   k_tile_slots<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->tile_off, v->rowlist);
@@ -180,11 +207,15 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
     FRICC_HIP(hipMemsetAsync(v->B, 0, nB * 4, s));
     if (nnz > 0) k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)v->B);
   } else {
+    const size_t nG = nB * 8;  // groups x TY x 8
     void* p_tmp;
-    FRICC_TRY(ws_get(22, nB * 4, &p_tmp));
-    FRICC_HIP(hipMemsetAsync(p_tmp, 0, nB * 4, s));
-    if (nnz > 0) k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)p_tmp);
-    if (nB > 0) k_f32_to_bf16<<<ceil_div((int64_t)nB, VNT), VNT, 0, s>>>((float*)p_tmp, (__hip_bfloat16*)v->B, nB);
+    FRICC_TRY(ws_get(22, nG * 4, &p_tmp));
+    FRICC_HIP(hipMemsetAsync(p_tmp, 0, nG * 4, s));
+    if (nnz > 0)
+      k_fill_grouped<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)p_tmp);
+    if (nG > 0)
+      k_split_f32<<<ceil_div((int64_t)nG, VNT), VNT, 0, s>>>((float*)p_tmp, (__hip_bfloat16*)v->B,
+                                                            (__hip_bfloat16*)v->Blo, nG);
   }
   FRICC_HIP(hipGetLastError());
   return OK;
@@ -204,7 +235,7 @@ int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, Vie
     std::unique_ptr<View> nv(new View());
     for (int q = 0; q < 4; q++) nv->pos[q] = pos[q];
     nv->TY = TY;
-    nv->kstep = dtype == DT_BF16 ? 32 : 16;
+    nv->kstep = dtype == DT_BF16 ? 4 : 16;  // BF16: groups of 8 rows, 4 groups per K step
     nv->dtype = dtype;
     v = nv.get();
     sp->views[key] = std::move(nv);

@@ -275,13 +275,14 @@ def _dev4(a, name, writeable=False):
 def contract_DTSpT(W, T, ouput, term, precision="fp64"):
     """ouput += term(W, T), in place (contract_SparseTensor4d_wrapper,
     src/sparse_tensor.cpp:349-382).  ``precision`` (not in the reference):
-    "fp64" or "fp32"."""
+    "fp64", "fp32" (FP32 MFMA) or "bf16" (BF16 MFMA, hi/lo split operands,
+    FP32 accumulate)."""
     if term not in _TERMS:
         raise ValueError(f"contraction term {term!r} not implemented")
     if not isinstance(T, SparseTensor4d):
         raise TypeError("T must be a SparseTensor4d")
-    if precision not in ("fp64", "fp32"):
-        raise ValueError("precision must be 'fp64' or 'fp32'")
+    if precision not in ("fp64", "fp32", "bf16"):
+        raise ValueError("precision must be 'fp64', 'fp32' or 'bf16'")
     Wd, ws, _ = _dev4(W, "W")
     Od, os_, host = _dev4(ouput, "ouput", writeable=True)
     if T.nnz == 0:

@@ -65,6 +65,19 @@ def eri_block(Ld, naux, nmo, ranges, out, strides, stream=None):
     return out
 
 
+def bf16_planes(A, R, X, stream=None):
+    """[hi | lo] BF16 planes of A viewed as (R, X) (fricc_split_bf16 layout)."""
+    import torch
+
+    A = A.contiguous()
+    assert A.numel() == R * X and A.dtype in (torch.float64, torch.float32)
+    src = _capi.FP64 if A.dtype == torch.float64 else _capi.FP32
+    out = torch.empty(2 * ((R + 7) // 8) * X * 8, dtype=torch.int16, device=A.device)
+    _capi.check(_capi.lib().fricc_split_bf16(ctypes.c_void_p(A.data_ptr()), src, int(R), int(X),
+                                             ctypes.c_void_p(out.data_ptr()), _capi.stream_ptr(stream)))
+    return out
+
+
 def _cstrides(shape):
     s = [1, 1, 1, 1]
     for i in (2, 1, 0):
@@ -137,6 +150,26 @@ class DeviceERIs:
 
     def get_ovvv(self):
         return self.ovvv
+
+    def sparse_operand(self, name, precision):
+        """Static A operand of a sparse contraction ("A1323", "A0112", "A0313",
+        "Gx", "Gy", "ladder") in the kernel layout for ``precision``
+        ("fp64" | "fp32" | "bf16"), cached."""
+        import torch
+
+        key = ("sparse", name, precision)
+        if key in self._prep:
+            return self._prep[key]
+        p64 = self.prepared(torch.float64)
+        A = self.ladder if name == "ladder" else p64[name]
+        if precision == "fp64":
+            out = A
+        elif precision == "fp32":
+            out = A.to(torch.float32).contiguous()
+        else:
+            out = bf16_planes(A, A.shape[0] * A.shape[1], A.shape[2] * A.shape[3])
+        self._prep[key] = out
+        return out
 
     def prepared(self, dtype):
         """Static operands in the layouts the update reads, cached per dtype."""

@@ -141,6 +141,17 @@ int fricc_eri_build(const double* L, int64_t naux, int64_t nmo, const int64_t rn
 int fricc_sparse_prepare(fricc_sparse* T, const char* term_or_spec, const int64_t W_shape[4],
                          const int64_t out_shape[4], int precision, void* stream);
 
+/* BF16 operand for FRICC_BF16 contractions: A[R][X] row-major (FP64 when
+ * src_precision == FRICC_FP64, FP32 when FRICC_FP32) -> out = [hi | lo], two
+ * planes of ceil(R/8)*X*8 bf16 each, element (r, x) at ((r/8)*X + x)*8 + r%8,
+ * hi = bf16(a), lo = bf16(a - hi).  out holds 2*ceil(R/8)*X*8 uint16.  No
+ * counterpart in the reference (FP64 only). */
+int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream);
+
+/* MFMA passes of FRICC_BF16 contractions: 3 = hi*hi + hi*lo + lo*hi (default,
+ * about 16 significant bits per product), 1 = hi*hi only. */
+int fricc_set_bf16_passes(int npass);
+
 /* Block ordering of the gathered contraction GEMM: 0 = x tiles fastest,
  * 1 = column tiles fastest (default).  Tuning knob; no counterpart in the
  * reference. */

@@ -121,6 +121,24 @@ def test_contraction_fp32(py_rccsd, term):
     npt.assert_allclose(out, ref, rtol=0, atol=2e-5 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("term", sorted(CASES))
+def test_contraction_bf16_split(py_rccsd, term):
+    """BF16 MFMA with hi/lo split operands: about 16 significant bits per
+    product, FP32 accumulation."""
+    no, nv = 7, 19
+    wshape, spec, alpha, oshape = CASES[term]
+    rng = np.random.default_rng(11)
+    w = rng.standard_normal(wshape(no, nv))
+    t2 = rng.standard_normal((no, no, nv, nv))
+    T = py_rccsd.SparseTensor4d(t2, t2.shape, t2.size // 5, "largest")
+    dense = np.zeros(t2.size)
+    dense[T.indices()] = T.values()
+    ref = alpha * np.einsum(spec, w, dense.reshape(t2.shape))
+    out = np.zeros(oshape(no, nv))
+    py_rccsd.contract_DTSpT(w, T, out, term, precision="bf16")
+    npt.assert_allclose(out, ref, rtol=0, atol=2e-4 * np.abs(ref).max())
+
+
 def test_strided_views(py_rccsd):
     """W and ouput may be arbitrary strided numpy views (src/sparse_tensor.cpp:353-354)."""
     no, nv = 5, 9
