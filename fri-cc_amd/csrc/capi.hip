@@ -120,12 +120,13 @@ static int dtype_of(int precision, int* dt) {
 
 static int g_npass = 3;
 
-static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, double* out, const int64_t os[4],
-                        hipStream_t s) {
+static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, void* out, const int64_t os[4],
+                        hipStream_t s, int of32 = 0) {
   View* v = nullptr;
   FRICC_TRY(get_view(T, p.pos, 64, dt, s, &v));
   OutMap om;
   om.out = out;
+  om.of32 = of32;
   om.Xh = p.Xh;
   om.Xl = p.Xl;
   om.oxh = os[p.opos[0]];
@@ -290,8 +291,8 @@ int fricc_contract_prepared(const char* term, const void* A, const int64_t W_sha
 }
 
 int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int prepared, const int64_t W_shape[4],
-                        const int64_t W_strides[4], fricc_sparse* T, double* out, const int64_t out_shape[4],
-                        const int64_t out_strides[4], int precision, void* stream) {
+                        const int64_t W_strides[4], fricc_sparse* T, void* out, int out_precision,
+                        const int64_t out_shape[4], const int64_t out_strides[4], int precision, void* stream) {
   FRICC_CHECK(T != nullptr && W_or_A != nullptr && out != nullptr, "contract_spec: null argument");
   char w[5], t[5], o[5];
   FRICC_TRY(parse_spec(spec, w, t, o));
@@ -299,19 +300,21 @@ int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int 
   FRICC_TRY(make_plan_spec(w, t, o, alpha, W_shape, T->dims, out_shape, &p));
   int dt;
   FRICC_TRY(dtype_of(precision, &dt));
+  FRICC_CHECK(out_precision == FRICC_FP64 || out_precision == FRICC_FP32, "contract_spec: output must be FP64 or FP32");
+  const int of32 = out_precision == FRICC_FP32;
   hipStream_t s = (hipStream_t)stream;
-  if (prepared) return run_contract(p, W_or_A, dt, T, out, out_strides, s);
+  if (prepared) return run_contract(p, W_or_A, dt, T, out, out_strides, s, of32);
   if (dt == DT_BF16) {
     const uint16_t* A2 = nullptr;
     FRICC_TRY(bf16_operand_from_w(p, (const double*)W_or_A, W_strides, s, &A2));
-    return run_contract(p, A2, dt, T, out, out_strides, s);
+    return run_contract(p, A2, dt, T, out, out_strides, s, of32);
   }
   const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
   void* A = nullptr;
   FRICC_TRY(ws_get(30, (size_t)R * X * (dt == DT_F64 ? 8 : 4), &A));
   const int64_t st[4] = {W_strides[p.wpos[0]], W_strides[p.wpos[1]], W_strides[p.wpos[2]], W_strides[p.wpos[3]]};
   FRICC_TRY(permute_operand((const double*)W_or_A, p.Rh, p.Rl, p.Xh, p.Xl, st, A, dt, s));
-  return run_contract(p, A, dt, T, out, out_strides, s);
+  return run_contract(p, A, dt, T, out, out_strides, s, of32);
 }
 
 int fricc_sparse_prepare(fricc_sparse* T, const char* spec, const int64_t W_shape[4], const int64_t out_shape[4],

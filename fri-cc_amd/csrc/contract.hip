@@ -19,10 +19,19 @@ typedef __attribute__((ext_vector_type(4))) double d4;
 typedef __attribute__((ext_vector_type(4))) float f4;
 
 struct OutDev {
-  double* out;
+  void* out;
+  int of32;
   uint32_t Xl, Yl;
   int64_t oxh, oxl, oyh, oyl;
 };
+
+// out[off] += alpha * v in the output's element type
+__device__ inline void out_add(const OutDev& om, int64_t off, double alpha, double v) {
+  if (om.of32)
+    ((float*)om.out)[off] += (float)(alpha * v);
+  else
+    ((double*)om.out)[off] += alpha * v;
+}
 
 template <typename T>
 struct Acc;
@@ -167,8 +176,8 @@ __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t 
         const int64_t x = xw + 16 * s + Acc<T>::row(lk, i);
         if (x < X) {
           const uint32_t xx = (uint32_t)x;
-          double* o = om.out + (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy;
-          *o += alpha * (double)acc[s][q][i];
+          out_add(om, (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy, alpha,
+                  (double)acc[s][q][i]);
         }
       }
     }
@@ -185,6 +194,7 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
   if (v->ntiles == 0) return OK;
   OutDev om;
   om.out = m.out;
+  om.of32 = m.of32;
   om.Xl = (uint32_t)m.Xl;
   om.Yl = (uint32_t)v->Yl;
   om.oxh = m.oxh;
@@ -309,8 +319,8 @@ __global__ __launch_bounds__(256) void k_ggemm_bf16(const uint4* __restrict__ Ah
         const int64_t x = xw + 16 * s + 4 * lk + i;
         if (x < X) {
           const uint32_t xx = (uint32_t)x;
-          double* o = om.out + (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy;
-          *o += alpha * (double)acc[s][q][i];
+          out_add(om, (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy, alpha,
+                  (double)acc[s][q][i]);
         }
       }
     }
@@ -325,6 +335,7 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   if (v->ntiles == 0) return OK;
   OutDev om;
   om.out = m.out;
+  om.of32 = m.of32;
   om.Xl = (uint32_t)m.Xl;
   om.Yl = (uint32_t)v->Yl;
   om.oxh = m.oxh;

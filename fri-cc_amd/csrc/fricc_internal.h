@@ -60,7 +60,8 @@ int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, Vie
 // A: row-major [R x X] in `adtype` (F64 or F32 here; BF16 uses its own path).
 // Output element (x, y) lives at out + (x/Xl)*oxh + (x%Xl)*oxl + (y/Yl)*oyh + (y%Yl)*oyl.
 struct OutMap {
-  double* out;
+  void* out;      // double*, or float* when of32
+  int of32 = 0;   // output element type: 0 FP64, 1 FP32
   int64_t Xh, Xl, oxh, oxl, oyh, oyl;
 };
 int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const OutMap& om, double alpha,

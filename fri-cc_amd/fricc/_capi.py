@@ -20,7 +20,7 @@ EXPORTS = (
     "fricc_abi_version", "fricc_last_error", "fricc_compress", "fricc_sample_systematic", "fricc_sparse_create", "fricc_sparse_destroy",
     "fricc_sparse_compress", "fricc_sparse_set", "fricc_sparse_info", "fricc_sparse_get", "fricc_sparse_device",
     "fricc_contract", "fricc_contract_prepared", "fricc_contract_spec", "fricc_eri_build", "fricc_set_gemm_order",
-    "fricc_sparse_prepare", "fricc_split_bf16", "fricc_set_bf16_passes",
+    "fricc_sparse_prepare", "fricc_split_bf16", "fricc_set_bf16_passes", "fricc_t2_finalize",
 )
 
 
@@ -82,11 +82,12 @@ def lib():
     L.fricc_contract.argtypes = [ctypes.c_char_p, _vp, I64x4, I64x4, _vp, _vp, I64x4, I64x4, ctypes.c_int, _vp]
     L.fricc_contract_prepared.argtypes = [ctypes.c_char_p, _vp, I64x4, _vp, _vp, I64x4, I64x4, ctypes.c_int, _vp]
     L.fricc_contract_spec.argtypes = [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.c_int, I64x4, I64x4, _vp, _vp,
-                                      I64x4, I64x4, ctypes.c_int, _vp]
+                                      ctypes.c_int, I64x4, I64x4, ctypes.c_int, _vp]
     L.fricc_eri_build.argtypes = [_vp, _i64, _i64, I64x8, I64x4, _vp, _vp]
     L.fricc_set_gemm_order.argtypes = [ctypes.c_int]
     L.fricc_split_bf16.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_set_bf16_passes.argtypes = [ctypes.c_int]
+    L.fricc_t2_finalize.argtypes = [ctypes.c_int, ctypes.c_int] + [_vp] * 3 + [ctypes.c_int] + [_vp] * 7 + [_vp]
     L.fricc_sparse_prepare.argtypes = [_vp, ctypes.c_char_p, I64x4, I64x4, ctypes.c_int, _vp]
     for name in EXPORTS:
         getattr(L, name)  # every declared symbol must resolve

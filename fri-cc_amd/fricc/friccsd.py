@@ -163,10 +163,35 @@ def _tr(t):
     return t.permute(1, 0, 3, 2)
 
 
+# Compressed terms as index specs with the sign they carry into the
+# symmetrised buffer S of update_amps (t2new += S + S.transpose(1,0,3,2)).
+_O3V3 = (("1302", "akic,kjcb->ijab", 2.0), ("1202", "akci,kjcb->ijab", -1.0),
+         ("1303", "akic,kjbc->ijab", -1.0), ("1203", "bkci,kjac->ijab", -1.0))
+_PREC_ID = {"fp64": 0, "fp32": 1}
+
+
+def _out_prec(t):
+    return _capi.FP32 if t.dtype == _torch().float32 else _capi.FP64
+
+
+def _contract_into(spec, alpha, A, wshape, T, out, precision):
+    """out += alpha * einsum(spec, W, T) with A the prepared operand of W."""
+    _capi.check(_capi.lib().fricc_contract_spec(
+        spec.encode(), float(alpha), _vp(A), 1, _capi.i64x4(wshape), _capi.i64x4([0, 0, 0, 0]), T.handle(),
+        _vp(out), _out_prec(out), _capi.i64x4(out.shape), _capi.i64x4(out.stride()), _capi.PRECISIONS[precision],
+        _capi.stream_ptr()))
+
+
 def update_amps(cc, t1, t2, eris):
     """One FRI-CCSD Jacobi step (fricc/friccsd.py:29-301) on the GPU.
 
-    Hirata et al., J. Chem. Phys. 120, 2581 (2004), eqs. 35-36."""
+    Hirata et al., J. Chem. Phys. 120, 2581 (2004), eqs. 35-36.  Same terms as
+    the reference, accumulated differently: every contribution the reference
+    adds as ``tmp + tmp.transpose(1,0,3,2)`` goes into one buffer S (GEMMs
+    accumulate in place, compressed contractions add with their sign), every
+    other T2 contribution into the output buffer, and one fused kernel adds
+    S + S^T and the t1*t1 vvvv term, divides by the denominators and reduces
+    the energy and the amplitude-change norm (fricc_t2_finalize)."""
     torch = _torch()
     st = cc.fri_settings
     m_keep = st["m_keep"]
@@ -178,8 +203,9 @@ def update_amps(cc, t1, t2, eris):
     tm = cc.timer if getattr(cc, "timer", None) is not None else _NoTimer()
     nocc, nvir = t1.shape
     o, v = nocc, nvir
+    ov = o * v
     f64 = torch.float64
-    cdt = cc.dense_dtype
+    cdt = cc.dense_dtype  # dense GEMMs and the S / W buffers
     P = eris.prepared(cdt)
     P64 = eris.prepared(f64)
     fock = eris.fock
@@ -247,16 +273,6 @@ def update_amps(cc, t1, t2, eris):
     t1new += torch.bmm(Aoo, t2.reshape(o * o, v, v).transpose(1, 2)).sum(0)
     t1new += -2 * torch.einsum("lcki,lc,ka->ia", ovoo, t1, t1) + torch.einsum("kcli,lc,ka->ia", ovoo, t1, t1)
 
-    # ---- T2 equation, t1-driven part (fricc/friccsd.py:178-187)
-    tmp = C(torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v))).reshape(o, o, v, v)
-    M = torch.einsum("kibc,jc->kibj", eris.oovv, t1)
-    tmp -= torch.einsum("ka,kibj->ijab", t1, M)
-    t2new = tmp + _tr(tmp)
-    tmp2 = torch.einsum("kcai,jc->akij", eris.ovvo, t1) + ovoo.permute(1, 3, 0, 2)
-    tmp = torch.einsum("akij,kb->ijab", tmp2, t1)
-    t2new -= tmp + _tr(tmp)
-    t2new += P64["ovovT"]
-
     # ---- Loo, Lvv (imd.Loo / imd.Lvv)
     Loo = Foo_u + fov @ t1.T + 2 * torch.einsum("lcki,lc->ki", ovoo, t1) - torch.einsum("kcli,lc->ki", ovoo, t1)
     Lvv = Fvv_u - t1.T @ fov + LvvT1
@@ -265,117 +281,126 @@ def update_amps(cc, t1, t2, eris):
 
     # ---- W intermediates (fricc/friccsd.py:194-207)
     t0 = time.perf_counter()
+    Woooo = torch.einsum("lcki,jc->klij", ovoo, t1) + torch.einsum("kclj,ic->klij", ovoo, t1)
+    Woooo = Woooo.contiguous()
     if "O^4V^2" in ccs:
-        Woooo = torch.einsum("lcki,jc->klij", ovoo, t1) + torch.einsum("kclj,ic->klij", ovoo, t1)
-        Woooo = Woooo.contiguous()
-        _contract("1323", eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
-        Woooo += torch.einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
-        Woooo += eris.oooo.permute(0, 2, 1, 3)
+        _contract_into("kcld,ijcd->klij", 1.0, eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
     else:
-        Woooo = torch.einsum("lcki,jc->klij", ovoo, t1) + torch.einsum("kclj,ic->klij", ovoo, t1)
         Woooo += torch.einsum("kcld,ijcd->klij", eris.ovov, t2)
-        Woooo += torch.einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
-        Woooo += eris.oooo.permute(0, 2, 1, 3)
-    # Wvoov as A_voov[k,c,i,a] = Wvoov[a,k,i,c]; Wvovo as A_vovo[k,c,i,a] = Wvovo[a,k,c,i]
-    A_voov = C(torch.matmul(t1c, P["P1"]))
-    A_voov -= torch.einsum("kcli,la->kcia", ovoo, t1)
-    A_voov += P64["ovvo_p"]
-    A_vovo = C(torch.matmul(t1c, P["P2"]))
-    A_vovo -= torch.einsum("lcki,la->kcia", ovoo, t1)
-    A_vovo += P64["oovv_p"]
+    Woooo += torch.einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
+    Woooo += eris.oooo.permute(0, 2, 1, 3)
+    # Wvoov as A_voov[k,c,i,a] = Wvoov[a,k,i,c];  Wvovo as A_vovo[k,c,i,a] = Wvovo[a,k,c,i]
+    t1b = t1c.expand(ov, o, v)
+    A_voov = torch.baddbmm(P["ovvo_p"].reshape(ov, o, v), t1b, P["P1"].reshape(ov, v, v))
+    A_vovo = torch.baddbmm(P["oovv_p"].reshape(ov, o, v), t1b, P["P2"].reshape(ov, v, v))
+    # - sum_l (kcli + sum_d ldkc t1[i,d]) t1[l,a];  - sum_l (lcki + sum_d lckd t1[i,d]) t1[l,a]
+    N1 = ovoo + torch.einsum("ldkc,id->kcli", eris.ovov, t1)
+    N2 = ovoo.permute(2, 1, 0, 3) + torch.einsum("lckd,id->kcli", eris.ovov, t1)
+    A_voov.baddbmm_(N1.to(cdt).reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
+    A_vovo.baddbmm_(N2.to(cdt).reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
+    A_voov = A_voov.reshape(o, v, o, v)
+    A_vovo = A_vovo.reshape(o, v, o, v)
     Wvoov = A_voov.permute(3, 0, 2, 1)  # [a,k,i,c] view
     Wvovo = A_vovo.permute(3, 0, 1, 2)  # [a,k,c,i] view
     if "O^3V^3" in ccs:
         A0112 = eris.sparse_operand("A0112", sp)
         A0313 = eris.sparse_operand("A0313", sp)
-        _contract("0112", A0112, (o, v, o, v), ts_alt, Wvoov, sp)
-        _contract("0313", A0313, (o, v, o, v), ts_alt, Wvoov, sp)
-        _contract("0113", A0112, (o, v, o, v), ts_alt, Wvoov, sp)
-        _contract("0312", A0313, (o, v, o, v), ts_alt, Wvovo, sp)
+        _contract_into("ldkc,ilda->akic", -0.5, A0112, (o, v, o, v), ts_alt, Wvoov, sp)
+        _contract_into("lckd,ilad->akic", -0.5, A0313, (o, v, o, v), ts_alt, Wvoov, sp)
+        _contract_into("ldkc,ilad->akic", 1.0, A0112, (o, v, o, v), ts_alt, Wvoov, sp)
+        _contract_into("lckd,ilda->akci", -0.5, A0313, (o, v, o, v), ts_alt, Wvovo, sp)
     else:
-        ovov64 = eris.ovov
-        Wvoov -= 0.5 * torch.einsum("ldkc,ilda->akic", ovov64, t2)
-        Wvoov -= 0.5 * torch.einsum("lckd,ilad->akic", ovov64, t2)
-        Wvoov += torch.einsum("ldkc,ilad->akic", ovov64, t2)
-        Wvovo -= 0.5 * torch.einsum("lckd,ilda->akci", ovov64, t2)
-    A_voov -= torch.einsum("lkci,la->kcia", torch.einsum("ldkc,id->lkci", eris.ovov, t1), t1)
-    A_vovo -= torch.einsum("lcki,la->kcia", torch.einsum("lckd,id->lcki", eris.ovov, t1), t1)
+        ovov64 = eris.ovov.to(cdt)
+        t2w = t2c
+        Wvoov -= 0.5 * torch.einsum("ldkc,ilda->akic", ovov64, t2w)
+        Wvoov -= 0.5 * torch.einsum("lckd,ilad->akic", ovov64, t2w)
+        Wvoov += torch.einsum("ldkc,ilad->akic", ovov64, t2w)
+        Wvovo -= 0.5 * torch.einsum("lckd,ilda->akci", ovov64, t2w)
     timings["W"] = time.perf_counter() - t0
 
-    # ---- O^4V^2 and vvvv terms (fricc/friccsd.py:210-235)
+    # ---- direct T2 terms into the output buffer acc (fricc/friccsd.py:210-235)
     t0 = time.perf_counter()
+    X = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)  # sum_cd (kd|ac) T[ij,cd]
+    Y = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)  # sum_cd (kc|bd) T[ij,cd]
+    if "O^2V^4" in ccs:
+        _contract_into("kdac,ijcd->ijka", 1.0, eris.sparse_operand("Gx", sp), (o, v, v, v), ts, X, sp)
+        _contract_into("kcbd,ijcd->ijkb", 1.0, eris.sparse_operand("Gy", sp), (o, v, v, v), ts, Y, sp)
+    else:
+        X += torch.einsum("kdac,ijcd->ijka", eris.ovvv, t2)
+        Y += torch.einsum("kcbd,ijcd->ijkb", eris.ovvv, t2)
     if "O^4V^2" in ccs:
-        t2new += torch.einsum("klij,ka,lb->ijab", Woooo, t1, t1)
-        _contract("0101", _dyn_operand(Woooo, o * o, o * o, sp), (o, o, o, o), ts, t2new, sp)
+        # klij,ka,lb -> ijab  ==  sum_l Q[ij,l,a] t1[l,b],  Q[ij,l,a] = sum_k Woooo[k,l,i,j] t1[k,a]
+        X -= torch.einsum("klij,ka->ijla", Woooo, t1)
+    # acc[ij,a,b] = -sum_k X[ij,k,a] t1[k,b] - sum_k t1[k,a] Y[ij,k,b]
+    acc = torch.bmm(X.reshape(o * o, o, v).transpose(1, 2), t1.expand(o * o, o, v)).neg_()
+    acc.baddbmm_(t1.T.expand(o * o, v, o), Y.reshape(o * o, o, v), alpha=-1)
+    acc = acc.reshape(o, o, v, v)
+    if "O^4V^2" in ccs:
+        _contract_into("klij,klab->ijab", 1.0, _dyn_operand(Woooo, o * o, o * o, sp), (o, o, o, o), ts, acc, sp)
     else:
-        tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
-        t2new += (Woooo.reshape(o * o, o * o).T @ tau.reshape(o * o, v * v)).reshape(o, o, v, v)
-    # sum_cd Wvvvv[a,b,c,d] = s_ab - sum_k t_kb g_ka - sum_k t_ka g_kb
-    G1 = P64["g_ka"].T @ t1
-    WSum = P64["s_ab"] - G1 - G1.T
+        tau = t2 + t1[:, None, :, None] * t1[None, :, None, :]
+        acc.reshape(o * o, v * v).addmm_(Woooo.reshape(o * o, o * o).T, tau.reshape(o * o, v * v))
     if cc.vvvv_t1t1 == "reference":
-        t2new += t1[:, None, :, None] * t1[None, :, None, :] * WSum[None, None]
+        # t_ia t_jb sum_cd W_abcd (fricc/friccsd.py:222, quirk A.1), added in the finalize kernel
+        G1 = P64["g_ka"].T @ t1
+        wsum = (P64["s_ab"] - G1 - G1.T).contiguous()
     else:
+        wsum = None
         tt = (t1[:, None, :, None] * t1[None, :, None, :]).reshape(o * o, v * v)
-        t2new += (tt @ P64["ladder"].reshape(v * v, v * v)).reshape(o, o, v, v)
+        acc.reshape(o * o, v * v).addmm_(tt, P64["ladder"].reshape(v * v, v * v))
         Xd = torch.einsum("kdac,ic,jd->ijka", eris.ovvv, t1, t1)
         Yd = torch.einsum("kcbd,ic,jd->ijkb", eris.ovvv, t1, t1)
-        t2new -= torch.einsum("ijka,kb->ijab", Xd, t1) + torch.einsum("ijkb,ka->ijab", Yd, t1)
-    X = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)
-    Y = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)
+        acc -= torch.einsum("ijka,kb->ijab", Xd, t1) + torch.einsum("ijkb,ka->ijab", Yd, t1)
     if "O^2V^4" in ccs:
         _prepare(ts, "2323", (v, v, v, v), (o, o, v, v), sp)
         tm.start("ladder")
-        _contract("2323", eris.sparse_operand("ladder", sp), (v, v, v, v), ts, t2new, sp)
+        _contract_into("abcd,ijcd->ijab", 1.0, eris.sparse_operand("ladder", sp), (v, v, v, v), ts, acc, sp)
         tm.stop("ladder")
-        _contract_spec("kdac,ijcd->ijka", 1.0, eris.sparse_operand("Gx", sp), (o, v, v, v), ts, X, sp)
-        _contract_spec("kcbd,ijcd->ijkb", 1.0, eris.sparse_operand("Gy", sp), (o, v, v, v), ts, Y, sp)
     else:
-        t2new += (t2.reshape(o * o, v * v) @ P64["ladder"].reshape(v * v, v * v)).reshape(o, o, v, v)
-        X += torch.einsum("kdac,ijcd->ijka", eris.ovvv, t2)
-        Y += torch.einsum("kcbd,ijcd->ijkb", eris.ovvv, t2)
-    t2new -= torch.einsum("ijka,kb->ijab", X, t1) + torch.einsum("ijkb,ka->ijab", Y, t1)
+        acc.reshape(o * o, v * v).addmm_(t2.reshape(o * o, v * v), P64["ladder"].reshape(v * v, v * v))
     timings["ladder"] = time.perf_counter() - t0
 
-    # ---- Lvv, Loo terms (fricc/friccsd.py:237-241)
-    tmp = C(torch.matmul(Lvv.to(cdt), t2c))
-    t2new += tmp + _tr(tmp)
-    tmp = C(Loo.to(cdt).T @ t2c.reshape(o, -1)).reshape(o, o, v, v)
-    t2new -= tmp + _tr(tmp)
-
-    # ---- O^3V^3 (fricc/friccsd.py:243-279)
+    # ---- symmetrised T2 terms into S (fricc/friccsd.py:178-186, 237-279)
     t0 = time.perf_counter()
+    # tmp = (ia|cb) t1[j,c] - sum_k oovv[k,i,b,c] t1[j,c] t1[k,a]
+    S = torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v)).reshape(o, o, v, v)
+    S3 = S.reshape(o * o, v, v)
+    M = torch.einsum("kibc,jc->ijkb", eris.oovv, t1).to(cdt)
+    S3.baddbmm_(t1c.T.expand(o * o, v, o), M.reshape(o * o, o, v), alpha=-1)
+    # - (kcai,jc + ovoo[i,a,j,k]) t1[k,b]
+    T2p = torch.einsum("kcai,jc->ijak", eris.ovvo, t1) + ovoo.permute(0, 2, 1, 3)
+    S3.baddbmm_(T2p.to(cdt).reshape(o * o, v, o), t1c.expand(o * o, o, v), alpha=-1)
+    # + Lvv[a,c] t2[i,j,c,b] - Loo[k,i] t2[k,j,a,b]
+    S3.baddbmm_(Lvv.to(cdt).expand(o * o, v, v), t2c.reshape(o * o, v, v))
+    S.reshape(o, o * v * v).addmm_(Loo.to(cdt).T, t2c.reshape(o, -1), alpha=-1)
     if "O^3V^3" in ccs:
         _prepare(ts, "1302", (v, o, o, v), (o, o, v, v), sp)
         _prepare(ts, "1303", (v, o, o, v), (o, o, v, v), sp)
         tm.start("O3V3")
-        Av = _dyn_operand(A_voov, o * v, o * v, sp)
-        Avo = _dyn_operand(A_vovo, o * v, o * v, sp)
-        tmp = torch.zeros_like(t2new)
-        _contract("1302", Av, (v, o, o, v), ts, tmp, sp)
-        _contract("1202", Avo, (v, o, v, o), ts, tmp, sp)
-        t2new += tmp + _tr(tmp)
-        tmp = torch.zeros_like(t2new)
-        _contract("1303", Av, (v, o, o, v), ts, tmp, sp)
-        t2new -= tmp + _tr(tmp)
-        tmp = torch.zeros_like(t2new)
-        _contract("1203", Avo, (v, o, v, o), ts, tmp, sp)
-        t2new -= tmp + _tr(tmp)
+        Av = _dyn_operand(A_voov, ov, ov, sp)
+        Avo = _dyn_operand(A_vovo, ov, ov, sp)
+        for (term, spec, alpha), A, ws in zip(_O3V3, (Av, Avo, Av, Avo),
+                                               ((v, o, o, v), (v, o, v, o), (v, o, o, v), (v, o, v, o))):
+            _contract_into(spec, alpha, A, ws, ts, S, sp)
         tm.stop("O3V3")
     else:
-        tmp = 2 * torch.einsum("akic,kjcb->ijab", Wvoov, t2)
-        tmp -= torch.einsum("akci,kjcb->ijab", Wvovo, t2)
-        t2new += tmp + _tr(tmp)
-        tmp = torch.einsum("akic,kjbc->ijab", Wvoov, t2)
-        t2new -= tmp + _tr(tmp)
-        tmp = torch.einsum("bkci,kjac->ijab", Wvovo, t2)
-        t2new -= tmp + _tr(tmp)
+        t2w = t2c
+        S += 2 * torch.einsum("akic,kjcb->ijab", Wvoov, t2w) - torch.einsum("akci,kjcb->ijab", Wvovo, t2w)
+        S -= torch.einsum("akic,kjbc->ijab", Wvoov, t2w) + torch.einsum("bkci,kjac->ijab", Wvovo, t2w)
     timings["O3V3"] = time.perf_counter() - t0
 
-    # ---- denominators (fricc/friccsd.py:281-284), FP64
-    eia = mo_e_o[:, None] - mo_e_v
-    t1new /= eia
-    t2new /= eia[:, None, :, None] + eia[None, :, None, :]
+    # ---- denominators (fricc/friccsd.py:281-284), energy and norm: FP64, fused
+    eia = (mo_e_o[:, None] - mo_e_v).contiguous()
+    t1new = (t1new / eia).contiguous()
+    sums = torch.empty(2, dtype=f64, device=t2.device)
+    _capi.check(_capi.lib().fricc_t2_finalize(
+        o, v, _vp(P64["ovovT"]), _vp(acc), _vp(S), _PREC_ID["fp32" if S.dtype == torch.float32 else "fp64"],
+        _vp(t1), _vp(wsum) if wsum is not None else None, _vp(eia), _vp(t2), _vp(P64["Lovov"]), _vp(t1new),
+        _vp(sums), _capi.stream_ptr()))
+    t2new = acc
+    e2, dn = sums.tolist()
+    cc._last_step = (t1new, t2new, 2 * float((fov * t1new).sum().item()) + e2,
+                     math.sqrt(float(((t1new - t1) ** 2).sum().item()) + dn))
     cc.sparse_info = (ts.info, ts_alt.info if ts_alt is not None else None)
     return t1new, t2new
 
@@ -437,10 +462,15 @@ def kernel(mycc, eris=None, t1=None, t2=None, max_cycle=50, tol=1e-8, tolnormt=1
         tc = time.perf_counter()
         mycc._iter = istep
         t1new, t2new = mycc.update_amps(t1, t2, eris)
-        normt = amplitudes_to_vector_norm(t1new, t2new, t1, t2)
+        last = getattr(mycc, "_last_step", None)
+        if last is not None and last[0] is t1new and last[1] is t2new:
+            enew, normt = last[2], last[3]  # computed in the fused finalize pass
+        else:
+            normt = amplitudes_to_vector_norm(t1new, t2new, t1, t2)
+            enew = mycc.energy(t1new, t2new, eris)
         t1, t2 = t1new, t2new
         t1new = t2new = None
-        eold, eccsd = eccsd, mycc.energy(t1, t2, eris)
+        eold, eccsd = eccsd, enew
         mycc.energies.append(eccsd)
         mycc.cycle_times.append(time.perf_counter() - tc)
         log.info("cycle = %d  E_corr(%s) = %.15g  dE = %.9g  norm(t1,t2) = %.6g", istep + 1,

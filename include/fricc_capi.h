@@ -119,11 +119,12 @@ int fricc_contract_prepared(const char* term, const void* A, const int64_t W_sha
  * axis, exactly two letters shared by W and T), out += alpha * einsum(spec).
  * prepared != 0: W_or_A is the kernel layout A[r][x] (see above) in the
  * precision's element type; otherwise an FP64 array with W_strides.
+ * out_precision: element type of out, FRICC_FP64 (double) or FRICC_FP32 (float).
  * Used for the t1-dressing of the ladder (cc_Wvvvv, fricc/friccsd.py:325-331)
  * applied to the sparse T instead of materialising W. */
 int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int prepared, const int64_t W_shape[4],
-                        const int64_t W_strides[4], fricc_sparse* T, double* out, const int64_t out_shape[4],
-                        const int64_t out_strides[4], int precision, void* stream);
+                        const int64_t W_strides[4], fricc_sparse* T, void* out, int out_precision,
+                        const int64_t out_shape[4], const int64_t out_strides[4], int precision, void* stream);
 
 /* (pq|rs) = sum_Q L[Q,p,q] L[Q,r,s] (sequential fma chain over Q) for
  * p in [rng[0],rng[1]), q in [rng[2],rng[3]), r in [rng[4],rng[5]),
@@ -151,6 +152,19 @@ int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, voi
 /* MFMA passes of FRICC_BF16 contractions: 3 = hi*hi + hi*lo + lo*hi (default,
  * about 16 significant bits per product), 1 = hi*hi only. */
 int fricc_set_bf16_passes(int npass);
+
+/* Fused end of the amplitude update: acc (in: direct T2 terms, out: t2new)
+ *   t2new = (ovovT + acc + S + S^T(ij,ab->ji,ba) + t1[i,a] t1[j,b] wsum[a,b])
+ *           / (eia[i,a] + eia[j,b])
+ * and sums[0] = sum (t2new + t1new t1new) * lovov, sums[1] = sum over the
+ * packed lower triangle of (t2new - t2old)^2 (device, FP64, block-ordered).
+ * ovovT, acc, t2old, lovov: [o][o][v][v] FP64; S: same shape, FP64
+ * (s_precision 0) or FP32 (1); t1, eia, t1new: [o][v]; wsum [v][v] or NULL.
+ * Replaces the symmetrisation, denominator, energy and norm passes of
+ * fricc/friccsd.py:239-284 and :418-427. */
+int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_precision,
+                      const double* t1, const double* wsum, const double* eia, const double* t2old,
+                      const double* lovov, const double* t1new, double* sums, void* stream);
 
 /* Block ordering of the gathered contraction GEMM: 0 = x tiles fastest,
  * 1 = column tiles fastest (default).  Tuning knob; no counterpart in the
