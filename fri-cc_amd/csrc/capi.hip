@@ -7,6 +7,7 @@
 namespace fricc {
 const char* last_error();
 extern int g_order;
+extern int g_bf16_width;
 
 // The 11 contraction terms as index strings (src/sparse_tensor.cpp:11-346;
 // the same identities the reference tests assert,
@@ -120,10 +121,14 @@ static int dtype_of(int precision, int* dt) {
 
 static int g_npass = 3;
 
+static int view_width(const Plan& p, int dt) {
+  return dt == DT_BF16 ? bf16_view_width(p.Rh * p.Rl, p.Xh * p.Xl) : 64;
+}
+
 static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, void* out, const int64_t os[4],
                         hipStream_t s, int of32 = 0) {
   View* v = nullptr;
-  FRICC_TRY(get_view(T, p.pos, 64, dt, s, &v));
+  FRICC_TRY(get_view(T, p.pos, view_width(p, dt), dt, s, &v));
   OutMap om;
   om.out = out;
   om.of32 = of32;
@@ -331,7 +336,7 @@ int fricc_sparse_prepare(fricc_sparse* T, const char* spec, const int64_t W_shap
   int dt;
   FRICC_TRY(dtype_of(precision, &dt));
   View* v = nullptr;
-  return get_view(T, p.pos, 64, dt, (hipStream_t)stream, &v);
+  return get_view(T, p.pos, view_width(p, dt), dt, (hipStream_t)stream, &v);
 }
 
 int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream) {
@@ -348,6 +353,13 @@ int fricc_set_bf16_passes(int npass) {
 
 // Block ordering of the gathered GEMM (0: x tiles fastest, 1: column tiles
 // fastest).  Tuning knob; no counterpart in the reference.
+int fricc_set_bf16_view_width(int width) {
+  FRICC_CHECK(width == 0 || (width > 0 && width % 128 == 0 && width <= 4096),
+              "set_bf16_view_width: 0 (automatic) or a multiple of 128 up to 4096");
+  g_bf16_width = width;
+  return OK;
+}
+
 int fricc_set_gemm_order(int order) {
   FRICC_CHECK(order == 0 || order == 1, "set_gemm_order: order must be 0 or 1");
   g_order = order;

@@ -362,26 +362,53 @@ __device__ inline double prob_of(double xv, uint64_t j, const CompScalars& c, do
   return inD ? 0.0 : (fabs(xv) / c.RK) * nsd;
 }
 
+// One tile (TILE doubles) of src staged in LDS with coalesced loads; thread t
+// then owns tile elements [t*ITEMS, t*ITEMS + ITEMS) (the blocked order the
+// prefix needs).  Element e sits at LDS slot e + e/ITEMS: the pad slot keeps
+// the blocked reads of a wave off one bank.
+constexpr int LTILE = TILE + TILE / ITEMS;
+__device__ inline void stage_tile(const double* __restrict__ src, int64_t base, int64_t n, double* lds) {
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const int e = r * NT + threadIdx.x;
+    const int64_t j = base + e;
+    lds[e + e / ITEMS] = j < n ? src[j] : 0.0;
+  }
+  __syncthreads();
+}
+__device__ inline double tile_elem(const double* lds, int r) {
+  const int e = threadIdx.x * ITEMS + r;
+  return lds[e + e / ITEMS];
+}
+
 // pin != nullptr: sample from the given probabilities (sample_systematic,
 // src/fri_utils.cpp:129-159) instead of deriving them from x and D.
+// Per tile: the exact sum of FXP(p) (tsum) and per thread (thsum, in thread
+// order), the latter so that emission recomputes only where it emits.
 __global__ void k_samp_tilesums(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
-                                CompScalars* sc, u128* __restrict__ tsum) {
+                                CompScalars* sc, u128* __restrict__ tsum, u128* __restrict__ thsum) {
   __shared__ u128 sm[NT / WAVE + 1];
+  __shared__ double lds[LTILE];
   const CompScalars c = *sc;
   if (c.n_sampled == 0) return;
   const double nsd = (double)c.n_sampled;
-  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  const int64_t tb = (int64_t)blockIdx.x * TILE;
+  const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
+  stage_tile(pin ? pin : x, tb, n, lds);
   u128 s = 0;
   uint32_t bad = 0;
+#pragma unroll
   for (int r = 0; r < ITEMS; r++) {
-    int64_t j = base + r;
+    const int64_t j = base + r;
     if (j < n) {
-      double p = pin ? pin[j] : prob_of(x[j], (uint64_t)j, c, nsd);
+      const double v = tile_elem(lds, r);
+      const double p = pin ? v : prob_of(v, (uint64_t)j, c, nsd);
       bad |= !(p <= 1.0);  // also catches NaN (remaining norm 0, quirk A.6)
       s += fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
     }
   }
   if (bad) atomicOr(&sc->pflag, 1u);
+  thsum[(int64_t)blockIdx.x * NT + threadIdx.x] = s;
   u128 tot;
   block_excl_scan_u128<NT>(s, &tot, sm);
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
@@ -397,40 +424,51 @@ __device__ inline int64_t first_k_at_least(double P, double u) {
   return k;
 }
 
+// Emission.  A thread's elements hold lattice points k+u exactly when its
+// whole range [P(first), P(last+1)) does (first_k_at_least of the rounded
+// prefix is monotone), so only those threads recompute their elements.
 __global__ void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
-                            const CompScalars* __restrict__ sc, const u128* __restrict__ tsum, double u,
-                            uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
+                            const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
+                            const u128* __restrict__ thsum, double u, uint32_t* __restrict__ out_idx,
+                            double* __restrict__ out_val) {
   __shared__ u128 sm[NT / WAVE + 1];
+  __shared__ double lds[LTILE];
   const CompScalars c = *sc;
   if (c.n_sampled == 0) return;
   const double nsd = (double)c.n_sampled;
   const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
-  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
-  double xv[ITEMS], p[ITEMS];
-  u128 f[ITEMS];
-  u128 s = 0;
-#pragma unroll
-  for (int r = 0; r < ITEMS; r++) {
-    int64_t j = base + r;
-    xv[r] = (j < n && x) ? x[j] : 0.0;
-    p[r] = j < n ? (pin ? pin[j] : prob_of(xv[r], (uint64_t)j, c, nsd)) : 0.0;
-    f[r] = fx_of(dbits(p[r] <= 1.0 ? p[r] : 0.0), 1023u);
-    s += f[r];
-  }
+  const int64_t tb = (int64_t)blockIdx.x * TILE;
+  const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
+  const u128 st = thsum[(int64_t)blockIdx.x * NT + threadIdx.x];
   u128 tot;
-  u128 pre = block_excl_scan_u128<NT>(s, &tot, sm) + tsum[blockIdx.x];
+  u128 pre = block_excl_scan_u128<NT>(st, &tot, sm) + tsum[blockIdx.x];
+  bool work = false;
+  {
+    const double Plo = u128_to_double(pre, -96), Phi = u128_to_double(pre + st, -96);
+    if (Phi > Plo) {
+      int64_t klo = first_k_at_least(Plo, u), khi = first_k_at_least(Phi, u);
+      if (khi > ns) khi = ns;
+      work = klo < khi;
+    }
+  }
+  if (!__syncthreads_or(work)) return;
+  stage_tile(pin ? pin : x, tb, n, lds);
+  if (!work) return;
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
-    int64_t j = base + r;
-    u128 nxt = pre + f[r];
-    if (j < n && f[r] != 0) {
+    const int64_t j = base + r;
+    const double v = tile_elem(lds, r);
+    const double p = j < n ? (pin ? v : prob_of(v, (uint64_t)j, c, nsd)) : 0.0;
+    const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
+    const u128 nxt = pre + f;
+    if (j < n && f != 0) {
       double Plo = u128_to_double(pre, -96), Phi = u128_to_double(nxt, -96);
       if (Phi > Plo) {
         int64_t klo = first_k_at_least(Plo, u), khi = first_k_at_least(Phi, u);
         if (khi > ns) khi = ns;
         for (int64_t k = klo; k < khi; k++) {
           out_idx[K + k] = (uint32_t)j;
-          if (out_val) out_val[K + k] = xv[r] / p[r];
+          if (out_val) out_val[K + k] = v / p;
         }
       }
     }
@@ -485,7 +523,7 @@ __global__ void k_init_scalars(CompScalars* sc, int64_t m) {
 
 // ---------------------------------------------------------------- driver
 enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
-       WS_BH = 8, WS_TS = 9, WS_TOT = 10 };
+       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11 };
 
 int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
              double* out_val, CompressInfo* info, hipStream_t s) {
@@ -573,9 +611,11 @@ int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32
   k_dfinal<<<1, NT, 0, s>>>(kA, vA, m, sc, ts);
   k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, -1, out_idx, out_val);
   // --- systematic sampling of the remainder
-  k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts);
+  void* p_ths;
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (u128*)p_ths);
   k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-  k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, u, out_idx, out_val);
+  k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (const u128*)p_ths, u, out_idx, out_val);
   k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot, u);
   FRICC_HIP(hipGetLastError());
   FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -617,9 +657,11 @@ int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, ui
   h.K = 0;
   h.n_sampled = (uint64_t)n_sample;
   FRICC_HIP(hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, s));
-  k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts);
+  void* p_ths;
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (u128*)p_ths);
   k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-  k_samp_emit<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, u, out_idx, nullptr);
+  k_samp_emit<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (const u128*)p_ths, u, out_idx, nullptr);
   k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot, u);
   FRICC_HIP(hipGetLastError());
   FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -217,10 +217,21 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
 }
 
 // ---------------------------------------------------------------------------
-// BF16 gathered contraction (v_mfma_f32_16x16x32_bf16, FP32 accumulate).
-// A and B come as hi/lo bf16 planes; the 8 rows of a group are contiguous
-// (16 bytes) so each lane loads its 8 K values of one x (or y) at once.
-// npass 3: hi*hi + hi*lo + lo*hi (about 16 significant bits per product).
+// BF16 gathered contraction on v_mfma_f32_16x16x32_bf16 (FP32 accumulate).
+//
+// A comes as hi/lo bf16 planes [ceil(R/8)][X][8]; B (the view) as FP32
+// [group][TY][8] and is split into hi = bf16(b), lo = bf16(b - hi) while it is
+// staged.  npass 3: hi*hi + hi*lo + lo*hi (about 16 significant bits per
+// product), npass 1: hi*hi.
+//
+// A workgroup (4 waves) owns a BX x BY output block: x in [bx*BX, +BX), y a
+// BY-wide slice of one view tile.  Per K step (KG gathered groups = 32 rows =
+// one MFMA K) the whole block stages A[KG][BX] and B[KG][BY] (hi and lo) in
+// LDS, each element fetched from global memory once per block; the next
+// step's global loads are in flight while the waves run their MFMAs from LDS.
+// Waves are 2 (x) x 2 (y); each owns 128 x 64 = 8 x 4 MFMA blocks.
+// Consecutive blocks on one XCD are the BY slices of one tile (same gathered
+// A rows, shared through that XCD's L2).
 typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
 
 __device__ inline bf8 as_bf8(uint4 u) {
@@ -229,130 +240,316 @@ __device__ inline bf8 as_bf8(uint4 u) {
   return r;
 }
 
-template <int NPASS>
-struct FragB16 {
-  uint4 ah[4], al[4], bh[4], bl[4];
+constexpr int BX = 256;
+constexpr int KG = 4;
+
+struct B16Args {
+  const uint4* Ahi;
+  const uint4* Alo;
+  int64_t X, Y;
+  const uint32_t* tile_off;
+  const uint32_t* glist;
+  const uint4* Bhi;  // view planes [group][TY][8] bf16
+  const uint4* Blo;
+  int64_t TY;
+  int64_t nbx, nyb;
+  int nsplit, order;
+  OutDev om;
+  double alpha;
+  float* partial;  // [nsplit][X][Y] when nsplit > 1
 };
 
-template <int NPASS>
-__device__ inline void load_b16(FragB16<NPASS>& f, const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
-                                int64_t X, const uint32_t* __restrict__ glist, const uint4* __restrict__ Bhi,
-                                const uint4* __restrict__ Blo, uint32_t kg, const int64_t xa[4], int lr) {
-  const int64_t ga = (int64_t)glist[kg] * X;
-  const int64_t gb = (int64_t)kg * GT_Y + lr;
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void gbl_void;
+
+// 16 bytes per lane from global memory straight into LDS: lane l of the wave
+// writes LDS byte address lds + 16*l (lds the same for the whole wave).
+// Written as one asm block (M0 set, the one wait state M0 needs before an LDS
+// DMA, the DMA) so that the compiler does not serialise the pipeline with
+// waits of its own; the block has no register outputs, and completion is
+// awaited explicitly with wait_vm before the data is read.
+__device__ inline void dma16(const uint4* g, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g),
+               "s"(__builtin_amdgcn_readfirstlane(lds))
+               : "m0");
+}
+
+__device__ inline uint32_t lds_addr(const void* p) { return (uint32_t)(uintptr_t)(lds_void*)p; }
+
+// s_waitcnt vmcnt(N), other counters untouched (gfx9 encoding)
+template <int N>
+__device__ inline void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <int BYB>
+struct StageMem {
+  uint4 A[2][KG][BX];   // [hi|lo][group][x]
+  uint4 B[2][KG][BYB];  // [hi|lo][group][y]
+};
+
+// Operands of the pipeline, copied out of the kernel arguments once.  The
+// row list is read through the constant address space: it is never written
+// while the kernel runs, so its loads stay scalar (no vector-memory wait).
+typedef const __attribute__((address_space(4))) uint32_t cu32;
+struct StageSrc {
+  const uint4* Ahi;
+  const uint4* Alo;
+  const uint4* Bhi;
+  const uint4* Blo;
+  cu32* glist;  // row list (constant address space)
+  int64_t X, TY;
+};
+
+// One pipeline stage: the KG gathered groups starting at k, A columns
+// [x0, +BX) and B columns [bcol, +BYB) of the tile, hi and lo planes.  Wave w
+// loads A column chunk w (64 columns) of every group and plane, and B column
+// chunk w%2 of groups w/2 and w/2+2; all selections are compile-time except
+// the wave's own offsets, so nothing is indexed through memory.
+template <int NPASS, int BYB>
+__device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t x0, int64_t bcol, StageMem<BYB>& sm, int w,
+                                   int lane) {
+  static_assert(BX == 256 && BYB == 128 && KG == 4, "wave assignment assumes 4 waves, BX 256, BYB 128");
+  const uint32_t g0 = p.glist[k], g1 = p.glist[k + 1], g2 = p.glist[k + 2], g3 = p.glist[k + 3];
+  const uint32_t la = lds_addr(&sm.A[0][0][0]), lb = lds_addr(&sm.B[0][0][0]);
+  int64_t x = x0 + w * 64 + lane;
+  if (x >= p.X) x = p.X - 1;
+  const uint32_t ga[4] = {g0, g1, g2, g3};
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    f.ah[s] = Ahi[ga + xa[s]];
-    if (NPASS > 1) f.al[s] = Alo[ga + xa[s]];
+  for (int gl = 0; gl < KG; gl++) {
+    const int64_t off = (int64_t)ga[gl] * p.X + x;
+    const uint32_t l = la + (uint32_t)((gl * BX + w * 64) * 16);
+    dma16(p.Ahi + off, l);
+    if (NPASS > 1) dma16(p.Alo + off, l + (uint32_t)(KG * BX * 16));
   }
+  const int yc = w & 1;
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    f.bh[q] = Bhi[gb + 16 * q];
-    if (NPASS > 1) f.bl[q] = Blo[gb + 16 * q];
+  for (int h = 0; h < 2; h++) {
+    const int gl = (w >> 1) + 2 * h;
+    const int64_t off = (int64_t)(k + gl) * p.TY + bcol + yc * 64 + lane;
+    const uint32_t l = lb + (uint32_t)((gl * BYB + yc * 64) * 16);
+    dma16(p.Bhi + off, l);
+    if (NPASS > 1) dma16(p.Blo + off, l + (uint32_t)(KG * BYB * 16));
   }
 }
 
-template <int NPASS>
-__device__ inline void mma_b16(f4 (&acc)[4][4], const FragB16<NPASS>& f) {
+// MFMAs of one stage: wave (wx, wy) owns x [wx*128, +128) and y [wy*BYB/2, +BYB/2)
+template <int NPASS, int BYB>
+__device__ inline void stage_compute(const StageMem<BYB>& sm, f4 (&acc)[8][BYB / 32], int wx, int wy, int lr, int lk) {
+  constexpr int NQ = BYB / 32;
+  uint4 bh[NQ], bl[NQ];
 #pragma unroll
-  for (int s = 0; s < 4; s++)
+  for (int q = 0; q < NQ; q++) {
+    bh[q] = sm.B[0][lk][wy * (BYB / 2) + 16 * q + lr];
+    if (NPASS > 1) bl[q] = sm.B[1][lk][wy * (BYB / 2) + 16 * q + lr];
+  }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(f.ah[s]), as_bf8(f.bh[q]), acc[s][q], 0, 0, 0);
+  for (int s = 0; s < 8; s++) {
+    const uint4 ah = sm.A[0][lk][wx * 128 + 16 * s + lr];
+    uint4 al;
+    if (NPASS > 1) al = sm.A[1][lk][wx * 128 + 16 * s + lr];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(ah), as_bf8(bh[q]), acc[s][q], 0, 0, 0);
       if (NPASS > 1) {
-        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(f.ah[s]), as_bf8(f.bl[q]), acc[s][q], 0, 0, 0);
-        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(f.al[s]), as_bf8(f.bh[q]), acc[s][q], 0, 0, 0);
+        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(ah), as_bf8(bl[q]), acc[s][q], 0, 0, 0);
+        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(al), as_bf8(bh[q]), acc[s][q], 0, 0, 0);
       }
     }
+  }
 }
 
-template <int NPASS>
-__global__ __launch_bounds__(256) void k_ggemm_bf16(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
-                                                    int64_t X, const uint32_t* __restrict__ tile_off,
-                                                    const uint32_t* __restrict__ glist, const uint4* __restrict__ Bhi,
-                                                    const uint4* __restrict__ Blo, int64_t nbx, int64_t ntiles,
-                                                    int64_t Y, OutDev om, double alpha, int order) {
-  int64_t bx, t;
-  xcd_remap(nbx, ntiles, order, bx, t);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int64_t xw = bx * GT_X + 64 * w;
-  const uint32_t g_beg = tile_off[t], g_end = tile_off[t + 1];  // groups; multiple of 4 per tile
-  f4 acc[4][4];
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int q = 0; q < 4; q++) acc[s][q] = f4{0, 0, 0, 0};
-  int64_t xa[4];
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    xa[s] = xw + 16 * s + lr;
-    if (xa[s] >= X) xa[s] = X - 1;
-  }
-  if (g_beg < g_end) {
-    FragB16<NPASS> f0, f1;
-    load_b16<NPASS>(f0, Ahi, Alo, X, glist, Bhi, Blo, g_beg + lk, xa, lr);
-    uint32_t kg = g_beg;
-    while (true) {
-      const bool more = kg + 4 < g_end;
-      if (more) load_b16<NPASS>(f1, Ahi, Alo, X, glist, Bhi, Blo, kg + 4 + lk, xa, lr);
-      mma_b16<NPASS>(acc, f0);
-      if (!more) break;
-      kg += 4;
-      const bool more2 = kg + 4 < g_end;
-      if (more2) load_b16<NPASS>(f0, Ahi, Alo, X, glist, Bhi, Blo, kg + 4 + lk, xa, lr);
-      mma_b16<NPASS>(acc, f1);
-      if (!more2) break;
-      kg += 4;
+// A workgroup of 4 waves owns a BX x BYB output block: x in [bx*BX, +BX), y a
+// BYB-wide slice of one view tile; each wave owns 128 x BYB/2.  K advances KG gathered
+// groups (32 rows, one MFMA K) per stage; stages are filled by global->LDS
+// DMA S-1 stages ahead of the MFMAs that read them, so no staging registers
+// are needed and every element is fetched once per workgroup.  Each stage
+// buffer is a separate LDS variable so in-flight DMA provably does not alias
+// the stage being read.  Consecutive workgroups on one XCD are the BYB slices
+// of one tile (same gathered A rows, shared through that XCD's L2).
+template <int NPASS, int BYB, int S>
+__global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
+  constexpr int NW = 4;
+  constexpr int NQ = BYB / 32;
+  constexpr int LW = (2 * KG * (BX + BYB) / 64) / NW;  // DMA instructions per wave per stage (12)
+  constexpr int LWP = NPASS > 1 ? LW : LW / 2;  // the lo planes are skipped with one pass
+  __shared__ StageMem<BYB> st[S];
+  static_assert(S == 2 || S == 3, "2 or 3 stages");
+  int64_t bx, yb;
+  int sp;
+  {
+    const int64_t nb = a.nbx * a.nyb * a.nsplit;
+    int64_t id = blockIdx.x;
+    const int64_t per = nb / 8;
+    if (per * 8 == nb) id = (id % 8) * per + id / 8;
+    sp = (int)(id % a.nsplit);
+    id /= a.nsplit;
+    if (a.order == 0) {
+      yb = id / a.nbx;
+      bx = id % a.nbx;
+    } else {
+      bx = id / a.nyb;
+      yb = id % a.nyb;
     }
   }
+  const int64_t sub = a.TY / BYB;
+  const int64_t t = yb / sub;
+  const int64_t bcol = (yb % sub) * BYB;  // first column of this block within the tile
+  const int64_t y0 = t * a.TY + bcol;
+  uint32_t g_beg = a.tile_off[t], g_end = a.tile_off[t + 1];
+  if (a.nsplit > 1) {
+    const uint32_t nst = (g_end - g_beg) / KG;
+    const uint32_t per = (nst + a.nsplit - 1) / a.nsplit;
+    const uint32_t s0 = min(nst, per * (uint32_t)sp), s1 = min(nst, per * (uint32_t)(sp + 1));
+    g_end = g_beg + s1 * KG;
+    g_beg = g_beg + s0 * KG;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wx = w & 1, wy = w >> 1;
+  const int64_t x0 = bx * BX;
+  StageSrc ps;
+  ps.Ahi = a.Ahi;
+  ps.Alo = a.Alo;
+  ps.Bhi = a.Bhi;
+  ps.Blo = a.Blo;
+  ps.glist = (cu32*)a.glist;
+  ps.X = a.X;
+  ps.TY = a.TY;
+  f4 acc[8][NQ];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int64_t y = t * GT_Y + 16 * q + lr;
-    if (y >= Y) continue;
+  for (int s = 0; s < 8; s++)
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[s][q] = f4{0, 0, 0, 0};
+  // Pipeline: stage i lives in buffer i % S.  Each step waits for this
+  // wave's DMA of the current stage (the next one may stay in flight),
+  // barriers (every wave's DMA of the stage landed; every wave is done with
+  // the buffer computed last step), refills that buffer with the stage S-1
+  // ahead, then runs the MFMAs of the current stage.  The fences around the
+  // barrier order LDS accesses only, so the explicit vmcnt wait is the only
+  // wait on the DMA.
+  uint32_t k = g_beg;
+  if (k < g_end) {
+#pragma unroll
+    for (int s = 0; s < S - 1; s++)
+      if (k + s * KG < g_end) stage_issue<NPASS, BYB>(ps, k + s * KG, x0, bcol, st[s], w, lane);
+    int cur = 0;
+    for (; k < g_end; k += KG) {
+      if (S > 2 && k + KG < g_end)
+        wait_vm<(S > 2 ? LWP : 0)>();
+      else
+        wait_vm<0>();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      const int nxt = cur == 0 ? S - 1 : cur - 1;  // (cur + S - 1) % S
+      if (k + (S - 1) * KG < g_end) stage_issue<NPASS, BYB>(ps, k + (S - 1) * KG, x0, bcol, st[nxt], w, lane);
+      stage_compute<NPASS, BYB>(st[cur], acc, wx, wy, lr, lk);
+      cur = cur == S - 1 ? 0 : cur + 1;
+    }
+  }
+  // D layout: lane holds column lane%16 of each 16x16 block, rows 4*(lane/16)+i
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int64_t y = y0 + wy * (BYB / 2) + 16 * q + lr;
+    if (y >= a.Y) continue;
     const uint32_t yy = (uint32_t)y;
-    const int64_t oy = (int64_t)(yy / om.Yl) * om.oyh + (int64_t)(yy % om.Yl) * om.oyl;
+    const int64_t oy = (int64_t)(yy / a.om.Yl) * a.om.oyh + (int64_t)(yy % a.om.Yl) * a.om.oyl;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < 8; s++) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int64_t x = xw + 16 * s + 4 * lk + i;
-        if (x < X) {
-          const uint32_t xx = (uint32_t)x;
-          out_add(om, (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy, alpha,
-                  (double)acc[s][q][i]);
+        const int64_t x = x0 + wx * 128 + 16 * s + 4 * lk + i;
+        if (x < a.X) {
+          if (a.nsplit > 1) {
+            a.partial[((int64_t)sp * a.X + x) * a.Y + y] = acc[s][q][i];
+          } else {
+            const uint32_t xx = (uint32_t)x;
+            out_add(a.om, (int64_t)(xx / a.om.Xl) * a.om.oxh + (int64_t)(xx % a.om.Xl) * a.om.oxl + oy, a.alpha,
+                    (double)acc[s][q][i]);
+          }
         }
       }
     }
   }
 }
 
+// out[x, y] += alpha * sum_s partial[s][x][y], splits added in order
+__global__ void k_splitk_reduce(const float* __restrict__ partial, int nsplit, int64_t X, int64_t Y, OutDev om,
+                                double alpha) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= X * Y) return;
+  float acc = 0.0f;
+  for (int s = 0; s < nsplit; s++) acc += partial[(int64_t)s * X * Y + e];
+  const uint32_t xx = (uint32_t)(e / Y), yy = (uint32_t)(e % Y);
+  out_add(om,
+          (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + (int64_t)(yy / om.Yl) * om.oyh +
+              (int64_t)(yy % om.Yl) * om.oyl,
+          alpha, (double)acc);
+}
+
+int g_bf16_width = 0;  // 0: by operand size
+
+int bf16_view_width(int64_t R, int64_t X) {
+  if (g_bf16_width > 0) return g_bf16_width;
+  // operands too large to stay in the caches are streamed once per tile:
+  // wider tiles cut that traffic (the ladder's A is R*X*4 bytes)
+  return (double)R * (double)X * 4.0 > 1.0e9 ? 512 : 128;
+}
+
 int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& m, double alpha,
                        int npass, hipStream_t s) {
-  FRICC_CHECK(v->TY == GT_Y && v->dtype == DT_BF16, "gathered_gemm_bf16: needs a BF16 view with tile width 64");
+  FRICC_CHECK(v->dtype == DT_BF16 && v->TY % 128 == 0, "gathered_gemm_bf16: needs a BF16 view with tile width k*128");
   FRICC_CHECK(X > 0 && X < ((int64_t)1 << 31), "gathered_gemm_bf16: bad X");
   FRICC_CHECK(npass == 1 || npass == 3, "gathered_gemm_bf16: npass must be 1 or 3");
   if (v->ntiles == 0) return OK;
-  OutDev om;
-  om.out = m.out;
-  om.of32 = m.of32;
-  om.Xl = (uint32_t)m.Xl;
-  om.Yl = (uint32_t)v->Yl;
-  om.oxh = m.oxh;
-  om.oxl = m.oxl;
-  om.oyh = m.oyh;
-  om.oyl = m.oyl;
+  B16Args a;
+  a.om.out = m.out;
+  a.om.of32 = m.of32;
+  a.om.Xl = (uint32_t)m.Xl;
+  a.om.Yl = (uint32_t)v->Yl;
+  a.om.oxh = m.oxh;
+  a.om.oxl = m.oxl;
+  a.om.oyh = m.oyh;
+  a.om.oyl = m.oyl;
   const int64_t Rg = (R + 7) / 8;
-  const uint4* Ahi = (const uint4*)A2;
-  const uint4* Alo = Ahi + Rg * X;
-  const int64_t nbx = ceil_div(X, GT_X);
-  const int64_t nb = nbx * v->ntiles;
+  a.Ahi = (const uint4*)A2;
+  a.Alo = a.Ahi + Rg * X;
+  a.X = X;
+  a.Y = v->Y;
+  a.tile_off = v->tile_off;
+  a.glist = v->rowlist;
+  a.Bhi = (const uint4*)v->B;
+  a.Blo = (const uint4*)v->Blo;
+  a.TY = v->TY;
+  a.nbx = ceil_div(X, BX);
+  // 128-wide y blocks, 3 stages in flight (144 KB of LDS, one workgroup per CU)
+  const int byb = 128;
+  a.nyb = v->ntiles * (v->TY / byb);
+  a.order = g_order;
+  a.alpha = alpha;
+  a.partial = nullptr;
+  // small grids: split each tile's K range, partial sums added in order
+  const int64_t nb0 = a.nbx * a.nyb;
+  int nsplit = 1;
+  if (nb0 < 512) {
+    nsplit = (int)std::min<int64_t>(16, ceil_div(512, nb0));
+    while (nsplit > 1 && (double)nsplit * X * v->Y * 4.0 > 1.0e9) nsplit--;
+  }
+  a.nsplit = nsplit;
+  if (nsplit > 1) {
+    void* p;
+    FRICC_TRY(ws_get(32, (size_t)nsplit * X * v->Y * 4, &p));
+    a.partial = (float*)p;
+  }
+  const int64_t nb = nb0 * nsplit;
   if (npass == 3)
-    k_ggemm_bf16<3><<<nb, 256, 0, s>>>(Ahi, Alo, X, v->tile_off, v->rowlist, (const uint4*)v->B, (const uint4*)v->Blo,
-                                       nbx, v->ntiles, v->Y, om, alpha, g_order);
+    k_ggemm_bf16<3, 128, 3><<<nb, 256, 0, s>>>(a);
   else
-    k_ggemm_bf16<1><<<nb, 256, 0, s>>>(Ahi, Alo, X, v->tile_off, v->rowlist, (const uint4*)v->B, (const uint4*)v->Blo,
-                                       nbx, v->ntiles, v->Y, om, alpha, g_order);
+    k_ggemm_bf16<1, 128, 3><<<nb, 256, 0, s>>>(a);
+  FRICC_HIP(hipGetLastError());
+  if (nsplit > 1) k_splitk_reduce<<<ceil_div(X * v->Y, 256), 256, 0, s>>>(a.partial, nsplit, X, v->Y, a.om, alpha);
   FRICC_HIP(hipGetLastError());
   return OK;
 }

@@ -31,12 +31,12 @@ struct View {
   int pos[4];          // rh, rl, yh, yl
   int64_t Rh, Rl, Yh, Yl, R, Y;
   int TY, kstep, dtype;
-  int64_t ntiles = 0, total_rows = 0, nnz = 0;
-  uint32_t* tile_off = nullptr;  // [ntiles + 1] first gathered row of each tile
-  uint32_t* rowlist = nullptr;   // [total_rows] r of each gathered row
-  void* B = nullptr;             // [total_rows x TY] values (dtype); BF16: hi plane [groups][TY][8]
+  int64_t ntiles = 0, nnz = 0;
+  uint32_t* tile_off = nullptr;  // [ntiles + 1] first gathered row of each tile (device; last = total)
+  uint32_t* rowlist = nullptr;   // [cap_rows] r of each gathered row (BF16: r/8 of each group)
+  void* B = nullptr;             // [rows x TY] values (dtype); BF16: hi plane, bf16 [groups][TY][8]
   void* Blo = nullptr;           // BF16 only: lo plane (value - hi), same layout
-  size_t cap_rows = 0;
+  size_t cap_rows = 0;           // worst-case rows the buffers hold
   uint64_t gen = 0;
   ~View();
 };
@@ -72,6 +72,8 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
 // ((r/8)*X + x)*8 + r%8.  npass 3 = hi*hi + hi*lo + lo*hi, 1 = hi*hi.
 int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& om, double alpha,
                        int npass, hipStream_t s);
+// tile width of BF16 views for an operand of R x X
+int bf16_view_width(int64_t R, int64_t X);
 // FP64/FP32 row-major A[R][X] -> [hi | lo] bf16 planes (layout above)
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s);
 

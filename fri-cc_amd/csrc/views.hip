@@ -119,7 +119,7 @@ __global__ void k_fill(const uint32_t* __restrict__ idx, const double* __restric
   atomicAdd(&B[(int64_t)k * g.TY + (y % g.TY)], (T)val[e]);
 }
 
-// BF16 view: accumulate into an FP32 [group][TY][8] buffer
+// BF16 view: accumulate into an FP32 [group][TY][8] buffer (split afterwards)
 __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* __restrict__ val, int64_t nnz, VGeom g,
                                const uint32_t* __restrict__ slot, float* __restrict__ B) {
   int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
@@ -131,18 +131,38 @@ __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* _
   atomicAdd(&B[((int64_t)k * g.TY + (y % g.TY)) * 8 + (r & 7)], (float)val[e]);
 }
 
-// f32 -> bf16 hi (round to nearest) and lo = bf16(v - hi)
-__global__ void k_split_f32(const float* __restrict__ in, __hip_bfloat16* __restrict__ hi, __hip_bfloat16* __restrict__ lo,
-                            int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x;
-  if (i < n) {
-    float v = in[i];
-    __hip_bfloat16 h = __float2bfloat16(v);
+// Zero the first (*rows) rows of a row-major buffer of `words` uint4 per row;
+// the row count is read on the device, so the host never waits for it.
+__global__ void k_zero_rows(const uint32_t* __restrict__ rows, int64_t words, uint4* __restrict__ B) {
+  const int64_t n = (int64_t)(*rows) * words;
+  const int64_t stride = (int64_t)gridDim.x * VNT;
+  for (int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x; i < n; i += stride) B[i] = make_uint4(0, 0, 0, 0);
+}
+
+// FP32 [rows][w] -> bf16 hi = bf16(v), lo = bf16(v - hi) for the first (*rows)
+// rows (row count read on the device), round to nearest even
+__global__ void k_split_rows(const uint32_t* __restrict__ rows, int64_t w, const float* __restrict__ in,
+                             __hip_bfloat16* __restrict__ hi, __hip_bfloat16* __restrict__ lo) {
+  const int64_t n = (int64_t)(*rows) * w;
+  const int64_t stride = (int64_t)gridDim.x * VNT;
+  for (int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x; i < n; i += stride) {
+    const float f = in[i];
+    const __hip_bfloat16 h = __float2bfloat16(f);
     hi[i] = h;
-    lo[i] = __float2bfloat16(v - __bfloat162float(h));
+    lo[i] = __float2bfloat16(f - __bfloat162float(h));
   }
 }
 
+// Worst-case gathered rows of a view: every tile holds at most all of its
+// slot rows and at most one row per entry, each padded to the K step.
+static size_t view_capacity(int64_t ntiles, int64_t slot_rows, int64_t nnz, int kstep) {
+  const int64_t pad = ntiles * (kstep - 1);
+  return (size_t)std::max<int64_t>(1, std::min(ntiles * slot_rows + pad, nnz + pad));
+}
+
+// Builds the view on the stream without a host round trip: storage is sized
+// for the worst case (at most the dense tensor, 4 or 8 bytes per element), the
+// per-tile offsets stay on the device and the GEMM reads them there.
 static int build_view(Sparse* sp, View* v, hipStream_t s) {
   const int64_t* d = sp->dims;
   v->Rh = d[v->pos[0]];
@@ -174,48 +194,44 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   uint32_t* slot = (uint32_t*)p_slot;
   uint32_t* cnt = (uint32_t*)p_cnt;
   if (!v->tile_off) FRICC_HIP(hipMalloc(&v->tile_off, (size_t)(v->ntiles + 1) * 4));
-  FRICC_HIP(hipMemsetAsync(slot, 0, nslot * 4, s));
   const int64_t nnz = sp->nnz;
-  if (nnz > 0) k_mark<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, nnz, g, slot);
-  k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
-  FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));
-  uint32_t total = 0;
-  FRICC_HIP(hipMemcpyAsync(&total, v->tile_off + v->ntiles, 4, hipMemcpyDeviceToHost, s));
-  FRICC_HIP(hipStreamSynchronize(s));
-  v->total_rows = total;
-  // bytes per gathered row (or group of 8 rows) of the B block
+  // bytes per gathered row (BF16 views: per group of 8 rows and plane, bf16 [TY][8])
   const size_t rowbytes = grouped ? (size_t)v->TY * 8 * 2 : (size_t)v->TY * (v->dtype == DT_F64 ? 8 : 4);
-  if (v->cap_rows < (size_t)total || !v->rowlist) {
+  const size_t cap = view_capacity(v->ntiles, g.R, nnz, v->kstep);
+  if (v->cap_rows < cap || !v->rowlist) {
     if (v->rowlist) FRICC_HIP(hipFree(v->rowlist));
     if (v->B) FRICC_HIP(hipFree(v->B));
     if (v->Blo) FRICC_HIP(hipFree(v->Blo));
     v->rowlist = nullptr;
     v->B = nullptr;
     v->Blo = nullptr;
-    size_t cap = (size_t)total + (size_t)total / 8 + 64;
     FRICC_HIP(hipMalloc(&v->rowlist, cap * 4));
     FRICC_HIP(hipMalloc(&v->B, cap * rowbytes));
     if (grouped) FRICC_HIP(hipMalloc(&v->Blo, cap * rowbytes));
     v->cap_rows = cap;
   }
+  FRICC_HIP(hipMemsetAsync(slot, 0, nslot * 4, s));
+  if (nnz > 0) k_mark<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, nnz, g, slot);
+  k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
+  FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));
   k_tile_slots<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->tile_off, v->rowlist);
-  const size_t nB = (size_t)total * v->TY;
-  if (v->dtype == DT_F64) {
-    FRICC_HIP(hipMemsetAsync(v->B, 0, nB * 8, s));
-    if (nnz > 0) k_fill<double><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (double*)v->B);
-  } else if (v->dtype == DT_F32) {
-    FRICC_HIP(hipMemsetAsync(v->B, 0, nB * 4, s));
-    if (nnz > 0) k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)v->B);
+  const uint32_t* total = v->tile_off + v->ntiles;  // gathered rows (groups), on the device
+  if (!grouped) {
+    k_zero_rows<<<1024, VNT, 0, s>>>(total, (int64_t)(rowbytes / 16), (uint4*)v->B);
+    if (nnz > 0) {
+      if (v->dtype == DT_F64)
+        k_fill<double><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (double*)v->B);
+      else
+        k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)v->B);
+    }
   } else {
-    const size_t nG = nB * 8;  // groups x TY x 8
+    const int64_t w = v->TY * 8;  // FP32 values per group
     void* p_tmp;
-    FRICC_TRY(ws_get(22, nG * 4, &p_tmp));
-    FRICC_HIP(hipMemsetAsync(p_tmp, 0, nG * 4, s));
+    FRICC_TRY(ws_get(22, cap * (size_t)w * 4, &p_tmp));
+    k_zero_rows<<<1024, VNT, 0, s>>>(total, w / 4, (uint4*)p_tmp);
     if (nnz > 0)
       k_fill_grouped<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)p_tmp);
-    if (nG > 0)
-      k_split_f32<<<ceil_div((int64_t)nG, VNT), VNT, 0, s>>>((float*)p_tmp, (__hip_bfloat16*)v->B,
-                                                            (__hip_bfloat16*)v->Blo, nG);
+    k_split_rows<<<2048, VNT, 0, s>>>(total, w, (const float*)p_tmp, (__hip_bfloat16*)v->B, (__hip_bfloat16*)v->Blo);
   }
   FRICC_HIP(hipGetLastError());
   return OK;

@@ -241,8 +241,8 @@ def update_amps(cc, t1, t2, eris):
     timings["compress"] = time.perf_counter() - t0
 
     # ---- Foo, Fvv, Fov (imd.cc_Foo / cc_Fvv / cc_Fov)
-    ovov = P["ovov"]
-    Fov_ = 2 * torch.einsum("kcld,ld->kc", ovov, t1c) - torch.einsum("kdlc,ld->kc", ovov, t1c)
+    # Fkc = sum_ld [2 (kc|ld) - (kd|lc)] t1[l,d] = sum_l Lov2[k,l] (c x d) @ t1[l]
+    Fov_ = torch.matmul(P["Lov2"], t1c.unsqueeze(-1)).sum(1).reshape(o, v)
     # Fki = sum_lcd Lov2[k,(lcd)] t2[i,(lcd)];  Fac = -sum_kl t2[kl] @ Lov[kl]
     Fki = _mm_splitk_nt(P["Lov2"].reshape(o, -1), t2c.reshape(o, -1)) + Fov_ @ t1c.T
     Fac = -torch.bmm(t2c.reshape(o * o, v, v), P["Lov"].reshape(o * o, v, v)).sum(0) - t1c.T @ Fov_
@@ -258,10 +258,15 @@ def update_amps(cc, t1, t2, eris):
     t1new = -2 * ((t1 @ fov.T) @ t1)
     t1new += t1 @ Fvv.T
     t1new -= Foo.T @ t1
-    t1new += 2 * torch.einsum("kc,kica->ia", Fov, t2) - torch.einsum("kc,ikca->ia", Fov, t2)
+    # 2 sum_kc Fov[k,c] t2[k,i,c,a] - sum_kc Fov[k,c] t2[i,k,c,a]  (batched matrix-vector, t2 read once each)
+    Fovc = Fov.to(cdt)
+    t1new += C(2 * torch.matmul(Fovc.view(o, 1, 1, v), t2c).sum(0).reshape(o, v)
+               - torch.matmul(Fovc.reshape(1, ov), t2c.reshape(o, ov, v)).reshape(o, v))
     t1new += (t1 @ Fov.T) @ t1
     t1new += fov
-    t1new += 2 * torch.einsum("kcai,kc->ia", eris.ovvo, t1) - torch.einsum("kiac,kc->ia", eris.oovv, t1)
+    # 2 sum_kc (kc|ai) t1[k,c] - sum_kc (ki|ac) t1[k,c]
+    t1new += C(2 * (t1c.reshape(1, ov) @ P["ovvo"].reshape(ov, v * o)).reshape(v, o).T
+               - torch.matmul(P["oovv"].reshape(o, ov, v), t1c.view(o, v, 1)).sum(0).reshape(o, v))
     # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] @ Z[(kcd),a]
     t1new += C(_mm_splitk(t2c.reshape(o, -1), P["Z"].reshape(-1, v)))
     # sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac)
@@ -270,7 +275,7 @@ def update_amps(cc, t1, t2, eris):
     ovoo = eris.ovoo
     # -2 lcki,klac + kcli,klac  ==  sum_kl Aoo[kl] (i x c) @ t2[kl]^T (c x a)
     Aoo = (-2 * ovoo.permute(2, 0, 3, 1) + ovoo.permute(0, 2, 3, 1)).reshape(o * o, o, v)
-    t1new += torch.bmm(Aoo, t2.reshape(o * o, v, v).transpose(1, 2)).sum(0)
+    t1new += C(torch.bmm(Aoo.to(cdt), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
     t1new += -2 * torch.einsum("lcki,lc,ka->ia", ovoo, t1, t1) + torch.einsum("kcli,lc,ka->ia", ovoo, t1, t1)
 
     # ---- Loo, Lvv (imd.Loo / imd.Lvv)
@@ -281,23 +286,26 @@ def update_amps(cc, t1, t2, eris):
 
     # ---- W intermediates (fricc/friccsd.py:194-207)
     t0 = time.perf_counter()
+    # U[k,c,l,i] = sum_d (kc|ld) t1[i,d]: with (ld|kc) = (kc|ld) and (kc|ai) = (ia|kc)
+    # it gives the t1 pieces of Woooo, of both A_vo intermediates and of T2p
+    U = (P["ovov"].reshape(ov * o, v) @ t1c.T).reshape(o, v, o, o)
     Woooo = torch.einsum("lcki,jc->klij", ovoo, t1) + torch.einsum("kclj,ic->klij", ovoo, t1)
     Woooo = Woooo.contiguous()
     if "O^4V^2" in ccs:
         _contract_into("kcld,ijcd->klij", 1.0, eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
     else:
         Woooo += torch.einsum("kcld,ijcd->klij", eris.ovov, t2)
-    Woooo += torch.einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
+    Woooo += C(torch.einsum("ic,kclj->klij", t1c, U))  # sum_cd (kc|ld) t1[i,c] t1[j,d]
     Woooo += eris.oooo.permute(0, 2, 1, 3)
     # Wvoov as A_voov[k,c,i,a] = Wvoov[a,k,i,c];  Wvovo as A_vovo[k,c,i,a] = Wvovo[a,k,c,i]
     t1b = t1c.expand(ov, o, v)
     A_voov = torch.baddbmm(P["ovvo_p"].reshape(ov, o, v), t1b, P["P1"].reshape(ov, v, v))
     A_vovo = torch.baddbmm(P["oovv_p"].reshape(ov, o, v), t1b, P["P2"].reshape(ov, v, v))
     # - sum_l (kcli + sum_d ldkc t1[i,d]) t1[l,a];  - sum_l (lcki + sum_d lckd t1[i,d]) t1[l,a]
-    N1 = ovoo + torch.einsum("ldkc,id->kcli", eris.ovov, t1)
-    N2 = ovoo.permute(2, 1, 0, 3) + torch.einsum("lckd,id->kcli", eris.ovov, t1)
-    A_voov.baddbmm_(N1.to(cdt).reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
-    A_vovo.baddbmm_(N2.to(cdt).reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
+    N1 = P["ovoo"] + U                                   # kcli + sum_d (ld|kc) t1[i,d]
+    N2 = P["ovoo"].permute(2, 1, 0, 3) + U.permute(2, 1, 0, 3)  # lcki + sum_d (lc|kd) t1[i,d]
+    A_voov.baddbmm_(N1.reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
+    A_vovo.baddbmm_(N2.reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
     A_voov = A_voov.reshape(o, v, o, v)
     A_vovo = A_vovo.reshape(o, v, o, v)
     Wvoov = A_voov.permute(3, 0, 2, 1)  # [a,k,i,c] view
@@ -365,11 +373,12 @@ def update_amps(cc, t1, t2, eris):
     # tmp = (ia|cb) t1[j,c] - sum_k oovv[k,i,b,c] t1[j,c] t1[k,a]
     S = torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v)).reshape(o, o, v, v)
     S3 = S.reshape(o * o, v, v)
-    M = torch.einsum("kibc,jc->ijkb", eris.oovv, t1).to(cdt)
+    # M[i,j,k,b] = sum_c (ki|bc) t1[j,c]
+    M = (P["oovv"].reshape(o * o * v, v) @ t1c.T).reshape(o, o, v, o).permute(1, 3, 0, 2)
     S3.baddbmm_(t1c.T.expand(o * o, v, o), M.reshape(o * o, o, v), alpha=-1)
-    # - (kcai,jc + ovoo[i,a,j,k]) t1[k,b]
-    T2p = torch.einsum("kcai,jc->ijak", eris.ovvo, t1) + ovoo.permute(0, 2, 1, 3)
-    S3.baddbmm_(T2p.to(cdt).reshape(o * o, v, o), t1c.expand(o * o, o, v), alpha=-1)
+    # - (sum_c (kc|ai) t1[j,c] + ovoo[i,a,j,k]) t1[k,b];  sum_c (ia|kc) t1[j,c] = U[i,a,k,j]
+    T2p = U.permute(0, 3, 1, 2) + P["ovoo"].permute(0, 2, 1, 3)
+    S3.baddbmm_(T2p.reshape(o * o, v, o), t1c.expand(o * o, o, v), alpha=-1)
     # + Lvv[a,c] t2[i,j,c,b] - Loo[k,i] t2[k,j,a,b]
     S3.baddbmm_(Lvv.to(cdt).expand(o * o, v, v), t2c.reshape(o * o, v, v))
     S.reshape(o, o * v * v).addmm_(Loo.to(cdt).T, t2c.reshape(o, -1), alpha=-1)

@@ -166,6 +166,11 @@ int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void
                       const double* t1, const double* wsum, const double* eia, const double* t2old,
                       const double* lovov, const double* t1new, double* sums, void* stream);
 
+/* Column-tile width of the gathered views of FRICC_BF16 contractions: 0 =
+ * automatic (512 for operands too large for the caches, else 128), or a
+ * multiple of 128.  Tuning knob; no counterpart in the reference. */
+int fricc_set_bf16_view_width(int width);
+
 /* Block ordering of the gathered contraction GEMM: 0 = x tiles fastest,
  * 1 = column tiles fastest (default).  Tuning knob; no counterpart in the
  * reference. */

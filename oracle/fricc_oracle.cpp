@@ -215,19 +215,26 @@ int orc_compress(const double* x, int64_t n, int64_t m, int method, int mode, do
   for (int64_t k = 0; k < K; k++) p[sig[k]] = 0.0;
   // p > 1 makes the reference exit (src/fri_utils.cpp:383-395); a NaN p
   // (remaining norm 0 when m exceeds the nonzeros, see the guard at :415)
-  // makes it emit index 0 with a non-finite value.  Both are reported.
-  for (int64_t i = 0; i < n; i++)
-    if (!(p[i] <= 1.0)) { info->status = 1; return 1; }
-  std::vector<double> P(n + 1);
-  if (mode == 0) {
-    orc_prefix_canonical(p.data(), n, P.data());
-  } else {
-    P[0] = 0.0;
-    std::partial_sum(p.begin(), p.end(), P.begin() + 1);
-  }
-  info->p_total = P[n];
+  // makes it emit index 0 with a non-finite value.  Both are reported.  With
+  // n' = 0 (D took all m entries) nothing is sampled and p is never used: the
+  // reference's check passes (NaN > 1 is false) and it returns D alone.
   std::vector<int64_t> S;
-  info->n_notfound = systematic_scan(ns, P.data(), n, u, S);
+  if (ns > 0) {
+    for (int64_t i = 0; i < n; i++)
+      if (!(p[i] <= 1.0)) { info->status = 1; return 1; }
+    std::vector<double> P(n + 1);
+    if (mode == 0) {
+      orc_prefix_canonical(p.data(), n, P.data());
+    } else {
+      P[0] = 0.0;
+      std::partial_sum(p.begin(), p.end(), P.begin() + 1);
+    }
+    info->p_total = P[n];
+    info->n_notfound = systematic_scan(ns, P.data(), n, u, S);
+  } else {
+    info->p_total = 0.0;
+    info->n_notfound = 0;
+  }
   // wrap-up (src/fri_utils.cpp:494-504)
   for (int64_t k = 0; k < K; k++) { out_idx[k] = sig[k]; out_val[k] = x[sig[k]]; }
   for (int64_t i = 0; i < ns; i++) {

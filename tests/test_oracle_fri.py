@@ -183,3 +183,20 @@ def test_planted_mistake_is_caught():
         wrong = np.searchsorted(P, uk, side="right") - 1
         wrong[uk >= P[-1]] = 0
         assert not np.array_equal(idx[K:], wrong)
+
+
+def test_m_equal_to_nonzeros_returns_d_alone():
+    """m = nnz(x): the D loop takes every nonzero, n' = 0, nothing is sampled
+    and no probability is used (the reference's p > 1 check passes on the NaN
+    p of R = 0 and it returns D, src/fri_utils.cpp:383-395,494-504).  One more
+    than the nonzeros leaves n' > 0 with R = 0, which is reported (quirk A.6)."""
+    rng = np.random.default_rng(3)
+    x = np.zeros(5000)
+    nz = rng.choice(5000, 800, replace=False)
+    x[nz] = rng.choice([1.0, -1.0, 0.5, 2.0 ** -30], 800)
+    for mode in (0, 1):
+        idx, val, info = lo.compress(x, 800, lo.FRI_SYSTEMATIC, 0.3, mode=mode)
+        assert info["status"] == 0 and info["n_D"] == 800 and info["n_sampled"] == 0
+        assert np.array_equal(np.sort(idx), np.sort(nz))
+        assert np.array_equal(val, x[idx])
+        assert lo.compress(x, 801, lo.FRI_SYSTEMATIC, 0.3, mode=mode)[2]["status"] == 1
