@@ -70,7 +70,7 @@ def run_mode(eris, cfg, mode, steps, warmup, seed):
     sec = cc.timer.ms()
     info = cc.sparse_info[0]
     cc.final_amplitudes = (t1, t2)
-    return {"state": (t1, t2), "it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "p50_ms": 1e3 * _pct(lat, 50),
+    return {"state": (t1, t2), "slots": cc._slots, "it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "p50_ms": 1e3 * _pct(lat, 50),
             "p99_ms": 1e3 * _pct(lat, 99), "sections_ms": {k: sum(v) / len(v) for k, v in sec.items() if v},
             "energies": energies, "n_D": info["n_D"], "n_sampled": info["n_sampled"]}
 
@@ -101,6 +101,169 @@ def precision_error(eris, cfg, mode, t1, t2, seed):
     return out
 
 
+# (W shape, output shape, number of output blocks the reference loops over)
+# for each contraction term, src/sparse_tensor.cpp:11-346
+def _term_geometry(o, v):
+    ovov, vo = (o, v, o, v), v * o
+    return {
+        "0101": ((o, o, o, o), (o, o, v, v), v * v), "2323": ((v, v, v, v), (o, o, v, v), o * o),
+        "1302": ((v, o, o, v), (o, o, v, v), vo), "1202": ((v, o, v, o), (o, o, v, v), vo),
+        "1303": ((v, o, o, v), (o, o, v, v), vo), "1203": ((v, o, v, o), (o, o, v, v), vo),
+        "1323": (ovov, (o, o, o, o), o * o), "0112": (ovov, (v, o, o, v), vo), "0313": (ovov, (v, o, o, v), vo),
+        "0113": (ovov, (v, o, o, v), vo), "0312": (ovov, (v, o, v, o), vo),
+    }
+
+
+def cpu_baseline_sample(x, idx, val, o, v, m, blocks_per_term=None, slice_len=1 << 23, log=None):
+    """CPU path of one FRI-CCSD iteration, timed on a bounded sample and
+    scaled to seconds per iteration.  The timed code is the CPU oracle's
+    restatement of the reference's native path (oracle/fricc_oracle.cpp):
+
+    * compression: the oracle's FAITHFUL mode (sequential 1-norm, D loop and
+      partial_sum, forward systematic scan) on the first ``slice_len`` entries
+      of T2 with m scaled to the slice, times N/slice_len, twice per iteration
+      (t2_sparse and t2_sparse_alt).  Its top-m selection is a partial sort;
+      the reference sorts all N (src/fri_utils.cpp:11-36), so this part is
+      optimistic for the CPU.
+    * the 11 sparse contractions: the reference's loop nests over the same AoS
+      entries (the iteration's real sampled T2), a contiguous range of output
+      blocks from the middle of each term's block range, scaled by the block
+      count (2323: by the entry count, its per-entry v^2 loop dominating).
+      W is a constant array of the reference's shape and C layout (the values
+      do not change the loop's work).
+    * the dense remainder: 11 O(o^2 v^3) contractions (fricc/friccsd.py:144-207
+      via PySCF), timed as 11 host DGEMMs of that size.
+
+    Returns (seconds per iteration, threads, description)."""
+    import numpy as np
+    import torch
+
+    from oracle import liboracle as lo
+
+    say = log or (lambda *a: None)
+    threads = lo.num_threads()
+    N = x.size
+    geo = _term_geometry(o, v)
+    tdims = (o, o, v, v)
+    bpt = blocks_per_term or 2 * threads
+    # compression
+    n_s = min(slice_len, N)
+    m_s = max(1, int(round(m * n_s / N)))
+    xs = np.ascontiguousarray(x[:n_s])
+    lo.compress(xs[: min(n_s, 1 << 16)], max(1, m_s // 128), lo.FRI_SYSTEMATIC, 0.5, mode=lo.FAITHFUL)
+    t0 = time.perf_counter()
+    info = lo.compress(xs, m_s, lo.FRI_SYSTEMATIC, 0.5, mode=lo.FAITHFUL)[2]
+    t_comp = (time.perf_counter() - t0) * N / n_s
+    say(f"compression sample {n_s} entries, m={m_s}: {t_comp:.2f} s per call (status {info['status']})")
+    # sparse contractions
+    idx4 = np.ascontiguousarray(lo.unpack(idx, tdims))
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    per_term = {}
+    for term, (wshape, oshape, nb) in geo.items():
+        W = torch.full(wshape, 1e-3, dtype=torch.float64).numpy()
+        out = np.zeros(oshape)
+        b = min(threads if (term == "2323" and blocks_per_term is None) else bpt, nb)  # 2323: one block per thread
+        lo_b = (nb - b) // 2
+        t0 = time.perf_counter()
+        lo.contract_ref(term, W, idx, val, tdims, out, lo_b, lo_b + b, idx4=idx4)
+        t = time.perf_counter() - t0
+        if term == "2323":
+            blk = idx4[:, 0] * o + idx4[:, 1]
+            e_r = int(np.count_nonzero((blk >= lo_b) & (blk < lo_b + b)))
+            scale = idx.size / max(e_r, 1)
+        else:
+            scale = nb / b
+        per_term[term] = t * scale
+        say(f"{term}: blocks [{lo_b}, {lo_b + b}) of {nb}: {t:.3f} s -> {per_term[term]:.1f} s")
+        del W, out
+    # dense remainder
+    A = np.random.default_rng(0).random((v, v))
+    B = np.random.default_rng(1).random((v, o * o * v))
+    A @ B
+    t0 = time.perf_counter()
+    A @ B
+    t_dense = 11 * (time.perf_counter() - t0)
+    total = 2 * t_comp + sum(per_term.values()) + t_dense
+    desc = (f"oracle FAITHFUL compression of {n_s} of {N} T2 entries (x2 per iteration, scaled by N); "
+            f"reference loop nests over the iteration's {idx.size} sampled entries, {bpt} output blocks per "
+            f"term ({threads} for 2323; scaled by block count, 2323 by entry count); dense remainder as 11 host DGEMMs "
+            f"{v}x{v}x{o * o * v}; {threads} threads")
+    return total, threads, desc
+
+
+COPY_BYTES = 1 << 30  # calibration copy for the traffic counters
+
+
+def traffic_child(args):
+    """Run under rocprofv3 --pmc by measure_traffic: one calibration copy of
+    COPY_BYTES, then the warm mixed-precision iteration whose ladder launch is
+    read."""
+    import ctypes
+
+    import torch
+
+    from fricc import _capi
+    from fricc.systems import CONFIGS, DeviceERIs, SyntheticSystem
+
+    torch.cuda.set_device(0)
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    _capi.check(_capi.lib().fricc_stream_copy(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                              COPY_BYTES, _capi.stream_ptr()))
+    torch.cuda.synchronize()
+    del a, b
+    cfg = CONFIGS[args.config]
+    sysm = SyntheticSystem.from_config(args.config)
+    eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
+    run_mode(eris, cfg, "mixed", 1, 2, args.seed)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(args):
+    """HBM bytes of one ladder launch from two rocprofv3 --pmc runs of
+    traffic_child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
+    by what the same counter reads for the COPY_BYTES copy (which moves
+    exactly COPY_BYTES each way).  Returns (dict or None, note)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    base = tempfile.mkdtemp(prefix="fricc_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(base, counter)
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
+               os.path.abspath(__file__), "--traffic-child", "--config", args.config, "--seed", str(args.seed)]
+        try:
+            subprocess.run(cmd, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env,
+                           cwd="/tmp", check=False)
+        except subprocess.TimeoutExpired:
+            return None, f"{counter} pass timed out"
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None, f"{counter} pass wrote no counters"
+        rows = list(csv.DictReader(open(files[0])))
+        copy = [float(r["Counter_Value"]) for r in rows if "k_stream_copy" in r["Kernel_Name"]]
+        gem = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]), float(r["Counter_Value"]))
+               for r in rows if "k_ggemm_bf16" in r["Kernel_Name"]]
+        if not copy or not gem:
+            return None, f"{counter} pass: copy or ladder dispatch missing"
+        got[counter] = (copy[0], max(gem)[1])  # the ladder is the longest gathered launch
+    shutil.rmtree(base, ignore_errors=True)
+    f_fetch = COPY_BYTES / (got["FETCH_SIZE"][0] * 1024.0)
+    f_write = COPY_BYTES / (got["WRITE_SIZE"][0] * 1024.0)
+    fetch = got["FETCH_SIZE"][1] * 1024.0 * f_fetch
+    write = got["WRITE_SIZE"][1] * 1024.0 * f_write
+    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "fetch_scale": f_fetch,
+            "write_scale": f_write}, "ok"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,7 +273,12 @@ def main():
     ap.add_argument("--modes", default="mixed,fp32,fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_child:
+        traffic_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path runs on one MI355X)",
               file=sys.stderr)
@@ -132,9 +300,10 @@ def main():
     res = {}
     for mode in args.modes.split(","):
         res[mode] = run_mode(eris, cfg, mode, args.steps, args.warmup, args.seed)
-        print(f"# {mode}: {json.dumps({k: v for k, v in res[mode].items() if k not in ('energies', 'state')})}",
+        print(f"# {mode}: {json.dumps({k: v for k, v in res[mode].items() if k not in ('energies', 'state', 'slots')})}",
               file=sys.stderr, flush=True)
     errs = {}
+    t1s = t2s = None
     if "fp64" in res:
         t1s, t2s = res[list(res)[0]]["state"]
         for mode in res:
@@ -143,10 +312,46 @@ def main():
                 print(f"# error {mode}: {errs[mode]}", file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in res else list(res)[0]
     r = res[head]
-    ladder_flops = 2.0 * m * v * v
+    N = o * o * v * v
     lad_ms = r["sections_ms"].get("ladder")
     prec = {"mixed": "bf16", "fp32": "fp32", "fp64": "fp64"}[head]
-    achieved = ladder_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None
+    # ladder 2323, one launch: 2 m v^2 flops; v^4 W elements + the FP64 output
+    # read and written (bf16 W: HBM-bound streaming W; FP32/FP64: MFMA-bound)
+    lad_flops = 2.0 * m * v * v
+    w_bytes = {"bf16": 2, "fp32": 4, "fp64": 8}[prec]
+    lad_bytes = float(v) ** 4 * w_bytes + 2.0 * N * 8
+    # achievable HBM bandwidth: the streaming copy kernel, HIP events on its stream
+    import ctypes
+
+    from fricc import _capi
+
+    ca = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    cb = torch.empty_like(ca)
+    cp = lambda: _capi.check(_capi.lib().fricc_stream_copy(ctypes.c_void_p(ca.data_ptr()),  # noqa: E731
+                                                           ctypes.c_void_p(cb.data_ptr()), COPY_BYTES,
+                                                           _capi.stream_ptr()))
+    cp()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        cp()
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2.0 * COPY_BYTES * 10 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    del ca, cb
+    if prec == "bf16":
+        roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm_bf16)", "bound": "hbm",
+                "achieved": lad_bytes / (lad_ms * 1e-3) / 1e9 if lad_ms else None, "peak": HBM_GBS, "unit": "GB/s"}
+    else:
+        roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm)", "bound": "mfma",
+                "achieved": lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None, "peak": PEAK_TFLOPS[prec],
+                "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
+    roof["traffic"] = None
+    roof.update({"launch_ms": lad_ms, "algorithmic_bytes_per_launch": lad_bytes,
+                 "algorithmic_flops_per_launch": lad_flops,
+                 "algorithmic_tflops": lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None,
+                 "achievable_hbm_gbs_copy": copy_gbs})
     out = {
         "metric": "FRI-CCSD iter/s at o=32,v=256,m=4e6 (fp32/bf16 vs fp64); p50/p99 solve latency",
         "value": r["it_per_s"],
@@ -165,10 +370,7 @@ def main():
                    "nocc": o, "nvir": v, "m_keep": m, "precision_mode": head},
         "iteration_latency_ms": {"p50": r["p50_ms"], "p99": r["p99_ms"]},
         "sections_ms": r["sections_ms"],
-        "roofline": {"kernel": "gathered ladder GEMM (2323)", "bound": "mfma",
-                     "achieved": achieved, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-                     "frac": (achieved / PEAK_TFLOPS[prec]) if achieved else None, "traffic": None,
-                     "algorithmic_flops_per_launch": ladder_flops},
+        "roofline": roof,
         "setup_s": t_setup,
         "n_D": r["n_D"],
         "n_sampled": r["n_sampled"],
@@ -180,6 +382,33 @@ def main():
         out["speedup_vs_fp64"] = r["it_per_s"] / res["fp64"]["it_per_s"]
     out["precision_error_vs_fp64"] = errs
     out["cpu_baseline"] = None
+    # host copies for the CPU leg, then free the GPU for the profiling passes
+    t2_host = res[head]["state"][1].cpu().numpy().ravel()
+    ts = res[head]["slots"][0]
+    ts_idx, ts_val = ts.indices(), ts.values()
+    for k in list(res):
+        res[k].pop("state", None)
+        res[k].pop("slots", None)
+    del eris, ts, sysm
+    t1s = t2s = None
+    import gc
+
+    gc.collect()
+    torch.cuda.empty_cache()
+    if not args.no_traffic and head == "mixed":
+        tr, note = measure_traffic(args)
+        print(f"# traffic: {tr} ({note})", file=sys.stderr, flush=True)
+        if tr is not None:
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_detail"] = tr
+        else:
+            roof["traffic_note"] = note
+    if not args.no_cpu_baseline:
+        t_cpu, thr, desc = cpu_baseline_sample(
+            t2_host, ts_idx, ts_val, o, v, m, log=lambda s: print(f"# cpu: {s}", file=sys.stderr, flush=True))
+        out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "iter/s", "cores": thr, "kind": "port",
+                               "s_per_iter": t_cpu, "sample": desc}
+        out["speedup_vs_cpu"] = r["it_per_s"] * t_cpu
     print(json.dumps(out), flush=True)
 
 

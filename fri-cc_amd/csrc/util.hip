@@ -113,4 +113,32 @@ int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, 
   return OK;
 }
 
+// Plain streaming copy, 16 bytes per lane per step: the reference point for
+// achievable HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE
+// counters of the other kernels.
+__global__ void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+int stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+  FRICC_CHECK(bytes >= 0 && bytes % 16 == 0, "stream_copy: bytes must be a multiple of 16");
+  FRICC_CHECK(((uintptr_t)src | (uintptr_t)dst) % 16 == 0, "stream_copy: pointers must be 16-byte aligned");
+  if (bytes == 0) return OK;
+  const int64_t n16 = bytes / 16;
+  const int64_t grid = std::min<int64_t>(ceil_div(n16, 256), 256 * 32);
+  k_stream_copy<<<grid, 256, 0, s>>>((const uint4*)src, (uint4*)dst, n16);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
 }  // namespace fricc
+
+extern "C" {
+// Device-to-device copy of `bytes` (a multiple of 16, 16-byte aligned
+// pointers) by a plain streaming kernel.  Bandwidth reference for the
+// roofline; no counterpart in the reference.
+int fricc_stream_copy(const void* src, void* dst, int64_t bytes, void* stream) {
+  return fricc::stream_copy(src, dst, bytes, (hipStream_t)stream);
+}
+}

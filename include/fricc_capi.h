@@ -171,6 +171,12 @@ int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void
  * multiple of 128.  Tuning knob; no counterpart in the reference. */
 int fricc_set_bf16_view_width(int width);
 
+/* Device-to-device copy of `bytes` (a multiple of 16; 16-byte aligned device
+ * pointers) by a plain streaming kernel: the reference point for achievable
+ * HBM bandwidth and for scaling hardware traffic counters of the other
+ * kernels.  No counterpart in the reference. */
+int fricc_stream_copy(const void* src, void* dst, int64_t bytes, void* stream);
+
 /* Block ordering of the gathered contraction GEMM: 0 = x tiles fastest,
  * 1 = column tiles fastest (default).  Tuning knob; no counterpart in the
  * reference. */

@@ -106,11 +106,13 @@ def unpack(idx, dims):
     return out
 
 
-def contract_ref(term, W, idx, val, tdims, out, block_lo=0, block_hi=-1):
+def contract_ref(term, W, idx, val, tdims, out, block_lo=0, block_hi=-1, idx4=None):
     """The reference's loop-nest kernels (src/sparse_tensor.cpp:11-346): rescans
-    every sparse entry per output block.  ``out`` is accumulated in place."""
+    every sparse entry per output block.  ``out`` is accumulated in place.
+    ``idx4`` (optional): the entries already unpacked to [nnz][4] int64, the
+    reference's AoS index layout (include/sparse_tensor.hpp:15-18)."""
     assert W.dtype == np.float64 and out.dtype == np.float64
-    idx4 = np.ascontiguousarray(unpack(idx, tdims))
+    idx4 = np.ascontiguousarray(unpack(idx, tdims) if idx4 is None else idx4, dtype=np.int64)
     val = np.ascontiguousarray(val, dtype=np.float64)
     wshape = np.array(W.shape, dtype=np.int64)
     ws = np.array([s // 8 for s in W.strides], dtype=np.int64)
