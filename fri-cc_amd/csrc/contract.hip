@@ -33,6 +33,26 @@ __device__ inline void out_add(const OutDev& om, int64_t off, double alpha, doub
     ((double*)om.out)[off] += alpha * v;
 }
 
+__device__ inline int64_t out_xoff(const OutDev& om, int64_t x) {
+  const uint32_t xx = (uint32_t)x;
+  return (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl;
+}
+
+// out[off[n]] += alpha * v[n] for the N outputs of one lane and column block
+// (ok[n] false: skipped).  All N values are read before any is written, so the
+// round trips overlap; one read-modify-write at a time would serialise them,
+// since the compiler cannot tell that the offsets differ.
+template <typename TO, int N>
+__device__ inline void out_add_batch(TO* __restrict__ out, const int64_t (&off)[N], const bool (&ok)[N], double alpha,
+                                     const double (&v)[N]) {
+  TO cur[N];
+#pragma unroll
+  for (int n = 0; n < N; n++) cur[n] = ok[n] ? out[off[n]] : (TO)0;
+#pragma unroll
+  for (int n = 0; n < N; n++)
+    if (ok[n]) out[off[n]] = cur[n] + (TO)(alpha * v[n]);
+}
+
 template <typename T>
 struct Acc;
 template <>
@@ -169,18 +189,22 @@ __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t 
     if (y >= Y) continue;
     const uint32_t yy = (uint32_t)y;
     const int64_t oy = (int64_t)(yy / om.Yl) * om.oyh + (int64_t)(yy % om.Yl) * om.oyl;
+    int64_t off[16];
+    bool ok[16];
+    double vv[16];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < 4; s++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int64_t x = xw + 16 * s + Acc<T>::row(lk, i);
-        if (x < X) {
-          const uint32_t xx = (uint32_t)x;
-          out_add(om, (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + oy, alpha,
-                  (double)acc[s][q][i]);
-        }
+        ok[4 * s + i] = x < X;
+        off[4 * s + i] = out_xoff(om, x < X ? x : 0) + oy;
+        vv[4 * s + i] = (double)acc[s][q][i];
       }
-    }
+    if (om.of32)
+      out_add_batch<float, 16>((float*)om.out, off, ok, alpha, vv);
+    else
+      out_add_batch<double, 16>((double*)om.out, off, ok, alpha, vv);
   }
 }
 
@@ -454,24 +478,34 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
   for (int q = 0; q < NQ; q++) {
     const int64_t y = y0 + wy * (BYB / 2) + 16 * q + lr;
     if (y >= a.Y) continue;
+    if (a.nsplit > 1) {
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int64_t x = x0 + wx * 128 + 16 * s + 4 * lk + i;
+          if (x < a.X) a.partial[((int64_t)sp * a.X + x) * a.Y + y] = acc[s][q][i];
+        }
+      continue;
+    }
     const uint32_t yy = (uint32_t)y;
     const int64_t oy = (int64_t)(yy / a.om.Yl) * a.om.oyh + (int64_t)(yy % a.om.Yl) * a.om.oyl;
+    int64_t off[32];
+    bool ok[32];
+    double vv[32];
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
+    for (int s = 0; s < 8; s++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int64_t x = x0 + wx * 128 + 16 * s + 4 * lk + i;
-        if (x < a.X) {
-          if (a.nsplit > 1) {
-            a.partial[((int64_t)sp * a.X + x) * a.Y + y] = acc[s][q][i];
-          } else {
-            const uint32_t xx = (uint32_t)x;
-            out_add(a.om, (int64_t)(xx / a.om.Xl) * a.om.oxh + (int64_t)(xx % a.om.Xl) * a.om.oxl + oy, a.alpha,
-                    (double)acc[s][q][i]);
-          }
-        }
+        ok[4 * s + i] = x < a.X;
+        off[4 * s + i] = out_xoff(a.om, x < a.X ? x : 0) + oy;
+        vv[4 * s + i] = (double)acc[s][q][i];
       }
-    }
+    if (a.om.of32)
+      out_add_batch<float, 32>((float*)a.om.out, off, ok, a.alpha, vv);
+    else
+      out_add_batch<double, 32>((double*)a.om.out, off, ok, a.alpha, vv);
   }
 }
 
