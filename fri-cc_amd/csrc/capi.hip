@@ -203,6 +203,7 @@ int fricc_sparse_destroy(fricc_sparse* sp) {
 
 int fricc_sparse_compress(fricc_sparse* sp, const double* x, int64_t m, int compression, double u, void* stream) {
   FRICC_CHECK(sp != nullptr, "sparse_compress: null tensor");
+  sp->pending = false;
   const int64_t n = sp->dims[0] * sp->dims[1] * sp->dims[2] * sp->dims[3];
   FRICC_TRY(sparse_reserve(sp, m));
   sp->gen++;
@@ -210,6 +211,33 @@ int fricc_sparse_compress(fricc_sparse* sp, const double* x, int64_t m, int comp
   int st = compress(x, n, m, compression, u, sp->idx, sp->val, &sp->info, (hipStream_t)stream);
   if (st == OK) sp->nnz = m;
   return st;
+}
+
+int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, int compression, const double* u_dev,
+                                void* stream) {
+  FRICC_CHECK(sp != nullptr && u_dev != nullptr, "sparse_compress_async: null argument");
+  const int64_t n = sp->dims[0] * sp->dims[1] * sp->dims[2] * sp->dims[3];
+  FRICC_TRY(sparse_reserve(sp, m));
+  if (!sp->dsc) FRICC_HIP(hipMalloc(&sp->dsc, comp_scalars_size()));
+  sp->gen++;
+  sp->nnz = 0;
+  void* sc = nullptr;
+  FRICC_TRY(compress_launch(x, n, m, compression, 0.0, u_dev, sp->idx, sp->val, sp->dsc, (hipStream_t)stream, &sc));
+  sp->nnz = m;
+  sp->pending = true;
+  return OK;
+}
+
+// Reads the scalars the last asynchronous compression left on the device
+// (every replay of a recorded graph rewrites them, so they are read each time).
+int fricc_sparse_status(fricc_sparse* sp, void* stream) {
+  FRICC_CHECK(sp != nullptr, "sparse_status: null tensor");
+  if (!sp->pending) return OK;
+  std::vector<char> h(comp_scalars_size());
+  hipStream_t s = (hipStream_t)stream;
+  FRICC_HIP(hipMemcpyAsync(h.data(), sp->dsc, h.size(), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  return comp_scalars_status(h.data(), &sp->info);
 }
 
 int fricc_sparse_set(fricc_sparse* sp, const int64_t* idx, const double* val, int64_t nnz, void* stream) {
@@ -229,6 +257,7 @@ int fricc_sparse_set(fricc_sparse* sp, const int64_t* idx, const double* val, in
     FRICC_HIP(hipStreamSynchronize(s));
   }
   sp->nnz = nnz;
+  sp->pending = false;
   memset(&sp->info, 0, sizeof(sp->info));
   sp->info.n_D = nnz;
   return OK;

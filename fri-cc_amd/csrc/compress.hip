@@ -17,7 +17,8 @@
 //   output  = [sigma_0..sigma_{|D|-1}, S_0..S_{n'-1}],
 //             values [x_D, x_S / p_S]
 //
-// Pipeline (one stream, two host syncs):
+// Pipeline (one stream, no host round trip; fricc_compress reads the scalars
+// back once at the end):
 //   A  histogram of the top 12 bits of |x| (+ non-finite detection, max exponent)
 //   sel choose the top-12-bit bucket b1 holding the m-th largest
 //   B  exact FX 1-norm + per-tile count of candidates (bucket >= b1)
@@ -45,6 +46,9 @@ struct CompScalars {
   uint64_t thr_key, thr_idx;
   uint64_t k_notfound;
   uint64_t n_notfound;
+  double u;        // the uniform of this call
+  uint32_t ubad;   // u outside [0, 1]
+  uint32_t pad;
 };
 
 // ---------------------------------------------------------------- pass A
@@ -178,75 +182,110 @@ __global__ void k_compact(const double* __restrict__ x, int64_t n, const CompSca
 }
 
 // ---------------------------------------------------------------- radix sort
-// Global histograms of all 8 byte digits (to skip constant digits).
-__global__ void k_sort_hist_all(const uint64_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[8][256];
-  for (int i = threadIdx.x; i < 8 * 256; i += NT) (&h[0][0])[i] = 0;
-  __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * NT;
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
-    uint64_t k = keys[i];
-#pragma unroll
-    for (int d = 0; d < 8; d++) atomicAdd(&h[d][(k >> (8 * d)) & 255], 1u);
+// Stable LSD radix sort of the n_cand candidates (count read on the device:
+// no host round trip).  A fixed grid strides over the live tiles; per digit
+// the per-tile counts live in blkhist[digit][tile] (row length ntmax).
+__device__ inline int64_t live_tiles(const CompScalars* sc) { return ((int64_t)sc->n_cand + TILE - 1) / TILE; }
+
+__global__ void k_sort_count(const uint64_t* __restrict__ keys, const CompScalars* __restrict__ sc, int shift,
+                             uint32_t* __restrict__ blkhist, int64_t ntmax) {
+  __shared__ uint32_t h[256];
+  const int64_t n = (int64_t)sc->n_cand, nta = live_tiles(sc);
+  for (int64_t t = blockIdx.x; t < nta; t += gridDim.x) {
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = t * TILE;
+    for (int r = 0; r < ITEMS; r++) {
+      int64_t i = base + (int64_t)r * NT + threadIdx.x;
+      if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    blkhist[(int64_t)threadIdx.x * ntmax + t] = h[threadIdx.x];
+    __syncthreads();
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 8 * 256; i += NT)
-    if ((&h[0][0])[i]) atomicAdd(&hist[i], (&h[0][0])[i]);
 }
 
-__global__ void k_sort_count(const uint64_t* __restrict__ keys, int64_t n, int shift,
-                             uint32_t* __restrict__ blkhist, int64_t nblk) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * TILE;
-  for (int r = 0; r < ITEMS; r++) {
-    int64_t i = base + (int64_t)r * NT + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+// Block d: total of digit d over the live tiles.
+__global__ void k_digit_total(const uint32_t* __restrict__ blkhist, int64_t ntmax, const CompScalars* __restrict__ sc,
+                              uint32_t* __restrict__ dtot) {
+  __shared__ uint32_t sm[NT / WAVE + 1];
+  const int64_t nta = live_tiles(sc);
+  const uint32_t* row = blkhist + (int64_t)blockIdx.x * ntmax;
+  uint32_t s = 0;
+  for (int64_t t = threadIdx.x; t < nta; t += NT) s += row[t];
+  uint32_t tot;
+  block_excl_scan_u32<NT>(s, &tot, sm);
+  if (threadIdx.x == 0) dtot[blockIdx.x] = tot;
+}
+
+// Block d: exclusive offsets of digit d per live tile, after all smaller digits.
+__global__ void k_digit_offsets(uint32_t* __restrict__ blkhist, int64_t ntmax, const CompScalars* __restrict__ sc,
+                                const uint32_t* __restrict__ dtot) {
+  __shared__ uint32_t sm[NT / WAVE + 1];
+  __shared__ uint32_t s_base;
+  const int64_t nta = live_tiles(sc);
+  if (threadIdx.x == 0) {
+    uint32_t b = 0;
+    for (uint32_t d = 0; d < blockIdx.x; d++) b += dtot[d];
+    s_base = b;
   }
   __syncthreads();
-  blkhist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+  uint32_t* row = blkhist + (int64_t)blockIdx.x * ntmax;
+  uint32_t run = s_base;
+  for (int64_t c0 = 0; c0 < nta; c0 += NT) {
+    const int64_t t = c0 + threadIdx.x;
+    const uint32_t v = t < nta ? row[t] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan_u32<NT>(v, &tot, sm);
+    if (t < nta) row[t] = run + ex;
+    run += tot;
+    __syncthreads();
+  }
 }
 
 // Stable scatter: tile order = index order (rounds of NT, wave order, lane order).
 __global__ void k_sort_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
-                               int shift, const uint32_t* __restrict__ blkoff, int64_t nblk) {
+                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                               const CompScalars* __restrict__ sc, int shift, const uint32_t* __restrict__ blkoff,
+                               int64_t ntmax) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[NT / WAVE][256];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  run[threadIdx.x] = blkoff[(int64_t)threadIdx.x * nblk + blockIdx.x];
-  const int64_t base = (int64_t)blockIdx.x * TILE;
+  const int64_t n = (int64_t)sc->n_cand, nta = live_tiles(sc);
   const uint64_t lt = (1ull << lane) - 1ull;
-  for (int r = 0; r < ITEMS; r++) {
-    for (int q = threadIdx.x; q < (NT / WAVE) * 256; q += NT) (&wcnt[0][0])[q] = 0;
-    __syncthreads();
-    int64_t i = base + (int64_t)r * NT + threadIdx.x;
-    bool valid = i < n;
-    uint64_t k = valid ? kin[i] : 0;
-    uint32_t v = valid ? vin[i] : 0;
-    uint32_t d = (uint32_t)((k >> shift) & 255);
-    uint64_t peers = __ballot(valid);
+  for (int64_t t = blockIdx.x; t < nta; t += gridDim.x) {
+    run[threadIdx.x] = blkoff[(int64_t)threadIdx.x * ntmax + t];
+    const int64_t base = t * TILE;
+    for (int r = 0; r < ITEMS; r++) {
+      for (int q = threadIdx.x; q < (NT / WAVE) * 256; q += NT) (&wcnt[0][0])[q] = 0;
+      __syncthreads();
+      int64_t i = base + (int64_t)r * NT + threadIdx.x;
+      bool valid = i < n;
+      uint64_t k = valid ? kin[i] : 0;
+      uint32_t v = valid ? vin[i] : 0;
+      uint32_t d = (uint32_t)((k >> shift) & 255);
+      uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
-      bool bit = (d >> b) & 1;
-      uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+      for (int b = 0; b < 8; b++) {
+        bool bit = (d >> b) & 1;
+        uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      uint32_t rank = (uint32_t)__popcll(peers & lt);
+      if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (valid) {
+        uint32_t off = run[d] + rank;
+        for (int q = 0; q < w; q++) off += wcnt[q][d];
+        kout[off] = k;
+        vout[off] = v;
+      }
+      __syncthreads();
+      uint32_t add = 0;
+      for (int q = 0; q < NT / WAVE; q++) add += wcnt[q][threadIdx.x];
+      run[threadIdx.x] += add;
+      __syncthreads();
     }
-    uint32_t rank = (uint32_t)__popcll(peers & lt);
-    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t off = run[d] + rank;
-      for (int q = 0; q < w; q++) off += wcnt[q][d];
-      kout[off] = k;
-      vout[off] = v;
-    }
-    __syncthreads();
-    uint32_t add = 0;
-    for (int q = 0; q < NT / WAVE; q++) add += wcnt[q][threadIdx.x];
-    run[threadIdx.x] += add;
-    __syncthreads();
   }
 }
 
@@ -429,12 +468,13 @@ __device__ inline int64_t first_k_at_least(double P, double u) {
 // prefix is monotone), so only those threads recompute their elements.
 __global__ void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
                             const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
-                            const u128* __restrict__ thsum, double u, uint32_t* __restrict__ out_idx,
+                            const u128* __restrict__ thsum, uint32_t* __restrict__ out_idx,
                             double* __restrict__ out_val) {
   __shared__ u128 sm[NT / WAVE + 1];
   __shared__ double lds[LTILE];
   const CompScalars c = *sc;
   if (c.n_sampled == 0) return;
+  const double u = c.u;
   const double nsd = (double)c.n_sampled;
   const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
   const int64_t tb = (int64_t)blockIdx.x * TILE;
@@ -477,7 +517,8 @@ __global__ void k_samp_emit(const double* __restrict__ x, const double* __restri
 }
 
 // Single thread: total interval end and the first not-found position.
-__global__ void k_samp_final(CompScalars* sc, const u128* __restrict__ total, double u) {
+__global__ void k_samp_final(CompScalars* sc, const u128* __restrict__ total) {
+  const double u = sc->u;
   if (sc->n_sampled == 0) {
     sc->Ptot = 0.0;
     sc->k_notfound = 0;
@@ -515,126 +556,150 @@ __global__ void k_emit_sorted(const double* __restrict__ x, const uint32_t* __re
   out_val[k] = x[j];
 }
 
-__global__ void k_init_scalars(CompScalars* sc, int64_t m) {
-  CompScalars z = {};
-  z.K = (unsigned long long)m;
-  *sc = z;
+// Scalars of a call (u from u_dev when given, else u_val) and, when hist is
+// given, the zeroed top-bit and digit histograms (a kernel rather than a
+// memset, so the call can be recorded in a graph).
+__global__ void k_init_scalars(CompScalars* sc, int64_t m, int64_t n_sampled, double u_val,
+                               const double* __restrict__ u_dev, uint32_t* __restrict__ hist, int nhist) {
+  if (threadIdx.x == 0) {
+    CompScalars z = {};
+    z.K = (unsigned long long)m;
+    z.n_sampled = (uint64_t)n_sampled;
+    const double u = u_dev ? *u_dev : u_val;
+    z.ubad = (u >= 0.0 && u <= 1.0) ? 0u : 1u;
+    z.u = z.ubad ? 0.0 : u;  // an out-of-range u is reported, never used
+    *sc = z;
+  }
+  if (hist)
+    for (int i = threadIdx.x; i < nhist; i += blockDim.x) hist[i] = 0u;
+}
+
+// Copy of the call's scalars for a later host read (compression statistics,
+// error flags) without a round trip now.
+__global__ void k_export_scalars(const CompScalars* __restrict__ sc, CompScalars* __restrict__ out) {
+  if (threadIdx.x == 0) *out = *sc;
 }
 
 // ---------------------------------------------------------------- driver
 enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
        WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11 };
 
-int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
-             double* out_val, CompressInfo* info, hipStream_t s) {
-  memset(info, 0, sizeof(*info));
-  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "compress: vector length must be in [1, 2^31)");
-  FRICC_CHECK(m > 0 && m <= n, "compress: need 0 < m <= n (fricc/friccsd.py:68-72)");
-  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC, "compress: unknown compression method");
-  FRICC_CHECK(u >= 0.0 && u <= 1.0, "compress: uniform must lie in [0, 1]");
-  void *p_sc, *p_hist, *p_tcnt, *p_k0, *p_v0, *p_k1, *p_v1, *p_sh, *p_tot;
-  const int64_t nt = ceil_div(n, TILE);
-  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
-  FRICC_TRY(ws_get(WS_HIST, 4096 * 4 + 8 * 256 * 4, &p_hist));
-  FRICC_TRY(ws_get(WS_TCNT, (size_t)(nt + 1) * 4, &p_tcnt));
-  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
-  CompScalars* sc = (CompScalars*)p_sc;
-  uint32_t* hist = (uint32_t*)p_hist;
-  uint32_t* hist8 = hist + 4096;
-  uint32_t* tcnt = (uint32_t*)p_tcnt;
+size_t comp_scalars_size() { return sizeof(CompScalars); }
 
-  k_init_scalars<<<1, 1, 0, s>>>(sc, m);
-  FRICC_HIP(hipMemsetAsync(hist, 0, 4096 * 4 + 8 * 256 * 4, s));
-  const int gA = (int)std::min<int64_t>(2048, ceil_div(n, NT * 8));
-  k_hist_top<<<gA, NT, 0, s>>>(x, n, hist);
-  k_select_top<<<1, NT, 0, s>>>(hist, m, sc);
-  k_norm_count<<<nt, NT, 0, s>>>(x, n, sc, tcnt);
-  FRICC_HIP(hipGetLastError());
-  CompScalars h;
-  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
-  FRICC_HIP(hipStreamSynchronize(s));
+// Host view of a call's scalars: statistics into info, and the call's status.
+int comp_scalars_status(const void* host_sc, CompressInfo* info) {
+  const CompScalars& h = *(const CompScalars*)host_sc;
+  if (info) {
+    memset(info, 0, sizeof(*info));
+    info->n_D = (int64_t)h.K;
+    info->n_sampled = (int64_t)h.n_sampled;
+    info->n_notfound = (int64_t)h.n_notfound;
+    info->norm0 = h.R0;
+    info->remaining_norm = h.RK;
+    info->p_total = h.Ptot;
+    info->n_candidates = (int64_t)h.n_cand;
+    info->nonfinite = (int32_t)h.nonfinite;
+  }
+  if (h.ubad) {
+    set_error("compress: uniform must lie in [0, 1]");
+    return EINVAL_;
+  }
   if (h.nonfinite) {
     set_error("compress: non-finite value in the input vector");
-    info->nonfinite = 1;
     return ENONFINITE;
   }
-  const int64_t nc = (int64_t)h.n_cand;
-  FRICC_TRY(scan_u32(tcnt, tcnt, nt, nullptr, s));
-  FRICC_TRY(ws_get(WS_K0, (size_t)nc * 8, &p_k0));
-  FRICC_TRY(ws_get(WS_V0, (size_t)nc * 4, &p_v0));
-  FRICC_TRY(ws_get(WS_K1, (size_t)nc * 8, &p_k1));
-  FRICC_TRY(ws_get(WS_V1, (size_t)nc * 4, &p_v1));
-  uint64_t* kA = (uint64_t*)p_k0;
-  uint32_t* vA = (uint32_t*)p_v0;
-  uint64_t* kB = (uint64_t*)p_k1;
-  uint32_t* vB = (uint32_t*)p_v1;
-  k_compact<<<nt, NT, 0, s>>>(x, n, sc, tcnt, kA, vA);
-
-  // --- sort candidates (stable LSD, skipping constant digits)
-  const int gH = (int)std::min<int64_t>(1024, ceil_div(nc, NT * 8));
-  k_sort_hist_all<<<gH, NT, 0, s>>>(kA, nc, hist8);
-  uint32_t h8[8 * 256];
-  FRICC_HIP(hipMemcpyAsync(h8, hist8, sizeof(h8), hipMemcpyDeviceToHost, s));
-  FRICC_HIP(hipStreamSynchronize(s));
-  const int64_t nblk = ceil_div(nc, TILE);
-  FRICC_TRY(ws_get(WS_SH, (size_t)nblk * 256 * 4, &p_sh));
-  uint32_t* bh = (uint32_t*)p_sh;
-  for (int d = 0; d < 8; d++) {
-    bool trivial = false;
-    for (int b = 0; b < 256; b++)
-      if ((int64_t)h8[d * 256 + b] == nc) trivial = true;
-    if (trivial) continue;
-    k_sort_count<<<nblk, NT, 0, s>>>(kA, nc, 8 * d, bh, nblk);
-    FRICC_TRY(scan_u32(bh, bh, nblk * 256, nullptr, s));
-    k_sort_scatter<<<nblk, NT, 0, s>>>(kA, vA, kB, vB, nc, 8 * d, bh, nblk);
-    std::swap(kA, kB);
-    std::swap(vA, vB);
-  }
-  FRICC_HIP(hipGetLastError());
-
-  if (method == COMP_LARGEST) {
-    k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, m, out_idx, out_val);
-    FRICC_HIP(hipGetLastError());
-    info->n_D = m;
-    info->n_sampled = 0;
-    return OK;
-  }
-
-  // --- D test over the first m sorted candidates
-  const int64_t ntm = ceil_div(m, TILE);
-  void* p_ts;
-  FRICC_TRY(ws_get(WS_TS, (size_t)(std::max(ntm, nt) + 1) * sizeof(u128), &p_ts));
-  u128* ts = (u128*)p_ts;
-  k_dfx_tilesums<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
-  k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, ntm, nullptr);
-  k_dtest<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
-  k_dfinal<<<1, NT, 0, s>>>(kA, vA, m, sc, ts);
-  k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, -1, out_idx, out_val);
-  // --- systematic sampling of the remainder
-  void* p_ths;
-  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
-  k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (u128*)p_ths);
-  k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-  k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (const u128*)p_ths, u, out_idx, out_val);
-  k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot, u);
-  FRICC_HIP(hipGetLastError());
-  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
-  FRICC_HIP(hipStreamSynchronize(s));
-  if (h.n_notfound > 0)
-    k_samp_notfound<<<ceil_div((int64_t)h.n_notfound, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
-  FRICC_HIP(hipGetLastError());
-  info->n_D = (int64_t)h.K;
-  info->n_sampled = (int64_t)h.n_sampled;
-  info->n_notfound = (int64_t)h.n_notfound;
-  info->norm0 = h.R0;
-  info->remaining_norm = h.RK;
-  info->p_total = h.Ptot;
-  info->n_candidates = nc;
   if (h.pflag) {
     set_error("compress: a sampling probability exceeds 1 (reference exits, src/fri_utils.cpp:383-395)");
     return EPROB;
   }
   return OK;
+}
+
+// The whole compression on stream s with no host round trip: every size the
+// host does not know (candidate count, |D|, samples) stays on the device and
+// the kernels read it there; buffers are sized for the worst case.  The
+// call's scalars end in the workspace (*sc_out) and, when export_sc is given,
+// in that device buffer.  u comes from u_dev when given, else u_val.
+int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
+                    uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s, void** sc_out) {
+  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "compress: vector length must be in [1, 2^31)");
+  FRICC_CHECK(m > 0 && m <= n, "compress: need 0 < m <= n (fricc/friccsd.py:68-72)");
+  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC, "compress: unknown compression method");
+  FRICC_CHECK(u_dev || (u_val >= 0.0 && u_val <= 1.0), "compress: uniform must lie in [0, 1]");
+  void *p_sc, *p_hist, *p_tcnt, *p_k0, *p_v0, *p_k1, *p_v1, *p_sh, *p_tot, *p_ts, *p_ths;
+  const int64_t nt = ceil_div(n, TILE);
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_HIST, (4096 + 256) * 4, &p_hist));
+  FRICC_TRY(ws_get(WS_TCNT, (size_t)(nt + 1) * 4, &p_tcnt));
+  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  FRICC_TRY(ws_get(WS_K0, (size_t)n * 8, &p_k0));
+  FRICC_TRY(ws_get(WS_V0, (size_t)n * 4, &p_v0));
+  FRICC_TRY(ws_get(WS_K1, (size_t)n * 8, &p_k1));
+  FRICC_TRY(ws_get(WS_V1, (size_t)n * 4, &p_v1));
+  FRICC_TRY(ws_get(WS_SH, (size_t)nt * 256 * 4, &p_sh));
+  FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  CompScalars* sc = (CompScalars*)p_sc;
+  uint32_t* hist = (uint32_t*)p_hist;
+  uint32_t* dtot = hist + 4096;
+  uint32_t* tcnt = (uint32_t*)p_tcnt;
+  uint64_t* kA = (uint64_t*)p_k0;
+  uint32_t* vA = (uint32_t*)p_v0;
+  uint64_t* kB = (uint64_t*)p_k1;
+  uint32_t* vB = (uint32_t*)p_v1;
+  uint32_t* bh = (uint32_t*)p_sh;
+  u128* ts = (u128*)p_ts;
+  *sc_out = sc;
+
+  k_init_scalars<<<1, NT, 0, s>>>(sc, m, 0, u_val, u_dev, hist, 4096);
+  const int gA = (int)std::min<int64_t>(2048, ceil_div(n, NT * 8));
+  k_hist_top<<<gA, NT, 0, s>>>(x, n, hist);
+  k_select_top<<<1, NT, 0, s>>>(hist, m, sc);
+  k_norm_count<<<nt, NT, 0, s>>>(x, n, sc, tcnt);
+  FRICC_TRY(scan_u32(tcnt, tcnt, nt, nullptr, s));
+  k_compact<<<nt, NT, 0, s>>>(x, n, sc, tcnt, kA, vA);
+  // --- sort candidates (stable LSD, 8 byte digits)
+  const int gS = (int)std::min<int64_t>(nt, 2048);
+  for (int d = 0; d < 8; d++) {
+    k_sort_count<<<gS, NT, 0, s>>>(kA, sc, 8 * d, bh, nt);
+    k_digit_total<<<256, NT, 0, s>>>(bh, nt, sc, dtot);
+    k_digit_offsets<<<256, NT, 0, s>>>(bh, nt, sc, dtot);
+    k_sort_scatter<<<gS, NT, 0, s>>>(kA, vA, kB, vB, sc, 8 * d, bh, nt);
+    std::swap(kA, kB);
+    std::swap(vA, vB);
+  }
+  FRICC_HIP(hipGetLastError());
+  if (method == COMP_LARGEST) {
+    k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, m, out_idx, out_val);
+  } else {
+    // --- D test over the first m sorted candidates
+    const int64_t ntm = ceil_div(m, TILE);
+    k_dfx_tilesums<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
+    k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, ntm, nullptr);
+    k_dtest<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
+    k_dfinal<<<1, NT, 0, s>>>(kA, vA, m, sc, ts);
+    k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, -1, out_idx, out_val);
+    // --- systematic sampling of the remainder
+    k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (u128*)p_ths);
+    k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
+    k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (const u128*)p_ths, out_idx, out_val);
+    k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
+    k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
+  }
+  if (export_sc) k_export_scalars<<<1, 64, 0, s>>>(sc, (CompScalars*)export_sc);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx, double* out_val,
+             CompressInfo* info, hipStream_t s) {
+  memset(info, 0, sizeof(*info));
+  void* sc = nullptr;
+  FRICC_TRY(compress_launch(x, n, m, method, u, nullptr, out_idx, out_val, nullptr, s, &sc));
+  CompScalars h;
+  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  return comp_scalars_status(&h, info);
 }
 
 // Systematic sampling of n_sample indices from given probabilities p[n]
@@ -646,29 +711,24 @@ int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, ui
   FRICC_CHECK(u >= 0.0 && u <= 1.0, "sample_systematic: uniform must lie in [0, 1]");
   if (n_notfound) *n_notfound = 0;
   if (n_sample == 0) return OK;
-  void *p_sc, *p_ts, *p_tot;
+  void *p_sc, *p_ts, *p_tot, *p_ths;
   const int64_t nt = ceil_div(n, TILE);
   FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
   FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
   FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
   CompScalars* sc = (CompScalars*)p_sc;
   u128* ts = (u128*)p_ts;
-  CompScalars h = {};
-  h.K = 0;
-  h.n_sampled = (uint64_t)n_sample;
-  FRICC_HIP(hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, s));
-  void* p_ths;
-  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  k_init_scalars<<<1, NT, 0, s>>>(sc, 0, n_sample, u, nullptr, nullptr, 0);
   k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (u128*)p_ths);
   k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-  k_samp_emit<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (const u128*)p_ths, u, out_idx, nullptr);
-  k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot, u);
+  k_samp_emit<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (const u128*)p_ths, out_idx, nullptr);
+  k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
+  k_samp_notfound<<<ceil_div(n_sample, NT), NT, 0, s>>>(p, sc, out_idx, nullptr);
   FRICC_HIP(hipGetLastError());
+  CompScalars h;
   FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
   FRICC_HIP(hipStreamSynchronize(s));
-  if (h.n_notfound > 0)
-    k_samp_notfound<<<ceil_div((int64_t)h.n_notfound, NT), NT, 0, s>>>(p, sc, out_idx, nullptr);
-  FRICC_HIP(hipGetLastError());
   if (n_notfound) *n_notfound = (int64_t)h.n_notfound;
   if (h.pflag) {
     set_error("sample_systematic: a probability exceeds 1");

@@ -20,6 +20,13 @@ struct CompressInfo {
 
 int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
              double* out_val, CompressInfo* info, hipStream_t s);
+// The same compression with no host round trip (u from u_dev when given);
+// the call's scalars go to export_sc (device, comp_scalars_size() bytes).
+int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
+                    uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s, void** sc_out);
+size_t comp_scalars_size();
+// statistics of a call's scalars (host copy) into info; returns the call's status
+int comp_scalars_status(const void* host_sc, CompressInfo* info);
 int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
                       int64_t* n_notfound, hipStream_t s);
 
@@ -47,6 +54,8 @@ struct Sparse {
   uint32_t* idx = nullptr;   // flat index ((i*d1+j)*d2+a)*d3+b
   double* val = nullptr;
   CompressInfo info{};
+  void* dsc = nullptr;   // device copy of the last asynchronous compression's scalars
+  bool pending = false;  // info/status of an asynchronous compression not read yet
   int device = 0;
   uint64_t gen = 1;  // bumped whenever the entries change
   std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<View>> views;

@@ -21,6 +21,7 @@ View::~View() {
 
 Sparse::~Sparse() {
   views.clear();
+  if (dsc) (void)hipFree(dsc);
   if (idx) (void)hipFree(idx);
   if (val) (void)hipFree(val);
 }
@@ -131,6 +132,13 @@ __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* _
   atomicAdd(&B[((int64_t)k * g.TY + (y % g.TY)) * 8 + (r & 7)], (float)val[e]);
 }
 
+// Zero n words (a kernel rather than a memset, so a view build can be
+// recorded in a graph)
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * VNT;
+  for (int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
 // Zero the first (*rows) rows of a row-major buffer of `words` uint4 per row;
 // the row count is read on the device, so the host never waits for it.
 __global__ void k_zero_rows(const uint32_t* __restrict__ rows, int64_t words, uint4* __restrict__ B) {
@@ -210,7 +218,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
     if (grouped) FRICC_HIP(hipMalloc(&v->Blo, cap * rowbytes));
     v->cap_rows = cap;
   }
-  FRICC_HIP(hipMemsetAsync(slot, 0, nslot * 4, s));
+  k_zero_u32<<<(unsigned)std::min<size_t>(2048, (nslot + VNT - 1) / VNT), VNT, 0, s>>>(slot, (int64_t)nslot);
   if (nnz > 0) k_mark<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, nnz, g, slot);
   k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
   FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));

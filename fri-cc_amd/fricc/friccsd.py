@@ -182,7 +182,57 @@ def _contract_into(spec, alpha, A, wshape, T, out, precision):
         _capi.stream_ptr()))
 
 
+def _check_m_keep(cc, t2):
+    m_keep = cc.fri_settings["m_keep"]
+    if m_keep > t2.numel() or m_keep == 0:
+        raise ValueError("Bad m_keep value! The following condition wasn't met 0 < m_keep <= t2.size")
+
+
+def _needs_alt(cc):
+    ccs = cc.fri_settings["compressed_contractions"]
+    return "O^3V^3" in ccs or "O^4V^2" in ccs
+
+
+def _draw_uniforms(cc):
+    """The uniforms of one step, in the reference's order (t2_sparse, then
+    t2_sparse_alt, fricc/friccsd.py:99-138); none for "largest"."""
+    if cc.fri_settings["compression"] != "fri":
+        return []
+    return [next(cc.uniforms) for _ in range(2 if _needs_alt(cc) else 1)]
+
+
+def _set_uniforms(cc, us):
+    """Write the step's uniforms into the device buffer the compressions read
+    (fill kernels: stream-ordered, no host round trip)."""
+    ud = cc._udev()
+    for k, u in enumerate(us):
+        ud[k].fill_(float(u))
+
+
+def _finish_step(cc, t1new, t2new, red):
+    """The one host read of a step: {E2, |dt2|^2, E1, |dt1|^2} and the
+    compression statuses (raises on non-finite amplitudes, p > 1)."""
+    e2, dn, e1, d1 = red.tolist()
+    info = []
+    for k in sorted(cc._slots):
+        if k in cc._used_slots:
+            info.append(cc._slots[k].sync_status())
+    cc._last_step = (t1new, t2new, e1 + e2, math.sqrt(d1 + dn))
+    cc.sparse_info = (info[0] if info else None, info[1] if len(info) > 1 else None)
+
+
 def update_amps(cc, t1, t2, eris):
+    """One FRI-CCSD Jacobi step (fricc/friccsd.py:29-301) on the GPU: draws the
+    step's uniforms, runs the device part (no host round trip) and reads back
+    the energy, the amplitude-change norm and the compression statuses once."""
+    _check_m_keep(cc, t2)
+    _set_uniforms(cc, _draw_uniforms(cc))
+    t1new, t2new, red = _update_amps_device(cc, t1, t2, eris)
+    _finish_step(cc, t1new, t2new, red)
+    return t1new, t2new
+
+
+def _update_amps_device(cc, t1, t2, eris):
     """One FRI-CCSD Jacobi step (fricc/friccsd.py:29-301) on the GPU.
 
     Hirata et al., J. Chem. Phys. 120, 2581 (2004), eqs. 35-36.  Same terms as
@@ -191,12 +241,15 @@ def update_amps(cc, t1, t2, eris):
     accumulate in place, compressed contractions add with their sign), every
     other T2 contribution into the output buffer, and one fused kernel adds
     S + S^T and the t1*t1 vvvv term, divides by the denominators and reduces
-    the energy and the amplitude-change norm (fricc_t2_finalize)."""
+    the energy and the amplitude-change norm (fricc_t2_finalize).
+
+    Nothing here waits for the GPU: the uniforms come from the device buffer
+    cc._udev(), compression statistics and errors stay on the device, and the
+    reductions are returned as a device tensor {E2, |dt2|^2, E1, |dt1|^2}, so
+    the whole call can be recorded as one graph."""
     torch = _torch()
     st = cc.fri_settings
     m_keep = st["m_keep"]
-    if m_keep > t2.numel() or m_keep == 0:
-        raise ValueError("Bad m_keep value! The following condition wasn't met 0 < m_keep <= t2.size")
     ccs = st["compressed_contractions"]
     sp = cc.sparse_precision
     timings = cc._timings = {}
@@ -224,19 +277,22 @@ def update_amps(cc, t1, t2, eris):
     t0 = time.perf_counter()
     tm.start("compress")
     comp = st["compression"]
-    draw = (lambda: next(cc.uniforms)) if comp == "fri" else (lambda: 0.0)
-    ts = cc._slot(0, t2.shape).recompress(t2, int(m_keep), comp, u=draw())
+    ud = cc._udev()
+    ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1])
+    cc._used_slots = {0}
     ts_alt = None
-    if "O^3V^3" in ccs or "O^4V^2" in ccs:
+    if _needs_alt(cc):
+        ualt = ud[1:2] if comp == "fri" else ud[0:1]
         if not st["unbiased"] or cc._iter < st["burn_in"]:
-            ts_alt = cc._slot(1, t2.shape).recompress(t2, int(m_keep), comp, u=draw())
+            ts_alt = cc._slot(1, t2.shape).recompress_async(t2, int(m_keep), comp, ualt)
         else:
             it = cc._iter - st["burn_in"]
             if it == 0:
                 cc._t2_avg = t2.clone()
             else:
                 cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
-            ts_alt = cc._slot(1, t2.shape).recompress(cc._t2_avg, int(m_keep), comp, u=draw())
+            ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt)
+        cc._used_slots = {0, 1}
     tm.stop("compress")
     timings["compress"] = time.perf_counter() - t0
 
@@ -401,17 +457,14 @@ def update_amps(cc, t1, t2, eris):
     # ---- denominators (fricc/friccsd.py:281-284), energy and norm: FP64, fused
     eia = (mo_e_o[:, None] - mo_e_v).contiguous()
     t1new = (t1new / eia).contiguous()
-    sums = torch.empty(2, dtype=f64, device=t2.device)
+    red = torch.empty(4, dtype=f64, device=t2.device)
     _capi.check(_capi.lib().fricc_t2_finalize(
         o, v, _vp(P64["ovovT"]), _vp(acc), _vp(S), _PREC_ID["fp32" if S.dtype == torch.float32 else "fp64"],
         _vp(t1), _vp(wsum) if wsum is not None else None, _vp(eia), _vp(t2), _vp(P64["Lovov"]), _vp(t1new),
-        _vp(sums), _capi.stream_ptr()))
-    t2new = acc
-    e2, dn = sums.tolist()
-    cc._last_step = (t1new, t2new, 2 * float((fov * t1new).sum().item()) + e2,
-                     math.sqrt(float(((t1new - t1) ** 2).sum().item()) + dn))
-    cc.sparse_info = (ts.info, ts_alt.info if ts_alt is not None else None)
-    return t1new, t2new
+        _vp(red), _capi.stream_ptr()))
+    red[2].copy_(2 * (fov * t1new).sum())
+    red[3].copy_(((t1new - t1) ** 2).sum())
+    return t1new, acc, red
 
 
 def energy(cc, t1, t2, eris):
@@ -449,6 +502,58 @@ def amplitudes_to_vector_norm(t1new, t2new, t1, t2):
     return float(torch.sqrt(((t1new - t1) ** 2).sum() + (tri ** 2).sum()).item())
 
 
+class GraphStep:
+    """``update_amps`` recorded once as a graph for this (nocc, nvir, m) shape
+    and replayed every iteration: one launch of the whole step instead of a
+    few hundred.  Inputs live in static buffers; per iteration the host writes
+    the step's uniforms (fill kernels, stream-ordered), replays, reads the
+    step's reductions and statuses once (``_finish_step``) and copies the
+    outputs into the inputs.  Needs the unbiased averaging off (its branch
+    depends on the iteration)."""
+
+    def __init__(self, cc, eris, t1, t2):
+        torch = _torch()
+        st = cc.fri_settings
+        if st["unbiased"] and _needs_alt(cc):
+            raise ValueError("graph mode needs unbiased averaging off")
+        if getattr(cc, "timer", None) is not None:
+            raise ValueError("graph mode cannot run with a section timer")
+        _check_m_keep(cc, t2)
+        self.cc, self.eris = cc, eris
+        self.t1 = t1.detach().clone().contiguous()
+        self.t2 = t2.detach().clone().contiguous()
+        # warm-up step on a side stream: allocates every workspace, view and
+        # operand the step needs, so the recorded step allocates nothing
+        # outside the graph's pool
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            _set_uniforms(cc, [0.5, 0.5])
+            _update_amps_device(cc, self.t1, self.t2, eris)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = _update_amps_device(cc, self.t1, self.t2, eris)
+        torch.cuda.synchronize()
+
+    def step(self, t1, t2):
+        if t1 is not self.t1:
+            self.t1.copy_(t1)
+        if t2 is not self.t2:
+            self.t2.copy_(t2)
+        cc = self.cc
+        _set_uniforms(cc, _draw_uniforms(cc))
+        self.graph.replay()
+        t1n, t2n, red = self.out
+        _finish_step(cc, t1n, t2n, red)
+        e, nt = cc._last_step[2], cc._last_step[3]
+        self.t1.copy_(t1n)
+        self.t2.copy_(t2n)
+        cc._last_step = (self.t1, self.t2, e, nt)
+        return self.t1, self.t2
+
+
 def kernel(mycc, eris=None, t1=None, t2=None, max_cycle=50, tol=1e-8, tolnormt=1e-6, verbose=None):
     """The iteration loop (fricc/friccsd.py:379-456), device-resident: t1, t2
     stay on the GPU; the energy and the convergence/NaN guards are read back
@@ -467,10 +572,14 @@ def kernel(mycc, eris=None, t1=None, t2=None, max_cycle=50, tol=1e-8, tolnormt=1
     mycc.energies = []
     mycc.cycle_times = []
     conv = False
+    runner = GraphStep(mycc, eris, t1, t2) if mycc.fri_settings.get("graph") else None
     for istep in range(max_cycle):
         tc = time.perf_counter()
         mycc._iter = istep
-        t1new, t2new = mycc.update_amps(t1, t2, eris)
+        if runner is not None:
+            t1new, t2new = runner.step(t1, t2)
+        else:
+            t1new, t2new = mycc.update_amps(t1, t2, eris)
         last = getattr(mycc, "_last_step", None)
         if last is not None and last[0] is t1new and last[1] is t2new:
             enew, normt = last[2], last[3]  # computed in the fused finalize pass
@@ -505,8 +614,9 @@ class FRICCSD:
     ``mf`` is a PySCF SCF object (when PySCF is importable), a
     ``systems.SyntheticSystem`` or a ready ``systems.DeviceERIs``.
     Extra settings keys (not in the reference): ``"precision"`` ("fp64" |
-    "mixed"), ``"seed"`` (srand seed for the uniform stream, default: none, as
-    the reference), ``"vvvv_t1t1"`` ("reference" | "rccsd")."""
+    "fp32" | "mixed"), ``"seed"`` (srand seed for the uniform stream, default:
+    none, as the reference), ``"vvvv_t1t1"`` ("reference" | "rccsd"),
+    ``"graph"`` (True: ``kernel`` records the step once and replays it)."""
 
     def __init__(self, mf, fri_settings=default_fri_settings, frozen=None, mo_coeff=None, mo_occ=None,
                  uniforms=None):
@@ -542,6 +652,8 @@ class FRICCSD:
         self.t1 = self.t2 = None
         self._iter = 0
         self._slots = {}
+        self._used_slots = set()
+        self._u_dev = None
         self._eris = None
         self.uniforms = uniforms if uniforms is not None else GlibcUniforms(settings.get("seed"))
         self.vvvv_t1t1 = settings["vvvv_t1t1"]
@@ -605,6 +717,12 @@ class FRICCSD:
         return (self.e_hf or 0.0) + (self.e_corr or 0.0)
 
     # ---- internals
+    def _udev(self):
+        """Device buffer of the step's uniforms (read by the compressions)."""
+        if self._u_dev is None:
+            self._u_dev = _torch().zeros(2, dtype=_torch().float64, device="cuda")
+        return self._u_dev
+
     def _slot(self, k, dims):
         s = self._slots.get(k)
         if s is None or tuple(s.dims) != tuple(dims):

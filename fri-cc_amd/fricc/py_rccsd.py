@@ -238,8 +238,34 @@ class SparseTensor4d:
         self.u = uu if method == _capi.FRI_SYSTEMATIC else None
         _capi.check(_capi.lib().fricc_sparse_compress(self._h, ctypes.c_void_p(x.data_ptr()), int(m), method, uu,
                                                       _capi.stream_ptr(stream)))
+        self._async = False
         self._refresh_info()
         return self
+
+    def recompress_async(self, x, m, compression, u_dev, stream=None):
+        """``recompress`` with no host round trip (device-resident loop, graph
+        capture): the uniform is read from ``u_dev`` (a one-element CUDA float64
+        tensor, unused for "largest"); statistics and errors wait for
+        ``sync_status``."""
+        x = _as_device_vector(x)
+        if compression not in ("largest", "fri"):
+            raise ValueError("compression must be 'largest' or 'fri'")
+        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
+        _capi.check(_capi.lib().fricc_sparse_compress_async(self._h, ctypes.c_void_p(x.data_ptr()), int(m), method,
+                                                            ctypes.c_void_p(u_dev.data_ptr()),
+                                                            _capi.stream_ptr(stream)))
+        self._host = None
+        self._async = True
+        return self
+
+    def sync_status(self, stream=None):
+        """Statistics and status of the last asynchronous compression (raises
+        on non-finite input, p > 1 or a bad uniform)."""
+        if getattr(self, "_async", False):
+            st = _capi.lib().fricc_sparse_status(self._h, _capi.stream_ptr(stream))
+            self._refresh_info()
+            _capi.check(st)
+        return self.info
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -81,6 +81,21 @@ int fricc_sparse_destroy(fricc_sparse* sp);
  * prod(dims) doubles) into sp, reusing its storage. */
 int fricc_sparse_compress(fricc_sparse* sp, const double* x, int64_t m, int compression, double u, void* stream);
 
+/* The same compression with no host round trip, for a device-resident loop
+ * (and graph capture): the uniform is read from device memory (u_dev, one
+ * double, in [0, 1]); statistics and errors (non-finite input, p > 1, u out of
+ * range) stay on the device until fricc_sparse_status.  No counterpart in the
+ * reference. */
+int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, int compression, const double* u_dev,
+                                void* stream);
+
+/* Waits for work on `stream`, reads the scalars the last asynchronous
+ * compression of sp left on the device (rewritten by every replay of a
+ * recorded graph), makes its statistics visible to fricc_sparse_info and
+ * returns its status (FRICC_OK, FRICC_ENONFINITE, FRICC_EPROB, or FRICC_EINVAL
+ * for u outside [0, 1]).  FRICC_OK when sp was last filled synchronously. */
+int fricc_sparse_status(fricc_sparse* sp, void* stream);
+
 /* set_element for all entries at once (include/sparse_tensor.hpp:127-136):
  * host arrays of nnz flat indices and values; indices must be < prod(dims). */
 int fricc_sparse_set(fricc_sparse* sp, const int64_t* idx, const double* val, int64_t nnz, void* stream);

@@ -146,3 +146,54 @@ def test_bad_settings(env):
     cc = FRICCSD(sysm, {"m_keep": 10 ** 9})
     with pytest.raises(ValueError):
         cc.update_amps(t1, t2, eris)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_graph_replay_matches_eager(env, prec):
+    """The step recorded once and replayed each iteration -- with host reads,
+    copies and an unrelated eager step on the same device between replays --
+    gives the eager trajectory exactly: energies, index sets and amplitudes,
+    iteration by iteration."""
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    from fricc.friccsd import GraphStep
+
+    o, v, naux = 4, 14, 40
+    us = list(np.random.default_rng(4).random(40))
+    base = {"m_keep": 1500, "compression": "fri", "sampling_method": "systematic", "precision": prec}
+    other = FRICCSD(SyntheticSystem(3, 9, 20, 2), {"m_keep": 200}, uniforms=iter(np.linspace(0.1, 0.9, 50)))
+    oe = other.ao2mo()
+    ot1, ot2 = other.get_init_guess(oe)
+    runs = {}
+    for graph in (False, True):
+        cc = FRICCSD(SyntheticSystem(o, v, naux, 8), base, uniforms=iter(us))
+        cc.level_shift = 2.0
+        eris = cc.ao2mo()
+        t1, t2 = cc.get_init_guess(eris)
+        runner = GraphStep(cc, eris, t1, t2) if graph else None
+        energies, idx = [], []
+        for it in range(5):
+            cc._iter = it
+            t1, t2 = runner.step(t1, t2) if graph else cc.update_amps(t1, t2, eris)
+            energies.append(cc._last_step[2])
+            idx.append(cc._slots[0].indices().copy())
+            other.update_amps(ot1, ot2, oe)  # eager work between replays
+        runs[graph] = (energies, idx, t2.cpu().numpy().copy())
+    assert runs[True][0] == runs[False][0]
+    for a, b in zip(runs[True][1], runs[False][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(runs[True][2], runs[False][2])
+
+
+def test_graph_mode_kernel_loop(env):
+    """FRICCSD(..., {"graph": True}).kernel() runs the recorded step."""
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    us = list(np.random.default_rng(5).random(40))
+    res = {}
+    for graph in (False, True):
+        cc = FRICCSD(SyntheticSystem(4, 12, 30, 3), {"m_keep": 900, "graph": graph}, uniforms=iter(us))
+        cc.level_shift = 2.0
+        cc.max_cycle = 4
+        cc.conv_tol = cc.conv_tol_normt = 0
+        cc.kernel()
+        res[graph] = list(cc.energies)
+    assert res[True] == res[False] and len(res[True]) == 4
