@@ -194,6 +194,52 @@ def cpu_baseline_sample(x, idx, val, o, v, m, blocks_per_term=None, slice_len=1 
 COPY_BYTES = 1 << 30  # calibration copy for the traffic counters
 
 
+def replica_bench(args):
+    """Batched replica trajectories of the o=20, v=180 system (naux 600,
+    seed 1): R replicas with m log-uniform in [2e5, 2e6] and Philox seeds
+    0..R-1, one graph launch per iteration for all of them, against the same
+    replicas stepped one at a time eagerly."""
+    import torch
+
+    from fricc.replicas import ReplicaBatch, log_uniform_m, run_alone
+    from fricc.systems import CONFIGS, DeviceERIs, SyntheticSystem
+
+    cfg = CONFIGS["c2"]
+    o, v = cfg["nocc"], cfg["nvir"]
+    sysm = SyntheticSystem.from_config("c2")
+    eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
+    R = args.replicas
+    m_list = log_uniform_m(R, 2e5, 2e6, seed=args.seed, cap=o * o * v * v)
+    seeds = list(range(R))
+    settings = {"compression": "fri", "sampling_method": "systematic", "precision": args.replica_precision}
+    batch = ReplicaBatch(eris, settings, m_list, seeds, level_shift=2.0)
+    t0 = time.perf_counter()
+    batch.step()  # records the graph
+    t_record = time.perf_counter() - t0
+    K = args.replica_iters
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        batch.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # the same work stepped eagerly, one replica at a time (a sample of replicas)
+    ns = min(4, R)
+    run_alone(eris, settings, m_list[0], seeds[0], 1, level_shift=2.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(ns):
+        run_alone(eris, settings, m_list[r], seeds[r], 2, level_shift=2.0)
+    torch.cuda.synchronize()
+    eager = (time.perf_counter() - t0) / (2 * ns)
+    s = batch.summary()
+    return {"system": f"synthetic DF nocc={o} nvir={v} naux={cfg['naux']} seed={cfg['seed']}, level_shift=2.0",
+            "replicas": R, "m_range": [min(m_list), max(m_list)], "precision": args.replica_precision,
+            "timed_iterations": K + 1, "replica_iter_per_s": R * K / dt, "ms_per_batch_iteration": 1e3 * dt / K,
+            "eager_replica_iter_per_s": 1.0 / eager, "record_s": t_record,
+            "E_corr_mean": s["mean"], "E_corr_stderr": s["stderr"]}
+
+
 def traffic_child(args):
     """Run under rocprofv3 --pmc by measure_traffic: one calibration copy of
     COPY_BYTES, then the warm mixed-precision iteration whose ladder launch is
@@ -264,6 +310,31 @@ def measure_traffic(args):
             "write_scale": f_write}, "ok"
 
 
+def serve_bench(args):
+    """Continuous-batching serving: Poisson arrivals (rate args.serve_rate per
+    second) of args.serve solves, (nocc, nvir) uniform over the given sizes,
+    m/|T2| uniform in [1%, 10%], each to |dE| < 1e-6 Ha or 50 iterations,
+    admission by an HBM budget of 240 GB, mixed precision."""
+    import numpy as np
+    import torch
+
+    from fricc import serving as sv
+
+    sizes = [tuple(int(x) for x in s.split("x")) for s in args.serve_sizes.split(",")]
+    reqs = sv.poisson_requests(args.serve, args.serve_rate, sizes, (0.01, 0.10), seed=args.seed)
+    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=16)
+    res = server.run(reqs)
+    st = sv.latency_stats(res, server.makespan)
+    st.update({"sizes": [f"{o}x{v}" for o, v in sizes], "rate_per_s": args.serve_rate,
+               "max_concurrent": server.max_concurrent, "hbm_budget_gb": 240,
+               "per_size": {f"{o}x{v}": {"n": sum(1 for r in res if (r.nocc, r.nvir) == (o, v)),
+                                         "p50_s": float(np.median([r.latency for r in res if (r.nocc, r.nvir) == (o, v)]
+                                                                  or [float("nan")]))}
+                            for o, v in sizes}})
+    torch.cuda.empty_cache()
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +345,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
+    ap.add_argument("--replicas", type=int, default=0, help="also run R batched replicas of the o=20 v=180 system")
+    ap.add_argument("--replica-iters", type=int, default=5)
+    ap.add_argument("--replica-precision", default="mixed")
+    ap.add_argument("--serve", type=int, default=0, help="also serve N solves arriving as a Poisson stream")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_child:
@@ -403,6 +480,13 @@ def main():
             roof["traffic_detail"] = tr
         else:
             roof["traffic_note"] = note
+    if args.serve > 0:
+        out["serving"] = serve_bench(args)
+        print(f"# serving: {out['serving']}", file=sys.stderr, flush=True)
+    if args.replicas > 0:
+        out["replica_batch"] = replica_bench(args)
+        print(f"# replicas: {out['replica_batch']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         t_cpu, thr, desc = cpu_baseline_sample(
             t2_host, ts_idx, ts_val, o, v, m, log=lambda s: print(f"# cpu: {s}", file=sys.stderr, flush=True))

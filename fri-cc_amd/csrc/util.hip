@@ -9,10 +9,45 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
-static Workspace g_ws[16];
+// Workspace arenas: per device, NARENA independent sets of grow-only slots.
+// Work that may run concurrently (solves on different streams) selects
+// different arenas; everything recorded or replayed for one solve uses that
+// solve's arena, so two solves never share scratch.
+constexpr int NARENA = 64;
+static Workspace g_ws[16][NARENA];
 static std::mutex g_ws_mu;
+static int g_arena = 0;
 
-// Grow-only per-device arena slot.  Contents are not preserved on growth.
+int ws_select(int arena) {
+  if (arena < 0 || arena >= NARENA) {
+    set_error("workspace arena out of range (0..63)");
+    return EINVAL_;
+  }
+  g_arena = arena;
+  return OK;
+}
+
+int ws_release(int arena) {
+  if (arena < 0 || arena >= NARENA) {
+    set_error("workspace arena out of range (0..63)");
+    return EINVAL_;
+  }
+  int dev = 0;
+  FRICC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& w = g_ws[dev][arena];
+  bool any = false;
+  for (int s = 0; s < 64; s++) any = any || w.ptr[s];
+  if (any) FRICC_HIP(hipDeviceSynchronize());
+  for (int s = 0; s < 64; s++) {
+    if (w.ptr[s]) FRICC_HIP(hipFree(w.ptr[s]));
+    w.ptr[s] = nullptr;
+    w.cap[s] = 0;
+  }
+  return OK;
+}
+
+// Grow-only slot of the selected arena.  Contents are not preserved on growth.
 int ws_get(int slot, size_t bytes, void** out) {
   int dev = 0;
   FRICC_HIP(hipGetDevice(&dev));
@@ -21,7 +56,7 @@ int ws_get(int slot, size_t bytes, void** out) {
     return EINVAL_;
   }
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace& w = g_ws[dev];
+  Workspace& w = g_ws[dev][g_arena];
   if (bytes == 0) bytes = 256;
   if (w.cap[slot] < bytes) {
     if (w.ptr[slot]) {
@@ -135,6 +170,12 @@ int stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
 }  // namespace fricc
 
 extern "C" {
+// Workspace arena of the following calls on this device (0..63, default 0).
+// No counterpart in the reference.
+int fricc_workspace_select(int arena) { return fricc::ws_select(arena); }
+// Frees an arena's buffers (waits for the device first).
+int fricc_workspace_release(int arena) { return fricc::ws_release(arena); }
+
 // Device-to-device copy of `bytes` (a multiple of 16, 16-byte aligned
 // pointers) by a plain streaming kernel.  Bandwidth reference for the
 // roofline; no counterpart in the reference.

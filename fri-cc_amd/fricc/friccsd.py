@@ -251,6 +251,7 @@ def _update_amps_device(cc, t1, t2, eris):
     st = cc.fri_settings
     m_keep = st["m_keep"]
     ccs = st["compressed_contractions"]
+    _capi.check(_capi.lib().fricc_workspace_select(int(cc.arena)))
     sp = cc.sparse_precision
     timings = cc._timings = {}
     tm = cc.timer if getattr(cc, "timer", None) is not None else _NoTimer()
@@ -537,14 +538,16 @@ class GraphStep:
             self.out = _update_amps_device(cc, self.t1, self.t2, eris)
         torch.cuda.synchronize()
 
-    def step(self, t1, t2):
-        if t1 is not self.t1:
-            self.t1.copy_(t1)
-        if t2 is not self.t2:
-            self.t2.copy_(t2)
+    def issue(self):
+        """Queue one iteration on the current stream (uniforms, replay); no wait."""
         cc = self.cc
         _set_uniforms(cc, _draw_uniforms(cc))
         self.graph.replay()
+
+    def complete(self):
+        """Read the iteration's energy, norm and statuses (waits for the current
+        stream), then queue the copy of its outputs into the inputs."""
+        cc = self.cc
         t1n, t2n, red = self.out
         _finish_step(cc, t1n, t2n, red)
         e, nt = cc._last_step[2], cc._last_step[3]
@@ -552,6 +555,14 @@ class GraphStep:
         self.t2.copy_(t2n)
         cc._last_step = (self.t1, self.t2, e, nt)
         return self.t1, self.t2
+
+    def step(self, t1, t2):
+        if t1 is not self.t1:
+            self.t1.copy_(t1)
+        if t2 is not self.t2:
+            self.t2.copy_(t2)
+        self.issue()
+        return self.complete()
 
 
 def kernel(mycc, eris=None, t1=None, t2=None, max_cycle=50, tol=1e-8, tolnormt=1e-6, verbose=None):
@@ -654,6 +665,7 @@ class FRICCSD:
         self._slots = {}
         self._used_slots = set()
         self._u_dev = None
+        self.arena = 0  # library scratch arena; solves that run concurrently need different ones
         self._eris = None
         self.uniforms = uniforms if uniforms is not None else GlibcUniforms(settings.get("seed"))
         self.vvvv_t1t1 = settings["vvvv_t1t1"]

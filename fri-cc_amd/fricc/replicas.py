@@ -1,0 +1,147 @@
+"""Independent FRI-CCSD replica trajectories advanced together.
+
+Replica r is an ordinary ``FRICCSD`` solve of one system with its own
+``m_keep`` and its own uniform stream (``PhiloxUniforms``: counter-based,
+independent per seed; glibc's global generator cannot serve several
+trajectories and its seeds 0 and 1 coincide).  The replicas share the
+system's integrals and prepared operands.
+
+``ReplicaBatch`` records one iteration of *every* replica -- each replica's
+whole device step (``friccsd._update_amps_device``) followed by the copy of its
+outputs into its inputs -- as a single graph, so one launch advances all R
+trajectories by one iteration.  Per iteration the host writes the R x 2
+uniforms with one copy from pinned memory, replays, and reads every replica's
+energy and norm in one transfer.  Each replica's arithmetic is exactly that
+of the replica run alone (same kernels, same order), so a replica's
+trajectory in the batch is bit-identical to its trajectory alone.
+
+The reference runs one trajectory per process and averages afterwards
+(examples/03_friccsd_unbiased.py; no batched form exists there).
+"""
+import math
+
+import numpy as np
+
+from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, _needs_alt, _update_amps_device
+from .uniforms import PhiloxUniforms
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def log_uniform_m(n, lo, hi, seed, cap=None):
+    """n values of m drawn log-uniformly in [lo, hi] (integers), optionally capped."""
+    rng = np.random.default_rng(seed)
+    m = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.int64)
+    if cap is not None:
+        m = np.minimum(m, cap)
+    return [int(x) for x in m]
+
+
+class ReplicaBatch:
+    """R replicas of one system (``eris``) stepped by one graph launch per iteration.
+
+    settings: FRICCSD settings shared by all replicas (precision, compression,
+    ...); ``m_list`` and ``seeds`` give each replica's m_keep and Philox seed."""
+
+    def __init__(self, eris, settings, m_list, seeds, level_shift=0.0):
+        torch = _torch()
+        if len(m_list) != len(seeds):
+            raise ValueError("one m and one seed per replica")
+        self.eris = eris
+        self.R = len(seeds)
+        self.reps = []
+        for m, seed in zip(m_list, seeds):
+            cc = FRICCSD(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
+            cc.level_shift = level_shift
+            if cc.fri_settings["unbiased"] and _needs_alt(cc):
+                raise ValueError("replica batches need the unbiased averaging off")
+            self.reps.append(cc)
+        # all uniforms of one iteration: device R x 2, written from pinned host memory
+        self.u_dev = torch.zeros(self.R, 2, dtype=torch.float64, device="cuda")
+        self.u_host = torch.zeros(self.R, 2, dtype=torch.float64, pin_memory=True)
+        for r, cc in enumerate(self.reps):
+            cc._u_dev = self.u_dev[r]
+        t1, t2 = self.reps[0].get_init_guess(eris)
+        for cc in self.reps:
+            _check_m_keep(cc, t2)
+        self.t1 = [t1.clone() for _ in range(self.R)]
+        self.t2 = [t2.clone() for _ in range(self.R)]
+        self.energies = []  # per iteration: R energies
+        self.graph = None
+
+    def _record(self):
+        torch = _torch()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # warm-up: every workspace, view and operand allocated
+            self.u_dev.fill_(0.5)
+            for r, cc in enumerate(self.reps):
+                _update_amps_device(cc, self.t1[r], self.t2[r], self.eris)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        self.out = []
+        with torch.cuda.graph(self.graph):
+            for r, cc in enumerate(self.reps):
+                t1n, t2n, red = _update_amps_device(cc, self.t1[r], self.t2[r], self.eris)
+                self.out.append((t1n, t2n, red))
+            self.red = torch.stack([o[2] for o in self.out])
+            for r, (t1n, t2n, _) in enumerate(self.out):
+                self.t1[r].copy_(t1n)
+                self.t2[r].copy_(t2n)
+        torch.cuda.synchronize()
+
+    def step(self):
+        """One iteration of every replica; returns the R energies."""
+        if self.graph is None:
+            self._record()
+        for r, cc in enumerate(self.reps):
+            us = _draw_uniforms(cc)
+            for k, u in enumerate(us):
+                self.u_host[r, k] = u
+        self.u_dev.copy_(self.u_host, non_blocking=True)
+        self.graph.replay()
+        red = self.red.cpu().numpy()  # one transfer: {E2, |dt2|^2, E1, |dt1|^2} per replica
+        en = []
+        for r, cc in enumerate(self.reps):
+            for k in sorted(cc._used_slots):
+                cc._slots[k].sync_status()
+            e = float(red[r, 0] + red[r, 2])
+            cc._last_step = (self.t1[r], self.t2[r], e, math.sqrt(float(red[r, 1] + red[r, 3])))
+            cc._iter += 1
+            en.append(e)
+        self.energies.append(en)
+        return en
+
+    def run(self, n_iter):
+        for _ in range(n_iter):
+            self.step()
+        return self.summary()
+
+    def summary(self, last=None):
+        """Replica mean and standard error of E_corr at the last iteration (or
+        averaged over the last ``last`` iterations per replica first)."""
+        e = np.asarray(self.energies)
+        x = e[-1] if not last else e[-last:].mean(axis=0)
+        se = x.std(ddof=1) / math.sqrt(len(x)) if len(x) > 1 else float("nan")
+        return {"mean": float(x.mean()), "stderr": float(se), "n_replicas": len(x), "n_iter": len(e)}
+
+
+def run_alone(eris, settings, m, seed, n_iter, level_shift=0.0):
+    """The same replica run by itself, eagerly (reference for the batch)."""
+    cc = FRICCSD(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
+    cc.level_shift = level_shift
+    t1, t2 = cc.get_init_guess(eris)
+    en = []
+    for it in range(n_iter):
+        cc._iter = it
+        t1, t2 = cc.update_amps(t1, t2, eris)
+        en.append(cc._last_step[2])
+    return en, t2
+
+
+__all__ = ["ReplicaBatch", "log_uniform_m", "run_alone"]

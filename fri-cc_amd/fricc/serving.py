@@ -1,0 +1,224 @@
+"""Continuous-batching FRI-CCSD solve server on one GPU.
+
+Requests -- a synthetic density-fitted system (nocc, nvir), m and a uniform
+seed -- arrive over time.  The server admits them in arrival order while the
+estimated HBM of all admitted solves stays within a budget.  Each admitted
+solve gets its own HIP stream, its own library scratch arena, device-resident
+state and its own recorded step (``friccsd.GraphStep``: the whole iteration is
+one graph launch).  The scheduler keeps every active solve's next iteration
+queued on its stream, so iterations of different solves run side by side on
+the device; whenever a solve's iteration has finished it reads that solve's
+energy (one small transfer), retires it when converged or out of iterations,
+and otherwise queues its next iteration at once.  Requests that arrived in
+the meantime are admitted as soon as memory frees up.
+
+A solve's arithmetic does not depend on what else runs (its own buffers,
+scratch and graph; no shared accumulators), so its energies are bit-identical
+alone and in the mix.
+
+The reference has no serving layer (one solve per Python process,
+fricc/friccsd.py:379-456); this is the device-resident loop of that kernel,
+scheduled many at a time.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .friccsd import FRICCSD, GraphStep
+from .uniforms import PhiloxUniforms
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@dataclass
+class Request:
+    id: int
+    arrival: float  # seconds after the start of the run
+    nocc: int
+    nvir: int
+    m: int
+    seed: int = 0  # uniform stream (Philox)
+    sys_seed: int = 1  # synthetic system
+    naux: int = 0  # 0: 3 * (nocc + nvir)
+    max_cycle: int = 50
+    tol: float = 1e-6  # |dE| to stop
+    level_shift: float = 2.0
+
+
+@dataclass
+class Result:
+    id: int
+    nocc: int
+    nvir: int
+    m: int
+    arrival: float
+    admitted: float
+    finished: float
+    iterations: int
+    converged: bool
+    energies: list = field(default_factory=list)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def poisson_requests(n, rate, sizes, mfrac=(0.01, 0.10), seed=0, max_cycle=50, tol=1e-6):
+    """n requests with exponential inter-arrival times (rate per second),
+    (nocc, nvir) drawn uniformly from ``sizes`` and m / |T2| uniform in mfrac."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        o, v = sizes[int(rng.integers(len(sizes)))]
+        m = max(1, int(rng.uniform(*mfrac) * o * o * v * v))
+        out.append(Request(i, float(t[i]), o, v, m, seed=int(rng.integers(1 << 31)), sys_seed=1 + (i % 3),
+                           max_cycle=max_cycle, tol=tol))
+    return out
+
+
+def estimate_bytes(o, v, precision="mixed"):
+    """HBM of one admitted solve in steady state: the FP64 integral blocks it
+    keeps, the operands prepared from them, the ladder (FP64 in the FP64 mode,
+    else only its BF16 planes: the FP64 copy is released after they are built),
+    amplitudes and graph temporaries, views and scratch.  Admission also needs
+    room for the FP64 ladder while it is built (v^4 doubles)."""
+    ov3, o2v2, v4 = float(o) * v ** 3, float(o * v) ** 2, float(v) ** 4
+    if precision == "fp64":
+        return 8.0 * v4 + 64.0 * ov3 + 300.0 * o2v2 + 1.0e9
+    return 4.0 * v4 + 32.0 * ov3 + 260.0 * o2v2 + 1.0e9
+
+
+class Solve:
+    def __init__(self, req, arena, precision):
+        torch = _torch()
+        from .systems import DeviceERIs, SyntheticSystem
+
+        self.req = req
+        self.stream = torch.cuda.Stream()
+        naux = req.naux or 3 * (req.nocc + req.nvir)
+        with torch.cuda.stream(self.stream):
+            sysm = SyntheticSystem(req.nocc, req.nvir, naux, req.sys_seed)
+            self.eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
+            cc = FRICCSD(self.eris, {"m_keep": int(req.m), "compression": "fri", "sampling_method": "systematic",
+                                     "precision": precision}, uniforms=PhiloxUniforms(req.seed))
+            cc.level_shift = req.level_shift
+            cc.arena = arena
+            self.cc = cc
+            t1, t2 = cc.get_init_guess(self.eris)
+            self.e = cc.energy(t1, t2, self.eris)
+            self.runner = GraphStep(cc, self.eris, t1, t2)
+            if precision != "fp64":
+                self.eris.release_fp64_ladder()
+        self.energies = []
+        self.done = False
+        self.converged = False
+
+    def issue(self):
+        torch = _torch()
+        with torch.cuda.stream(self.stream):
+            self.cc._iter = len(self.energies)
+            self.runner.issue()
+
+    def ready(self):
+        return self.stream.query()
+
+    def complete(self):
+        """Read the finished iteration; True when the solve is done."""
+        torch = _torch()
+        with torch.cuda.stream(self.stream):
+            self.runner.complete()
+        e = self.cc._last_step[2]
+        de = e - self.e
+        self.e = e
+        self.energies.append(e)
+        if abs(de) < self.req.tol:
+            self.converged = True
+        self.done = self.converged or len(self.energies) >= self.req.max_cycle or not math.isfinite(e)
+        return self.done
+
+
+class Server:
+    """Admission by HBM budget, per-solve streams and arenas, continuous batching."""
+
+    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8):
+        self.budget = float(hbm_budget)
+        self.precision = precision
+        self.max_active = min(int(max_active), 63)
+
+    def run(self, requests, clock=time.perf_counter):
+        torch = _torch()
+        from . import _capi
+
+        pending = sorted(requests, key=lambda r: r.arrival)
+        free_arenas = list(range(1, self.max_active + 1))
+        active, results, used = [], [], 0.0
+        self.max_concurrent = 0
+        t0 = clock()
+        while pending or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now and free_arenas:
+                need = estimate_bytes(pending[0].nocc, pending[0].nvir, self.precision)
+                if active and used + need > self.budget:
+                    break
+                req = pending.pop(0)
+                s = Solve(req, free_arenas.pop(0), self.precision)
+                s.admitted = clock() - t0
+                s.need = need
+                used += need
+                s.issue()
+                active.append(s)
+                self.max_concurrent = max(self.max_concurrent, len(active))
+            if not active:
+                time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
+                continue
+            progressed = False
+            for s in list(active):
+                if not s.ready():
+                    continue
+                progressed = True
+                if s.complete():
+                    fin = clock() - t0
+                    results.append(Result(s.req.id, s.req.nocc, s.req.nvir, s.req.m, s.req.arrival, s.admitted, fin,
+                                          len(s.energies), s.converged, list(s.energies)))
+                    active.remove(s)
+                    used -= s.need
+                    free_arenas.append(s.cc.arena)
+                    free_arenas.sort()
+                    del s
+                    torch.cuda.empty_cache()
+                else:
+                    s.issue()
+            if not progressed:
+                time.sleep(5e-5)
+        torch.cuda.synchronize()
+        for a in range(1, self.max_active + 1):
+            _capi.check(_capi.lib().fricc_workspace_release(a))
+        self.makespan = clock() - t0
+        return sorted(results, key=lambda r: r.id)
+
+
+def latency_stats(results, makespan):
+    lat = np.array([r.latency for r in results])
+    return {"solves": len(results), "solves_per_s": len(results) / makespan if makespan > 0 else None,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_iterations": float(np.mean([r.iterations for r in results])),
+            "converged": int(sum(r.converged for r in results)), "makespan_s": makespan}
+
+
+def solve_alone(req, precision="mixed"):
+    """The request run by itself (same recorded step), for comparison."""
+    torch = _torch()
+    s = Solve(req, 0, precision)
+    s.issue()
+    while True:
+        s.stream.synchronize()
+        if s.complete():
+            return list(s.energies)
+        s.issue()

@@ -78,6 +78,23 @@ def bf16_planes(A, R, X, stream=None):
     return out
 
 
+class _LazyOperands(dict):
+    """Operands of one dtype, built on first access."""
+
+    def __init__(self, eris, dtype):
+        super().__init__()
+        self.eris, self.dtype = eris, dtype
+
+    def __missing__(self, name):
+        alias = {"Lovov": "Lov2", "Lov2": "Lovov"}.get(name)
+        if alias is not None and dict.__contains__(self, alias):
+            val = dict.__getitem__(self, alias)
+        else:
+            val = self.eris._build(name, self.dtype)
+        self[name] = val
+        return val
+
+
 def _cstrides(shape):
     s = [1, 1, 1, 1]
     for i in (2, 1, 0):
@@ -93,6 +110,7 @@ class DeviceERIs:
 
     def __init__(self):
         self._prep = {}
+        self._s_ab = None
 
     @classmethod
     def from_factors(cls, L, mo_energy, nocc, fock=None, build_ladder=True):
@@ -154,63 +172,101 @@ class DeviceERIs:
     def sparse_operand(self, name, precision):
         """Static A operand of a sparse contraction ("A1323", "A0112", "A0313",
         "Gx", "Gy", "ladder") in the kernel layout for ``precision``
-        ("fp64" | "fp32" | "bf16"), cached."""
+        ("fp64" | "fp32" | "bf16"), cached.  For FP32/BF16 the FP64 layout is
+        built only transiently."""
         import torch
 
         key = ("sparse", name, precision)
         if key in self._prep:
             return self._prep[key]
-        p64 = self.prepared(torch.float64)
-        A = self.ladder if name == "ladder" else p64[name]
         if precision == "fp64":
-            out = A
-        elif precision == "fp32":
-            out = A.to(torch.float32).contiguous()
+            out = self.ladder if name == "ladder" else self.prepared(torch.float64)[name]
         else:
-            out = bf16_planes(A, A.shape[0] * A.shape[1], A.shape[2] * A.shape[3])
+            A = self.ladder if name == "ladder" else self._build(name, torch.float64)
+            if A is None:
+                raise RuntimeError("the FP64 ladder was released; its BF16/FP32 operand must be built before")
+            if precision == "fp32":
+                out = A.to(torch.float32).contiguous()
+            else:
+                out = bf16_planes(A, A.shape[0] * A.shape[1], A.shape[2] * A.shape[3])
+            del A
         self._prep[key] = out
         return out
 
-    def prepared(self, dtype):
-        """Static operands in the layouts the update reads, cached per dtype."""
+    def _build(self, name, dtype):
+        """One static operand in the layout the update reads (see prepared)."""
         import torch
 
-        key = str(dtype)
-        if key in self._prep:
-            return self._prep[key]
         o, v = self.nocc, self.nvir
-        c = lambda a: a.to(dtype).contiguous()
+        c = lambda a: a.to(dtype).contiguous()  # noqa: E731
         ovov, ovvv = self.ovov, self.ovvv
-        p = {}
-        # sparse-contraction operands A[r][x] (kernel layout, see fricc_contract_prepared)
-        p["A1323"] = c(ovov.permute(1, 3, 0, 2))          # [c,d,k,l] = (kc|ld)
-        p["A0112"] = c(ovov)                              # [l,d,k,c] = (ld|kc)
-        p["A0313"] = c(ovov.permute(0, 3, 2, 1))          # [l,d,k,c] = (lc|kd)
-        p["Gx"] = c(ovvv.permute(3, 1, 0, 2))             # [c,d,k,a] = (kd|ac)
-        p["Gy"] = c(ovvv.permute(1, 3, 0, 2))             # [c,d,k,b] = (kc|bd)
-        # dense-remainder operands
-        P1 = ovvv.permute(0, 1, 3, 2)                     # [k,c,d,a] = (kc|ad)
-        P2 = ovvv.permute(0, 3, 1, 2)                     # [k,c,d,a] = (kd|ac)
-        p["P1"] = c(P1)
-        p["P2"] = c(P2)
-        p["Z"] = c(2 * P2 - P1)
-        p["Qv"] = c(ovvv.permute(0, 2, 1, 3))             # [i,c,a,b] = (ia|cb)
-        p["Lov"] = c(2 * ovov.permute(0, 2, 3, 1) - ovov.permute(0, 2, 1, 3))   # [k,l,d,c]
-        p["Lov2"] = c(2 * ovov.permute(0, 2, 1, 3) - ovov.permute(0, 2, 3, 1))  # [k,l,c,d]
-        p["Lovov"] = c(2 * ovov.permute(0, 2, 1, 3) - ovov.permute(0, 2, 3, 1))  # [i,j,a,b]
-        p["ovovT"] = c(ovov.permute(0, 2, 1, 3))           # [i,j,a,b] = (ia|jb)
-        p["ovvo_p"] = c(self.ovvo.permute(0, 1, 3, 2))     # [k,c,i,a] = (kc|ai)
-        p["oovv_p"] = c(self.oovv.permute(0, 3, 1, 2))     # [k,c,i,a] = (ki|ac)
-        p["ovov"] = c(ovov)
-        p["ovoo"] = c(self.ovoo)
-        p["oovv"] = c(self.oovv)
-        p["ovvo"] = c(self.ovvo)
-        p["oooo"] = c(self.oooo)
-        if self.ladder is not None:
-            if dtype == torch.float64:
-                p["ladder"] = self.ladder
-            # sum over (c,d) of the bare part of Wvvvv (ac|bd)
-            p["s_ab"] = self.ladder.reshape(v * v, v * v).sum(0).reshape(v, v).to(dtype)
-        p["g_ka"] = ovvv.sum(dim=(1, 3)).to(dtype)         # sum_cd (kd|ac) = sum_cd (kc|ad)
-        self._prep[key] = p
-        return p
+        if name == "A1323":
+            return c(ovov.permute(1, 3, 0, 2))          # [c,d,k,l] = (kc|ld)
+        if name in ("A0112", "ovov"):
+            return c(ovov)                              # [l,d,k,c] = (ld|kc)
+        if name == "A0313":
+            return c(ovov.permute(0, 3, 2, 1))          # [l,d,k,c] = (lc|kd)
+        if name == "Gx":
+            return c(ovvv.permute(3, 1, 0, 2))          # [c,d,k,a] = (kd|ac)
+        if name == "Gy":
+            return c(ovvv.permute(1, 3, 0, 2))          # [c,d,k,b] = (kc|bd)
+        if name == "P1":
+            return c(ovvv.permute(0, 1, 3, 2))          # [k,c,d,a] = (kc|ad)
+        if name == "P2":
+            return c(ovvv.permute(0, 3, 1, 2))          # [k,c,d,a] = (kd|ac)
+        if name == "Z":
+            return c(2 * ovvv.permute(0, 3, 1, 2) - ovvv.permute(0, 1, 3, 2))
+        if name == "Qv":
+            return c(ovvv.permute(0, 2, 1, 3))          # [i,c,a,b] = (ia|cb)
+        if name == "Lov":
+            return c(2 * ovov.permute(0, 2, 3, 1) - ovov.permute(0, 2, 1, 3))   # [k,l,d,c]
+        if name in ("Lov2", "Lovov"):
+            # [k,l,c,d] = 2 (kc|ld) - (kd|lc): one tensor serves both names
+            return c(2 * ovov.permute(0, 2, 1, 3) - ovov.permute(0, 2, 3, 1))
+        if name == "ovovT":
+            return c(ovov.permute(0, 2, 1, 3))          # [i,j,a,b] = (ia|jb)
+        if name == "ovvo_p":
+            return c(self.ovvo.permute(0, 1, 3, 2))     # [k,c,i,a] = (kc|ai)
+        if name == "oovv_p":
+            return c(self.oovv.permute(0, 3, 1, 2))     # [k,c,i,a] = (ki|ac)
+        if name in ("ovoo", "oovv", "ovvo", "oooo"):
+            return c(getattr(self, name))
+        if name == "ladder":
+            if dtype != torch.float64 or self.ladder is None:
+                raise KeyError("the FP64 ladder operand is not available")
+            return self.ladder
+        if name == "s_ab":
+            # sum over (c,d) of the bare part of Wvvvv, (ac|bd)
+            if self.ladder is None:
+                if self._s_ab is None:
+                    raise KeyError("s_ab needs the ladder")
+                return self._s_ab.to(dtype)
+            return self.ladder.reshape(v * v, v * v).sum(0).reshape(v, v).to(dtype)
+        if name == "g_ka":
+            return ovvv.sum(dim=(1, 3)).to(dtype)       # sum_cd (kd|ac) = sum_cd (kc|ad)
+        raise KeyError(name)
+
+    def prepared(self, dtype):
+        """Static operands in the layouts the update reads, per dtype, each built
+        on first use and cached (Lov2 and Lovov share one tensor)."""
+        key = str(dtype)
+        if key not in self._prep:
+            self._prep[key] = _LazyOperands(self, dtype)
+        return self._prep[key]
+
+    def release_fp64_ladder(self):
+        """Free the FP64 ladder (v^4 doubles) once the reduced-precision operand
+        and its column sums are built: a solve that runs in FP32/BF16 only
+        needs those."""
+        import torch
+
+        if self.ladder is None:
+            return
+        v = self.nvir
+        self._s_ab = self.ladder.reshape(v * v, v * v).sum(0).reshape(v, v)
+        for d in list(self._prep):
+            if not isinstance(d, tuple) and "ladder" in self._prep[d]:
+                del self._prep[d]["ladder"]
+        self._prep.pop(("sparse", "ladder", "fp64"), None)
+        self.ladder = None
+        torch.cuda.empty_cache()

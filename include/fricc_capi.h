@@ -186,6 +186,16 @@ int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void
  * multiple of 128.  Tuning knob; no counterpart in the reference. */
 int fricc_set_bf16_view_width(int width);
 
+/* Scratch arenas.  The library keeps grow-only scratch (compression buffers,
+ * view slot tables, partial sums) per device in 64 numbered arenas; calls use
+ * the arena selected last (0 by default).  Work that can run concurrently on
+ * different streams -- solves served side by side -- must use different
+ * arenas; a recorded graph keeps the arena it was recorded with.
+ * fricc_workspace_release frees an arena's buffers (it waits for the device).
+ * No counterpart in the reference. */
+int fricc_workspace_select(int arena);
+int fricc_workspace_release(int arena);
+
 /* Device-to-device copy of `bytes` (a multiple of 16; 16-byte aligned device
  * pointers) by a plain streaming kernel: the reference point for achievable
  * HBM bandwidth and for scaling hardware traffic counters of the other
