@@ -32,7 +32,7 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
 
-def run_mode(eris, cfg, mode, steps, warmup, seed):
+def run_mode(eris, cfg, mode, steps, warmup, seed, solver=None):
     import torch
 
     from fricc.friccsd import FRICCSD, SectionTimer
@@ -40,7 +40,7 @@ def run_mode(eris, cfg, mode, steps, warmup, seed):
 
     settings = {"m_keep": cfg["m_keep"], "compression": "fri", "sampling_method": "systematic",
                 "precision": mode}
-    cc = FRICCSD(eris, settings, uniforms=GlibcUniforms(seed))
+    cc = (solver or FRICCSD)(eris, settings, uniforms=GlibcUniforms(seed))
     cc.level_shift = 2.0
     t1, t2 = cc.get_init_guess(eris)
     e = cc.energy(t1, t2, eris)
@@ -323,10 +323,15 @@ def serve_bench(args):
     sizes = [tuple(int(x) for x in s.split("x")) for s in args.serve_sizes.split(",")]
     reqs = sv.poisson_requests(args.serve, args.serve_rate, sizes, (0.01, 0.10), seed=args.seed)
     server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=16)
+    torch.cuda.empty_cache()
+    at_start = {"allocated": torch.cuda.memory_allocated(), "device_free": torch.cuda.mem_get_info()[0]}
+    print(f"# serving: memory at start {at_start}", file=sys.stderr, flush=True)
     res = server.run(reqs)
     st = sv.latency_stats(res, server.makespan)
     st.update({"sizes": [f"{o}x{v}" for o, v in sizes], "rate_per_s": args.serve_rate,
                "max_concurrent": server.max_concurrent, "hbm_budget_gb": 240,
+               "footprint_gb": {f"{o}x{v}": round(b / 1e9, 2) for (o, v), b in server.measured.items()},
+               "memory_at_start": at_start, "admission_log": server.log[:64],
                "per_size": {f"{o}x{v}": {"n": sum(1 for r in res if (r.nocc, r.nvir) == (o, v)),
                                          "p50_s": float(np.median([r.latency for r in res if (r.nocc, r.nvir) == (o, v)]
                                                                   or [float("nan")]))}
@@ -350,7 +355,9 @@ def main():
     ap.add_argument("--replica-precision", default="mixed")
     ap.add_argument("--serve", type=int, default=0, help="also serve N solves arriving as a Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
+    ap.add_argument("--serve-sizes", default="10x90,20x180,32x256,48x352")
+    ap.add_argument("--lccsd-modes", default="mixed,fp64", help="FRI-LCCSD iterations timed beside ('' for none)")
+    ap.add_argument("--lccsd-steps", type=int, default=3)
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_child:
@@ -379,6 +386,16 @@ def main():
         res[mode] = run_mode(eris, cfg, mode, args.steps, args.warmup, args.seed)
         print(f"# {mode}: {json.dumps({k: v for k, v in res[mode].items() if k not in ('energies', 'state', 'slots')})}",
               file=sys.stderr, flush=True)
+    lccsd = {}
+    if args.lccsd_modes:
+        from fricc.frilccsd import FRILCCSD
+
+        for mode in args.lccsd_modes.split(","):
+            rl = run_mode(eris, cfg, mode, args.lccsd_steps, 1, args.seed, solver=FRILCCSD)
+            lccsd[mode] = {"value": rl["it_per_s"], "ms_per_step": rl["ms_per_step"], "p50_ms": rl["p50_ms"],
+                           "E_corr_last": rl["energies"][-1]}
+            del rl
+            print(f"# FRI-LCCSD {mode}: {lccsd[mode]}", file=sys.stderr, flush=True)
     errs = {}
     t1s = t2s = None
     if "fp64" in res:
@@ -458,6 +475,9 @@ def main():
     if "fp64" in res and head != "fp64":
         out["speedup_vs_fp64"] = r["it_per_s"] / res["fp64"]["it_per_s"]
     out["precision_error_vs_fp64"] = errs
+    if lccsd:
+        out["frilccsd"] = {"unit": "iter/s", "modes": lccsd,
+                           "workload": "FRI-LCCSD iteration on the same system, m and uniforms"}
     out["cpu_baseline"] = None
     # host copies for the CPU leg, then free the GPU for the profiling passes
     t2_host = res[head]["state"][1].cpu().numpy().ravel()
@@ -472,6 +492,8 @@ def main():
 
     gc.collect()
     torch.cuda.empty_cache()
+    print(f"# memory after the timed modes: allocated {torch.cuda.memory_allocated() / 1e9:.2f} GB, "
+          f"device free {torch.cuda.mem_get_info()[0] / 1e9:.2f} GB", file=sys.stderr, flush=True)
     if not args.no_traffic and head == "mixed":
         tr, note = measure_traffic(args)
         print(f"# traffic: {tr} ({note})", file=sys.stderr, flush=True)

@@ -198,7 +198,7 @@ def _draw_uniforms(cc):
     t2_sparse_alt, fricc/friccsd.py:99-138); none for "largest"."""
     if cc.fri_settings["compression"] != "fri":
         return []
-    return [next(cc.uniforms) for _ in range(2 if _needs_alt(cc) else 1)]
+    return [next(cc.uniforms) for _ in range(cc._n_compressions())]
 
 
 def _set_uniforms(cc, us):
@@ -227,7 +227,7 @@ def update_amps(cc, t1, t2, eris):
     the energy, the amplitude-change norm and the compression statuses once."""
     _check_m_keep(cc, t2)
     _set_uniforms(cc, _draw_uniforms(cc))
-    t1new, t2new, red = _update_amps_device(cc, t1, t2, eris)
+    t1new, t2new, red = cc._device_step(t1, t2, eris)
     _finish_step(cc, t1new, t2new, red)
     return t1new, t2new
 
@@ -503,6 +503,21 @@ def amplitudes_to_vector_norm(t1new, t2new, t1, t2):
     return float(torch.sqrt(((t1new - t1) ** 2).sum() + (tri ** 2).sum()).item())
 
 
+_SIDE_STREAMS = {}
+
+
+def _side_stream():
+    """One warm-up stream per device, reused: the BLAS library keeps a
+    workspace for every stream it has run on, so a new stream per recording
+    would leave one more workspace behind each time."""
+    torch = _torch()
+    dev = torch.cuda.current_device()
+    st = _SIDE_STREAMS.get(dev)
+    if st is None:
+        st = _SIDE_STREAMS[dev] = torch.cuda.Stream()
+    return st
+
+
 class GraphStep:
     """``update_amps`` recorded once as a graph for this (nocc, nvir, m) shape
     and replayed every iteration: one launch of the whole step instead of a
@@ -515,7 +530,7 @@ class GraphStep:
     def __init__(self, cc, eris, t1, t2):
         torch = _torch()
         st = cc.fri_settings
-        if st["unbiased"] and _needs_alt(cc):
+        if cc._iteration_dependent():
             raise ValueError("graph mode needs unbiased averaging off")
         if getattr(cc, "timer", None) is not None:
             raise ValueError("graph mode cannot run with a section timer")
@@ -526,16 +541,16 @@ class GraphStep:
         # warm-up step on a side stream: allocates every workspace, view and
         # operand the step needs, so the recorded step allocates nothing
         # outside the graph's pool
-        side = torch.cuda.Stream()
+        side = _side_stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            _set_uniforms(cc, [0.5, 0.5])
-            _update_amps_device(cc, self.t1, self.t2, eris)
+            _set_uniforms(cc, [0.5] * cc._n_compressions())
+            cc._device_step(self.t1, self.t2, eris)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.out = _update_amps_device(cc, self.t1, self.t2, eris)
+            self.out = cc._device_step(self.t1, self.t2, eris)
         torch.cuda.synchronize()
 
     def issue(self):
@@ -710,6 +725,20 @@ class FRICCSD:
     def update_amps(self, t1, t2, eris):
         return update_amps(self, t1, t2, eris)
 
+    # ---- step hooks shared with GraphStep, replica batches and the server
+    def _device_step(self, t1, t2, eris):
+        """The device part of update_amps: (t1new, t2new, reductions), no host wait."""
+        return _update_amps_device(self, t1, t2, eris)
+
+    def _n_compressions(self):
+        """FRI compressions (uniforms) per step: t2_sparse, and t2_sparse_alt
+        when an O^3V^3 / O^4V^2 intermediate is compressed."""
+        return 2 if _needs_alt(self) else 1
+
+    def _iteration_dependent(self):
+        """True when a step's work depends on the iteration count (unbiased averaging)."""
+        return bool(self.fri_settings.get("unbiased")) and _needs_alt(self)
+
     def amplitudes_to_vector_norm(self, t1new, t2new, t1, t2):
         return amplitudes_to_vector_norm(t1new, t2new, t1, t2)
 
@@ -732,7 +761,7 @@ class FRICCSD:
     def _udev(self):
         """Device buffer of the step's uniforms (read by the compressions)."""
         if self._u_dev is None:
-            self._u_dev = _torch().zeros(2, dtype=_torch().float64, device="cuda")
+            self._u_dev = _torch().zeros(8, dtype=_torch().float64, device="cuda")
         return self._u_dev
 
     def _slot(self, k, dims):

@@ -1,0 +1,210 @@
+"""Device-resident FRI-LCCSD: ``FRILCCSD``, its ``update_amps`` and energy.
+
+Mirrors ``fricc/frilccsd.py`` of the reference: the closed-shell linearised
+CCSD of Taube & Bartlett, J. Chem. Phys. 130, 144112 (2009), eqs. 13 and 15,
+with T2 FRI-compressed for the 2323, 0101 and four O^3V^3 contractions
+(fricc/frilccsd.py:27-230), optionally one independent compression per
+compressed term (``"Independent Compressions"``, :233-457, drawn in the order
+2323, 0101, 1302, 1202, 1303, 1203) and optionally LCCD (``"LCCD"``: T1 held
+at zero).  The energy is ``lin_cc_energy`` (:460-476).  The iteration loop is
+FRICCSD's (the reference class inherits it, :479-519).
+
+The intermediates are the bare integral blocks, so every compressed term
+reads a static operand prepared once per system: the ladder (ac|bd) (the same
+operand FRI-CCSD uses), (ki|lj), (kc|ai) and (ki|ac).  The device step has
+FRI-CCSD's shape -- symmetrised terms into one buffer S, direct terms into
+``acc``, one fused finalize that adds S + S^T, divides by the denominators and
+reduces the energy and norm -- and records as one graph the same way.
+Precision modes as in FRICCSD.
+"""
+import copy
+import logging
+
+from . import _capi
+from .friccsd import (FRICCSD, _check_m_keep, _contract_into, _mm_splitk, _prepare, _set_uniforms, _torch, _vp,
+                      default_fri_settings as _ccsd_defaults, _finish_step, _draw_uniforms, _O3V3, _PREC_ID)
+
+log = logging.getLogger("fricc")
+
+default_fri_settings = {
+    "m_keep": 1000,
+    "compression": "fri",
+    "sampling_method": "systematic",
+    "verbose": False,
+    "compressed_contractions": ["O^2V^4", "O^3V^3", "O^4V^2"],
+    "LCCD": False,
+    "Independent Compressions": False,
+}
+
+# compressed terms in the order the independent variant draws their tensors
+# (fricc/frilccsd.py:333-342 builds them as a list, used as [0] .. [5])
+TERM_ORDER = ("2323", "0101", "1302", "1202", "1303", "1203")
+
+
+def update_amps(cc, t1, t2, eris):
+    """One FRI-LCCSD Jacobi step on the GPU (fricc/frilccsd.py:27-230 and
+    :233-457): uniforms drawn, device step, one read of the energy, norm and
+    compression statuses."""
+    _check_m_keep(cc, t2)
+    _set_uniforms(cc, _draw_uniforms(cc))
+    t1new, t2new, red = cc._device_step(t1, t2, eris)
+    _finish_step(cc, t1new, t2new, red)
+    return t1new, t2new
+
+
+def _update_amps_lccsd_device(cc, t1, t2, eris):
+    """Device part of the step; returns (t1new, t2new, {E2, |dt2|^2, E1, |dt1|^2})
+    without waiting for the GPU."""
+    torch = _torch()
+    st = cc.fri_settings
+    m_keep = int(st["m_keep"])
+    ccs = st["compressed_contractions"]
+    _capi.check(_capi.lib().fricc_workspace_select(int(cc.arena)))
+    sp = cc.sparse_precision
+    o, v = t1.shape
+    ov = o * v
+    f64 = torch.float64
+    cdt = cc.dense_dtype
+    P = eris.prepared(cdt)
+    P64 = eris.prepared(f64)
+    fock = eris.fock
+    mo_e_o = eris.mo_energy[:o]
+    mo_e_v = eris.mo_energy[o:] + cc.level_shift
+    fov = fock[:o, o:]
+    t1_in = t1.contiguous()
+    t2 = t2.contiguous()
+    lccd = bool(st["LCCD"])
+    t1 = torch.zeros_like(t1_in) if lccd else t1_in
+    t1c = t1.to(cdt)
+    t2c = t2.to(cdt) if cdt != f64 else t2
+    C = lambda a: a.to(f64) if cdt != f64 else a  # noqa: E731
+
+    # ---- compression: one tensor, or one per compressed term
+    comp = st["compression"]
+    ud = cc._udev()
+    if st["Independent Compressions"] and comp == "fri":
+        ts = {}
+        for k, term in enumerate(TERM_ORDER):
+            ts[term] = cc._slot(k, t2.shape).recompress_async(t2, m_keep, comp, ud[k:k + 1])
+        cc._used_slots = set(range(len(TERM_ORDER)))
+    else:
+        one = cc._slot(0, t2.shape).recompress_async(t2, m_keep, comp, ud[0:1])
+        ts = dict.fromkeys(TERM_ORDER, one)
+        cc._used_slots = {0}
+
+    Foo = fock[:o, :o].clone()
+    Fvv = fock[o:, o:].clone()
+    Foo.diagonal().sub_(mo_e_o)
+    Fvv.diagonal().sub_(mo_e_v)
+
+    # ---- T1 equation (fricc/frilccsd.py:126-154)
+    if lccd:
+        t1new = torch.zeros_like(t1)
+    else:
+        t1new = fov.clone()
+        t1new += t1 @ Fvv.T
+        t1new -= Foo.T @ t1
+        t1new += C(2 * (t1c.reshape(1, ov) @ P["ovvo"].reshape(ov, v * o)).reshape(v, o).T
+                   - torch.matmul(P["oovv"].reshape(o, ov, v), t1c.view(o, v, 1)).sum(0).reshape(o, v))
+        # 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] @ Z[(kcd),a]
+        t1new += C(_mm_splitk(t2c.reshape(o, -1), P["Z"].reshape(-1, v)))
+        ovoo = P["ovoo"]
+        Aoo = (-2 * ovoo.permute(2, 0, 3, 1) + ovoo.permute(0, 2, 3, 1)).reshape(o * o, o, v)
+        t1new += C(torch.bmm(Aoo, t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
+        fovc = fov.to(cdt)
+        t1new += C(2 * torch.matmul(fovc.view(o, 1, 1, v), t2c).sum(0).reshape(o, v)
+                   - torch.matmul(fovc.reshape(1, ov), t2c.reshape(o, ov, v)).reshape(o, v))
+
+    # ---- symmetrised T2 terms into S (fricc/frilccsd.py:168-224 "tmp + tmp^T")
+    # Term 8: sum_c (ia|bc) t1[j,c];  Term 7: - sum_k (ia|jk) t1[k,b]
+    S = torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v)).reshape(o, o, v, v)
+    S3 = S.reshape(o * o, v, v)
+    S3 -= torch.matmul(P["ovoo"].permute(0, 2, 1, 3), t1c).reshape(o * o, v, v)
+    # Term 3: Fvv[a,c] t2[i,j,c,b];  Term 2: - Foo[k,i] t2[k,j,a,b]
+    S3.baddbmm_(Fvv.to(cdt).expand(o * o, v, v), t2c.reshape(o * o, v, v))
+    S.reshape(o, o * v * v).addmm_(Foo.to(cdt).T, t2c.reshape(o, -1), alpha=-1)
+    # Term 6: O^3V^3 with Wvoov[a,k,i,c] = (kc|ai), Wvovo[a,k,c,i] = (ki|ac)
+    if "O^3V^3" in ccs:
+        Av = eris.sparse_operand("ovvo_p", sp)
+        Avo = eris.sparse_operand("oovv_p", sp)
+        for (term, spec, alpha), A, ws in zip(_O3V3, (Av, Avo, Av, Avo),
+                                               ((v, o, o, v), (v, o, v, o), (v, o, o, v), (v, o, v, o))):
+            _prepare(ts[term], term, ws, (o, o, v, v), sp)
+            _contract_into(spec, alpha, A, ws, ts[term], S, sp)
+    else:
+        Wvoov = P["ovvo_p"].permute(3, 0, 2, 1)   # [a,k,i,c]
+        Wvovo = P["oovv_p"].permute(3, 0, 1, 2)   # [a,k,c,i]
+        S += 2 * torch.einsum("akic,kjcb->ijab", Wvoov, t2c) - torch.einsum("akci,kjcb->ijab", Wvovo, t2c)
+        S -= torch.einsum("akic,kjbc->ijab", Wvoov, t2c) + torch.einsum("bkci,kjac->ijab", Wvovo, t2c)
+
+    # ---- direct T2 terms into acc (FP64): Term 4 (ladder) and Term 5 (0101)
+    acc = torch.zeros((o, o, v, v), dtype=f64, device=t2.device)
+    if "O^2V^4" in ccs:
+        _prepare(ts["2323"], "2323", (v, v, v, v), (o, o, v, v), sp)
+        _contract_into("abcd,ijcd->ijab", 1.0, eris.sparse_operand("ladder", sp), (v, v, v, v), ts["2323"], acc, sp)
+    else:
+        if eris.ladder is None:
+            raise RuntimeError("the uncompressed O^2V^4 term needs the FP64 ladder")
+        acc.reshape(o * o, v * v).addmm_(t2.reshape(o * o, v * v), P64["ladder"].reshape(v * v, v * v))
+    if "O^4V^2" in ccs:
+        _contract_into("klij,klab->ijab", 1.0, eris.sparse_operand("Woooo_b", sp), (o, o, o, o), ts["0101"], acc,
+                       sp)
+    else:
+        acc.reshape(o * o, v * v).addmm_(P64["Woooo_b"].reshape(o * o, o * o).T, t2.reshape(o * o, v * v))
+
+    # ---- denominators, linearised energy and norm: FP64, fused
+    eia = (mo_e_o[:, None] - mo_e_v).contiguous()
+    t1new = (t1new / eia).contiguous()
+    red = torch.empty(4, dtype=f64, device=t2.device)
+    zero = torch.zeros_like(t1new)  # the linearised energy has no t1*t1 part
+    _capi.check(_capi.lib().fricc_t2_finalize(
+        o, v, _vp(P64["ovovT"]), _vp(acc), _vp(S), _PREC_ID["fp32" if S.dtype == torch.float32 else "fp64"],
+        _vp(t1), None, _vp(eia), _vp(t2), _vp(P64["Lovov"]), _vp(zero), _vp(red), _capi.stream_ptr()))
+    red[2].copy_(2 * (fov * t1new).sum())
+    red[3].copy_(((t1new - t1_in) ** 2).sum())
+    return t1new, acc, red
+
+
+def energy(cc, t1, t2, eris):
+    """lin_cc_energy (fricc/frilccsd.py:460-476), FP64:
+    2 sum f_ia t_ia + sum t2_ijab [2 (ia|jb) - (ib|ja)]."""
+    o = t1.shape[0]
+    P = eris.prepared(_torch().float64)
+    e = 2 * (eris.fock[:o, o:] * t1).sum() + (t2 * P["Lovov"]).sum()
+    return float(e.item())
+
+
+class FRILCCSD(FRICCSD):
+    """FRI-LCCSD driver (fricc/frilccsd.py:479-519).  Settings: the reference's
+    keys ("LCCD", "Independent Compressions", ...) plus this package's
+    "precision", "seed" and "graph"."""
+
+    def __init__(self, mf, fri_settings=default_fri_settings, frozen=None, mo_coeff=None, mo_occ=None,
+                 uniforms=None):
+        settings = copy.deepcopy(fri_settings)
+        for k, val in default_fri_settings.items():
+            settings.setdefault(k, val)
+        # keys FRICCSD reads that the linearised update does not use
+        settings.setdefault("burn_in", _ccsd_defaults["burn_in"])
+        settings.setdefault("unbiased", False)
+        if settings["unbiased"]:
+            raise ValueError("FRILCCSD has no unbiased averaging")
+        super().__init__(mf, settings, frozen=frozen, mo_coeff=mo_coeff, mo_occ=mo_occ, uniforms=uniforms)
+
+    def update_amps(self, t1, t2, eris):
+        return update_amps(self, t1, t2, eris)
+
+    def energy(self, t1, t2, eris=None):
+        return energy(self, t1, t2, eris or self.ao2mo())
+
+    def _device_step(self, t1, t2, eris):
+        return _update_amps_lccsd_device(self, t1, t2, eris)
+
+    def _n_compressions(self):
+        return len(TERM_ORDER) if self.fri_settings["Independent Compressions"] else 1
+
+    def _iteration_dependent(self):
+        return False
+
+
+__all__ = ["FRILCCSD", "update_amps", "energy", "default_fri_settings", "TERM_ORDER"]

@@ -9,7 +9,7 @@ system's integrals and prepared operands.
 ``ReplicaBatch`` records one iteration of *every* replica -- each replica's
 whole device step (``friccsd._update_amps_device``) followed by the copy of its
 outputs into its inputs -- as a single graph, so one launch advances all R
-trajectories by one iteration.  Per iteration the host writes the R x 2
+trajectories by one iteration.  Per iteration the host writes the R x k
 uniforms with one copy from pinned memory, replays, and reads every replica's
 energy and norm in one transfer.  Each replica's arithmetic is exactly that
 of the replica run alone (same kernels, same order), so a replica's
@@ -22,7 +22,7 @@ import math
 
 import numpy as np
 
-from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, _needs_alt, _update_amps_device
+from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, _side_stream
 from .uniforms import PhiloxUniforms
 
 
@@ -44,10 +44,11 @@ def log_uniform_m(n, lo, hi, seed, cap=None):
 class ReplicaBatch:
     """R replicas of one system (``eris``) stepped by one graph launch per iteration.
 
-    settings: FRICCSD settings shared by all replicas (precision, compression,
-    ...); ``m_list`` and ``seeds`` give each replica's m_keep and Philox seed."""
+    settings: solver settings shared by all replicas (precision, compression,
+    ...); ``m_list`` and ``seeds`` give each replica's m_keep and Philox seed;
+    ``solver``: FRICCSD (default) or FRILCCSD."""
 
-    def __init__(self, eris, settings, m_list, seeds, level_shift=0.0):
+    def __init__(self, eris, settings, m_list, seeds, level_shift=0.0, solver=FRICCSD):
         torch = _torch()
         if len(m_list) != len(seeds):
             raise ValueError("one m and one seed per replica")
@@ -55,14 +56,15 @@ class ReplicaBatch:
         self.R = len(seeds)
         self.reps = []
         for m, seed in zip(m_list, seeds):
-            cc = FRICCSD(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
+            cc = solver(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
             cc.level_shift = level_shift
-            if cc.fri_settings["unbiased"] and _needs_alt(cc):
+            if cc._iteration_dependent():
                 raise ValueError("replica batches need the unbiased averaging off")
             self.reps.append(cc)
-        # all uniforms of one iteration: device R x 2, written from pinned host memory
-        self.u_dev = torch.zeros(self.R, 2, dtype=torch.float64, device="cuda")
-        self.u_host = torch.zeros(self.R, 2, dtype=torch.float64, pin_memory=True)
+        # all uniforms of one iteration: device R x k, written from pinned host memory
+        nu = max(2, max(cc._n_compressions() for cc in self.reps))
+        self.u_dev = torch.zeros(self.R, nu, dtype=torch.float64, device="cuda")
+        self.u_host = torch.zeros(self.R, nu, dtype=torch.float64, pin_memory=True)
         for r, cc in enumerate(self.reps):
             cc._u_dev = self.u_dev[r]
         t1, t2 = self.reps[0].get_init_guess(eris)
@@ -75,19 +77,19 @@ class ReplicaBatch:
 
     def _record(self):
         torch = _torch()
-        side = torch.cuda.Stream()
+        side = _side_stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):  # warm-up: every workspace, view and operand allocated
             self.u_dev.fill_(0.5)
             for r, cc in enumerate(self.reps):
-                _update_amps_device(cc, self.t1[r], self.t2[r], self.eris)
+                cc._device_step(self.t1[r], self.t2[r], self.eris)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         self.out = []
         with torch.cuda.graph(self.graph):
             for r, cc in enumerate(self.reps):
-                t1n, t2n, red = _update_amps_device(cc, self.t1[r], self.t2[r], self.eris)
+                t1n, t2n, red = cc._device_step(self.t1[r], self.t2[r], self.eris)
                 self.out.append((t1n, t2n, red))
             self.red = torch.stack([o[2] for o in self.out])
             for r, (t1n, t2n, _) in enumerate(self.out):
@@ -131,9 +133,9 @@ class ReplicaBatch:
         return {"mean": float(x.mean()), "stderr": float(se), "n_replicas": len(x), "n_iter": len(e)}
 
 
-def run_alone(eris, settings, m, seed, n_iter, level_shift=0.0):
+def run_alone(eris, settings, m, seed, n_iter, level_shift=0.0, solver=FRICCSD):
     """The same replica run by itself, eagerly (reference for the batch)."""
-    cc = FRICCSD(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
+    cc = solver(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
     cc.level_shift = level_shift
     t1, t2 = cc.get_init_guess(eris)
     en = []

@@ -86,26 +86,62 @@ def poisson_requests(n, rate, sizes, mfrac=(0.01, 0.10), seed=0, max_cycle=50, t
 def estimate_bytes(o, v, precision="mixed"):
     """HBM of one admitted solve in steady state: the FP64 integral blocks it
     keeps, the operands prepared from them, the ladder (FP64 in the FP64 mode,
-    else only its BF16 planes: the FP64 copy is released after they are built),
-    amplitudes and graph temporaries, views and scratch.  Admission also needs
-    room for the FP64 ladder while it is built (v^4 doubles)."""
+    else only its BF16 planes, built slice by slice from the factors),
+    amplitudes and graph temporaries, views and scratch."""
     ov3, o2v2, v4 = float(o) * v ** 3, float(o * v) ** 2, float(v) ** 4
     if precision == "fp64":
-        return 8.0 * v4 + 64.0 * ov3 + 300.0 * o2v2 + 1.0e9
-    return 4.0 * v4 + 32.0 * ov3 + 260.0 * o2v2 + 1.0e9
+        return 8.0 * v4 + 64.0 * ov3 + 340.0 * o2v2 + 2.0e9
+    return 4.0 * v4 + 32.0 * ov3 + 300.0 * o2v2 + 2.0e9
+
+
+def transient_bytes(o, v, precision="mixed"):
+    """Extra HBM while a solve is set up: one FP64 ladder slice and its
+    converted copy, and the FP64 source of one reduced-precision operand."""
+    if precision == "fp64":
+        return 8.0 * float(o) * v ** 3 + 0.5e9
+    return 2.0 * (1 << 31) + 8.0 * float(o) * v ** 3 + 0.5e9
+
+
+def device_available():
+    """Bytes held by nothing: free device memory plus what the caching
+    allocator holds unused (a footprint is the drop of this figure)."""
+    torch = _torch()
+    free, _ = torch.cuda.mem_get_info()
+    return free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+
+
+def device_free(need):
+    """Free device memory for a request of ``need`` bytes.  Memory the caching
+    allocator holds unused may be split into pieces no large block fits in,
+    so only memory the device reports free counts; when that is short the
+    allocator's unused segments are returned to the device first."""
+    torch = _torch()
+    free, _ = torch.cuda.mem_get_info()
+    if free < need:
+        torch.cuda.empty_cache()
+        free, _ = torch.cuda.mem_get_info()
+    return free
 
 
 class Solve:
-    def __init__(self, req, arena, precision):
+    """One admitted request: its system, solver, recorded step and stream.
+    ``stream``: the stream to run on (a new one when None; the server reuses
+    one per arena, since the BLAS library keeps a workspace per stream it has
+    seen)."""
+
+    def __init__(self, req, arena, precision, stream=None):
         torch = _torch()
         from .systems import DeviceERIs, SyntheticSystem
 
         self.req = req
-        self.stream = torch.cuda.Stream()
+        self.stream = stream if stream is not None else torch.cuda.Stream()
         naux = req.naux or 3 * (req.nocc + req.nvir)
         with torch.cuda.stream(self.stream):
             sysm = SyntheticSystem(req.nocc, req.nvir, naux, req.sys_seed)
-            self.eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
+            # reduced-precision solves never hold the FP64 ladder: its operand
+            # is built slice by slice from the factors
+            self.eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc,
+                                                build_ladder=(precision == "fp64"))
             cc = FRICCSD(self.eris, {"m_keep": int(req.m), "compression": "fri", "sampling_method": "systematic",
                                      "precision": precision}, uniforms=PhiloxUniforms(req.seed))
             cc.level_shift = req.level_shift
@@ -114,8 +150,6 @@ class Solve:
             t1, t2 = cc.get_init_guess(self.eris)
             self.e = cc.energy(t1, t2, self.eris)
             self.runner = GraphStep(cc, self.eris, t1, t2)
-            if precision != "fp64":
-                self.eris.release_fp64_ladder()
         self.energies = []
         self.done = False
         self.converged = False
@@ -145,12 +179,73 @@ class Solve:
 
 
 class Server:
-    """Admission by HBM budget, per-solve streams and arenas, continuous batching."""
+    """Admission by HBM budget, per-solve streams and arenas, continuous batching.
+
+    A request is admitted while (a) the admitted solves' footprints plus its
+    own stay within ``hbm_budget`` and (b) the device still has room for its
+    footprint and set-up transients.  A footprint is the larger of
+    ``estimate_bytes`` and the most the device lost to an earlier set-up of
+    the same shape (measured around ``Solve``)."""
+
+    margin = 2.0e9  # device memory left free beyond each admission's need
 
     def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8):
         self.budget = float(hbm_budget)
         self.precision = precision
         self.max_active = min(int(max_active), 63)
+        self.measured = {}
+        self._streams = {}  # one stream per arena, reused by the solves that get the arena
+        self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
+
+    def _need(self, req):
+        key = (req.nocc, req.nvir)
+        return max(estimate_bytes(req.nocc, req.nvir, self.precision), self.measured.get(key, 0.0))
+
+    def _reclaim(self, idle):
+        """Free the library scratch of idle arenas (grow-only buffers a retired
+        solve left behind).  Waits for the device once."""
+        from . import _capi
+
+        torch = _torch()
+        rel = sorted(self._dirty & set(idle))
+        for a in rel:
+            _capi.check(_capi.lib().fricc_workspace_release(a))
+            self._dirty.discard(a)
+        torch.cuda.empty_cache()
+        return bool(rel)
+
+    def _fits(self, req, used, active, idle):
+        need = self._need(req)
+        if active and used + need > self.budget:
+            return False
+        want = need + transient_bytes(req.nocc, req.nvir, self.precision) + self.margin
+        if device_free(want) >= want:
+            return True
+        if self._reclaim(idle) and _torch().cuda.mem_get_info()[0] >= want:
+            return True
+        return not active
+
+    def _stream(self, arena):
+        st = self._streams.get(arena)
+        if st is None:
+            st = self._streams[arena] = _torch().cuda.Stream()
+        return st
+
+    def _try_solve(self, req, arena):
+        """Set up a solve; None when the device ran out of memory doing so
+        (its partial allocations are released)."""
+        torch = _torch()
+        try:
+            return Solve(req, arena, self.precision, stream=self._stream(arena))
+        except (torch.OutOfMemoryError, MemoryError, RuntimeError) as e:
+            if isinstance(e, RuntimeError) and "out of memory" not in str(e) and "hipMalloc" not in str(e):
+                raise
+        import gc
+
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        return None
 
     def run(self, requests, clock=time.perf_counter):
         torch = _torch()
@@ -160,25 +255,50 @@ class Server:
         free_arenas = list(range(1, self.max_active + 1))
         active, results, used = [], [], 0.0
         self.max_concurrent = 0
+        self.deferred = 0
+        self._dirty = set()  # arenas whose scratch may hold memory
+        wait_retire = False  # a set-up ran out of memory: admit again after a retirement
         t0 = clock()
         while pending or active:
             now = clock() - t0
-            while pending and pending[0].arrival <= now and free_arenas:
-                need = estimate_bytes(pending[0].nocc, pending[0].nvir, self.precision)
-                if active and used + need > self.budget:
+            while pending and pending[0].arrival <= now and free_arenas and not wait_retire:
+                if not self._fits(pending[0], used, active, free_arenas):
                     break
-                req = pending.pop(0)
-                s = Solve(req, free_arenas.pop(0), self.precision)
+                req = pending[0]
+                before = device_available()
+                arena = free_arenas.pop(0)
+                self._dirty.add(arena)
+                s = self._try_solve(req, arena)
+                if s is None and not active and self._reclaim(free_arenas + [arena]):
+                    before = device_available()
+                    self._dirty.add(arena)
+                    s = self._try_solve(req, arena)
+                if s is None:
+                    free_arenas.insert(0, arena)
+                    if not active:
+                        raise MemoryError(f"solve {req.id} ({req.nocc}x{req.nvir}) does not fit on the device alone")
+                    key = (req.nocc, req.nvir)
+                    self.measured[key] = max(self.measured.get(key, 0.0), 1.25 * self._need(req))
+                    self.deferred += 1
+                    self.log.append((round(clock() - t0, 3), "deferred", req.id, req.nocc, req.nvir, None, used,
+                                     torch.cuda.mem_get_info()[0]))
+                    wait_retire = True
+                    break
+                pending.pop(0)
                 s.admitted = clock() - t0
-                s.need = need
-                used += need
+                key = (req.nocc, req.nvir)
+                self.measured[key] = max(self.measured.get(key, 0.0), float(before - device_available()))
+                s.need = self._need(req)
+                used += s.need
+                self.log.append((round(s.admitted, 3), "admit", req.id, req.nocc, req.nvir, s.need, used,
+                                 _torch().cuda.mem_get_info()[0]))
                 s.issue()
                 active.append(s)
                 self.max_concurrent = max(self.max_concurrent, len(active))
             if not active:
                 time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
-            progressed = False
+            progressed = retired = False
             for s in list(active):
                 if not s.ready():
                     continue
@@ -189,12 +309,16 @@ class Server:
                                           len(s.energies), s.converged, list(s.energies)))
                     active.remove(s)
                     used -= s.need
+                    self.log.append((round(fin, 3), "retire", s.req.id, s.req.nocc, s.req.nvir, s.need, used, None))
                     free_arenas.append(s.cc.arena)
                     free_arenas.sort()
-                    del s
-                    torch.cuda.empty_cache()
+                    wait_retire = False
+                    retired = True
                 else:
                     s.issue()
+            s = None
+            if retired:
+                torch.cuda.empty_cache()  # the retired solves' buffers back to the device
             if not progressed:
                 time.sleep(5e-5)
         torch.cuda.synchronize()
@@ -212,10 +336,16 @@ def latency_stats(results, makespan):
             "converged": int(sum(r.converged for r in results)), "makespan_s": makespan}
 
 
+_ALONE = {}
+
+
 def solve_alone(req, precision="mixed"):
     """The request run by itself (same recorded step), for comparison."""
     torch = _torch()
-    s = Solve(req, 0, precision)
+    dev = torch.cuda.current_device()
+    if dev not in _ALONE:
+        _ALONE[dev] = torch.cuda.Stream()
+    s = Solve(req, 0, precision, stream=_ALONE[dev])
     s.issue()
     while True:
         s.stream.synchronize()

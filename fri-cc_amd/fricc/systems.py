@@ -12,6 +12,7 @@ which is not available here.  The benchmark systems are synthetic:
 * Fock = diag(orbital energies), so f_ov = 0 and the MP2 guess has t1 = 0.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -79,18 +80,20 @@ def bf16_planes(A, R, X, stream=None):
 
 
 class _LazyOperands(dict):
-    """Operands of one dtype, built on first access."""
+    """Operands of one dtype, built on first access.  Holds its integrals
+    weakly: the integrals own this cache, and a reference cycle would keep
+    every device buffer of a dropped system alive until a cyclic collection."""
 
     def __init__(self, eris, dtype):
         super().__init__()
-        self.eris, self.dtype = eris, dtype
+        self._eris, self.dtype = weakref.ref(eris), dtype
 
     def __missing__(self, name):
         alias = {"Lovov": "Lov2", "Lov2": "Lovov"}.get(name)
         if alias is not None and dict.__contains__(self, alias):
             val = dict.__getitem__(self, alias)
         else:
-            val = self.eris._build(name, self.dtype)
+            val = self._eris()._build(name, self.dtype)
         self[name] = val
         return val
 
@@ -111,6 +114,8 @@ class DeviceERIs:
     def __init__(self):
         self._prep = {}
         self._s_ab = None
+        self.ladder = None
+        self.L = None
 
     @classmethod
     def from_factors(cls, L, mo_energy, nocc, fock=None, build_ladder=True):
@@ -181,6 +186,8 @@ class DeviceERIs:
             return self._prep[key]
         if precision == "fp64":
             out = self.ladder if name == "ladder" else self.prepared(torch.float64)[name]
+        elif name == "ladder" and self.ladder is None and self.L is not None:
+            out = self._reduced_ladder(precision)
         else:
             A = self.ladder if name == "ladder" else self._build(name, torch.float64)
             if A is None:
@@ -231,20 +238,68 @@ class DeviceERIs:
             return c(self.oovv.permute(0, 3, 1, 2))     # [k,c,i,a] = (ki|ac)
         if name in ("ovoo", "oovv", "ovvo", "oooo"):
             return c(getattr(self, name))
+        if name == "Woooo_b":
+            return c(self.oooo.permute(0, 2, 1, 3))     # [k,l,i,j] = (ki|lj)
         if name == "ladder":
             if dtype != torch.float64 or self.ladder is None:
                 raise KeyError("the FP64 ladder operand is not available")
             return self.ladder
         if name == "s_ab":
             # sum over (c,d) of the bare part of Wvvvv, (ac|bd)
-            if self.ladder is None:
-                if self._s_ab is None:
-                    raise KeyError("s_ab needs the ladder")
-                return self._s_ab.to(dtype)
-            return self.ladder.reshape(v * v, v * v).sum(0).reshape(v, v).to(dtype)
+            if self.ladder is not None:
+                return self.ladder.reshape(v * v, v * v).sum(0).reshape(v, v).to(dtype)
+            if self._s_ab is None:
+                if self.L is None:
+                    raise KeyError("s_ab needs the ladder or the factors")
+                # sum_cd sum_Q L[Q,a,c] L[Q,b,d] = sum_Q M[Q,a] M[Q,b], M = sum_c L[Q,a,c]
+                M = self.L[:, o:, o:].sum(2)
+                self._s_ab = (M.T @ M).contiguous()
+            return self._s_ab.to(dtype)
         if name == "g_ka":
             return ovvv.sum(dim=(1, 3)).to(dtype)       # sum_cd (kd|ac) = sum_cd (kc|ad)
         raise KeyError(name)
+
+    def _reduced_ladder(self, precision, chunk_bytes=1 << 31):
+        """BF16 planes (or FP32 copy) of the ladder W[c,d,a,b] = (ac|bd) built
+        from the factors in slices of c, so the FP64 ladder (v^4 doubles) never
+        exists whole: each slice's FP64 rows are evaluated, converted into
+        their place in the operand and dropped.  The elements are those of the
+        whole-ladder build (each integral is the same FMA chain, each
+        conversion elementwise)."""
+        import torch
+
+        o, v, nmo = self.nocc, self.nvir, self.nmo
+        X = v * v
+        cc = max(1, int(chunk_bytes // (8 * v ** 3)))
+        while (cc * v) % 8 and cc < v:  # slice rows cover whole 8-row groups
+            cc += 1
+        cc = min(cc, v)
+        dev = self.L.device
+        if precision == "fp32":
+            out = torch.empty((v, v, v, v), dtype=torch.float32, device=dev)
+        else:
+            G = (X + 7) // 8
+            out = torch.empty(2 * G * X * 8, dtype=torch.int16, device=dev)
+            hi, lo = out[: G * X * 8].view(G, X * 8), out[G * X * 8:].view(G, X * 8)
+        Wc = torch.empty((cc, v, v, v), dtype=torch.float64, device=dev)
+        for c0 in range(0, v, cc):
+            c1 = min(v, c0 + cc)
+            n = c1 - c0
+            W = Wc[:n]
+            # W[c,d,a,b] = (ac|bd): p=a, q=c, r=b, s=d
+            eri_block(self.L, self.naux, nmo, ((o, nmo), (o + c0, o + c1), (o, nmo), (o, nmo)), W,
+                      [v, v ** 3, 1, v * v])
+            rows = W.view(n * v, X)
+            if precision == "fp32":
+                out.view(X, X)[c0 * v:c1 * v].copy_(rows)
+            else:
+                part = bf16_planes(rows, n * v, X)
+                g0, gn = (c0 * v) // 8, (n * v + 7) // 8
+                hi[g0:g0 + gn].copy_(part[: gn * X * 8].view(gn, X * 8))
+                lo[g0:g0 + gn].copy_(part[gn * X * 8:].view(gn, X * 8))
+                del part
+        del Wc, W, rows
+        return out
 
     def prepared(self, dtype):
         """Static operands in the layouts the update reads, per dtype, each built

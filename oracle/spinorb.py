@@ -140,3 +140,58 @@ def energy(T1, T2, asym, f, nocc):
     g = asym[o, o, v, v]
     return float(np.einsum("ia,ia", f[o, v], T1) + 0.25 * np.einsum("ijab,ijab", g, T2)
                  + 0.5 * np.einsum("ijab,ia,jb", g, T1, T1))
+
+
+def update_linear(T1, T2, asym, f, nocc, lccd=False):
+    """One Jacobi step of linearised CCSD (every term of ``update`` that is
+    at most linear in T1 and T2; Taube & Bartlett, J. Chem. Phys. 130, 144112
+    (2009), eqs. 13 and 15).  ``lccd``: T1 is held at zero (LCCD)."""
+    so = 2 * nocc
+    o = slice(0, so)
+    v = slice(so, None)
+    fov = f[o, v]
+    foo = f[o, o]
+    fvv = f[v, v]
+    nv = fvv.shape[0]
+    if lccd:
+        T1 = np.zeros_like(T1)
+    Fae = fvv * (1 - np.eye(nv))
+    Fmi = foo * (1 - np.eye(so))
+
+    r1 = fov.copy()
+    r1 += np.einsum("ie,ae->ia", T1, Fae)
+    r1 -= np.einsum("ma,mi->ia", T1, Fmi)
+    r1 += np.einsum("imae,me->ia", T2, fov)
+    r1 -= np.einsum("nf,naif->ia", T1, asym[o, v, o, v])
+    r1 -= 0.5 * np.einsum("imef,maef->ia", T2, asym[o, v, v, v])
+    r1 -= 0.5 * np.einsum("mnae,mnie->ia", T2, asym[o, o, o, v])
+
+    r2 = asym[o, o, v, v].copy()
+    x = np.einsum("ijae,be->ijab", T2, Fae)
+    r2 += x - x.transpose(0, 1, 3, 2)
+    x = np.einsum("imab,mj->ijab", T2, Fmi)
+    r2 -= x - x.transpose(1, 0, 2, 3)
+    r2 += 0.5 * np.einsum("mnab,mnij->ijab", T2, asym[o, o, o, o])
+    r2 += 0.5 * np.einsum("ijef,abef->ijab", T2, asym[v, v, v, v])
+    x = np.einsum("imae,mbej->ijab", T2, asym[o, v, v, o])
+    r2 += x - x.transpose(0, 1, 3, 2) - x.transpose(1, 0, 2, 3) + x.transpose(1, 0, 3, 2)
+    x = np.einsum("ie,abej->ijab", T1, asym[v, v, v, o])
+    r2 += x - x.transpose(1, 0, 2, 3)
+    x = np.einsum("ma,mbij->ijab", T1, asym[o, v, o, o])
+    r2 -= x - x.transpose(0, 1, 3, 2)
+
+    eo = np.diag(foo)
+    ev = np.diag(fvv)
+    D1 = eo[:, None] - ev[None, :]
+    D2 = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+    if lccd:
+        return np.zeros_like(T1), r2 / D2
+    return r1 / D1, r2 / D2
+
+
+def energy_linear(T1, T2, asym, f, nocc):
+    """Linearised CC energy: sum f_ia t_ia + 1/4 sum <ij||ab> t_ijab."""
+    so = 2 * nocc
+    o = slice(0, so)
+    v = slice(so, None)
+    return float(np.einsum("ia,ia", f[o, v], T1) + 0.25 * np.einsum("ijab,ijab", asym[o, o, v, v], T2))

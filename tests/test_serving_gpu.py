@@ -51,3 +51,40 @@ def test_budget_limits_admission(serving):
     # served one after another in arrival order
     fin = [r.finished for r in res]
     assert fin == sorted(fin)
+
+
+def test_dropped_solve_frees_its_memory(serving):
+    """A retired solve's device buffers go back as soon as it is dropped (no
+    reference cycle holds them until a cyclic collection), and a second solve
+    on the same stream leaves nothing behind either (per-stream library
+    workspaces are created once)."""
+    import gc
+
+    import torch
+
+    sv = serving
+    req = sv.Request(0, 0.0, 6, 40, m=5000, seed=3, sys_seed=2, max_cycle=2, tol=0.0)
+    st = torch.cuda.Stream()
+
+    def run_once():
+        s = sv.Solve(req, 1, "mixed", stream=st)
+        s.issue()
+        s.stream.synchronize()
+        s.complete()
+        return s
+
+    del_first = run_once()
+    del del_first
+    gc.collect()
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    gc.disable()
+    try:
+        s = run_once()
+        held = torch.cuda.memory_allocated() - base
+        assert held > (8 << 20), held
+        del s
+        torch.cuda.synchronize()
+        assert torch.cuda.memory_allocated() - base <= (64 << 10), (held, torch.cuda.memory_allocated() - base)
+    finally:
+        gc.enable()
