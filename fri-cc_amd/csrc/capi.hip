@@ -149,7 +149,7 @@ static int bf16_operand_from_w(const Plan& p, const double* W, const int64_t W_s
   void* A = nullptr;
   void* A2 = nullptr;
   FRICC_TRY(ws_get(30, (size_t)R * X * 4, &A));
-  FRICC_TRY(ws_get(31, (size_t)((R + 7) / 8) * X * 8 * 2 * 2, &A2));
+  FRICC_TRY(ws_get(31, (size_t)R * bf16_xblocks(X) * 512 * 2, &A2));
   const int64_t st[4] = {W_strides[p.wpos[0]], W_strides[p.wpos[1]], W_strides[p.wpos[2]], W_strides[p.wpos[3]]};
   FRICC_TRY(permute_operand(W, p.Rh, p.Rl, p.Xh, p.Xl, st, A, DT_F32, s));
   FRICC_TRY(split_bf16(A, DT_F32, R, X, (uint16_t*)A2, s));

@@ -243,20 +243,33 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
 // ---------------------------------------------------------------------------
 // BF16 gathered contraction on v_mfma_f32_16x16x32_bf16 (FP32 accumulate).
 //
-// A comes as hi/lo bf16 planes [ceil(R/8)][X][8]; B (the view) as FP32
-// [group][TY][8] and is split into hi = bf16(b), lo = bf16(b - hi) while it is
-// staged.  npass 3: hi*hi + hi*lo + lo*hi (about 16 significant bits per
-// product), npass 1: hi*hi.
+// A comes as 1 KB records [X/256][R][hi 256 | lo 256] bf16 (X rounded up to
+// a multiple of 256, pad columns zero): the block of columns one workgroup
+// reads of row r is one contiguous record, and the rows of one column block
+// lie together, so the rows a K step gathers stay close in memory.  B (the
+// view) comes as hi/lo bf16 planes [k/8][TY][8] over the tile's gathered
+// rows k.  npass 3: hi*hi + hi*lo + lo*hi (about 16
+// significant bits per product), npass 1: hi*hi.
+//
+// Rows are gathered one by one (not in fixed groups of consecutive rows), so
+// a tile's K range holds only rows that carry entries of that tile.  Per K
+// step the block stages the 32 gathered rows of A, columns [x0, +BX), as
+// row-major LDS rows (one 1 KB DMA per row: hi then lo), and B's four
+// 8-row groups.  The MFMA A fragment wants 8 consecutive k per lane for one
+// x; ds_read_b64_tr_b16 delivers exactly that from k-major LDS rows: within
+// each 16-lane group, lane t receives element t%4 of the 8-byte chunks that
+// lanes t/4, t/4+4, t/4+8, t/4+12 address, so lane s addressing row
+// k0 + s/4, columns 4*(s%4).. hands lane t the column t of rows k0..k0+3.
+// Two such reads give rows k0..k0+7.
 //
 // A workgroup (4 waves) owns a BX x BY output block: x in [bx*BX, +BX), y a
-// BY-wide slice of one view tile.  Per K step (KG gathered groups = 32 rows =
-// one MFMA K) the whole block stages A[KG][BX] and B[KG][BY] (hi and lo) in
-// LDS, each element fetched from global memory once per block; the next
-// step's global loads are in flight while the waves run their MFMAs from LDS.
-// Waves are 2 (x) x 2 (y); each owns 128 x 64 = 8 x 4 MFMA blocks.
-// Consecutive blocks on one XCD are the BY slices of one tile (same gathered
-// A rows, shared through that XCD's L2).
+// BY-wide slice of one view tile; waves are 2 (x) x 2 (y), each owning
+// 128 x 64 = 8 x 4 MFMA blocks.  Consecutive blocks on one XCD are the BY
+// slices of one tile (same gathered A rows, shared through that XCD's L2).
 typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s lds_v4s;
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
 
 __device__ inline bf8 as_bf8(uint4 u) {
   bf8 r;
@@ -264,16 +277,32 @@ __device__ inline bf8 as_bf8(uint4 u) {
   return r;
 }
 
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+__device__ inline bf8 cat_bf8(v4s a, v4s b) {
+  const v8s c = __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf8, c);
+}
+
 constexpr int BX = 256;
-constexpr int KG = 4;
+constexpr int KG = 4;                  // 8-row groups of B per K step
+constexpr int KR = 32;                 // gathered rows per K step (one MFMA K)
+constexpr int A_ROW = 2 * BX * 2;      // staged bytes of one A row: hi | lo
+constexpr int A_PITCH = A_ROW + 32;    // LDS pitch of a staged A row
+// byte offset of staged row k: rows k and k+8 would meet the same banks at
+// this pitch, so each group of 8 rows starts 128 bytes (half the banks)
+// further on than the pitch alone would place it
+constexpr int A_GROUP = 8 * A_PITCH + 128;
+__device__ constexpr uint32_t a_row_off(int k) { return (uint32_t)((k >> 3) * A_GROUP + (k & 7) * A_PITCH); }
+constexpr int A_STAGE = (KR / 8) * A_GROUP;
+static_assert(7 * A_PITCH + A_ROW <= A_GROUP, "staged rows must not overlap");
 
 struct B16Args {
-  const uint4* Ahi;
-  const uint4* Alo;
-  int64_t X, Y;
+  const uint16_t* A;  // [X/256][R][hi 256 | lo 256]
+  int64_t R, X, Y;
   const uint32_t* tile_off;
   const uint32_t* glist;
-  const uint4* Bhi;  // view planes [group][TY][8] bf16
+  const uint4* Bhi;  // view planes [k/8][TY][8] bf16
   const uint4* Blo;
   int64_t TY;
   int64_t nbx, nyb;
@@ -292,7 +321,7 @@ typedef __attribute__((address_space(1))) void gbl_void;
 // DMA, the DMA) so that the compiler does not serialise the pipeline with
 // waits of its own; the block has no register outputs, and completion is
 // awaited explicitly with wait_vm before the data is read.
-__device__ inline void dma16(const uint4* g, uint32_t lds) {
+__device__ inline void dma16(const void* g, uint32_t lds) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g),
                "s"(__builtin_amdgcn_readfirstlane(lds))
                : "m0");
@@ -309,7 +338,7 @@ __device__ inline void wait_vm() {
 
 template <int BYB>
 struct StageMem {
-  uint4 A[2][KG][BX];   // [hi|lo][group][x]
+  uint8_t A[A_STAGE];   // staged rows [k]: hi BX | lo BX (bf16), at a_row_off(k)
   uint4 B[2][KG][BYB];  // [hi|lo][group][y]
 };
 
@@ -318,44 +347,41 @@ struct StageMem {
 // while the kernel runs, so its loads stay scalar (no vector-memory wait).
 typedef const __attribute__((address_space(4))) uint32_t cu32;
 struct StageSrc {
-  const uint4* Ahi;
-  const uint4* Alo;
+  const uint16_t* A;  // this lane's 16 bytes of record (column block, row 0)
   const uint4* Bhi;
   const uint4* Blo;
   cu32* glist;  // row list (constant address space)
-  int64_t X, TY;
+  int64_t TY;
 };
 
-// One pipeline stage: the KG gathered groups starting at k, A columns
-// [x0, +BX) and B columns [bcol, +BYB) of the tile, hi and lo planes.  Wave w
-// loads A column chunk w (64 columns) of every group and plane, and B column
-// chunk w%2 of groups w/2 and w/2+2; all selections are compile-time except
-// the wave's own offsets, so nothing is indexed through memory.
+// One pipeline stage: gathered rows [k, k+32) of the tile, the A record of
+// the workgroup's column block and B columns [bcol, +BYB), hi and lo planes.
+// Wave w stages A rows 8w..8w+7 (one 1 KB record per DMA: lanes 0-31 the hi
+// half, 32-63 the lo half) and B column chunk w%2 of groups w/2 and w/2+2.
 template <int NPASS, int BYB>
-__device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t x0, int64_t bcol, StageMem<BYB>& sm, int w,
-                                   int lane) {
-  static_assert(BX == 256 && BYB == 128 && KG == 4, "wave assignment assumes 4 waves, BX 256, BYB 128");
-  const uint32_t g0 = p.glist[k], g1 = p.glist[k + 1], g2 = p.glist[k + 2], g3 = p.glist[k + 3];
-  const uint32_t la = lds_addr(&sm.A[0][0][0]), lb = lds_addr(&sm.B[0][0][0]);
-  int64_t x = x0 + w * 64 + lane;
-  if (x >= p.X) x = p.X - 1;
-  const uint32_t ga[4] = {g0, g1, g2, g3};
+__device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t bcol, StageMem<BYB>& sm, int w, int lane) {
+  static_assert(BX == 256 && BYB == 128 && KG == 4 && KR == 32, "wave assignment assumes 4 waves, BX 256, BYB 128");
+  const uint32_t la = lds_addr(&sm.A[0]), lb = lds_addr(&sm.B[0][0][0]);
+  cu32* rl = p.glist + k + 8 * w;
+  const uint32_t r0 = rl[0], r1 = rl[1], r2 = rl[2], r3 = rl[3], r4 = rl[4], r5 = rl[5], r6 = rl[6], r7 = rl[7];
+  const uint32_t rr[8] = {r0, r1, r2, r3, r4, r5, r6, r7};
 #pragma unroll
-  for (int gl = 0; gl < KG; gl++) {
-    const int64_t off = (int64_t)ga[gl] * p.X + x;
-    const uint32_t l = la + (uint32_t)((gl * BX + w * 64) * 16);
-    dma16(p.Ahi + off, l);
-    if (NPASS > 1) dma16(p.Alo + off, l + (uint32_t)(KG * BX * 16));
-  }
+  for (int i = 0; i < 8; i++) dma16(p.A + (int64_t)rr[i] * (2 * BX), la + a_row_off(8 * w + i));
   const int yc = w & 1;
+  const uint32_t kg = k >> 3;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int gl = (w >> 1) + 2 * h;
-    const int64_t off = (int64_t)(k + gl) * p.TY + bcol + yc * 64 + lane;
+    const int64_t off = (int64_t)(kg + gl) * p.TY + bcol + yc * 64 + lane;
     const uint32_t l = lb + (uint32_t)((gl * BYB + yc * 64) * 16);
     dma16(p.Bhi + off, l);
     if (NPASS > 1) dma16(p.Blo + off, l + (uint32_t)(KG * BYB * 16));
   }
+}
+
+template <int BYB>
+__device__ inline v4s tr_read(const StageMem<BYB>& sm, uint32_t off) {
+  return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s*)((lds_u8*)(lds_void*)&sm.A[0] + off));
 }
 
 // MFMAs of one stage: wave (wx, wy) owns x [wx*128, +128) and y [wy*BYB/2, +BYB/2)
@@ -368,36 +394,40 @@ __device__ inline void stage_compute(const StageMem<BYB>& sm, f4 (&acc)[8][BYB /
     bh[q] = sm.B[0][lk][wy * (BYB / 2) + 16 * q + lr];
     if (NPASS > 1) bl[q] = sm.B[1][lk][wy * (BYB / 2) + 16 * q + lr];
   }
+  // this lane's transposed-read address: row 8*lk + lr/4 (+4 for the second
+  // read), columns 4*(lr%4).. of each 16-column block
+  const uint32_t r0 = a_row_off(8 * lk + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
+  const uint32_t r1 = a_row_off(8 * lk + 4 + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
 #pragma unroll
   for (int s = 0; s < 8; s++) {
-    const uint4 ah = sm.A[0][lk][wx * 128 + 16 * s + lr];
-    uint4 al;
-    if (NPASS > 1) al = sm.A[1][lk][wx * 128 + 16 * s + lr];
+    const uint32_t c = (uint32_t)((wx * 128 + 16 * s) * 2);
+    const bf8 ah = cat_bf8(tr_read(sm, r0 + c), tr_read(sm, r1 + c));
+    bf8 al;
+    if (NPASS > 1) al = cat_bf8(tr_read(sm, r0 + BX * 2 + c), tr_read(sm, r1 + BX * 2 + c));
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-      acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(ah), as_bf8(bh[q]), acc[s][q], 0, 0, 0);
+      acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf8(bh[q]), acc[s][q], 0, 0, 0);
       if (NPASS > 1) {
-        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(ah), as_bf8(bl[q]), acc[s][q], 0, 0, 0);
-        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(al), as_bf8(bh[q]), acc[s][q], 0, 0, 0);
+        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf8(bl[q]), acc[s][q], 0, 0, 0);
+        acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf8(bh[q]), acc[s][q], 0, 0, 0);
       }
     }
   }
 }
 
 // A workgroup of 4 waves owns a BX x BYB output block: x in [bx*BX, +BX), y a
-// BYB-wide slice of one view tile; each wave owns 128 x BYB/2.  K advances KG gathered
-// groups (32 rows, one MFMA K) per stage; stages are filled by global->LDS
-// DMA S-1 stages ahead of the MFMAs that read them, so no staging registers
-// are needed and every element is fetched once per workgroup.  Each stage
-// buffer is a separate LDS variable so in-flight DMA provably does not alias
-// the stage being read.  Consecutive workgroups on one XCD are the BYB slices
-// of one tile (same gathered A rows, shared through that XCD's L2).
+// BYB-wide slice of one view tile; each wave owns 128 x BYB/2.  K advances 32
+// gathered rows (one MFMA K) per stage; stages are filled by global->LDS DMA
+// S-1 stages ahead of the MFMAs that read them, so no staging registers are
+// needed and every element is fetched once per workgroup.  Each stage buffer
+// is a separate LDS variable so in-flight DMA provably does not alias the
+// stage being read.  Consecutive workgroups on one XCD are the BYB slices of
+// one tile (same gathered A rows, shared through that XCD's L2).
 template <int NPASS, int BYB, int S>
-__global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
-  constexpr int NW = 4;
+__global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   constexpr int NQ = BYB / 32;
-  constexpr int LW = (2 * KG * (BX + BYB) / 64) / NW;  // DMA instructions per wave per stage (12)
-  constexpr int LWP = NPASS > 1 ? LW : LW / 2;  // the lo planes are skipped with one pass
+  constexpr int LW = 8 + 4;                     // DMA instructions per wave per stage
+  constexpr int LWP = NPASS > 1 ? LW : LW - 2;  // the lo planes of B are skipped with one pass
   __shared__ StageMem<BYB> st[S];
   static_assert(S == 2 || S == 3, "2 or 3 stages");
   int64_t bx, yb;
@@ -423,23 +453,22 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
   const int64_t y0 = t * a.TY + bcol;
   uint32_t g_beg = a.tile_off[t], g_end = a.tile_off[t + 1];
   if (a.nsplit > 1) {
-    const uint32_t nst = (g_end - g_beg) / KG;
+    const uint32_t nst = (g_end - g_beg) / KR;
     const uint32_t per = (nst + a.nsplit - 1) / a.nsplit;
     const uint32_t s0 = min(nst, per * (uint32_t)sp), s1 = min(nst, per * (uint32_t)(sp + 1));
-    g_end = g_beg + s1 * KG;
-    g_beg = g_beg + s0 * KG;
+    g_end = g_beg + s1 * KR;
+    g_beg = g_beg + s0 * KR;
   }
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int wx = w & 1, wy = w >> 1;
   const int64_t x0 = bx * BX;
   StageSrc ps;
-  ps.Ahi = a.Ahi;
-  ps.Alo = a.Alo;
+  ps.A = a.A + (bx * a.R) * (2 * BX) + 8 * lane;
   ps.Bhi = a.Bhi;
   ps.Blo = a.Blo;
   ps.glist = (cu32*)a.glist;
-  ps.X = a.X;
   ps.TY = a.TY;
   f4 acc[8][NQ];
 #pragma unroll
@@ -457,10 +486,10 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
   if (k < g_end) {
 #pragma unroll
     for (int s = 0; s < S - 1; s++)
-      if (k + s * KG < g_end) stage_issue<NPASS, BYB>(ps, k + s * KG, x0, bcol, st[s], w, lane);
+      if (k + s * KR < g_end) stage_issue<NPASS, BYB>(ps, k + s * KR, bcol, st[s], w, lane);
     int cur = 0;
-    for (; k < g_end; k += KG) {
-      if (S > 2 && k + KG < g_end)
+    for (; k < g_end; k += KR) {
+      if (S > 2 && k + KR < g_end)
         wait_vm<(S > 2 ? LWP : 0)>();
       else
         wait_vm<0>();
@@ -468,7 +497,7 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
       __builtin_amdgcn_s_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
       const int nxt = cur == 0 ? S - 1 : cur - 1;  // (cur + S - 1) % S
-      if (k + (S - 1) * KG < g_end) stage_issue<NPASS, BYB>(ps, k + (S - 1) * KG, x0, bcol, st[nxt], w, lane);
+      if (k + (S - 1) * KR < g_end) stage_issue<NPASS, BYB>(ps, k + (S - 1) * KR, bcol, st[nxt], w, lane);
       stage_compute<NPASS, BYB>(st[cur], acc, wx, wy, lr, lk);
       cur = cur == S - 1 ? 0 : cur + 1;
     }
@@ -490,22 +519,25 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_bf16(B16Args a) {
     }
     const uint32_t yy = (uint32_t)y;
     const int64_t oy = (int64_t)(yy / a.om.Yl) * a.om.oyh + (int64_t)(yy % a.om.Yl) * a.om.oyl;
-    int64_t off[32];
-    bool ok[32];
-    double vv[32];
 #pragma unroll
-    for (int s = 0; s < 8; s++)
+    for (int hh = 0; hh < 2; hh++) {  // two batches of 16 read-modify-writes
+      int64_t off[16];
+      bool ok[16];
+      double vv[16];
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int64_t x = x0 + wx * 128 + 16 * s + 4 * lk + i;
-        ok[4 * s + i] = x < a.X;
-        off[4 * s + i] = out_xoff(a.om, x < a.X ? x : 0) + oy;
-        vv[4 * s + i] = (double)acc[s][q][i];
-      }
-    if (a.om.of32)
-      out_add_batch<float, 32>((float*)a.om.out, off, ok, a.alpha, vv);
-    else
-      out_add_batch<double, 32>((double*)a.om.out, off, ok, a.alpha, vv);
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int64_t x = x0 + wx * 128 + 16 * (4 * hh + s) + 4 * lk + i;
+          ok[4 * s + i] = x < a.X;
+          off[4 * s + i] = out_xoff(a.om, x < a.X ? x : 0) + oy;
+          vv[4 * s + i] = (double)acc[4 * hh + s][q][i];
+        }
+      if (a.om.of32)
+        out_add_batch<float, 16>((float*)a.om.out, off, ok, a.alpha, vv);
+      else
+        out_add_batch<double, 16>((double*)a.om.out, off, ok, a.alpha, vv);
+    }
   }
 }
 
@@ -547,9 +579,8 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   a.om.oxl = m.oxl;
   a.om.oyh = m.oyh;
   a.om.oyl = m.oyl;
-  const int64_t Rg = (R + 7) / 8;
-  a.Ahi = (const uint4*)A2;
-  a.Alo = a.Ahi + Rg * X;
+  a.A = A2;
+  a.R = R;
   a.X = X;
   a.Y = v->Y;
   a.tile_off = v->tile_off;
@@ -558,7 +589,7 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   a.Blo = (const uint4*)v->Blo;
   a.TY = v->TY;
   a.nbx = ceil_div(X, BX);
-  // 128-wide y blocks, 3 stages in flight (144 KB of LDS, one workgroup per CU)
+  // 128-wide y blocks, 3 stages in flight (149 KB of LDS, one workgroup per CU)
   const int byb = 128;
   a.nyb = v->ntiles * (v->TY / byb);
   a.order = g_order;
@@ -588,39 +619,32 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   return OK;
 }
 
-// A[R][X] (f64 or f32) -> hi/lo planes [(R+7)/8][X][8]; rows past R are zero.
+// A[R][X] (f64 or f32) -> records [X/256][R][hi 256 | lo 256] bf16 (X
+// rounded up to a multiple of 256, pad columns zero); hi = bf16(a),
+// lo = bf16(a - hi)
 template <typename T>
-__global__ void k_split_bf16(const T* __restrict__ A, int64_t R, int64_t X, __hip_bfloat16* __restrict__ hi,
-                             __hip_bfloat16* __restrict__ lo) {
-  const int64_t gx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t Rg = (R + 7) / 8;
-  if (gx >= Rg * X) return;
-  const int64_t g = gx / X, x = gx % X;
-  __hip_bfloat16 h[8], l[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const int64_t r = g * 8 + j;
-    const float v = r < R ? (float)A[r * X + x] : 0.0f;
-    h[j] = __float2bfloat16(v);
-    l[j] = __float2bfloat16(v - __bfloat162float(h[j]));
+__global__ void k_split_bf16(const T* __restrict__ A, int64_t R, int64_t X, int64_t n,
+                             __hip_bfloat16* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    // e = (xb * R + r) * 256 + xi
+    const int64_t xi = e & 255, rec = e >> 8;
+    const int64_t r = rec % R, x = (rec / R) * 256 + xi;
+    const float v = x < X ? (float)A[r * X + x] : 0.0f;
+    const __hip_bfloat16 h = __float2bfloat16(v);
+    out[rec * 512 + xi] = h;
+    out[rec * 512 + 256 + xi] = __float2bfloat16(v - __bfloat162float(h));
   }
-  uint4 uh, ul;
-  __builtin_memcpy(&uh, h, 16);
-  __builtin_memcpy(&ul, l, 16);
-  ((uint4*)hi)[gx] = uh;
-  ((uint4*)lo)[gx] = ul;
 }
 
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s) {
-  const int64_t Rg = (R + 7) / 8;
-  const int64_t n = Rg * X;
+  const int64_t n = R * bf16_xblocks(X) * 256;
   if (n == 0) return OK;
-  __hip_bfloat16* hi = (__hip_bfloat16*)A2;
-  __hip_bfloat16* lo = hi + n * 8;
+  const int64_t grid = std::min<int64_t>(ceil_div(n, 256), 256 * 64);
   if (src_dtype == DT_F64)
-    k_split_bf16<double><<<ceil_div(n, 256), 256, 0, s>>>((const double*)A, R, X, hi, lo);
+    k_split_bf16<double><<<grid, 256, 0, s>>>((const double*)A, R, X, n, (__hip_bfloat16*)A2);
   else if (src_dtype == DT_F32)
-    k_split_bf16<float><<<ceil_div(n, 256), 256, 0, s>>>((const float*)A, R, X, hi, lo);
+    k_split_bf16<float><<<grid, 256, 0, s>>>((const float*)A, R, X, n, (__hip_bfloat16*)A2);
   else
     FRICC_CHECK(false, "split_bf16: source must be FP64 or FP32");
   FRICC_HIP(hipGetLastError());

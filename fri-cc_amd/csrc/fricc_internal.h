@@ -40,8 +40,8 @@ struct View {
   int TY, kstep, dtype;
   int64_t ntiles = 0, nnz = 0;
   uint32_t* tile_off = nullptr;  // [ntiles + 1] first gathered row of each tile (device; last = total)
-  uint32_t* rowlist = nullptr;   // [cap_rows] r of each gathered row (BF16: r/8 of each group)
-  void* B = nullptr;             // [rows x TY] values (dtype); BF16: hi plane, bf16 [groups][TY][8]
+  uint32_t* rowlist = nullptr;   // [cap_rows] r of each gathered row
+  void* B = nullptr;             // [rows x TY] values (dtype); BF16: hi plane, bf16 [rows/8][TY][8]
   void* Blo = nullptr;           // BF16 only: lo plane (value - hi), same layout
   size_t cap_rows = 0;           // worst-case rows the buffers hold
   uint64_t gen = 0;
@@ -76,15 +76,18 @@ struct OutMap {
 int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const OutMap& om, double alpha,
                   hipStream_t s);
 
-// BF16 views gather aligned groups of 8 rows (r/8); A is given as two planes
-// [hi | lo], each [ceil(R/8)][X][8] bf16 with element (r, x) at
-// ((r/8)*X + x)*8 + r%8.  npass 3 = hi*hi + hi*lo + lo*hi, 1 = hi*hi.
+// BF16 views gather single rows; their planes interleave each 8 consecutive
+// gathered rows per column.  A is given as 1 KB records
+// [bf16_xblocks(X)][R][hi 256 | lo 256] bf16.  npass 3 = hi*hi + hi*lo +
+// lo*hi, 1 = hi*hi.
 int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& om, double alpha,
                        int npass, hipStream_t s);
 // tile width of BF16 views for an operand of R x X
 int bf16_view_width(int64_t R, int64_t X);
 // FP64/FP32 row-major A[R][X] -> [hi | lo] bf16 planes (layout above)
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s);
+// 256-column blocks of a BF16 operand (its size is R * blocks * 512 bf16)
+inline int64_t bf16_xblocks(int64_t X) { return (X + 255) / 256; }
 
 // A[r, x] = W[rh*s0 + rl*s1 + xh*s2 + xl*s3] (r = rh*Rl + rl, x = xh*Xl + xl) in dtype.
 int permute_operand(const double* W, int64_t Rh, int64_t Rl, int64_t Xh, int64_t Xl, const int64_t s[4],

@@ -42,8 +42,7 @@ struct VGeom {
   uint32_t d1, d2, d3;
   int rh, rl, yh, yl;
   uint32_t Rl, Yl, TY;
-  uint32_t R;       // rows of the slot table (groups for BF16 views)
-  uint32_t gshift;  // log2 of the row group (3 for BF16 views, else 0)
+  uint32_t R;  // rows of the slot table
 };
 
 __device__ inline void view_rc(uint32_t f, const VGeom& g, uint32_t& r, uint32_t& y) {
@@ -63,7 +62,7 @@ __global__ void k_mark(const uint32_t* __restrict__ idx, int64_t nnz, VGeom g, u
   if (e >= nnz) return;
   uint32_t r, y;
   view_rc(idx[e], g, r, y);
-  slot[(int64_t)(y / g.TY) * g.R + (r >> g.gshift)] = 1u;
+  slot[(int64_t)(y / g.TY) * g.R + r] = 1u;
 }
 
 __global__ void k_tile_count(const uint32_t* __restrict__ slot, uint32_t R, int kstep, uint32_t* __restrict__ cnt) {
@@ -120,7 +119,9 @@ __global__ void k_fill(const uint32_t* __restrict__ idx, const double* __restric
   atomicAdd(&B[(int64_t)k * g.TY + (y % g.TY)], (T)val[e]);
 }
 
-// BF16 view: accumulate into an FP32 [group][TY][8] buffer (split afterwards)
+// BF16 view: accumulate into an FP32 buffer laid out [k/8][TY][8] by gathered
+// row k (eight consecutive gathered rows interleaved per column), split
+// into the bf16 planes afterwards
 __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* __restrict__ val, int64_t nnz, VGeom g,
                                const uint32_t* __restrict__ slot, float* __restrict__ B) {
   int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
@@ -128,8 +129,8 @@ __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* _
   uint32_t r, y;
   view_rc(idx[e], g, r, y);
   uint32_t t = y / g.TY;
-  uint32_t k = slot[(int64_t)t * g.R + (r >> 3)];
-  atomicAdd(&B[((int64_t)k * g.TY + (y % g.TY)) * 8 + (r & 7)], (float)val[e]);
+  uint32_t k = slot[(int64_t)t * g.R + r];
+  atomicAdd(&B[((int64_t)(k >> 3) * g.TY + (y % g.TY)) * 8 + (k & 7)], (float)val[e]);
 }
 
 // Zero n words (a kernel rather than a memset, so a view build can be
@@ -193,8 +194,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   g.Yl = (uint32_t)v->Yl;
   g.TY = (uint32_t)v->TY;
   const bool grouped = v->dtype == DT_BF16;
-  g.gshift = grouped ? 3u : 0u;
-  g.R = (uint32_t)(grouped ? (v->R + 7) / 8 : v->R);
+  g.R = (uint32_t)v->R;
   void *p_slot, *p_cnt;
   const size_t nslot = (size_t)v->ntiles * g.R;
   FRICC_TRY(ws_get(20, nslot * 4, &p_slot));
@@ -203,8 +203,8 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   uint32_t* cnt = (uint32_t*)p_cnt;
   if (!v->tile_off) FRICC_HIP(hipMalloc(&v->tile_off, (size_t)(v->ntiles + 1) * 4));
   const int64_t nnz = sp->nnz;
-  // bytes per gathered row (BF16 views: per group of 8 rows and plane, bf16 [TY][8])
-  const size_t rowbytes = grouped ? (size_t)v->TY * 8 * 2 : (size_t)v->TY * (v->dtype == DT_F64 ? 8 : 4);
+  // bytes per gathered row (BF16 views: per plane)
+  const size_t rowbytes = (size_t)v->TY * (v->dtype == DT_F64 ? 8 : v->dtype == DT_F32 ? 4 : 2);
   const size_t cap = view_capacity(v->ntiles, g.R, nnz, v->kstep);
   if (v->cap_rows < cap || !v->rowlist) {
     if (v->rowlist) FRICC_HIP(hipFree(v->rowlist));
@@ -223,7 +223,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
   FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));
   k_tile_slots<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->tile_off, v->rowlist);
-  const uint32_t* total = v->tile_off + v->ntiles;  // gathered rows (groups), on the device
+  const uint32_t* total = v->tile_off + v->ntiles;  // gathered rows, on the device
   if (!grouped) {
     k_zero_rows<<<1024, VNT, 0, s>>>(total, (int64_t)(rowbytes / 16), (uint4*)v->B);
     if (nnz > 0) {
@@ -233,7 +233,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
         k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)v->B);
     }
   } else {
-    const int64_t w = v->TY * 8;  // FP32 values per group
+    const int64_t w = v->TY;  // FP32 values per gathered row
     void* p_tmp;
     FRICC_TRY(ws_get(22, cap * (size_t)w * 4, &p_tmp));
     k_zero_rows<<<1024, VNT, 0, s>>>(total, w / 4, (uint4*)p_tmp);
@@ -259,7 +259,7 @@ int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, Vie
     std::unique_ptr<View> nv(new View());
     for (int q = 0; q < 4; q++) nv->pos[q] = pos[q];
     nv->TY = TY;
-    nv->kstep = dtype == DT_BF16 ? 4 : 16;  // BF16: groups of 8 rows, 4 groups per K step
+    nv->kstep = dtype == DT_BF16 ? 32 : 16;  // rows per MFMA K step
     nv->dtype = dtype;
     v = nv.get();
     sp->views[key] = std::move(nv);

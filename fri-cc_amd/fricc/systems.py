@@ -66,14 +66,20 @@ def eri_block(Ld, naux, nmo, ranges, out, strides, stream=None):
     return out
 
 
+def bf16_xblocks(X):
+    """256-column blocks of a BF16 operand."""
+    return (X + 255) // 256
+
+
 def bf16_planes(A, R, X, stream=None):
-    """[hi | lo] BF16 planes of A viewed as (R, X) (fricc_split_bf16 layout)."""
+    """BF16 hi/lo operand of A viewed as (R, X): 1 KB records
+    [X/256][R][hi 256 | lo 256] (fricc_split_bf16 layout)."""
     import torch
 
     A = A.contiguous()
     assert A.numel() == R * X and A.dtype in (torch.float64, torch.float32)
     src = _capi.FP64 if A.dtype == torch.float64 else _capi.FP32
-    out = torch.empty(2 * ((R + 7) // 8) * X * 8, dtype=torch.int16, device=A.device)
+    out = torch.empty(R * bf16_xblocks(X) * 512, dtype=torch.int16, device=A.device)
     _capi.check(_capi.lib().fricc_split_bf16(ctypes.c_void_p(A.data_ptr()), src, int(R), int(X),
                                              ctypes.c_void_p(out.data_ptr()), _capi.stream_ptr(stream)))
     return out
@@ -270,17 +276,14 @@ class DeviceERIs:
 
         o, v, nmo = self.nocc, self.nvir, self.nmo
         X = v * v
-        cc = max(1, int(chunk_bytes // (8 * v ** 3)))
-        while (cc * v) % 8 and cc < v:  # slice rows cover whole 8-row groups
-            cc += 1
-        cc = min(cc, v)
+        Xb = bf16_xblocks(X)
+        cc = min(v, max(1, int(chunk_bytes // (8 * v ** 3))))
         dev = self.L.device
         if precision == "fp32":
             out = torch.empty((v, v, v, v), dtype=torch.float32, device=dev)
         else:
-            G = (X + 7) // 8
-            out = torch.empty(2 * G * X * 8, dtype=torch.int16, device=dev)
-            hi, lo = out[: G * X * 8].view(G, X * 8), out[G * X * 8:].view(G, X * 8)
+            out = torch.empty(X * Xb * 512, dtype=torch.int16, device=dev)
+            rec = out.view(Xb, X, 512)  # [column block][row (c,d)][hi | lo]
         Wc = torch.empty((cc, v, v, v), dtype=torch.float64, device=dev)
         for c0 in range(0, v, cc):
             c1 = min(v, c0 + cc)
@@ -294,9 +297,7 @@ class DeviceERIs:
                 out.view(X, X)[c0 * v:c1 * v].copy_(rows)
             else:
                 part = bf16_planes(rows, n * v, X)
-                g0, gn = (c0 * v) // 8, (n * v + 7) // 8
-                hi[g0:g0 + gn].copy_(part[: gn * X * 8].view(gn, X * 8))
-                lo[g0:g0 + gn].copy_(part[gn * X * 8:].view(gn, X * 8))
+                rec[:, c0 * v:c1 * v].copy_(part.view(Xb, n * v, 512))
                 del part
         del Wc, W, rows
         return out

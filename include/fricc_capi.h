@@ -158,10 +158,11 @@ int fricc_sparse_prepare(fricc_sparse* T, const char* term_or_spec, const int64_
                          const int64_t out_shape[4], int precision, void* stream);
 
 /* BF16 operand for FRICC_BF16 contractions: A[R][X] row-major (FP64 when
- * src_precision == FRICC_FP64, FP32 when FRICC_FP32) -> out = [hi | lo], two
- * planes of ceil(R/8)*X*8 bf16 each, element (r, x) at ((r/8)*X + x)*8 + r%8,
- * hi = bf16(a), lo = bf16(a - hi).  out holds 2*ceil(R/8)*X*8 uint16.  No
- * counterpart in the reference (FP64 only). */
+ * src_precision == FRICC_FP64, FP32 when FRICC_FP32) -> out = 1 KB records
+ * [ceil(X/256)][R][hi 256 | lo 256] bf16: element (r, x) has hi = bf16(a) at
+ * ((x/256)*R + r)*512 + x%256 and lo = bf16(a - hi) 256 further; columns
+ * past X are zero.  out holds R*ceil(X/256)*512 uint16.  No counterpart in
+ * the reference (FP64 only). */
 int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream);
 
 /* MFMA passes of FRICC_BF16 contractions: 3 = hi*hi + hi*lo + lo*hi (default,
