@@ -13,6 +13,8 @@
 // 128 B segments) and one B value per 16-column block.
 #include "fricc_internal.h"
 
+#include <utility>
+
 namespace fricc {
 
 typedef __attribute__((ext_vector_type(4))) double d4;
@@ -301,6 +303,7 @@ struct B16Args {
   const uint16_t* A;  // [X/256][R][hi 256 | lo 256]
   int64_t R, X, Y;
   const uint32_t* tile_off;
+  const uint32_t* tile_order;  // tiles by decreasing row count
   const uint32_t* glist;
   const uint4* Bhi;  // view planes [k/8][TY][8] bf16
   const uint4* Blo;
@@ -357,21 +360,25 @@ struct StageSrc {
 // One pipeline stage: gathered rows [k, k+32) of the tile, the A record of
 // the workgroup's column block and B columns [bcol, +BYB), hi and lo planes.
 // Wave w stages A rows 8w..8w+7 (one 1 KB record per DMA: lanes 0-31 the hi
-// half, 32-63 the lo half) and B column chunk w%2 of groups w/2 and w/2+2.
+// half, 32-63 the lo half) and a quarter of the B chunks (64 columns of one
+// 8-row group and plane per DMA).
 template <int NPASS, int BYB>
 __device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t bcol, StageMem<BYB>& sm, int w, int lane) {
-  static_assert(BX == 256 && BYB == 128 && KG == 4 && KR == 32, "wave assignment assumes 4 waves, BX 256, BYB 128");
+  static_assert(BX == 256 && (BYB == 128 || BYB == 256) && KG == 4 && KR == 32,
+                "wave assignment assumes 4 waves, BX 256, BYB 128 or 256");
   const uint32_t la = lds_addr(&sm.A[0]), lb = lds_addr(&sm.B[0][0][0]);
   cu32* rl = p.glist + k + 8 * w;
   const uint32_t r0 = rl[0], r1 = rl[1], r2 = rl[2], r3 = rl[3], r4 = rl[4], r5 = rl[5], r6 = rl[6], r7 = rl[7];
   const uint32_t rr[8] = {r0, r1, r2, r3, r4, r5, r6, r7};
 #pragma unroll
   for (int i = 0; i < 8; i++) dma16(p.A + (int64_t)rr[i] * (2 * BX), la + a_row_off(8 * w + i));
-  const int yc = w & 1;
+  constexpr int NCH = BYB / 64;          // 64-column chunks per group
+  constexpr int PER = KG * NCH / 4;      // chunks per wave
   const uint32_t kg = k >> 3;
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int gl = (w >> 1) + 2 * h;
+  for (int j = 0; j < PER; j++) {
+    const int c = w * PER + j;
+    const int gl = c / NCH, yc = c % NCH;
     const int64_t off = (int64_t)(kg + gl) * p.TY + bcol + yc * 64 + lane;
     const uint32_t l = lb + (uint32_t)((gl * BYB + yc * 64) * 16);
     dma16(p.Bhi + off, l);
@@ -423,11 +430,62 @@ __device__ inline void stage_compute(const StageMem<BYB>& sm, f4 (&acc)[8][BYB /
 // is a separate LDS variable so in-flight DMA provably does not alias the
 // stage being read.  Consecutive workgroups on one XCD are the BYB slices of
 // one tile (same gathered A rows, shared through that XCD's L2).
+// Epilogue of one 16-column block q of a wave's accumulators (compile-time q,
+// so the accumulators stay in registers): split-K partials, or
+// out += alpha * acc in two batches of 16 read-modify-writes.
+template <int Q, int NQ>
+__device__ inline void bf16_epilogue_q(const B16Args& a, const f4 (&acc)[8][NQ], int64_t x0, int64_t y0, int wx, int wy,
+                                       int lr, int lk, int sp, int byb) {
+  const int64_t y = y0 + wy * (byb / 2) + 16 * Q + lr;
+  if (y >= a.Y) return;
+  if (a.nsplit > 1) {
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int64_t x = x0 + wx * 128 + 16 * s + 4 * lk + i;
+        if (x < a.X) a.partial[((int64_t)sp * a.X + x) * a.Y + y] = acc[s][Q][i];
+      }
+    return;
+  }
+  const uint32_t yy = (uint32_t)y;
+  const int64_t oy = (int64_t)(yy / a.om.Yl) * a.om.oyh + (int64_t)(yy % a.om.Yl) * a.om.oyl;
+#pragma unroll
+  for (int hh = 0; hh < 2; hh++) {
+    int64_t off[16];
+    bool ok[16];
+    double vv[16];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int64_t x = x0 + wx * 128 + 16 * (4 * hh + s) + 4 * lk + i;
+        ok[4 * s + i] = x < a.X;
+        off[4 * s + i] = out_xoff(a.om, x < a.X ? x : 0) + oy;
+        vv[4 * s + i] = (double)acc[4 * hh + s][Q][i];
+      }
+    if (a.om.of32)
+      out_add_batch<float, 16>((float*)a.om.out, off, ok, a.alpha, vv);
+    else
+      out_add_batch<double, 16>((double*)a.om.out, off, ok, a.alpha, vv);
+    // one batch at a time: interleaving batches would need registers for all
+    // their addresses and values at once
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int NQ, size_t... Qs>
+__device__ inline void bf16_epilogue(std::index_sequence<Qs...>, const B16Args& a, const f4 (&acc)[8][NQ], int64_t x0,
+                                     int64_t y0, int wx, int wy, int lr, int lk, int sp) {
+  (bf16_epilogue_q<(int)Qs, NQ>(a, acc, x0, y0, wx, wy, lr, lk, sp, NQ * 32), ...);
+}
+
 template <int NPASS, int BYB, int S>
 __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   constexpr int NQ = BYB / 32;
-  constexpr int LW = 8 + 4;                     // DMA instructions per wave per stage
-  constexpr int LWP = NPASS > 1 ? LW : LW - 2;  // the lo planes of B are skipped with one pass
+  constexpr int PER = KG * (BYB / 64) / 4;          // B chunks per wave per stage
+  constexpr int LW = 8 + 2 * PER;                   // DMA instructions per wave per stage
+  constexpr int LWP = NPASS > 1 ? LW : LW - PER;    // the lo planes of B are skipped with one pass
   __shared__ StageMem<BYB> st[S];
   static_assert(S == 2 || S == 3, "2 or 3 stages");
   int64_t bx, yb;
@@ -448,7 +506,7 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
     }
   }
   const int64_t sub = a.TY / BYB;
-  const int64_t t = yb / sub;
+  const int64_t t = a.tile_order[yb / sub];
   const int64_t bcol = (yb % sub) * BYB;  // first column of this block within the tile
   const int64_t y0 = t * a.TY + bcol;
   uint32_t g_beg = a.tile_off[t], g_end = a.tile_off[t + 1];
@@ -503,42 +561,7 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
     }
   }
   // D layout: lane holds column lane%16 of each 16x16 block, rows 4*(lane/16)+i
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const int64_t y = y0 + wy * (BYB / 2) + 16 * q + lr;
-    if (y >= a.Y) continue;
-    if (a.nsplit > 1) {
-#pragma unroll
-      for (int s = 0; s < 8; s++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int64_t x = x0 + wx * 128 + 16 * s + 4 * lk + i;
-          if (x < a.X) a.partial[((int64_t)sp * a.X + x) * a.Y + y] = acc[s][q][i];
-        }
-      continue;
-    }
-    const uint32_t yy = (uint32_t)y;
-    const int64_t oy = (int64_t)(yy / a.om.Yl) * a.om.oyh + (int64_t)(yy % a.om.Yl) * a.om.oyl;
-#pragma unroll
-    for (int hh = 0; hh < 2; hh++) {  // two batches of 16 read-modify-writes
-      int64_t off[16];
-      bool ok[16];
-      double vv[16];
-#pragma unroll
-      for (int s = 0; s < 4; s++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int64_t x = x0 + wx * 128 + 16 * (4 * hh + s) + 4 * lk + i;
-          ok[4 * s + i] = x < a.X;
-          off[4 * s + i] = out_xoff(a.om, x < a.X ? x : 0) + oy;
-          vv[4 * s + i] = (double)acc[4 * hh + s][q][i];
-        }
-      if (a.om.of32)
-        out_add_batch<float, 16>((float*)a.om.out, off, ok, a.alpha, vv);
-      else
-        out_add_batch<double, 16>((double*)a.om.out, off, ok, a.alpha, vv);
-    }
-  }
+  bf16_epilogue<NQ>(std::make_index_sequence<NQ>{}, a, acc, x0, y0, wx, wy, lr, lk, sp);
 }
 
 // out[x, y] += alpha * sum_s partial[s][x][y], splits added in order
@@ -584,13 +607,16 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   a.X = X;
   a.Y = v->Y;
   a.tile_off = v->tile_off;
+  a.tile_order = v->tile_order;
   a.glist = v->rowlist;
   a.Bhi = (const uint4*)v->B;
   a.Blo = (const uint4*)v->Blo;
   a.TY = v->TY;
   a.nbx = ceil_div(X, BX);
-  // 128-wide y blocks, 3 stages in flight (149 KB of LDS, one workgroup per CU)
-  const int byb = 128;
+  // tiles of 256+ columns: 256 x 256 output blocks, 2 stages (131 KB of LDS);
+  // else 256 x 128 blocks, 3 stages (149 KB); one workgroup per CU.  The
+  // larger block reads a third fewer operand bytes per MFMA from L2.
+  const int byb = v->TY % 256 == 0 ? 256 : 128;
   a.nyb = v->ntiles * (v->TY / byb);
   a.order = g_order;
   a.alpha = alpha;
@@ -609,10 +635,17 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
     a.partial = (float*)p;
   }
   const int64_t nb = nb0 * nsplit;
-  if (npass == 3)
-    k_ggemm_bf16<3, 128, 3><<<nb, 256, 0, s>>>(a);
-  else
-    k_ggemm_bf16<1, 128, 3><<<nb, 256, 0, s>>>(a);
+  if (byb == 256) {
+    if (npass == 3)
+      k_ggemm_bf16<3, 256, 2><<<nb, 256, 0, s>>>(a);
+    else
+      k_ggemm_bf16<1, 256, 2><<<nb, 256, 0, s>>>(a);
+  } else {
+    if (npass == 3)
+      k_ggemm_bf16<3, 128, 3><<<nb, 256, 0, s>>>(a);
+    else
+      k_ggemm_bf16<1, 128, 3><<<nb, 256, 0, s>>>(a);
+  }
   FRICC_HIP(hipGetLastError());
   if (nsplit > 1) k_splitk_reduce<<<ceil_div(X * v->Y, 256), 256, 0, s>>>(a.partial, nsplit, X, v->Y, a.om, alpha);
   FRICC_HIP(hipGetLastError());

@@ -40,6 +40,7 @@ struct View {
   int TY, kstep, dtype;
   int64_t ntiles = 0, nnz = 0;
   uint32_t* tile_off = nullptr;  // [ntiles + 1] first gathered row of each tile (device; last = total)
+  uint32_t* tile_order = nullptr;  // [ntiles] tiles by decreasing row count (device)
   uint32_t* rowlist = nullptr;   // [cap_rows] r of each gathered row
   void* B = nullptr;             // [rows x TY] values (dtype); BF16: hi plane, bf16 [rows/8][TY][8]
   void* Blo = nullptr;           // BF16 only: lo plane (value - hi), same layout

@@ -14,6 +14,7 @@ constexpr int VNT = 256;
 
 View::~View() {
   if (tile_off) (void)hipFree(tile_off);
+  if (tile_order) (void)hipFree(tile_order);
   if (rowlist) (void)hipFree(rowlist);
   if (B) (void)hipFree(B);
   if (Blo) (void)hipFree(Blo);
@@ -133,6 +134,20 @@ __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* _
   atomicAdd(&B[((int64_t)(k >> 3) * g.TY + (y % g.TY)) * 8 + (k & 7)], (float)val[e]);
 }
 
+// Tiles by decreasing gathered-row count (ties by index): order[rank] = tile.
+// Workgroups take tiles in this order, so the longest K ranges start first.
+__global__ void k_tile_order(const uint32_t* __restrict__ tile_off, int64_t ntiles, uint32_t* __restrict__ order) {
+  for (int64_t t = (int64_t)blockIdx.x * VNT + threadIdx.x; t < ntiles; t += (int64_t)gridDim.x * VNT) {
+    const uint32_t c = tile_off[t + 1] - tile_off[t];
+    int64_t rank = 0;
+    for (int64_t u = 0; u < ntiles; u++) {
+      const uint32_t cu = tile_off[u + 1] - tile_off[u];
+      rank += (cu > c) || (cu == c && u < t);
+    }
+    order[rank] = (uint32_t)t;
+  }
+}
+
 // Zero n words (a kernel rather than a memset, so a view build can be
 // recorded in a graph)
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
@@ -202,6 +217,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   uint32_t* slot = (uint32_t*)p_slot;
   uint32_t* cnt = (uint32_t*)p_cnt;
   if (!v->tile_off) FRICC_HIP(hipMalloc(&v->tile_off, (size_t)(v->ntiles + 1) * 4));
+  if (!v->tile_order) FRICC_HIP(hipMalloc(&v->tile_order, (size_t)std::max<int64_t>(v->ntiles, 1) * 4));
   const int64_t nnz = sp->nnz;
   // bytes per gathered row (BF16 views: per plane)
   const size_t rowbytes = (size_t)v->TY * (v->dtype == DT_F64 ? 8 : v->dtype == DT_F32 ? 4 : 2);
@@ -223,6 +239,8 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
   FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));
   k_tile_slots<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->tile_off, v->rowlist);
+  k_tile_order<<<(unsigned)std::min<int64_t>(ceil_div(v->ntiles, VNT), 64), VNT, 0, s>>>(v->tile_off, v->ntiles,
+                                                                                        v->tile_order);
   const uint32_t* total = v->tile_off + v->ntiles;  // gathered rows, on the device
   if (!grouped) {
     k_zero_rows<<<1024, VNT, 0, s>>>(total, (int64_t)(rowbytes / 16), (uint4*)v->B);
