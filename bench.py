@@ -322,7 +322,11 @@ def serve_bench(args):
 
     sizes = [tuple(int(x) for x in s.split("x")) for s in args.serve_sizes.split(",")]
     reqs = sv.poisson_requests(args.serve, args.serve_rate, sizes, (0.01, 0.10), seed=args.seed)
-    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=16)
+    def progress(rec):
+        t, ev, rid, o, v = rec[:5]
+        print(f"# serve t={t:.2f}s {ev} {rid} ({o}x{v})", file=sys.stderr, flush=True)
+
+    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=16, progress=progress)
     torch.cuda.empty_cache()
     at_start = {"allocated": torch.cuda.memory_allocated(), "device_free": torch.cuda.mem_get_info()[0]}
     print(f"# serving: memory at start {at_start}", file=sys.stderr, flush=True)
@@ -355,7 +359,7 @@ def main():
     ap.add_argument("--replica-precision", default="mixed")
     ap.add_argument("--serve", type=int, default=0, help="also serve N solves arriving as a Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--serve-sizes", default="10x90,20x180,32x256,48x352")
+    ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
     ap.add_argument("--lccsd-modes", default="mixed,fp64", help="FRI-LCCSD iterations timed beside ('' for none)")
     ap.add_argument("--lccsd-steps", type=int, default=3)
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)

@@ -182,20 +182,27 @@ class Server:
     """Admission by HBM budget, per-solve streams and arenas, continuous batching.
 
     A request is admitted while (a) the admitted solves' footprints plus its
-    own stay within ``hbm_budget`` and (b) the device still has room for its
-    footprint and set-up transients.  A footprint is the larger of
+    own stay within ``hbm_budget``, (b) the device still has room for its
+    footprint and set-up transients and (c) it needs at most half the budget
+    or nothing else is running.  A footprint is the larger of
     ``estimate_bytes`` and the most the device lost to an earlier set-up of
     the same shape (measured around ``Solve``)."""
 
     margin = 2.0e9  # device memory left free beyond each admission's need
 
-    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8):
+    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8, progress=None):
+        self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision
         self.max_active = min(int(max_active), 63)
         self.measured = {}
         self._streams = {}  # one stream per arena, reused by the solves that get the arena
         self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
+
+    def _note(self, rec):
+        self.log.append(rec)
+        if self.progress is not None:
+            self.progress(rec)
 
     def _need(self, req):
         key = (req.nocc, req.nvir)
@@ -217,6 +224,10 @@ class Server:
     def _fits(self, req, used, active, idle):
         need = self._need(req)
         if active and used + need > self.budget:
+            return False
+        if active and need > 0.5 * self.budget:
+            # a solve that takes most of the budget is admitted alone: its
+            # set-up beside running solves is not validated on the device
             return False
         want = need + transient_bytes(req.nocc, req.nvir, self.precision) + self.margin
         if device_free(want) >= want:
@@ -280,7 +291,7 @@ class Server:
                     key = (req.nocc, req.nvir)
                     self.measured[key] = max(self.measured.get(key, 0.0), 1.25 * self._need(req))
                     self.deferred += 1
-                    self.log.append((round(clock() - t0, 3), "deferred", req.id, req.nocc, req.nvir, None, used,
+                    self._note((round(clock() - t0, 3), "deferred", req.id, req.nocc, req.nvir, None, used,
                                      torch.cuda.mem_get_info()[0]))
                     wait_retire = True
                     break
@@ -290,7 +301,7 @@ class Server:
                 self.measured[key] = max(self.measured.get(key, 0.0), float(before - device_available()))
                 s.need = self._need(req)
                 used += s.need
-                self.log.append((round(s.admitted, 3), "admit", req.id, req.nocc, req.nvir, s.need, used,
+                self._note((round(s.admitted, 3), "admit", req.id, req.nocc, req.nvir, s.need, used,
                                  _torch().cuda.mem_get_info()[0]))
                 s.issue()
                 active.append(s)
@@ -309,7 +320,7 @@ class Server:
                                           len(s.energies), s.converged, list(s.energies)))
                     active.remove(s)
                     used -= s.need
-                    self.log.append((round(fin, 3), "retire", s.req.id, s.req.nocc, s.req.nvir, s.need, used, None))
+                    self._note((round(fin, 3), "retire", s.req.id, s.req.nocc, s.req.nvir, s.need, used, None))
                     free_arenas.append(s.cc.arena)
                     free_arenas.sort()
                     wait_retire = False

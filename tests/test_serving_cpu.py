@@ -23,3 +23,16 @@ def test_memory_model_monotone():
     assert a < b < c and c < 240e9
     # the FP64 mode keeps the FP64 ladder and FP64 operands
     assert sv.estimate_bytes(32, 256, "fp64") > c
+
+
+def test_large_solve_is_admitted_alone():
+    """A request needing more than half the budget waits until nothing else
+    runs; smaller requests are still checked against the budget.  (Both
+    refusals are decided before any device query.)"""
+    big = sv.Request(0, 0.0, 48, 352, m=1)
+    small = sv.Request(1, 0.0, 10, 90, m=1)
+    server = sv.Server(hbm_budget=1.05 * sv.estimate_bytes(48, 352), precision="mixed")
+    running = [object()]
+    assert sv.estimate_bytes(48, 352) > 0.5 * server.budget
+    assert not server._fits(big, used=1.0, active=running, idle=[])
+    assert not server._fits(small, used=server.budget, active=running, idle=[])
