@@ -371,8 +371,9 @@ def _update_amps_device(cc, t1, t2, eris):
         A0112 = eris.sparse_operand("A0112", sp)
         A0313 = eris.sparse_operand("A0313", sp)
         _contract_into("ldkc,ilda->akic", -0.5, A0112, (o, v, o, v), ts_alt, Wvoov, sp)
-        _contract_into("lckd,ilad->akic", -0.5, A0313, (o, v, o, v), ts_alt, Wvoov, sp)
-        _contract_into("ldkc,ilad->akic", 1.0, A0112, (o, v, o, v), ts_alt, Wvoov, sp)
+        # 0313 and 0113 read the same view of T with operands in the same
+        # [(l,d)][(k,c)] layout: one contraction with (ld|kc) - 1/2 (lc|kd)
+        _contract_into("ldkc,ilad->akic", 1.0, eris.sparse_operand("A0113c", sp), (o, v, o, v), ts_alt, Wvoov, sp)
         _contract_into("lckd,ilda->akci", -0.5, A0313, (o, v, o, v), ts_alt, Wvovo, sp)
     else:
         ovov64 = eris.ovov.to(cdt)
@@ -443,11 +444,15 @@ def _update_amps_device(cc, t1, t2, eris):
         _prepare(ts, "1302", (v, o, o, v), (o, o, v, v), sp)
         _prepare(ts, "1303", (v, o, o, v), (o, o, v, v), sp)
         tm.start("O3V3")
+        # 1302 and 1202 read the same view of T with operands in the same
+        # [(k,c)][(i,a)] layout: one contraction with 2 Wvoov - Wvovo
+        A12 = _dyn_operand(2.0 * A_voov - A_vovo, ov, ov, sp)
+        _contract_into("akic,kjcb->ijab", 1.0, A12, (v, o, o, v), ts, S, sp)
+        del A12
         Av = _dyn_operand(A_voov, ov, ov, sp)
         Avo = _dyn_operand(A_vovo, ov, ov, sp)
-        for (term, spec, alpha), A, ws in zip(_O3V3, (Av, Avo, Av, Avo),
-                                               ((v, o, o, v), (v, o, v, o), (v, o, o, v), (v, o, v, o))):
-            _contract_into(spec, alpha, A, ws, ts, S, sp)
+        _contract_into("akic,kjbc->ijab", -1.0, Av, (v, o, o, v), ts, S, sp)    # 1303
+        _contract_into("bkci,kjac->ijab", -1.0, Avo, (v, o, v, o), ts, S, sp)   # 1203
         tm.stop("O3V3")
     else:
         t2w = t2c

@@ -127,8 +127,15 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
     if "O^3V^3" in ccs:
         Av = eris.sparse_operand("ovvo_p", sp)
         Avo = eris.sparse_operand("oovv_p", sp)
-        for (term, spec, alpha), A, ws in zip(_O3V3, (Av, Avo, Av, Avo),
-                                               ((v, o, o, v), (v, o, v, o), (v, o, o, v), (v, o, v, o))):
+        terms = list(zip(_O3V3, (Av, Avo, Av, Avo), ((v, o, o, v), (v, o, v, o), (v, o, o, v), (v, o, v, o))))
+        if ts["1302"] is ts["1202"]:
+            # one compressed tensor: 1302 and 1202 read the same view with
+            # operands in the same layout, so they are one contraction with
+            # the bare 2 Wvoov - Wvovo
+            _prepare(ts["1302"], "1302", (v, o, o, v), (o, o, v, v), sp)
+            _contract_into("akic,kjcb->ijab", 1.0, eris.sparse_operand("A12b", sp), (v, o, o, v), ts["1302"], S, sp)
+            terms = terms[2:]
+        for (term, spec, alpha), A, ws in terms:
             _prepare(ts[term], term, ws, (o, o, v, v), sp)
             _contract_into(spec, alpha, A, ws, ts[term], S, sp)
     else:

@@ -219,6 +219,11 @@ class DeviceERIs:
             return c(ovov)                              # [l,d,k,c] = (ld|kc)
         if name == "A0313":
             return c(ovov.permute(0, 3, 2, 1))          # [l,d,k,c] = (lc|kd)
+        if name == "A0113c":
+            return c(ovov - 0.5 * ovov.permute(0, 3, 2, 1))  # [l,d,k,c] = (ld|kc) - 1/2 (lc|kd)
+        if name == "A12b":
+            # [k,c,i,a] = 2 (kc|ai) - (ki|ac): bare 2 Wvoov - Wvovo of the linearised update
+            return c(2.0 * self.ovvo.permute(0, 1, 3, 2) - self.oovv.permute(0, 3, 1, 2))
         if name == "Gx":
             return c(ovvv.permute(3, 1, 0, 2))          # [c,d,k,a] = (kd|ac)
         if name == "Gy":
