@@ -386,8 +386,10 @@ def _update_amps_device(cc, t1, t2, eris):
 
     # ---- direct T2 terms into the output buffer acc (fricc/friccsd.py:210-235)
     t0 = time.perf_counter()
-    X = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)  # sum_cd (kd|ac) T[ij,cd]
-    Y = torch.zeros((o, o, o, v), dtype=f64, device=t2.device)  # sum_cd (kc|bd) T[ij,cd]
+    # the ladder's t1 dressing (Gx/Gy contractions and their t1 products) in
+    # the dense-GEMM precision; added to the FP64 accumulator once
+    X = torch.zeros((o, o, o, v), dtype=cdt, device=t2.device)  # sum_cd (kd|ac) T[ij,cd]
+    Y = torch.zeros((o, o, o, v), dtype=cdt, device=t2.device)  # sum_cd (kc|bd) T[ij,cd]
     if "O^2V^4" in ccs:
         _contract_into("kdac,ijcd->ijka", 1.0, eris.sparse_operand("Gx", sp), (o, v, v, v), ts, X, sp)
         _contract_into("kcbd,ijcd->ijkb", 1.0, eris.sparse_operand("Gy", sp), (o, v, v, v), ts, Y, sp)
@@ -396,11 +398,11 @@ def _update_amps_device(cc, t1, t2, eris):
         Y += torch.einsum("kcbd,ijcd->ijkb", eris.ovvv, t2)
     if "O^4V^2" in ccs:
         # klij,ka,lb -> ijab  ==  sum_l Q[ij,l,a] t1[l,b],  Q[ij,l,a] = sum_k Woooo[k,l,i,j] t1[k,a]
-        X -= torch.einsum("klij,ka->ijla", Woooo, t1)
+        X -= torch.einsum("klij,ka->ijla", Woooo.to(cdt), t1c)
     # acc[ij,a,b] = -sum_k X[ij,k,a] t1[k,b] - sum_k t1[k,a] Y[ij,k,b]
-    acc = torch.bmm(X.reshape(o * o, o, v).transpose(1, 2), t1.expand(o * o, o, v)).neg_()
-    acc.baddbmm_(t1.T.expand(o * o, v, o), Y.reshape(o * o, o, v), alpha=-1)
-    acc = acc.reshape(o, o, v, v)
+    acc = torch.bmm(X.reshape(o * o, o, v).transpose(1, 2), t1c.expand(o * o, o, v)).neg_()
+    acc.baddbmm_(t1c.T.expand(o * o, v, o), Y.reshape(o * o, o, v), alpha=-1)
+    acc = C(acc).reshape(o, o, v, v)
     if "O^4V^2" in ccs:
         _contract_into("klij,klab->ijab", 1.0, _dyn_operand(Woooo, o * o, o * o, sp), (o, o, o, o), ts, acc, sp)
     else:
