@@ -354,10 +354,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
-    ap.add_argument("--replicas", type=int, default=0, help="also run R batched replicas of the o=20 v=180 system")
-    ap.add_argument("--replica-iters", type=int, default=5)
+    ap.add_argument("--replicas", type=int, default=16, help="batched replicas of the o=20 v=180 system (0: none)")
+    ap.add_argument("--replica-iters", type=int, default=3)
     ap.add_argument("--replica-precision", default="mixed")
-    ap.add_argument("--serve", type=int, default=0, help="also serve N solves arriving as a Poisson stream")
+    ap.add_argument("--serve", type=int, default=8, help="serve N solves arriving as a Poisson stream (0: none)")
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
     ap.add_argument("--lccsd-modes", default="mixed,fp64", help="FRI-LCCSD iterations timed beside ('' for none)")
@@ -508,6 +508,11 @@ def main():
             roof["traffic_note"] = note
     if args.serve > 0:
         out["serving"] = serve_bench(args)
+        sv_ = out["serving"]
+        out["solve_latency_s"] = {"p50": sv_["p50_s"], "p99": sv_["p99_s"], "solves_per_s": sv_["solves_per_s"],
+                                  "workload": f"{sv_['solves']} synthetic DF solves, Poisson arrivals at "
+                                              f"{sv_['rate_per_s']}/s, sizes {sv_['sizes']}, m/|T2| in [1%, 10%], "
+                                              f"each to |dE| < 1e-6 Ha or 50 iterations, mixed precision"}
         print(f"# serving: {out['serving']}", file=sys.stderr, flush=True)
     if args.replicas > 0:
         out["replica_batch"] = replica_bench(args)
