@@ -494,25 +494,25 @@ __global__ void k_samp_emit(const double* __restrict__ x, const double* __restri
   if (!__syncthreads_or(work)) return;
   stage_tile(pin ? pin : x, tb, n, lds);
   if (!work) return;
+  // Walk the lattice alongside the elements: kc = first k with fl(k+u) >= P
+  // of the running prefix.  Element j owns the k with P_j <= fl(k+u) <
+  // P_{j+1}, i.e. those kc passes while fl(kc+u) < P_{j+1} (the same set as
+  // first_k_at_least(P_j) .. first_k_at_least(P_{j+1}), by monotonicity).
+  int64_t kc = first_k_at_least(u128_to_double(pre, -96), u);
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
     const int64_t j = base + r;
     const double v = tile_elem(lds, r);
     const double p = j < n ? (pin ? v : prob_of(v, (uint64_t)j, c, nsd)) : 0.0;
     const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
-    const u128 nxt = pre + f;
-    if (j < n && f != 0) {
-      double Plo = u128_to_double(pre, -96), Phi = u128_to_double(nxt, -96);
-      if (Phi > Plo) {
-        int64_t klo = first_k_at_least(Plo, u), khi = first_k_at_least(Phi, u);
-        if (khi > ns) khi = ns;
-        for (int64_t k = klo; k < khi; k++) {
-          out_idx[K + k] = (uint32_t)j;
-          if (out_val) out_val[K + k] = v / p;
-        }
+    pre += f;
+    if (f != 0) {
+      const double Phi = u128_to_double(pre, -96);
+      for (; kc < ns && (double)kc + u < Phi; kc++) {
+        out_idx[K + kc] = (uint32_t)j;
+        if (out_val) out_val[K + kc] = v / p;
       }
     }
-    pre = nxt;
   }
 }
 
