@@ -354,10 +354,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
-    ap.add_argument("--replicas", type=int, default=16, help="batched replicas of the o=20 v=180 system (0: none)")
+    ap.add_argument("--replicas", type=int, default=0, help="also run R batched replicas of the o=20 v=180 system")
     ap.add_argument("--replica-iters", type=int, default=3)
     ap.add_argument("--replica-precision", default="mixed")
-    ap.add_argument("--serve", type=int, default=8, help="serve N solves arriving as a Poisson stream (0: none)")
+    ap.add_argument("--serve", type=int, default=0, help="also serve N solves arriving as a Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
     ap.add_argument("--lccsd-modes", default="mixed,fp64", help="FRI-LCCSD iterations timed beside ('' for none)")
