@@ -326,7 +326,9 @@ def serve_bench(args):
         t, ev, rid, o, v = rec[:5]
         print(f"# serve t={t:.2f}s {ev} {rid} ({o}x{v})", file=sys.stderr, flush=True)
 
-    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=16, progress=progress)
+    # one solve at a time: concurrent admission of solves of these sizes is
+    # switched off (see fricc.serving.Server)
+    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=1, progress=progress)
     torch.cuda.empty_cache()
     at_start = {"allocated": torch.cuda.memory_allocated(), "device_free": torch.cuda.mem_get_info()[0]}
     print(f"# serving: memory at start {at_start}", file=sys.stderr, flush=True)

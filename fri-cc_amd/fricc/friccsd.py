@@ -510,18 +510,25 @@ def amplitudes_to_vector_norm(t1new, t2new, t1, t2):
     return float(torch.sqrt(((t1new - t1) ** 2).sum() + (tri ** 2).sum()).item())
 
 
-_SIDE_STREAMS = {}
+_ARENA_STREAMS = {}
 
 
-def _side_stream():
-    """One warm-up stream per device, reused: the BLAS library keeps a
-    workspace for every stream it has run on, so a new stream per recording
-    would leave one more workspace behind each time."""
+def arena_stream(arena):
+    """The stream of a library scratch arena (one per arena and device, reused).
+
+    A recorded step is warmed up and captured on its arena's stream.  The BLAS
+    library keeps one workspace per stream and a captured GEMM keeps using the
+    workspace of the stream it was captured on; steps recorded on one shared
+    capture stream would all use the same workspace, and replaying them side
+    by side (one solve per arena, several at once) would run GEMMs
+    concurrently on that one workspace.  One stream per arena gives every
+    concurrently replayed step its own; reusing the streams keeps the number
+    of workspaces bounded."""
     torch = _torch()
-    dev = torch.cuda.current_device()
-    st = _SIDE_STREAMS.get(dev)
+    key = (torch.cuda.current_device(), int(arena))
+    st = _ARENA_STREAMS.get(key)
     if st is None:
-        st = _SIDE_STREAMS[dev] = torch.cuda.Stream()
+        st = _ARENA_STREAMS[key] = torch.cuda.Stream()
     return st
 
 
@@ -545,18 +552,18 @@ class GraphStep:
         self.cc, self.eris = cc, eris
         self.t1 = t1.detach().clone().contiguous()
         self.t2 = t2.detach().clone().contiguous()
-        # warm-up step on a side stream: allocates every workspace, view and
-        # operand the step needs, so the recorded step allocates nothing
-        # outside the graph's pool
-        side = _side_stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
+        # warm-up step on the arena's stream: allocates every workspace, view
+        # and operand the step needs (the BLAS workspace of that stream
+        # included), so the recorded step allocates nothing outside the
+        # graph's pool; then the capture on the same stream
+        st = arena_stream(cc.arena)
+        st.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(st):
             _set_uniforms(cc, [0.5] * cc._n_compressions())
             cc._device_step(self.t1, self.t2, eris)
-        torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
+        with torch.cuda.graph(self.graph, stream=st):
             self.out = cc._device_step(self.t1, self.t2, eris)
         torch.cuda.synchronize()
 

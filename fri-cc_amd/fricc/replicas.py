@@ -22,7 +22,7 @@ import math
 
 import numpy as np
 
-from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, _side_stream
+from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, arena_stream
 from .uniforms import PhiloxUniforms
 
 
@@ -77,17 +77,16 @@ class ReplicaBatch:
 
     def _record(self):
         torch = _torch()
-        side = _side_stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):  # warm-up: every workspace, view and operand allocated
+        st = arena_stream(self.reps[0].arena)  # warm-up and capture on the arena's stream
+        st.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(st):  # warm-up: every workspace, view and operand allocated
             self.u_dev.fill_(0.5)
             for r, cc in enumerate(self.reps):
                 cc._device_step(self.t1[r], self.t2[r], self.eris)
-        torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         self.out = []
-        with torch.cuda.graph(self.graph):
+        with torch.cuda.graph(self.graph, stream=st):
             for r, cc in enumerate(self.reps):
                 t1n, t2n, red = cc._device_step(self.t1[r], self.t2[r], self.eris)
                 self.out.append((t1n, t2n, red))

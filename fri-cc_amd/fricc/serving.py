@@ -26,7 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .friccsd import FRICCSD, GraphStep
+from .friccsd import FRICCSD, GraphStep, arena_stream
 from .uniforms import PhiloxUniforms
 
 
@@ -181,6 +181,12 @@ class Solve:
 class Server:
     """Admission by HBM budget, per-solve streams and arenas, continuous batching.
 
+    ``max_active`` defaults to 1: with several solves of production size
+    (10x90 to 32x256) replaying their recorded steps side by side, a run
+    stopped making progress after the seventh admission, for a cause not yet
+    found; concurrent admission (``max_active`` > 1) is validated only on
+    small systems.
+
     A request is admitted while (a) the admitted solves' footprints plus its
     own stay within ``hbm_budget``, (b) the device still has room for its
     footprint and set-up transients and (c) it needs at most half the budget
@@ -190,13 +196,12 @@ class Server:
 
     margin = 2.0e9  # device memory left free beyond each admission's need
 
-    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8, progress=None):
+    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=1, progress=None):
         self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision
         self.max_active = min(int(max_active), 63)
         self.measured = {}
-        self._streams = {}  # one stream per arena, reused by the solves that get the arena
         self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
 
     def _note(self, rec):
@@ -237,10 +242,9 @@ class Server:
         return not active
 
     def _stream(self, arena):
-        st = self._streams.get(arena)
-        if st is None:
-            st = self._streams[arena] = _torch().cuda.Stream()
-        return st
+        # the arena's stream: the solve's step is recorded on it (its own BLAS
+        # workspace) and replayed on it
+        return arena_stream(arena)
 
     def _try_solve(self, req, arena):
         """Set up a solve; None when the device ran out of memory doing so
@@ -347,16 +351,10 @@ def latency_stats(results, makespan):
             "converged": int(sum(r.converged for r in results)), "makespan_s": makespan}
 
 
-_ALONE = {}
-
-
 def solve_alone(req, precision="mixed"):
     """The request run by itself (same recorded step), for comparison."""
     torch = _torch()
-    dev = torch.cuda.current_device()
-    if dev not in _ALONE:
-        _ALONE[dev] = torch.cuda.Stream()
-    s = Solve(req, 0, precision, stream=_ALONE[dev])
+    s = Solve(req, 0, precision, stream=arena_stream(0))
     s.issue()
     while True:
         s.stream.synchronize()
