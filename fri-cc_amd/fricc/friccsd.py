@@ -333,10 +333,14 @@ def _update_amps_device(cc, t1, t2, eris):
     # -2 lcki,klac + kcli,klac  ==  sum_kl Aoo[kl] (i x c) @ t2[kl]^T (c x a)
     Aoo = (-2 * ovoo.permute(2, 0, 3, 1) + ovoo.permute(0, 2, 3, 1)).reshape(o * o, o, v)
     t1new += C(torch.bmm(Aoo.to(cdt), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
-    t1new += -2 * torch.einsum("lcki,lc,ka->ia", ovoo, t1, t1) + torch.einsum("kcli,lc,ka->ia", ovoo, t1, t1)
+    # Pk[k,i] = sum_lc (lc|ki) t1[l,c] and Qk[k,i] = sum_lc (kc|li) t1[l,c], shared
+    # by the t1 equation and Loo; plain matrix-vector products over ovoo
+    Pk = torch.mv(ovoo.permute(2, 3, 0, 1).reshape(o * o, ov), t1.reshape(ov)).reshape(o, o)
+    Qk = torch.mv(ovoo.permute(0, 3, 1, 2).reshape(o * o, ov), t1.T.reshape(ov)).reshape(o, o)
+    t1new += (Qk - 2 * Pk).T @ t1  # -2 lcki,lc,ka + kcli,lc,ka
 
     # ---- Loo, Lvv (imd.Loo / imd.Lvv)
-    Loo = Foo_u + fov @ t1.T + 2 * torch.einsum("lcki,lc->ki", ovoo, t1) - torch.einsum("kcli,lc->ki", ovoo, t1)
+    Loo = Foo_u + fov @ t1.T + 2 * Pk - Qk
     Lvv = Fvv_u - t1.T @ fov + LvvT1
     Loo.diagonal().sub_(mo_e_o)
     Lvv.diagonal().sub_(mo_e_v)
