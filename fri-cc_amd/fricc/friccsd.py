@@ -329,14 +329,13 @@ def _update_amps_device(cc, t1, t2, eris):
     # sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac)
     LvvT1 = C(torch.matmul(t1c.reshape(o, 1, 1, v), P["Z"]).sum(0).reshape(v, v).T)
     t1new += t1 @ LvvT1.T
-    ovoo = eris.ovoo
     # -2 lcki,klac + kcli,klac  ==  sum_kl Aoo[kl] (i x c) @ t2[kl]^T (c x a)
-    Aoo = (-2 * ovoo.permute(2, 0, 3, 1) + ovoo.permute(0, 2, 3, 1)).reshape(o * o, o, v)
-    t1new += C(torch.bmm(Aoo.to(cdt), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
+    t1new += C(torch.bmm(P["Aoo"].reshape(o * o, o, v), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
     # Pk[k,i] = sum_lc (lc|ki) t1[l,c] and Qk[k,i] = sum_lc (kc|li) t1[l,c], shared
-    # by the t1 equation and Loo; plain matrix-vector products over ovoo
-    Pk = torch.mv(ovoo.permute(2, 3, 0, 1).reshape(o * o, ov), t1.reshape(ov)).reshape(o, o)
-    Qk = torch.mv(ovoo.permute(0, 3, 1, 2).reshape(o * o, ov), t1.T.reshape(ov)).reshape(o, o)
+    # by the t1 equation and Loo: matrix-vector products over row-major copies
+    # of ovoo, one output element per row
+    Pk = torch.mv(P64["ovoo_P"].reshape(o * o, ov), t1.reshape(ov)).reshape(o, o)
+    Qk = torch.mv(P64["ovoo_Q"].reshape(o * o, ov), t1.T.contiguous().reshape(ov)).reshape(o, o)
     t1new += (Qk - 2 * Pk).T @ t1  # -2 lcki,lc,ka + kcli,lc,ka
 
     # ---- Loo, Lvv (imd.Loo / imd.Lvv)
@@ -350,8 +349,11 @@ def _update_amps_device(cc, t1, t2, eris):
     # U[k,c,l,i] = sum_d (kc|ld) t1[i,d]: with (ld|kc) = (kc|ld) and (kc|ai) = (ia|kc)
     # it gives the t1 pieces of Woooo, of both A_vo intermediates and of T2p
     U = (P["ovov"].reshape(ov * o, v) @ t1c.T).reshape(o, v, o, o)
-    Woooo = torch.einsum("lcki,jc->klij", ovoo, t1) + torch.einsum("kclj,ic->klij", ovoo, t1)
-    Woooo = Woooo.contiguous()
+    # lcki,jc->klij + kclj,ic->klij from one product G[p,q,r,x] = sum_c (pc|qr) t1[x,c]:
+    # the first term is G[l,k,i,j], the second G[k,l,j,i]
+    G = (P64["ovoo_Y"].reshape(o * o * o, v) @ t1.T).reshape(o, o, o, o)
+    Woooo = G.permute(1, 0, 2, 3).contiguous()
+    Woooo += G.permute(0, 1, 3, 2)
     if "O^4V^2" in ccs:
         _contract_into("kcld,ijcd->klij", 1.0, eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
     else:

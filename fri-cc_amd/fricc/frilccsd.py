@@ -108,9 +108,7 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
                    - torch.matmul(P["oovv"].reshape(o, ov, v), t1c.view(o, v, 1)).sum(0).reshape(o, v))
         # 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] @ Z[(kcd),a]
         t1new += C(_mm_splitk(t2c.reshape(o, -1), P["Z"].reshape(-1, v)))
-        ovoo = P["ovoo"]
-        Aoo = (-2 * ovoo.permute(2, 0, 3, 1) + ovoo.permute(0, 2, 3, 1)).reshape(o * o, o, v)
-        t1new += C(torch.bmm(Aoo, t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
+        t1new += C(torch.bmm(P["Aoo"].reshape(o * o, o, v), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
         fovc = fov.to(cdt)
         t1new += C(2 * torch.matmul(fovc.view(o, 1, 1, v), t2c).sum(0).reshape(o, v)
                    - torch.matmul(fovc.reshape(1, ov), t2c.reshape(o, ov, v)).reshape(o, v))
@@ -119,7 +117,7 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
     # Term 8: sum_c (ia|bc) t1[j,c];  Term 7: - sum_k (ia|jk) t1[k,b]
     S = torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v)).reshape(o, o, v, v)
     S3 = S.reshape(o * o, v, v)
-    S3 -= torch.matmul(P["ovoo"].permute(0, 2, 1, 3), t1c).reshape(o * o, v, v)
+    S3 -= (P["ovoo_ijak"].reshape(o * o * v, o) @ t1c).reshape(o * o, v, v)
     # Term 3: Fvv[a,c] t2[i,j,c,b];  Term 2: - Foo[k,i] t2[k,j,a,b]
     S3.baddbmm_(Fvv.to(cdt).expand(o * o, v, v), t2c.reshape(o * o, v, v))
     S.reshape(o, o * v * v).addmm_(Foo.to(cdt).T, t2c.reshape(o, -1), alpha=-1)

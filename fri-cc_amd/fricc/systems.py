@@ -247,6 +247,18 @@ class DeviceERIs:
             return c(self.ovvo.permute(0, 1, 3, 2))     # [k,c,i,a] = (kc|ai)
         if name == "oovv_p":
             return c(self.oovv.permute(0, 3, 1, 2))     # [k,c,i,a] = (ki|ac)
+        if name == "ovoo_P":
+            return c(self.ovoo.permute(2, 3, 0, 1))     # [k,i,l,c] = (lc|ki)
+        if name == "ovoo_Q":
+            return c(self.ovoo.permute(0, 3, 1, 2))     # [k,i,c,l] = (kc|li)
+        if name == "ovoo_Y":
+            return c(self.ovoo.permute(0, 2, 3, 1))     # [l,k,i,c] = (lc|ki)
+        if name == "ovoo_ijak":
+            return c(self.ovoo.permute(0, 2, 1, 3))     # [i,j,a,k] = (ia|jk)
+        if name == "Aoo":
+            # [k,l,i,c] = -2 (lc|ki) + (kc|li)
+            Y = self.ovoo.permute(0, 2, 3, 1)
+            return c(-2 * Y.permute(1, 0, 2, 3) + Y)
         if name in ("ovoo", "oovv", "ovvo", "oooo"):
             return c(getattr(self, name))
         if name == "Woooo_b":
