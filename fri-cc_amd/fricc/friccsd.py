@@ -579,12 +579,16 @@ class GraphStep:
         _set_uniforms(cc, _draw_uniforms(cc))
         self.graph.replay()
 
-    def complete(self):
+    def complete(self, before_copy=None):
         """Read the iteration's energy, norm and statuses (waits for the current
-        stream), then queue the copy of its outputs into the inputs."""
+        stream), then queue the copy of its outputs into the inputs.
+        ``before_copy`` (optional) is called between the two, with the current
+        stream, so a scheduler can order the copies after other work."""
         cc = self.cc
         t1n, t2n, red = self.out
         _finish_step(cc, t1n, t2n, red)
+        if before_copy is not None:
+            before_copy()
         e, nt = cc._last_step[2], cc._last_step[3]
         self.t1.copy_(t1n)
         self.t2.copy_(t2n)

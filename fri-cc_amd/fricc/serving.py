@@ -163,11 +163,13 @@ class Solve:
     def ready(self):
         return self.stream.query()
 
-    def complete(self):
-        """Read the finished iteration; True when the solve is done."""
+    def complete(self, order=None):
+        """Read the finished iteration; True when the solve is done.  With an
+        ``order`` (DeviceOrder), the copies that follow the read are queued
+        after the work queued before them."""
         torch = _torch()
         with torch.cuda.stream(self.stream):
-            self.runner.complete()
+            self.runner.complete(None if order is None else (lambda: order.begin(self.stream)))
         e = self.cc._last_step[2]
         de = e - self.e
         self.e = e
@@ -178,14 +180,42 @@ class Solve:
         return self.done
 
 
+class DeviceOrder:
+    """Runs the device work of all solves one after another, in the order it
+    is queued, whatever stream it is on: work queued on a stream first waits
+    for the end of the work queued before it (``begin``), and its own end is
+    what the next work waits for (``end``).  The host never waits for it."""
+
+    def __init__(self):
+        self.tail = None
+
+    def begin(self, stream):
+        if self.tail is not None:
+            stream.wait_event(self.tail)
+
+    def end(self, stream):
+        ev = _torch().cuda.Event()
+        ev.record(stream)
+        self.tail = ev
+
+
 class Server:
     """Admission by HBM budget, per-solve streams and arenas, continuous batching.
 
-    ``max_active`` defaults to 1: with several solves of production size
-    (10x90 to 32x256) replaying their recorded steps side by side, a run
-    stopped making progress after the seventh admission, for a cause not yet
-    found; concurrent admission (``max_active`` > 1) is validated only on
-    small systems.
+    Up to ``max_active`` solves are admitted at once and their iterations
+    interleave: whenever one solve's iteration has finished, its next one is
+    queued behind whatever is queued already.  With ``overlap`` False (the
+    default) the device runs one solve's work at a time (DeviceOrder), and a
+    solve's set-up starts once the device is idle.  The BLAS library's GEMM
+    kernels (stream-K: workgroups of one grid wait for partial tiles from
+    other workgroups of the same grid) assume that their whole grid can be
+    resident at once; work of other solves running beside them -- other such
+    grids, or the sparse GEMM that holds a whole CU per workgroup -- breaks
+    that assumption, and two grids each waiting for their own unscheduled
+    workgroups block each other.  With ``overlap`` True, solves' iterations
+    run side by side on the device; that is validated only on small systems
+    (a run of several production-size solves side by side stopped making
+    progress).
 
     A request is admitted while (a) the admitted solves' footprints plus its
     own stay within ``hbm_budget``, (b) the device still has room for its
@@ -196,11 +226,12 @@ class Server:
 
     margin = 2.0e9  # device memory left free beyond each admission's need
 
-    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=1, progress=None):
+    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8, progress=None, overlap=False):
         self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision
         self.max_active = min(int(max_active), 63)
+        self.overlap = bool(overlap)
         self.measured = {}
         self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
 
@@ -273,6 +304,7 @@ class Server:
         self.deferred = 0
         self._dirty = set()  # arenas whose scratch may hold memory
         wait_retire = False  # a set-up ran out of memory: admit again after a retirement
+        order = None if self.overlap else DeviceOrder()
         t0 = clock()
         while pending or active:
             now = clock() - t0
@@ -280,6 +312,8 @@ class Server:
                 if not self._fits(pending[0], used, active, free_arenas):
                     break
                 req = pending[0]
+                if order is not None:
+                    torch.cuda.synchronize()  # the set-up runs with nothing else on the device
                 before = device_available()
                 arena = free_arenas.pop(0)
                 self._dirty.add(arena)
@@ -307,7 +341,11 @@ class Server:
                 used += s.need
                 self._note((round(s.admitted, 3), "admit", req.id, req.nocc, req.nvir, s.need, used,
                                  _torch().cuda.mem_get_info()[0]))
+                if order is not None:
+                    order.begin(s.stream)
                 s.issue()
+                if order is not None:
+                    order.end(s.stream)
                 active.append(s)
                 self.max_concurrent = max(self.max_concurrent, len(active))
             if not active:
@@ -318,7 +356,8 @@ class Server:
                 if not s.ready():
                     continue
                 progressed = True
-                if s.complete():
+                done = s.complete(order)
+                if done:
                     fin = clock() - t0
                     results.append(Result(s.req.id, s.req.nocc, s.req.nvir, s.req.m, s.req.arrival, s.admitted, fin,
                                           len(s.energies), s.converged, list(s.energies)))
@@ -331,6 +370,8 @@ class Server:
                     retired = True
                 else:
                     s.issue()
+                if order is not None:
+                    order.end(s.stream)
             s = None
             if retired:
                 torch.cuda.empty_cache()  # the retired solves' buffers back to the device

@@ -1,6 +1,7 @@
-"""Serving loop: solves admitted by an HBM budget run side by side (own
-streams, scratch arenas and recorded steps); each solve's energies are the
-same alone and in the mix."""
+"""Serving loop: solves admitted by an HBM budget interleave their
+iterations (own streams, scratch arenas and recorded steps), one solve's work
+at a time on the device or side by side; each solve's energies are the same
+alone and in the mix."""
 import numpy as np
 import pytest
 
@@ -27,11 +28,12 @@ def _requests(sv):
     return reqs
 
 
+@pytest.mark.parametrize("overlap", [False, True])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_alone_equals_in_the_mix(serving, prec):
+def test_alone_equals_in_the_mix(serving, prec, overlap):
     sv = serving
     reqs = _requests(sv)
-    server = sv.Server(hbm_budget=1e12, precision=prec, max_active=8)
+    server = sv.Server(hbm_budget=1e12, precision=prec, max_active=8, overlap=overlap)
     res = server.run(reqs)
     assert len(res) == len(reqs) and server.max_concurrent > 1
     for r, q in zip(res, reqs):
