@@ -326,16 +326,18 @@ def serve_bench(args):
         t, ev, rid, o, v = rec[:5]
         print(f"# serve t={t:.2f}s {ev} {rid} ({o}x{v})", file=sys.stderr, flush=True)
 
-    # up to 8 solves admitted, their iterations interleaved, one solve's work
-    # at a time on the device (fricc.serving.Server, overlap off)
-    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=args.serve_active, progress=progress)
+    # up to 8 solves admitted, their iterations interleaved (shortest remaining
+    # work first by default), one solve's work at a time on the device
+    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=args.serve_active, progress=progress,
+                       policy=args.serve_policy)
     torch.cuda.empty_cache()
     at_start = {"allocated": torch.cuda.memory_allocated(), "device_free": torch.cuda.mem_get_info()[0]}
     print(f"# serving: memory at start {at_start}", file=sys.stderr, flush=True)
     res = server.run(reqs)
     st = sv.latency_stats(res, server.makespan)
     st.update({"sizes": [f"{o}x{v}" for o, v in sizes], "rate_per_s": args.serve_rate,
-               "max_active": args.serve_active, "device_order": "one solve at a time (interleaved iterations)",
+               "max_active": args.serve_active, "policy": args.serve_policy,
+               "device_order": "one solve at a time (interleaved iterations)",
                "max_concurrent": server.max_concurrent, "hbm_budget_gb": 240,
                "footprint_gb": {f"{o}x{v}": round(b / 1e9, 2) for (o, v), b in server.measured.items()},
                "memory_at_start": at_start, "admission_log": server.log[:64],
@@ -363,6 +365,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0, help="also serve N solves arriving as a Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-active", type=int, default=8, help="solves admitted at once (interleaved)")
+    ap.add_argument("--serve-policy", default="srpt", choices=("srpt", "rr"),
+                    help="next iteration: shortest remaining work first, or round robin")
     ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
     ap.add_argument("--lccsd-modes", default="mixed,fp64", help="FRI-LCCSD iterations timed beside ('' for none)")
     ap.add_argument("--lccsd-steps", type=int, default=3)

@@ -153,6 +153,8 @@ class Solve:
         self.energies = []
         self.done = False
         self.converged = False
+        self.t_iter = None  # device time of the last iteration (ms), when measured
+        self.ready_at = 0.0
 
     def issue(self):
         torch = _torch()
@@ -194,19 +196,23 @@ class DeviceOrder:
             stream.wait_event(self.tail)
 
     def end(self, stream):
-        ev = _torch().cuda.Event()
+        ev = _torch().cuda.Event(enable_timing=True)
         ev.record(stream)
         self.tail = ev
+        return ev
 
 
 class Server:
     """Admission by HBM budget, per-solve streams and arenas, continuous batching.
 
     Up to ``max_active`` solves are admitted at once and their iterations
-    interleave: whenever one solve's iteration has finished, its next one is
-    queued behind whatever is queued already.  With ``overlap`` False (the
-    default) the device runs one solve's work at a time (DeviceOrder), and a
-    solve's set-up starts once the device is idle.  The BLAS library's GEMM
+    interleave.  With ``overlap`` False (the default) the device runs one
+    solve's work at a time (DeviceOrder), a solve's set-up starts once the
+    device is idle, and ``policy`` picks the next iteration: "srpt" keeps
+    ``depth`` iterations queued and gives the device to the solve with the
+    least remaining work (shortest remaining processing time first, which
+    minimises the mean latency of a single server); "rr" queues every solve's
+    next iteration as soon as its previous one has finished (round robin).  The BLAS library's GEMM
     kernels (stream-K: workgroups of one grid wait for partial tiles from
     other workgroups of the same grid) assume that their whole grid can be
     resident at once; work of other solves running beside them -- other such
@@ -226,12 +232,18 @@ class Server:
 
     margin = 2.0e9  # device memory left free beyond each admission's need
 
-    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8, progress=None, overlap=False):
+    depth = 2  # iterations queued on the device at once ("srpt"): one runs while the host reads the other
+
+    def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8, progress=None, overlap=False,
+                 policy="srpt"):
+        if policy not in ("srpt", "rr"):
+            raise ValueError(f"unknown policy {policy!r} (srpt, rr)")
         self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision
         self.max_active = min(int(max_active), 63)
         self.overlap = bool(overlap)
+        self.policy = policy
         self.measured = {}
         self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
 
@@ -293,6 +305,27 @@ class Server:
         torch.cuda.empty_cache()
         return None
 
+    def _priority(self, s):
+        """Order in which waiting solves get the device (smallest first): a
+        solve not yet measured first (its first iteration measures it), then
+        the least remaining work, iterations left times the solve's measured
+        iteration time; "rr": the order in which they became ready."""
+        if self.policy == "rr":
+            return (1, s.ready_at)
+        if s.t_iter is None:
+            return (0, s.ready_at)
+        return (1, (s.req.max_cycle - len(s.energies)) * s.t_iter)
+
+    def _queue(self, s, order):
+        """Queue the solve's next iteration behind the work queued so far and
+        time it on the device."""
+        torch = _torch()
+        order.begin(s.stream)
+        s.ev0 = torch.cuda.Event(enable_timing=True)
+        s.ev0.record(s.stream)
+        s.issue()
+        s.ev1 = order.end(s.stream)
+
     def run(self, requests, clock=time.perf_counter):
         torch = _torch()
         from . import _capi
@@ -300,11 +333,13 @@ class Server:
         pending = sorted(requests, key=lambda r: r.arrival)
         free_arenas = list(range(1, self.max_active + 1))
         active, results, used = [], [], 0.0
+        queued, waiting = [], []  # iteration on the device / ready for its next one
         self.max_concurrent = 0
         self.deferred = 0
         self._dirty = set()  # arenas whose scratch may hold memory
         wait_retire = False  # a set-up ran out of memory: admit again after a retirement
         order = None if self.overlap else DeviceOrder()
+        depth = self.max_active if (order is None or self.policy == "rr") else self.depth
         t0 = clock()
         while pending or active:
             now = clock() - t0
@@ -330,33 +365,34 @@ class Server:
                     self.measured[key] = max(self.measured.get(key, 0.0), 1.25 * self._need(req))
                     self.deferred += 1
                     self._note((round(clock() - t0, 3), "deferred", req.id, req.nocc, req.nvir, None, used,
-                                     torch.cuda.mem_get_info()[0]))
+                                torch.cuda.mem_get_info()[0]))
                     wait_retire = True
                     break
                 pending.pop(0)
-                s.admitted = clock() - t0
+                s.admitted = s.ready_at = clock() - t0
                 key = (req.nocc, req.nvir)
                 self.measured[key] = max(self.measured.get(key, 0.0), float(before - device_available()))
                 s.need = self._need(req)
                 used += s.need
                 self._note((round(s.admitted, 3), "admit", req.id, req.nocc, req.nvir, s.need, used,
-                                 _torch().cuda.mem_get_info()[0]))
-                if order is not None:
-                    order.begin(s.stream)
-                s.issue()
-                if order is not None:
-                    order.end(s.stream)
+                            torch.cuda.mem_get_info()[0]))
                 active.append(s)
+                waiting.append(s)
                 self.max_concurrent = max(self.max_concurrent, len(active))
             if not active:
                 time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
             progressed = retired = False
-            for s in list(active):
+            for s in list(queued):
                 if not s.ready():
                     continue
                 progressed = True
+                queued.remove(s)
+                if order is not None:
+                    s.t_iter = s.ev0.elapsed_time(s.ev1)
                 done = s.complete(order)
+                if order is not None:
+                    order.end(s.stream)  # the copies of the outputs into the inputs
                 if done:
                     fin = clock() - t0
                     results.append(Result(s.req.id, s.req.nocc, s.req.nvir, s.req.m, s.req.arrival, s.admitted, fin,
@@ -369,9 +405,18 @@ class Server:
                     wait_retire = False
                     retired = True
                 else:
+                    s.ready_at = clock() - t0
+                    waiting.append(s)
+            while waiting and len(queued) < depth:
+                progressed = True
+                if order is None:
+                    s = waiting.pop(0)
                     s.issue()
-                if order is not None:
-                    order.end(s.stream)
+                else:
+                    s = min(waiting, key=self._priority)
+                    waiting.remove(s)
+                    self._queue(s, order)
+                queued.append(s)
             s = None
             if retired:
                 torch.cuda.empty_cache()  # the retired solves' buffers back to the device

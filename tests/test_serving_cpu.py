@@ -36,3 +36,27 @@ def test_large_solve_is_admitted_alone():
     assert sv.estimate_bytes(48, 352) > 0.5 * server.budget
     assert not server._fits(big, used=1.0, active=running, idle=[])
     assert not server._fits(small, used=server.budget, active=running, idle=[])
+
+
+def test_srpt_priority():
+    """Shortest remaining work first: an unmeasured solve (its first
+    iteration) before any measured one, then iterations left x measured
+    iteration time; round robin keeps the order in which solves became ready."""
+    from types import SimpleNamespace as NS
+
+    def solve(max_cycle, done, t_iter, ready_at):
+        return NS(req=NS(max_cycle=max_cycle), energies=[0.0] * done, t_iter=t_iter, ready_at=ready_at)
+
+    big_late = solve(50, 45, 30.0, 0.1)     # 5 left x 30 ms = 150
+    small = solve(50, 10, 2.0, 0.3)         # 40 left x 2 ms = 80
+    fresh = solve(50, 0, None, 0.4)
+    big = solve(50, 0, 30.0, 0.2)           # 1500
+    server = sv.Server(policy="srpt")
+    order = sorted([big_late, small, fresh, big], key=server._priority)
+    assert order == [fresh, small, big_late, big]
+    rr = sv.Server(policy="rr")
+    assert sorted([big_late, small, fresh, big], key=rr._priority) == [big_late, big, small, fresh]
+    import pytest
+
+    with pytest.raises(ValueError):
+        sv.Server(policy="lifo")

@@ -28,12 +28,12 @@ def _requests(sv):
     return reqs
 
 
-@pytest.mark.parametrize("overlap", [False, True])
+@pytest.mark.parametrize("overlap,policy", [(False, "srpt"), (False, "rr"), (True, "rr")])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_alone_equals_in_the_mix(serving, prec, overlap):
+def test_alone_equals_in_the_mix(serving, prec, overlap, policy):
     sv = serving
     reqs = _requests(sv)
-    server = sv.Server(hbm_budget=1e12, precision=prec, max_active=8, overlap=overlap)
+    server = sv.Server(hbm_budget=1e12, precision=prec, max_active=8, overlap=overlap, policy=policy)
     res = server.run(reqs)
     assert len(res) == len(reqs) and server.max_concurrent > 1
     for r, q in zip(res, reqs):
