@@ -22,6 +22,8 @@ scheduled many at a time.
 """
 import math
 import time
+from collections import OrderedDict
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -123,21 +125,60 @@ def device_free(need):
     return free
 
 
+def system_key(req):
+    """(nocc, nvir, naux, sys_seed): the synthetic system a request names."""
+    return (req.nocc, req.nvir, req.naux or 3 * (req.nocc + req.nvir), req.sys_seed)
+
+
+class SystemCache:
+    """Host factors of the systems requests name.  Each is generated on a
+    worker thread as soon as a request for it arrives (numpy releases the
+    interpreter lock while it fills arrays, so the scheduler keeps queueing
+    iterations meanwhile) and kept for later requests of the same system,
+    least recently requested first out beyond ``max_bytes``."""
+
+    def __init__(self, max_bytes=16e9):
+        self.max_bytes = float(max_bytes)
+        self.entries = OrderedDict()  # system_key -> Future of SyntheticSystem
+        self._pool = None
+
+    def request(self, req):
+        """The future of the request's system (submitted on first request)."""
+        from .systems import SyntheticSystem
+
+        k = system_key(req)
+        fut = self.entries.get(k)
+        if fut is None:
+            if self._pool is None:
+                self._pool = ThreadPoolExecutor(max_workers=1)
+            fut = self.entries[k] = self._pool.submit(SyntheticSystem, *k)
+        self.entries.move_to_end(k)
+        total = sum(f.result().L.nbytes for f in self.entries.values() if f.done() and f.exception() is None)
+        for key in list(self.entries)[:-1]:
+            if total <= self.max_bytes:
+                break
+            f = self.entries[key]
+            if f.done():
+                if f.exception() is None:
+                    total -= f.result().L.nbytes
+                del self.entries[key]
+        return fut
+
+
 class Solve:
     """One admitted request: its system, solver, recorded step and stream.
     ``stream``: the stream to run on (a new one when None; the server reuses
     one per arena, since the BLAS library keeps a workspace per stream it has
-    seen)."""
+    seen).  ``system``: the request's SyntheticSystem when already generated."""
 
-    def __init__(self, req, arena, precision, stream=None):
+    def __init__(self, req, arena, precision, stream=None, system=None):
         torch = _torch()
         from .systems import DeviceERIs, SyntheticSystem
 
         self.req = req
         self.stream = stream if stream is not None else torch.cuda.Stream()
-        naux = req.naux or 3 * (req.nocc + req.nvir)
         with torch.cuda.stream(self.stream):
-            sysm = SyntheticSystem(req.nocc, req.nvir, naux, req.sys_seed)
+            sysm = system if system is not None else SyntheticSystem(*system_key(req))
             # reduced-precision solves never hold the FP64 ladder: its operand
             # is built slice by slice from the factors
             self.eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc,
@@ -244,6 +285,7 @@ class Server:
         self.max_active = min(int(max_active), 63)
         self.overlap = bool(overlap)
         self.policy = policy
+        self.systems = SystemCache()
         self.measured = {}
         self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
 
@@ -289,12 +331,12 @@ class Server:
         # workspace) and replayed on it
         return arena_stream(arena)
 
-    def _try_solve(self, req, arena):
+    def _try_solve(self, req, arena, system=None):
         """Set up a solve; None when the device ran out of memory doing so
         (its partial allocations are released)."""
         torch = _torch()
         try:
-            return Solve(req, arena, self.precision, stream=self._stream(arena))
+            return Solve(req, arena, self.precision, stream=self._stream(arena), system=system)
         except (torch.OutOfMemoryError, MemoryError, RuntimeError) as e:
             if isinstance(e, RuntimeError) and "out of memory" not in str(e) and "hipMalloc" not in str(e):
                 raise
@@ -330,7 +372,9 @@ class Server:
         torch = _torch()
         from . import _capi
 
-        pending = sorted(requests, key=lambda r: r.arrival)
+        by_arrival = sorted(requests, key=lambda r: r.arrival)
+        pending = list(by_arrival)
+        arrived = 0  # requests (in arrival order) whose system has been asked for
         free_arenas = list(range(1, self.max_active + 1))
         active, results, used = [], [], 0.0
         queued, waiting = [], []  # iteration on the device / ready for its next one
@@ -343,20 +387,28 @@ class Server:
         t0 = clock()
         while pending or active:
             now = clock() - t0
+            # a request's system is generated on the host as soon as it arrives
+            while arrived < len(by_arrival) and by_arrival[arrived].arrival <= now:
+                self.systems.request(by_arrival[arrived])
+                arrived += 1
             while pending and pending[0].arrival <= now and free_arenas and not wait_retire:
+                sysf = self.systems.request(pending[0])
+                if not sysf.done() and active:
+                    break  # keep the device fed until the system is ready
                 if not self._fits(pending[0], used, active, free_arenas):
                     break
                 req = pending[0]
+                system = sysf.result()
                 if order is not None:
                     torch.cuda.synchronize()  # the set-up runs with nothing else on the device
                 before = device_available()
                 arena = free_arenas.pop(0)
                 self._dirty.add(arena)
-                s = self._try_solve(req, arena)
+                s = self._try_solve(req, arena, system)
                 if s is None and not active and self._reclaim(free_arenas + [arena]):
                     before = device_available()
                     self._dirty.add(arena)
-                    s = self._try_solve(req, arena)
+                    s = self._try_solve(req, arena, system)
                 if s is None:
                     free_arenas.insert(0, arena)
                     if not active:

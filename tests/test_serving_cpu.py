@@ -60,3 +60,23 @@ def test_srpt_priority():
 
     with pytest.raises(ValueError):
         sv.Server(policy="lifo")
+
+
+def test_system_cache_keeps_recent_systems():
+    """Requests of one system share its host factors (generated once, on the
+    worker thread); beyond the byte bound the least recently requested
+    system is dropped."""
+    import numpy as np
+
+    from fricc.systems import SyntheticSystem
+
+    one = 39 * 13 * 13 * 8  # L of a 3x10 system, naux 39
+    cache = sv.SystemCache(max_bytes=2.5 * one)
+    reqs = [sv.Request(i, 0.0, 3, 10, m=1, sys_seed=s) for i, s in enumerate([1, 2, 1, 3])]
+    f0 = cache.request(reqs[0])
+    np.testing.assert_array_equal(f0.result().L, SyntheticSystem(3, 10, 39, 1).L)
+    cache.request(reqs[1]).result()
+    assert cache.request(reqs[2]) is f0
+    cache.request(reqs[3]).result()
+    cache.request(reqs[3])
+    assert list(cache.entries) == [(3, 10, 39, 1), (3, 10, 39, 3)]
