@@ -208,8 +208,14 @@ int fricc_sparse_compress(fricc_sparse* sp, const double* x, int64_t m, int comp
   FRICC_TRY(sparse_reserve(sp, m));
   sp->gen++;
   sp->nnz = 0;
-  int st = compress(x, n, m, compression, u, sp->idx, sp->val, &sp->info, (hipStream_t)stream);
-  if (st == OK) sp->nnz = m;
+  sp->dup_dev = nullptr;
+  sp->dup_host = 1;
+  int dup = 1;
+  int st = compress(x, n, m, compression, u, sp->idx, sp->val, &sp->info, (hipStream_t)stream, &dup);
+  if (st == OK) {
+    sp->nnz = m;
+    sp->dup_host = dup;
+  }
   return st;
 }
 
@@ -225,6 +231,8 @@ int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, in
   FRICC_TRY(compress_launch(x, n, m, compression, 0.0, u_dev, sp->idx, sp->val, sp->dsc, (hipStream_t)stream, &sc));
   sp->nnz = m;
   sp->pending = true;
+  sp->dup_host = 0;
+  sp->dup_dev = comp_scalars_dup_flag(sp->dsc);
   return OK;
 }
 
@@ -258,6 +266,8 @@ int fricc_sparse_set(fricc_sparse* sp, const int64_t* idx, const double* val, in
   }
   sp->nnz = nnz;
   sp->pending = false;
+  sp->dup_host = 1;  // entries set by the caller may repeat an index
+  sp->dup_dev = nullptr;
   memset(&sp->info, 0, sizeof(sp->info));
   sp->info.n_D = nnz;
   return OK;

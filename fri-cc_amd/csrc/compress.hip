@@ -48,7 +48,7 @@ struct CompScalars {
   uint64_t n_notfound;
   double u;        // the uniform of this call
   uint32_t ubad;   // u outside [0, 1]
-  uint32_t pad;
+  uint32_t dup;    // the output may repeat an index (two samples in one interval, or samples past the end)
 };
 
 // ---------------------------------------------------------------- pass A
@@ -467,7 +467,7 @@ __device__ inline int64_t first_k_at_least(double P, double u) {
 // whole range [P(first), P(last+1)) does (first_k_at_least of the rounded
 // prefix is monotone), so only those threads recompute their elements.
 __global__ void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
-                            const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
+                            CompScalars* sc, const u128* __restrict__ tsum,
                             const u128* __restrict__ thsum, uint32_t* __restrict__ out_idx,
                             double* __restrict__ out_val) {
   __shared__ u128 sm[NT / WAVE + 1];
@@ -508,10 +508,12 @@ __global__ void k_samp_emit(const double* __restrict__ x, const double* __restri
     pre += f;
     if (f != 0) {
       const double Phi = u128_to_double(pre, -96);
-      for (; kc < ns && (double)kc + u < Phi; kc++) {
+      int ne = 0;
+      for (; kc < ns && (double)kc + u < Phi; kc++, ne++) {
         out_idx[K + kc] = (uint32_t)j;
         if (out_val) out_val[K + kc] = v / p;
       }
+      if (ne > 1) sc->dup = 1u;  // j emitted more than once
     }
   }
 }
@@ -531,6 +533,7 @@ __global__ void k_samp_final(CompScalars* sc, const u128* __restrict__ total) {
   sc->Ptot = Ptot;
   sc->k_notfound = (uint64_t)(kn < ns ? kn : ns);
   sc->n_notfound = (uint64_t)(kn < ns ? ns - kn : 0);
+  if (kn < ns) sc->dup = 1u;  // the not-found samples all take index 0
 }
 
 // Positions past the last interval keep index 0 (src/fri_utils.cpp:132).
@@ -585,6 +588,8 @@ enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_
        WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11 };
 
 size_t comp_scalars_size() { return sizeof(CompScalars); }
+
+const uint32_t* comp_scalars_dup_flag(const void* dev_sc) { return &((const CompScalars*)dev_sc)->dup; }
 
 // Host view of a call's scalars: statistics into info, and the call's status.
 int comp_scalars_status(const void* host_sc, CompressInfo* info) {
@@ -692,13 +697,14 @@ int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_
 }
 
 int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx, double* out_val,
-             CompressInfo* info, hipStream_t s) {
+             CompressInfo* info, hipStream_t s, int* dup) {
   memset(info, 0, sizeof(*info));
   void* sc = nullptr;
   FRICC_TRY(compress_launch(x, n, m, method, u, nullptr, out_idx, out_val, nullptr, s, &sc));
   CompScalars h;
   FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
   FRICC_HIP(hipStreamSynchronize(s));
+  if (dup) *dup = h.dup != 0;
   return comp_scalars_status(&h, info);
 }
 

@@ -18,13 +18,16 @@ struct CompressInfo {
   int32_t nonfinite, pad;
 };
 
+// dup (optional): set when the output may repeat an index
 int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
-             double* out_val, CompressInfo* info, hipStream_t s);
+             double* out_val, CompressInfo* info, hipStream_t s, int* dup = nullptr);
 // The same compression with no host round trip (u from u_dev when given);
 // the call's scalars go to export_sc (device, comp_scalars_size() bytes).
 int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
                     uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s, void** sc_out);
 size_t comp_scalars_size();
+// device address of the "output may repeat an index" word of a call's scalars
+const uint32_t* comp_scalars_dup_flag(const void* dev_sc);
 // statistics of a call's scalars (host copy) into info; returns the call's status
 int comp_scalars_status(const void* host_sc, CompressInfo* info);
 int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
@@ -57,6 +60,10 @@ struct Sparse {
   CompressInfo info{};
   void* dsc = nullptr;   // device copy of the last asynchronous compression's scalars
   bool pending = false;  // info/status of an asynchronous compression not read yet
+  // whether the entries may repeat a flat index (then views add them up):
+  // dup_host, or the device word dup_dev when set (asynchronous compressions)
+  int dup_host = 1;
+  const uint32_t* dup_dev = nullptr;
   int device = 0;
   uint64_t gen = 1;  // bumped whenever the entries change
   std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<View>> views;

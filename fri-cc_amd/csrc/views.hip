@@ -108,30 +108,46 @@ __global__ void k_tile_slots(uint32_t* __restrict__ slot, uint32_t R, const uint
   for (uint32_t k = run + threadIdx.x; k < end; k += VNT) rowlist[k] = 0u;
 }
 
+// Entries with distinct flat indices land on distinct elements of the zeroed
+// block and are stored; entries that may repeat an index (dup) are added.
+__device__ inline bool may_repeat(const uint32_t* dup_dev, int dup_host) {
+  return dup_host != 0 || (dup_dev != nullptr && *dup_dev != 0u);
+}
+
 template <typename T>
 __global__ void k_fill(const uint32_t* __restrict__ idx, const double* __restrict__ val, int64_t nnz, VGeom g,
-                       const uint32_t* __restrict__ slot, T* __restrict__ B) {
+                       const uint32_t* __restrict__ slot, const uint32_t* __restrict__ dup_dev, int dup_host,
+                       T* __restrict__ B) {
   int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
   if (e >= nnz) return;
   uint32_t r, y;
   view_rc(idx[e], g, r, y);
   uint32_t t = y / g.TY;
   uint32_t k = slot[(int64_t)t * g.R + r];
-  atomicAdd(&B[(int64_t)k * g.TY + (y % g.TY)], (T)val[e]);
+  T* dst = &B[(int64_t)k * g.TY + (y % g.TY)];
+  if (may_repeat(dup_dev, dup_host))
+    atomicAdd(dst, (T)val[e]);
+  else
+    *dst = (T)val[e];
 }
 
 // BF16 view: accumulate into an FP32 buffer laid out [k/8][TY][8] by gathered
 // row k (eight consecutive gathered rows interleaved per column), split
 // into the bf16 planes afterwards
 __global__ void k_fill_grouped(const uint32_t* __restrict__ idx, const double* __restrict__ val, int64_t nnz, VGeom g,
-                               const uint32_t* __restrict__ slot, float* __restrict__ B) {
+                               const uint32_t* __restrict__ slot, const uint32_t* __restrict__ dup_dev, int dup_host,
+                               float* __restrict__ B) {
   int64_t e = (int64_t)blockIdx.x * VNT + threadIdx.x;
   if (e >= nnz) return;
   uint32_t r, y;
   view_rc(idx[e], g, r, y);
   uint32_t t = y / g.TY;
   uint32_t k = slot[(int64_t)t * g.R + r];
-  atomicAdd(&B[((int64_t)(k >> 3) * g.TY + (y % g.TY)) * 8 + (k & 7)], (float)val[e]);
+  float* dst = &B[((int64_t)(k >> 3) * g.TY + (y % g.TY)) * 8 + (k & 7)];
+  if (may_repeat(dup_dev, dup_host))
+    atomicAdd(dst, (float)val[e]);
+  else
+    *dst = (float)val[e];
 }
 
 // Tiles by decreasing gathered-row count (ties by index): order[rank] = tile.
@@ -246,9 +262,11 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
     k_zero_rows<<<1024, VNT, 0, s>>>(total, (int64_t)(rowbytes / 16), (uint4*)v->B);
     if (nnz > 0) {
       if (v->dtype == DT_F64)
-        k_fill<double><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (double*)v->B);
+        k_fill<double><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, sp->dup_dev,
+                                                           sp->dup_host, (double*)v->B);
       else
-        k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)v->B);
+        k_fill<float><<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, sp->dup_dev,
+                                                          sp->dup_host, (float*)v->B);
     }
   } else {
     const int64_t w = v->TY;  // FP32 values per gathered row
@@ -256,7 +274,8 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
     FRICC_TRY(ws_get(22, cap * (size_t)w * 4, &p_tmp));
     k_zero_rows<<<1024, VNT, 0, s>>>(total, w / 4, (uint4*)p_tmp);
     if (nnz > 0)
-      k_fill_grouped<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, (float*)p_tmp);
+      k_fill_grouped<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, sp->val, nnz, g, slot, sp->dup_dev,
+                                                         sp->dup_host, (float*)p_tmp);
     k_split_rows<<<2048, VNT, 0, s>>>(total, w, (const float*)p_tmp, (__hip_bfloat16*)v->B, (__hip_bfloat16*)v->Blo);
   }
   FRICC_HIP(hipGetLastError());

@@ -191,3 +191,29 @@ def test_empty_constructor(py_rccsd):
     T = py_rccsd.SparseTensor4d([2, 3, 4, 5], 7)
     assert T.nnz == 7
     assert T.get_element(6) == ([0, 0, 0, 0], 0.0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("term", ["2323", "1302", "0112"])
+def test_repeated_indices_add_up(py_rccsd, term, precision):
+    """Entries set by the caller may repeat a flat index; every contraction
+    adds them (the reference loops over the entries, src/sparse_tensor.cpp:63-77),
+    so the result is that of the dense tensor with repeats summed."""
+    no, nv = 5, 11
+    wshape, spec, alpha, oshape = CASES[term]
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(wshape(no, nv))
+    size = no * no * nv * nv
+    idx = rng.integers(0, size, 400)
+    idx[:50] = idx[50:100]  # 50 repeats
+    idx[100:110] = 0
+    val = rng.standard_normal(idx.size)
+    T = py_rccsd.SparseTensor4d([no, no, nv, nv], idx.size)
+    T.set_entries(idx, val)
+    dense = np.zeros(size)
+    np.add.at(dense, idx, val)
+    ref = alpha * np.einsum(spec, w, dense.reshape(no, no, nv, nv))
+    out = np.zeros(oshape(no, nv))
+    py_rccsd.contract_DTSpT(w, T, out, term, precision=precision)
+    tol = {"fp64": 1e-12, "fp32": 2e-5, "bf16": 2e-4}[precision]
+    npt.assert_allclose(out, ref, rtol=0, atol=tol * np.abs(ref).max())
