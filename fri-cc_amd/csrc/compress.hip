@@ -222,16 +222,11 @@ __global__ void k_digit_total(const uint32_t* __restrict__ blkhist, int64_t ntma
 __global__ void k_digit_offsets(uint32_t* __restrict__ blkhist, int64_t ntmax, const CompScalars* __restrict__ sc,
                                 const uint32_t* __restrict__ dtot) {
   __shared__ uint32_t sm[NT / WAVE + 1];
-  __shared__ uint32_t s_base;
   const int64_t nta = live_tiles(sc);
-  if (threadIdx.x == 0) {
-    uint32_t b = 0;
-    for (uint32_t d = 0; d < blockIdx.x; d++) b += dtot[d];
-    s_base = b;
-  }
-  __syncthreads();
+  // start of digit blockIdx.x: the totals of all smaller digits
+  uint32_t run;
+  block_excl_scan_u32<NT>(threadIdx.x < blockIdx.x ? dtot[threadIdx.x] : 0u, &run, sm);
   uint32_t* row = blkhist + (int64_t)blockIdx.x * ntmax;
-  uint32_t run = s_base;
   for (int64_t c0 = 0; c0 < nta; c0 += NT) {
     const int64_t t = c0 + threadIdx.x;
     const uint32_t v = t < nta ? row[t] : 0u;
@@ -243,49 +238,83 @@ __global__ void k_digit_offsets(uint32_t* __restrict__ blkhist, int64_t ntmax, c
   }
 }
 
-// Stable scatter: tile order = index order (rounds of NT, wave order, lane order).
-__global__ void k_sort_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                               const CompScalars* __restrict__ sc, int shift, const uint32_t* __restrict__ blkoff,
-                               int64_t ntmax) {
-  __shared__ uint32_t run[256];
+// Stable scatter: tile order = index order (rounds of NT, wave order, lane
+// order).  Each tile is first put in digit order in LDS (stable), then written
+// out so that consecutive threads write consecutive places of a digit's run.
+__global__ __launch_bounds__(NT) void k_sort_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                     const CompScalars* __restrict__ sc, int shift,
+                                                     const uint32_t* __restrict__ blkoff, int64_t ntmax) {
+  __shared__ uint64_t sk[TILE];
+  __shared__ uint32_t sv[TILE];
+  __shared__ uint32_t gbase[256];   // this tile's first place in each digit's run (global)
+  __shared__ uint32_t lstart[256];  // first place of each digit in the tile's digit order
+  __shared__ uint32_t lrun[256];    // places of each digit taken so far
   __shared__ uint32_t wcnt[NT / WAVE][256];
+  __shared__ uint32_t sm[NT / WAVE + 1];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   const int64_t n = (int64_t)sc->n_cand, nta = live_tiles(sc);
   const uint64_t lt = (1ull << lane) - 1ull;
   for (int64_t t = blockIdx.x; t < nta; t += gridDim.x) {
-    run[threadIdx.x] = blkoff[(int64_t)threadIdx.x * ntmax + t];
     const int64_t base = t * TILE;
+    const int cnt = (int)(n - base < TILE ? n - base : TILE);
+    gbase[threadIdx.x] = blkoff[(int64_t)threadIdx.x * ntmax + t];
+    lrun[threadIdx.x] = 0;
+    uint64_t k[ITEMS];
+    uint32_t v[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+      const int e = r * NT + threadIdx.x;
+      k[r] = e < cnt ? kin[base + e] : 0;
+      v[r] = e < cnt ? vin[base + e] : 0;
+    }
+    // digit counts of the tile -> local starts
+    wcnt[0][threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++)
+      if (r * NT + (int)threadIdx.x < cnt) atomicAdd(&wcnt[0][(k[r] >> shift) & 255], 1u);
+    __syncthreads();
+    uint32_t tot;
+    lstart[threadIdx.x] = block_excl_scan_u32<NT>(wcnt[0][threadIdx.x], &tot, sm);
+    // stable placement, one round of NT elements at a time
+#pragma unroll
     for (int r = 0; r < ITEMS; r++) {
       for (int q = threadIdx.x; q < (NT / WAVE) * 256; q += NT) (&wcnt[0][0])[q] = 0;
       __syncthreads();
-      int64_t i = base + (int64_t)r * NT + threadIdx.x;
-      bool valid = i < n;
-      uint64_t k = valid ? kin[i] : 0;
-      uint32_t v = valid ? vin[i] : 0;
-      uint32_t d = (uint32_t)((k >> shift) & 255);
+      const bool valid = r * NT + (int)threadIdx.x < cnt;
+      const uint32_t d = (uint32_t)((k[r] >> shift) & 255);
       uint64_t peers = __ballot(valid);
 #pragma unroll
       for (int b = 0; b < 8; b++) {
-        bool bit = (d >> b) & 1;
-        uint64_t bal = __ballot(bit);
+        const bool bit = (d >> b) & 1;
+        const uint64_t bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
       }
-      uint32_t rank = (uint32_t)__popcll(peers & lt);
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
       if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
       __syncthreads();
       if (valid) {
-        uint32_t off = run[d] + rank;
-        for (int q = 0; q < w; q++) off += wcnt[q][d];
-        kout[off] = k;
-        vout[off] = v;
+        uint32_t pos = lstart[d] + lrun[d] + rank;
+        for (int q = 0; q < w; q++) pos += wcnt[q][d];
+        sk[pos] = k[r];
+        sv[pos] = v[r];
       }
       __syncthreads();
       uint32_t add = 0;
       for (int q = 0; q < NT / WAVE; q++) add += wcnt[q][threadIdx.x];
-      run[threadIdx.x] += add;
+      lrun[threadIdx.x] += add;
       __syncthreads();
     }
+    // write out in digit order
+    for (int p = threadIdx.x; p < cnt; p += NT) {
+      const uint64_t kk = sk[p];
+      const uint32_t d = (uint32_t)((kk >> shift) & 255);
+      const uint32_t o = gbase[d] + (uint32_t)p - lstart[d];
+      kout[o] = kk;
+      vout[o] = sv[p];
+    }
+    __syncthreads();
   }
 }
 
