@@ -492,6 +492,36 @@ __device__ inline int64_t first_k_at_least(double P, double u) {
   return k;
 }
 
+// (T, tie) with  a < u128_to_double(pre, -96)  <=>  pre > T || (pre == T && tie)
+// for a double a in [0, 2^32): the rounded prefix exceeds a exactly when the
+// prefix lies above the midpoint between a and the next double, or on it when
+// round-to-nearest-even goes up there (a's significand odd).  The midpoint in
+// units of 2^-96 is (2m+1) 2^(E+95) for a = m 2^E; when it has a fractional
+// part, "above" is "above its floor" and no tie can occur.
+__device__ inline void above_threshold(double a, u128& T, bool& tie) {
+  const uint64_t b = dbits(a);
+  if (b == 0) {  // a == 0: any positive prefix rounds above it
+    T = 0;
+    tie = false;
+    return;
+  }
+  int ex = (int)(b >> 52) & 0x7ff;
+  uint64_t m = b & ((1ull << 52) - 1);
+  if (ex == 0)
+    ex = 1;
+  else
+    m |= 1ull << 52;
+  const int sh = ex - 1075 + 95;  // a = m 2^(ex-1075); sh <= 74 for a < 2^32
+  const u128 q = (u128)(2 * m + 1);
+  if (sh >= 0) {
+    T = q << sh;
+    tie = (m & 1ull) != 0;
+  } else {
+    T = -sh >= 128 ? (u128)0 : q >> (-sh);
+    tie = false;
+  }
+}
+
 // Emission.  A thread's elements hold lattice points k+u exactly when its
 // whole range [P(first), P(last+1)) does (first_k_at_least of the rounded
 // prefix is monotone), so only those threads recompute their elements.
@@ -527,7 +557,12 @@ __global__ void k_samp_emit(const double* __restrict__ x, const double* __restri
   // of the running prefix.  Element j owns the k with P_j <= fl(k+u) <
   // P_{j+1}, i.e. those kc passes while fl(kc+u) < P_{j+1} (the same set as
   // first_k_at_least(P_j) .. first_k_at_least(P_{j+1}), by monotonicity).
+  // fl(kc+u) < P_{j+1} is tested on the exact prefix against the threshold of
+  // fl(kc+u) (above_threshold), so the prefix is rounded to a double only here
   int64_t kc = first_k_at_least(u128_to_double(pre, -96), u);
+  u128 T;
+  bool tie;
+  above_threshold((double)kc + u, T, tie);
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
     const int64_t j = base + r;
@@ -536,11 +571,13 @@ __global__ void k_samp_emit(const double* __restrict__ x, const double* __restri
     const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
     pre += f;
     if (f != 0) {
-      const double Phi = u128_to_double(pre, -96);
       int ne = 0;
-      for (; kc < ns && (double)kc + u < Phi; kc++, ne++) {
+      while (kc < ns && (pre > T || (pre == T && tie))) {
         out_idx[K + kc] = (uint32_t)j;
         if (out_val) out_val[K + kc] = v / p;
+        kc++;
+        ne++;
+        above_threshold((double)kc + u, T, tie);
       }
       if (ne > 1) sc->dup = 1u;  // j emitted more than once
     }
