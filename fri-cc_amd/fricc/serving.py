@@ -249,11 +249,14 @@ class Server:
     Up to ``max_active`` solves are admitted at once and their iterations
     interleave.  With ``overlap`` False (the default) the device runs one
     solve's work at a time (DeviceOrder), a solve's set-up starts once the
-    device is idle, and ``policy`` picks the next iteration: "srpt" keeps
-    ``depth`` iterations queued and gives the device to the solve with the
-    least remaining work (shortest remaining processing time first, which
-    minimises the mean latency of a single server); "rr" queues every solve's
-    next iteration as soon as its previous one has finished (round robin).  The BLAS library's GEMM
+    device is idle, and ``policy`` picks the next iteration: "srpt" gives the
+    device to the solve with the least remaining work (shortest remaining
+    processing time first, which minimises the mean latency of a single
+    server), with up to ``depth`` iterations queued so the device does not
+    wait for the host -- but never one that would hold back the next
+    iteration of a solve in flight that comes first; "rr" queues every
+    solve's next iteration as soon as its previous one has finished (round
+    robin).  The BLAS library's GEMM
     kernels (stream-K: workgroups of one grid wait for partial tiles from
     other workgroups of the same grid) assume that their whole grid can be
     resident at once; work of other solves running beside them -- other such
@@ -358,6 +361,20 @@ class Server:
             return (0, s.ready_at)
         return (1, (s.req.max_cycle - len(s.energies)) * s.t_iter)
 
+    def _may_follow(self, s, queued):
+        """Whether s may be queued behind the iterations in flight: not when a
+        solve in flight would be picked before s for its next iteration (s
+        would hold that iteration back by one of its own)."""
+        if self.policy != "srpt":
+            return True
+        for q in queued:
+            left = q.req.max_cycle - len(q.energies) - 1
+            if q.t_iter is None or left <= 0:
+                continue
+            if self._priority(s) > (1, left * q.t_iter):
+                return False
+        return True
+
     def _queue(self, s, order):
         """Queue the solve's next iteration behind the work queued so far and
         time it on the device."""
@@ -460,15 +477,17 @@ class Server:
                     s.ready_at = clock() - t0
                     waiting.append(s)
             while waiting and len(queued) < depth:
-                progressed = True
                 if order is None:
                     s = waiting.pop(0)
                     s.issue()
                 else:
                     s = min(waiting, key=self._priority)
+                    if queued and not self._may_follow(s, queued):
+                        break
                     waiting.remove(s)
                     self._queue(s, order)
                 queued.append(s)
+                progressed = True
             s = None
             if retired:
                 torch.cuda.empty_cache()  # the retired solves' buffers back to the device

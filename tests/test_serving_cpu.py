@@ -80,3 +80,22 @@ def test_system_cache_keeps_recent_systems():
     cache.request(reqs[3]).result()
     cache.request(reqs[3])
     assert list(cache.entries) == [(3, 10, 39, 1), (3, 10, 39, 3)]
+
+
+def test_srpt_second_slot_never_delays_the_shorter_solve():
+    """With a short solve in flight, a long solve's iteration is not queued
+    behind it (it would delay the short solve's next iteration by a long
+    one); the other way round it is, and a solve in flight on its last
+    iteration holds nothing back."""
+    from types import SimpleNamespace as NS
+
+    def solve(done, t_iter):
+        return NS(req=NS(max_cycle=50), energies=[0.0] * done, t_iter=t_iter, ready_at=0.0)
+
+    server = sv.Server(policy="srpt")
+    short, long_ = solve(10, 2.0), solve(10, 250.0)
+    assert not server._may_follow(long_, [short])
+    assert server._may_follow(short, [long_])
+    assert server._may_follow(long_, [solve(49, 2.0)])
+    assert server._may_follow(long_, [solve(0, None)])
+    assert sv.Server(policy="rr")._may_follow(long_, [short])
