@@ -5,12 +5,13 @@ seed -- arrive over time.  The server admits them in arrival order while the
 estimated HBM of all admitted solves stays within a budget.  Each admitted
 solve gets its own HIP stream, its own library scratch arena, device-resident
 state and its own recorded step (``friccsd.GraphStep``: the whole iteration is
-one graph launch).  The scheduler keeps every active solve's next iteration
-queued on its stream, so iterations of different solves run side by side on
-the device; whenever a solve's iteration has finished it reads that solve's
-energy (one small transfer), retires it when converged or out of iterations,
-and otherwise queues its next iteration at once.  Requests that arrived in
-the meantime are admitted as soon as memory frees up.
+one graph launch).  The iterations of the admitted solves interleave on the
+device, one solve's work at a time (by default the solve with the least
+remaining work goes next); whenever a solve's iteration has finished the
+scheduler reads that solve's energy (one small transfer), retires it when
+converged or out of iterations, and otherwise lines up its next iteration.
+Requests that arrived in the meantime are admitted as soon as memory frees
+up; their systems are generated on the host while the device works.
 
 A solve's arithmetic does not depend on what else runs (its own buffers,
 scratch and graph; no shared accumulators), so its energies are bit-identical
@@ -113,15 +114,22 @@ def device_available():
 
 
 def device_free(need):
-    """Free device memory for a request of ``need`` bytes.  Memory the caching
-    allocator holds unused may be split into pieces no large block fits in,
-    so only memory the device reports free counts; when that is short the
-    allocator's unused segments are returned to the device first."""
+    """Device memory a request of ``need`` bytes can count on: what the device
+    reports free, plus what the caching allocator holds unused when the two
+    together cover the request (a retired solve's buffers, which the next
+    solve of its shape reuses; the allocator returns unused segments to the
+    device by itself when an allocation does not fit them).  Otherwise the
+    unused segments are returned to the device first and its free memory
+    counts."""
     torch = _torch()
     free, _ = torch.cuda.mem_get_info()
-    if free < need:
-        torch.cuda.empty_cache()
-        free, _ = torch.cuda.mem_get_info()
+    if free >= need:
+        return free
+    avail = device_available()
+    if avail >= need:
+        return avail
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
     return free
 
 
@@ -451,7 +459,7 @@ class Server:
             if not active:
                 time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
-            progressed = retired = False
+            progressed = False
             for s in list(queued):
                 if not s.ready():
                     continue
@@ -472,7 +480,6 @@ class Server:
                     free_arenas.append(s.cc.arena)
                     free_arenas.sort()
                     wait_retire = False
-                    retired = True
                 else:
                     s.ready_at = clock() - t0
                     waiting.append(s)
@@ -489,8 +496,9 @@ class Server:
                 queued.append(s)
                 progressed = True
             s = None
-            if retired:
-                torch.cuda.empty_cache()  # the retired solves' buffers back to the device
+            # a retired solve's buffers stay in the allocator's cache, where the
+            # next solve of that shape finds them; an admission that finds the
+            # device short returns the cache first (device_free)
             if not progressed:
                 time.sleep(5e-5)
         torch.cuda.synchronize()
