@@ -384,6 +384,14 @@ int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, voi
   return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream);
 }
 
+int fricc_split_bf16_lincomb(const void* A, const void* B, int src_precision, double alpha, double beta, int64_t R,
+                             int64_t X, void* out, void* stream) {
+  FRICC_CHECK(A != nullptr && B != nullptr, "split_bf16_lincomb: null operand");
+  int dt;
+  FRICC_TRY(dtype_of(src_precision, &dt));
+  return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream, B, alpha, beta);
+}
+
 int fricc_set_bf16_passes(int npass) {
   FRICC_CHECK(npass == 1 || npass == 3, "set_bf16_passes: 1 or 3");
   g_npass = npass;

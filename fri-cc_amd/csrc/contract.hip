@@ -655,29 +655,35 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
 // A[R][X] (f64 or f32) -> records [X/256][R][hi 256 | lo 256] bf16 (X
 // rounded up to a multiple of 256, pad columns zero); hi = bf16(a),
 // lo = bf16(a - hi)
+// With B: the split of alpha*A + beta*B, formed in the source precision (one
+// rounding of the sum of the two products; both exact for alpha, beta = 2, -1).
 template <typename T>
-__global__ void k_split_bf16(const T* __restrict__ A, int64_t R, int64_t X, int64_t n,
-                             __hip_bfloat16* __restrict__ out) {
+__global__ void k_split_bf16(const T* __restrict__ A, const T* __restrict__ B, T alpha, T beta, int64_t R,
+                             int64_t X, int64_t n, __hip_bfloat16* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
     // e = (xb * R + r) * 256 + xi
     const int64_t xi = e & 255, rec = e >> 8;
     const int64_t r = rec % R, x = (rec / R) * 256 + xi;
-    const float v = x < X ? (float)A[r * X + x] : 0.0f;
+    float v = 0.0f;
+    if (x < X) v = B ? (float)(alpha * A[r * X + x] + beta * B[r * X + x]) : (float)A[r * X + x];
     const __hip_bfloat16 h = __float2bfloat16(v);
     out[rec * 512 + xi] = h;
     out[rec * 512 + 256 + xi] = __float2bfloat16(v - __bfloat162float(h));
   }
 }
 
-int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s) {
+int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s, const void* B,
+               double alpha, double beta) {
   const int64_t n = R * bf16_xblocks(X) * 256;
   if (n == 0) return OK;
   const int64_t grid = std::min<int64_t>(ceil_div(n, 256), 256 * 64);
   if (src_dtype == DT_F64)
-    k_split_bf16<double><<<grid, 256, 0, s>>>((const double*)A, R, X, n, (__hip_bfloat16*)A2);
+    k_split_bf16<double><<<grid, 256, 0, s>>>((const double*)A, (const double*)B, alpha, beta, R, X, n,
+                                              (__hip_bfloat16*)A2);
   else if (src_dtype == DT_F32)
-    k_split_bf16<float><<<grid, 256, 0, s>>>((const float*)A, R, X, n, (__hip_bfloat16*)A2);
+    k_split_bf16<float><<<grid, 256, 0, s>>>((const float*)A, (const float*)B, (float)alpha, (float)beta, R, X,
+                                             n, (__hip_bfloat16*)A2);
   else
     FRICC_CHECK(false, "split_bf16: source must be FP64 or FP32");
   FRICC_HIP(hipGetLastError());

@@ -22,7 +22,7 @@ EXPORTS = (
     "fricc_contract", "fricc_contract_prepared", "fricc_contract_spec", "fricc_eri_build", "fricc_set_gemm_order",
     "fricc_sparse_prepare", "fricc_split_bf16", "fricc_set_bf16_passes", "fricc_t2_finalize",
     "fricc_set_bf16_view_width", "fricc_stream_copy", "fricc_sparse_compress_async", "fricc_sparse_status",
-    "fricc_workspace_select", "fricc_workspace_release",
+    "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
 )
 
 
@@ -88,6 +88,8 @@ def lib():
     L.fricc_eri_build.argtypes = [_vp, _i64, _i64, I64x8, I64x4, _vp, _vp]
     L.fricc_set_gemm_order.argtypes = [ctypes.c_int]
     L.fricc_split_bf16.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
+    L.fricc_split_bf16_lincomb.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i64, _i64, _vp,
+                                           _vp]
     L.fricc_set_bf16_passes.argtypes = [ctypes.c_int]
     L.fricc_set_bf16_view_width.argtypes = [ctypes.c_int]
     L.fricc_stream_copy.argtypes = [_vp, _vp, _i64, _vp]

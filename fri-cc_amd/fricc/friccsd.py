@@ -454,7 +454,12 @@ def _update_amps_device(cc, t1, t2, eris):
         tm.start("O3V3")
         # 1302 and 1202 read the same view of T with operands in the same
         # [(k,c)][(i,a)] layout: one contraction with 2 Wvoov - Wvovo
-        A12 = _dyn_operand(2.0 * A_voov - A_vovo, ov, ov, sp)
+        if sp == "bf16":
+            from .systems import bf16_planes_lincomb
+
+            A12 = bf16_planes_lincomb(A_voov, A_vovo, 2.0, -1.0, ov, ov)
+        else:
+            A12 = _dyn_operand(2.0 * A_voov - A_vovo, ov, ov, sp)
         _contract_into("akic,kjcb->ijab", 1.0, A12, (v, o, o, v), ts, S, sp)
         del A12
         Av = _dyn_operand(A_voov, ov, ov, sp)

@@ -85,6 +85,21 @@ def bf16_planes(A, R, X, stream=None):
     return out
 
 
+def bf16_planes_lincomb(A, B, alpha, beta, R, X, stream=None):
+    """bf16_planes of alpha*A + beta*B without forming the sum (fricc_split_bf16_lincomb)."""
+    import torch
+
+    A, B = A.contiguous(), B.contiguous()
+    assert A.shape == B.shape and A.dtype == B.dtype and A.numel() == R * X
+    assert A.dtype in (torch.float64, torch.float32)
+    src = _capi.FP64 if A.dtype == torch.float64 else _capi.FP32
+    out = torch.empty(R * bf16_xblocks(X) * 512, dtype=torch.int16, device=A.device)
+    _capi.check(_capi.lib().fricc_split_bf16_lincomb(ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(B.data_ptr()),
+                                                     src, float(alpha), float(beta), int(R), int(X),
+                                                     ctypes.c_void_p(out.data_ptr()), _capi.stream_ptr(stream)))
+    return out
+
+
 class _LazyOperands(dict):
     """Operands of one dtype, built on first access.  Holds its integrals
     weakly: the integrals own this cache, and a reference cycle would keep

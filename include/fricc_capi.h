@@ -165,6 +165,14 @@ int fricc_sparse_prepare(fricc_sparse* T, const char* term_or_spec, const int64_
  * the reference (FP64 only). */
 int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream);
 
+/* fricc_split_bf16 of alpha*A + beta*B (A, B: same shape and precision; the
+ * combination formed in that precision), so a combined operand needs no
+ * temporary.  No counterpart in the reference (its update forms
+ * 2*Wvoov - Wvovo implicitly by adding two contractions,
+ * fricc/friccsd.py:247-251). */
+int fricc_split_bf16_lincomb(const void* A, const void* B, int src_precision, double alpha, double beta, int64_t R,
+                             int64_t X, void* out, void* stream);
+
 /* MFMA passes of FRICC_BF16 contractions: 3 = hi*hi + hi*lo + lo*hi (default,
  * about 16 significant bits per product), 1 = hi*hi only. */
 int fricc_set_bf16_passes(int npass);

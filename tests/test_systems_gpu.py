@@ -71,3 +71,23 @@ def test_step_without_fp64_ladder(eris_pair, prec):
     (a2, ea), (b2, eb) = res
     assert (a2 - b2).abs().max().item() <= 1e-12 * a2.abs().max().item()
     assert abs(ea - eb) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_split_of_a_combination_equals_split_of_the_sum(dtype):
+    """fricc_split_bf16_lincomb(A, B, 2, -1) gives the planes of 2A - B formed
+    first (both products exact, one rounding of the sum), with a column count
+    that is not a multiple of the 256-column record."""
+    import torch
+
+    from fricc.systems import bf16_planes, bf16_planes_lincomb
+
+    g = torch.Generator(device="cuda").manual_seed(4)
+    dt = getattr(torch, dtype)
+    R, X = 37, 300
+    A = torch.randn(R, X, generator=g, device="cuda", dtype=dt)
+    B = torch.randn(R, X, generator=g, device="cuda", dtype=dt)
+    got = bf16_planes_lincomb(A, B, 2.0, -1.0, R, X)
+    want = bf16_planes(2.0 * A - B, R, X)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
