@@ -217,3 +217,29 @@ def test_repeated_indices_add_up(py_rccsd, term, precision):
     py_rccsd.contract_DTSpT(w, T, out, term, precision=precision)
     tol = {"fp64": 1e-12, "fp32": 2e-5, "bf16": 2e-4}[precision]
     npt.assert_allclose(out, ref, rtol=0, atol=tol * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("width", [128, 256, 512])
+@pytest.mark.parametrize("term", ["2323", "1302", "0112"])
+def test_bf16_view_widths(py_rccsd, term, width):
+    """The BF16 kernels at every view (tile) width the automatic rule picks:
+    128 (256-wide blocks of 128 columns), 256 and 512 (256x256 blocks, one or
+    two per tile), on shapes with several tiles and a ragged last one."""
+    from fricc import _capi
+
+    no, nv = 23, 12
+    wshape, spec, alpha, oshape = CASES[term]
+    rng = np.random.default_rng(13)
+    w = rng.standard_normal(wshape(no, nv))
+    t2 = rng.standard_normal((no, no, nv, nv))
+    T = py_rccsd.SparseTensor4d(t2, t2.shape, t2.size // 6, "largest")
+    dense = np.zeros(t2.size)
+    dense[T.indices()] = T.values()
+    ref = alpha * np.einsum(spec, w, dense.reshape(t2.shape))
+    out = np.zeros(oshape(no, nv))
+    _capi.check(_capi.lib().fricc_set_bf16_view_width(width))
+    try:
+        py_rccsd.contract_DTSpT(w, T, out, term, precision="bf16")
+    finally:
+        _capi.check(_capi.lib().fricc_set_bf16_view_width(0))
+    npt.assert_allclose(out, ref, rtol=0, atol=2e-4 * np.abs(ref).max())
