@@ -66,6 +66,7 @@ class Result:
     iterations: int
     converged: bool
     energies: list = field(default_factory=list)
+    error: str = ""  # why the solve was refused or stopped, if it was
 
     @property
     def latency(self):
@@ -429,11 +430,21 @@ class Server:
                 before = device_available()
                 arena = free_arenas.pop(0)
                 self._dirty.add(arena)
-                s = self._try_solve(req, arena, system)
-                if s is None and not active and self._reclaim(free_arenas + [arena]):
-                    before = device_available()
-                    self._dirty.add(arena)
+                try:
                     s = self._try_solve(req, arena, system)
+                    if s is None and not active and self._reclaim(free_arenas + [arena]):
+                        before = device_available()
+                        self._dirty.add(arena)
+                        s = self._try_solve(req, arena, system)
+                except (ValueError, RuntimeError) as err:
+                    # the solver refused the request (e.g. m > |T2|): answer it, go on
+                    pending.pop(0)
+                    free_arenas.insert(0, arena)
+                    fin = clock() - t0
+                    results.append(Result(req.id, req.nocc, req.nvir, req.m, req.arrival, fin, fin, 0, False,
+                                          error=f"{type(err).__name__}: {err}"))
+                    self._note((round(fin, 3), "refused", req.id, req.nocc, req.nvir, None, used, None))
+                    continue
                 if s is None:
                     free_arenas.insert(0, arena)
                     if not active:
@@ -467,13 +478,17 @@ class Server:
                 queued.remove(s)
                 if order is not None:
                     s.t_iter = s.ev0.elapsed_time(s.ev1)
-                done = s.complete(order)
+                error = ""
+                try:
+                    done = s.complete(order)
+                except RuntimeError as err:  # a status the step reported (non-finite amplitudes, p > 1)
+                    done, error = True, f"{type(err).__name__}: {err}"
                 if order is not None:
                     order.end(s.stream)  # the copies of the outputs into the inputs
                 if done:
                     fin = clock() - t0
                     results.append(Result(s.req.id, s.req.nocc, s.req.nvir, s.req.m, s.req.arrival, s.admitted, fin,
-                                          len(s.energies), s.converged, list(s.energies)))
+                                          len(s.energies), s.converged, list(s.energies), error=error))
                     active.remove(s)
                     used -= s.need
                     self._note((round(fin, 3), "retire", s.req.id, s.req.nocc, s.req.nvir, s.need, used, None))
@@ -509,11 +524,15 @@ class Server:
 
 
 def latency_stats(results, makespan):
-    lat = np.array([r.latency for r in results])
-    return {"solves": len(results), "solves_per_s": len(results) / makespan if makespan > 0 else None,
+    """Throughput and latency of the solves that ran (refused or failed ones
+    are counted apart)."""
+    ok = [r for r in results if not r.error]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    return {"solves": len(ok), "solves_per_s": len(ok) / makespan if makespan > 0 else None,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_iterations": float(np.mean([r.iterations for r in results])),
-            "converged": int(sum(r.converged for r in results)), "makespan_s": makespan}
+            "mean_iterations": float(np.mean([r.iterations for r in ok])) if ok else 0.0,
+            "converged": int(sum(r.converged for r in ok)), "failed": len(results) - len(ok),
+            "makespan_s": makespan}
 
 
 def solve_alone(req, precision="mixed"):

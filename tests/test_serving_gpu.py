@@ -90,3 +90,20 @@ def test_dropped_solve_frees_its_memory(serving):
         assert torch.cuda.memory_allocated() - base <= (64 << 10), (held, torch.cuda.memory_allocated() - base)
     finally:
         gc.enable()
+
+
+def test_refused_request_does_not_stop_the_others(serving):
+    """A request the solver refuses at set-up (m > |T2|, fricc/friccsd.py:68-72)
+    is answered with its error; the solves around it run as alone."""
+    sv = serving
+    reqs = _requests(sv)[:3]
+    o, v = reqs[1].nocc, reqs[1].nvir
+    reqs[1].m = o * o * v * v + 1
+    server = sv.Server(hbm_budget=1e12, precision="mixed", max_active=8)
+    res = server.run(reqs)
+    assert len(res) == 3
+    assert "m_keep" in res[1].error and res[1].iterations == 0
+    for r, q in ((res[0], reqs[0]), (res[2], reqs[2])):
+        assert not r.error and r.energies == sv.solve_alone(q, precision="mixed")
+    st = sv.latency_stats(res, server.makespan)
+    assert st["solves"] == 2 and st["failed"] == 1
