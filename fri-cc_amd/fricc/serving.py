@@ -288,7 +288,7 @@ class Server:
     depth = 2  # iterations queued on the device at once ("srpt"): one runs while the host reads the other
 
     def __init__(self, hbm_budget=240e9, precision="mixed", max_active=8, progress=None, overlap=False,
-                 policy="srpt"):
+                 policy="srpt", record_timeline=False):
         if policy not in ("srpt", "rr"):
             raise ValueError(f"unknown policy {policy!r} (srpt, rr)")
         self.progress = progress  # called with each admission / retirement / deferral record
@@ -297,6 +297,9 @@ class Server:
         self.max_active = min(int(max_active), 63)
         self.overlap = bool(overlap)
         self.policy = policy
+        # (request id, start event, end event) of every queued iteration when
+        # asked for (one solve's work at a time only)
+        self.timeline = [] if record_timeline else None
         self.systems = SystemCache()
         self.measured = {}
         self.log = []  # (time, event, id, nocc, nvir, need, used, device_free)
@@ -393,6 +396,8 @@ class Server:
         s.ev0.record(s.stream)
         s.issue()
         s.ev1 = order.end(s.stream)
+        if self.timeline is not None:
+            self.timeline.append((s.req.id, s.ev0, s.ev1))
 
     def run(self, requests, clock=time.perf_counter):
         torch = _torch()

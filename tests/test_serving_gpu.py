@@ -107,3 +107,22 @@ def test_refused_request_does_not_stop_the_others(serving):
         assert not r.error and r.energies == sv.solve_alone(q, precision="mixed")
     st = sv.latency_stats(res, server.makespan)
     assert st["solves"] == 2 and st["failed"] == 1
+
+
+def test_one_solves_work_at_a_time(serving):
+    """With overlap off, the iterations of different solves occupy disjoint
+    intervals of device time (each waits for the end of the work queued
+    before it), while the solves still interleave."""
+    sv = serving
+    server = sv.Server(hbm_budget=1e12, precision="mixed", max_active=8, record_timeline=True)
+    res = server.run(_requests(sv))
+    assert len(res) == 8 and server.max_concurrent > 1
+    t0 = server.timeline[0][1]  # the first queued iteration starts the chain
+    iv = sorted((t0.elapsed_time(a), t0.elapsed_time(b), rid) for rid, a, b in server.timeline)
+    assert len(iv) == sum(r.iterations for r in res)
+    for (s1, e1, _), (s2, _, _) in zip(iv, iv[1:]):
+        assert s2 >= e1 - 1e-3, (s1, e1, s2)
+    # interleaved: some solve's iterations come in more than one run
+    ids = [rid for _, _, rid in iv]
+    runs = 1 + sum(a != b for a, b in zip(ids, ids[1:]))
+    assert runs > len(set(ids))
