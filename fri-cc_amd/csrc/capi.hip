@@ -236,6 +236,27 @@ int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, in
   return OK;
 }
 
+int fricc_sparse_compress_pair_async(fricc_sparse* sp, fricc_sparse* sp2, const double* x, int64_t m, int compression,
+                                     const double* u_dev, const double* u_dev2, void* stream) {
+  FRICC_CHECK(sp != nullptr && sp2 != nullptr && sp != sp2 && u_dev != nullptr && u_dev2 != nullptr,
+              "sparse_compress_pair_async: two distinct tensors and two uniforms needed");
+  for (int q = 0; q < 4; q++)
+    FRICC_CHECK(sp->dims[q] == sp2->dims[q], "sparse_compress_pair_async: the tensors' dimensions differ");
+  FRICC_TRY(fricc_sparse_compress_async(sp, x, m, compression, u_dev, stream));
+  const int64_t n = sp2->dims[0] * sp2->dims[1] * sp2->dims[2] * sp2->dims[3];
+  FRICC_TRY(sparse_reserve(sp2, m));
+  if (!sp2->dsc) FRICC_HIP(hipMalloc(&sp2->dsc, comp_scalars_size()));
+  sp2->gen++;
+  sp2->nnz = 0;
+  FRICC_TRY(compress_resample(x, n, m, compression, u_dev2, sp->idx, sp->val, sp2->idx, sp2->val, sp2->dsc,
+                              (hipStream_t)stream));
+  sp2->nnz = m;
+  sp2->pending = true;
+  sp2->dup_host = 0;
+  sp2->dup_dev = comp_scalars_dup_flag(sp2->dsc);
+  return OK;
+}
+
 // Reads the scalars the last asynchronous compression left on the device
 // (every replay of a recorded graph rewrites them, so they are read each time).
 int fricc_sparse_status(fricc_sparse* sp, void* stream) {

@@ -651,7 +651,7 @@ __global__ void k_export_scalars(const CompScalars* __restrict__ sc, CompScalars
 
 // ---------------------------------------------------------------- driver
 enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
-       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11 };
+       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11, WS_SC2 = 12 };
 
 size_t comp_scalars_size() { return sizeof(CompScalars); }
 
@@ -754,6 +754,64 @@ int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_
     k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (u128*)p_ths);
     k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
     k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (const u128*)p_ths, out_idx, out_val);
+    k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
+    k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
+  }
+  if (export_sc) k_export_scalars<<<1, 64, 0, s>>>(sc, (CompScalars*)export_sc);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+// Scalars of a second sample of the same compression: everything but the
+// uniform (and what the emission derives from it) is the first one's.
+__global__ void k_resample_scalars(const CompScalars* __restrict__ src, const double* __restrict__ u_dev,
+                                   CompScalars* __restrict__ sc) {
+  if (threadIdx.x != 0) return;
+  CompScalars c = *src;
+  const double u = *u_dev;
+  c.ubad = (u >= 0.0 && u <= 1.0) ? 0u : 1u;
+  c.u = c.ubad ? 0.0 : u;
+  c.dup = 0u;
+  c.Ptot = 0.0;
+  c.k_notfound = 0;
+  c.n_notfound = 0;
+  *sc = c;
+}
+
+// The D part of the first sample (|D| read on the device).
+__global__ void k_copy_kept(const uint32_t* __restrict__ idx, const double* __restrict__ val,
+                            const CompScalars* __restrict__ sc, int64_t m, uint32_t* __restrict__ out_idx,
+                            double* __restrict__ out_val) {
+  const int64_t K = (int64_t)sc->K;
+  for (int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x; k < K && k < m; k += (int64_t)gridDim.x * NT) {
+    out_idx[k] = idx[k];
+    out_val[k] = val[k];
+  }
+}
+
+// A second FRI sample of the vector compress_launch has just compressed (same
+// x, n, m, method, stream and workspace arena, nothing in between): the
+// selection, |D|, the remaining norm and the exact prefix sums of p do not
+// depend on the uniform, so only the systematic emission runs again, with
+// u from u_dev.  Its output is bit for bit that of a full compression with
+// that uniform.  idx/val: the first sample's output (its D part is copied).
+int compress_resample(const double* x, int64_t n, int64_t m, int method, const double* u_dev, const uint32_t* idx,
+                      const double* val, uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s) {
+  FRICC_CHECK(u_dev != nullptr, "compress: resampling needs a device uniform");
+  const int64_t nt = ceil_div(n, TILE);
+  void *p_sc, *p_sc2, *p_tot, *p_ts, *p_ths;
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_SC2, sizeof(CompScalars), &p_sc2));
+  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  CompScalars* sc = (CompScalars*)p_sc2;
+  k_resample_scalars<<<1, 64, 0, s>>>((const CompScalars*)p_sc, u_dev, sc);
+  // the kept part (all m entries for "largest", where K stays m: the second
+  // sample is the first)
+  k_copy_kept<<<(unsigned)std::min<int64_t>(ceil_div(m, NT), 2048), NT, 0, s>>>(idx, val, sc, m, out_idx, out_val);
+  if (method != COMP_LARGEST) {
+    k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, (const u128*)p_ts, (const u128*)p_ths, out_idx, out_val);
     k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
     k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
   }

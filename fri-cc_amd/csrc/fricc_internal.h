@@ -25,6 +25,10 @@ int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32
 // the call's scalars go to export_sc (device, comp_scalars_size() bytes).
 int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
                     uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s, void** sc_out);
+// A second sample, with the uniform u_dev, of the vector compress_launch has
+// just compressed into idx/val (same x, n, m, method, stream and arena).
+int compress_resample(const double* x, int64_t n, int64_t m, int method, const double* u_dev, const uint32_t* idx,
+                      const double* val, uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s);
 size_t comp_scalars_size();
 // device address of the "output may repeat an index" word of a call's scalars
 const uint32_t* comp_scalars_dup_flag(const void* dev_sc);

@@ -279,20 +279,26 @@ def _update_amps_device(cc, t1, t2, eris):
     tm.start("compress")
     comp = st["compression"]
     ud = cc._udev()
-    ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1])
-    cc._used_slots = {0}
     ts_alt = None
-    if _needs_alt(cc):
+    if _needs_alt(cc) and (not st["unbiased"] or cc._iter < st["burn_in"]):
+        # t2_sparse and t2_sparse_alt: two samples of the same T2 (the second
+        # uniform for the second), which share everything but the sampling
         ualt = ud[1:2] if comp == "fri" else ud[0:1]
-        if not st["unbiased"] or cc._iter < st["burn_in"]:
-            ts_alt = cc._slot(1, t2.shape).recompress_async(t2, int(m_keep), comp, ualt)
+        ts, ts_alt = cc._slot(0, t2.shape).recompress_pair_async(cc._slot(1, t2.shape), t2, int(m_keep), comp,
+                                                                 ud[0:1], ualt)
+        cc._used_slots = {0, 1}
+    else:
+        ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1])
+        cc._used_slots = {0}
+    if _needs_alt(cc) and ts_alt is None:
+        # unbiased averaging after burn-in: t2_sparse_alt samples the running average
+        ualt = ud[1:2] if comp == "fri" else ud[0:1]
+        it = cc._iter - st["burn_in"]
+        if it == 0:
+            cc._t2_avg = t2.clone()
         else:
-            it = cc._iter - st["burn_in"]
-            if it == 0:
-                cc._t2_avg = t2.clone()
-            else:
-                cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
-            ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt)
+            cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
+        ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt)
         cc._used_slots = {0, 1}
     tm.stop("compress")
     timings["compress"] = time.perf_counter() - t0

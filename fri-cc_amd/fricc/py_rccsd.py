@@ -258,6 +258,23 @@ class SparseTensor4d:
         self._async = True
         return self
 
+    def recompress_pair_async(self, other, x, m, compression, u_dev, u_dev2, stream=None):
+        """``recompress_async`` of x into this tensor (uniform u_dev) and into
+        ``other`` (uniform u_dev2), sharing everything but the sampling: each
+        tensor ends bit for bit as its own ``recompress_async`` would leave it.
+        Returns (self, other)."""
+        x = _as_device_vector(x)
+        if compression not in ("largest", "fri"):
+            raise ValueError("compression must be 'largest' or 'fri'")
+        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
+        _capi.check(_capi.lib().fricc_sparse_compress_pair_async(
+            self._h, other._h, ctypes.c_void_p(x.data_ptr()), int(m), method, ctypes.c_void_p(u_dev.data_ptr()),
+            ctypes.c_void_p(u_dev2.data_ptr()), _capi.stream_ptr(stream)))
+        for t in (self, other):
+            t._host = None
+            t._async = True
+        return self, other
+
     def sync_status(self, stream=None):
         """Statistics and status of the last asynchronous compression (raises
         on non-finite input, p > 1 or a bad uniform)."""

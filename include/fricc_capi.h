@@ -89,6 +89,16 @@ int fricc_sparse_compress(fricc_sparse* sp, const double* x, int64_t m, int comp
 int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, int compression, const double* u_dev,
                                 void* stream);
 
+/* Two samples of one vector: fricc_sparse_compress_async of x into sp with
+ * the uniform at u_dev, then a second sample into sp2 with the uniform at
+ * u_dev2.  The selection of D, the remaining norm and the prefix sums of p do
+ * not depend on the uniform and are computed once; each tensor's entries and
+ * statistics are bit for bit those of its own fricc_sparse_compress_async.
+ * Replaces the two constructions of the same T2 in update_amps,
+ * t2_sparse and t2_sparse_alt (fricc/friccsd.py:99-138). */
+int fricc_sparse_compress_pair_async(fricc_sparse* sp, fricc_sparse* sp2, const double* x, int64_t m, int compression,
+                                     const double* u_dev, const double* u_dev2, void* stream);
+
 /* Waits for work on `stream`, reads the scalars the last asynchronous
  * compression of sp left on the device (rewritten by every replay of a
  * recorded graph), makes its statistics visible to fricc_sparse_info and

@@ -168,3 +168,32 @@ def test_error_paths(mods):
         _gpu(py_rccsd, z, 50, "fri", 0.5)
     with pytest.raises(ValueError):
         _gpu(py_rccsd, x, 1001, "fri", 0.5)
+
+
+@pytest.mark.parametrize("comp", ["fri", "largest"])
+def test_pair_equals_two_compressions(mods, comp):
+    """recompress_pair_async (one selection, two samples) leaves each tensor
+    bit for bit as its own recompress_async does, and as the oracle's
+    compression with that uniform, including u = 1 (samples past the last
+    interval take index 0) and u = 0."""
+    py_rccsd, lo, torch = mods
+    x = _vec(300_000, 5)
+    m = 20_000
+    xd = torch.from_numpy(x).cuda()
+    dims = (1, 1, 1, x.size)
+    method = lo.FRI_SYSTEMATIC if comp == "fri" else lo.LARGEST
+    keys = ("n_D", "n_sampled", "n_notfound", "norm0", "remaining_norm", "p_total", "n_candidates")
+    for u1, u2 in ((0.25, 0.75), (0.5, 1.0), (1.0, 0.0)):
+        ud = torch.tensor([u1, u2], dtype=torch.float64, device="cuda")
+        A, B = py_rccsd.SparseTensor4d.empty(dims), py_rccsd.SparseTensor4d.empty(dims)
+        A.recompress_pair_async(B, xd, m, comp, ud[0:1], ud[1:2])
+        infos = (A.sync_status(), B.sync_status())
+        for T, info, k, u in ((A, infos[0], 0, u1), (B, infos[1], 1, u2)):
+            S = py_rccsd.SparseTensor4d.empty(dims)
+            S.recompress_async(xd, m, comp, ud[k:k + 1])
+            sinfo = S.sync_status()
+            assert np.array_equal(T.indices(), S.indices()), (u1, u2, k)
+            assert np.array_equal(T.values().view(np.int64), S.values().view(np.int64)), (u1, u2, k)
+            assert all(info[q] == sinfo[q] for q in keys), (info, sinfo)
+            oi, ov, _ = lo.compress(x, m, method, u)
+            assert np.array_equal(T.indices(), oi) and np.array_equal(T.values().view(np.int64), ov.view(np.int64))
