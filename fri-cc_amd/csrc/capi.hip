@@ -236,6 +236,22 @@ int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, in
   return OK;
 }
 
+// Sample k >= 1 of one compression into sp2 (see compress_resample).
+static int resample_into(fricc_sparse* first, fricc_sparse* sp2, const double* x, int64_t m, int compression,
+                         const double* u_dev, hipStream_t s) {
+  const int64_t n = sp2->dims[0] * sp2->dims[1] * sp2->dims[2] * sp2->dims[3];
+  FRICC_TRY(sparse_reserve(sp2, m));
+  if (!sp2->dsc) FRICC_HIP(hipMalloc(&sp2->dsc, comp_scalars_size()));
+  sp2->gen++;
+  sp2->nnz = 0;
+  FRICC_TRY(compress_resample(x, n, m, compression, u_dev, first->idx, first->val, sp2->idx, sp2->val, sp2->dsc, s));
+  sp2->nnz = m;
+  sp2->pending = true;
+  sp2->dup_host = 0;
+  sp2->dup_dev = comp_scalars_dup_flag(sp2->dsc);
+  return OK;
+}
+
 int fricc_sparse_compress_pair_async(fricc_sparse* sp, fricc_sparse* sp2, const double* x, int64_t m, int compression,
                                      const double* u_dev, const double* u_dev2, void* stream) {
   FRICC_CHECK(sp != nullptr && sp2 != nullptr && sp != sp2 && u_dev != nullptr && u_dev2 != nullptr,
@@ -243,17 +259,21 @@ int fricc_sparse_compress_pair_async(fricc_sparse* sp, fricc_sparse* sp2, const 
   for (int q = 0; q < 4; q++)
     FRICC_CHECK(sp->dims[q] == sp2->dims[q], "sparse_compress_pair_async: the tensors' dimensions differ");
   FRICC_TRY(fricc_sparse_compress_async(sp, x, m, compression, u_dev, stream));
-  const int64_t n = sp2->dims[0] * sp2->dims[1] * sp2->dims[2] * sp2->dims[3];
-  FRICC_TRY(sparse_reserve(sp2, m));
-  if (!sp2->dsc) FRICC_HIP(hipMalloc(&sp2->dsc, comp_scalars_size()));
-  sp2->gen++;
-  sp2->nnz = 0;
-  FRICC_TRY(compress_resample(x, n, m, compression, u_dev2, sp->idx, sp->val, sp2->idx, sp2->val, sp2->dsc,
-                              (hipStream_t)stream));
-  sp2->nnz = m;
-  sp2->pending = true;
-  sp2->dup_host = 0;
-  sp2->dup_dev = comp_scalars_dup_flag(sp2->dsc);
+  return resample_into(sp, sp2, x, m, compression, u_dev2, (hipStream_t)stream);
+}
+
+int fricc_sparse_compress_multi_async(fricc_sparse* const* sps, int count, const double* x, int64_t m,
+                                      int compression, const double* u_dev, void* stream) {
+  FRICC_CHECK(sps != nullptr && count >= 1 && u_dev != nullptr, "sparse_compress_multi_async: bad arguments");
+  for (int k = 0; k < count; k++) {
+    FRICC_CHECK(sps[k] != nullptr, "sparse_compress_multi_async: null tensor");
+    for (int l = 0; l < k; l++) FRICC_CHECK(sps[l] != sps[k], "sparse_compress_multi_async: tensors must differ");
+    for (int q = 0; q < 4; q++)
+      FRICC_CHECK(sps[k]->dims[q] == sps[0]->dims[q], "sparse_compress_multi_async: the tensors' dimensions differ");
+  }
+  FRICC_TRY(fricc_sparse_compress_async(sps[0], x, m, compression, u_dev, stream));
+  for (int k = 1; k < count; k++)
+    FRICC_TRY(resample_into(sps[0], sps[k], x, m, compression, u_dev + k, (hipStream_t)stream));
   return OK;
 }
 

@@ -23,7 +23,7 @@ EXPORTS = (
     "fricc_sparse_prepare", "fricc_split_bf16", "fricc_set_bf16_passes", "fricc_t2_finalize",
     "fricc_set_bf16_view_width", "fricc_stream_copy", "fricc_sparse_compress_async", "fricc_sparse_status",
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
-    "fricc_sparse_compress_pair_async",
+    "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async",
 )
 
 
@@ -96,6 +96,7 @@ def lib():
     L.fricc_stream_copy.argtypes = [_vp, _vp, _i64, _vp]
     L.fricc_sparse_compress_async.argtypes = [_vp, _vp, _i64, ctypes.c_int, _vp, _vp]
     L.fricc_sparse_compress_pair_async.argtypes = [_vp, _vp, _vp, _i64, ctypes.c_int, _vp, _vp, _vp]
+    L.fricc_sparse_compress_multi_async.argtypes = [_vp, ctypes.c_int, _vp, _i64, ctypes.c_int, _vp, _vp]
     L.fricc_sparse_status.argtypes = [_vp, _vp]
     L.fricc_workspace_select.argtypes = [ctypes.c_int]
     L.fricc_workspace_release.argtypes = [ctypes.c_int]

@@ -21,6 +21,7 @@ import copy
 import logging
 
 from . import _capi
+from .py_rccsd import SparseTensor4d
 from .friccsd import (FRICCSD, _check_m_keep, _contract_into, _mm_splitk, _prepare, _set_uniforms, _torch, _vp,
                       default_fri_settings as _ccsd_defaults, _finish_step, _draw_uniforms, _O3V3, _PREC_ID)
 
@@ -83,9 +84,11 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
     comp = st["compression"]
     ud = cc._udev()
     if st["Independent Compressions"] and comp == "fri":
-        ts = {}
-        for k, term in enumerate(TERM_ORDER):
-            ts[term] = cc._slot(k, t2.shape).recompress_async(t2, m_keep, comp, ud[k:k + 1])
+        # one sample of the same T2 per term, uniforms in term order: one
+        # selection shared by all samples
+        slots = [cc._slot(k, t2.shape) for k in range(len(TERM_ORDER))]
+        SparseTensor4d.compress_multi_async(slots, t2, m_keep, comp, ud[:len(TERM_ORDER)])
+        ts = dict(zip(TERM_ORDER, slots))
         cc._used_slots = set(range(len(TERM_ORDER)))
     else:
         one = cc._slot(0, t2.shape).recompress_async(t2, m_keep, comp, ud[0:1])

@@ -275,6 +275,26 @@ class SparseTensor4d:
             t._async = True
         return self, other
 
+    @staticmethod
+    def compress_multi_async(tensors, x, m, compression, u_dev, stream=None):
+        """``recompress_async`` of x into every tensor of ``tensors``, tensor k
+        with the uniform u_dev[k] (a CUDA float64 tensor of len(tensors)
+        consecutive values), sharing everything but the sampling."""
+        x = _as_device_vector(x)
+        if compression not in ("largest", "fri"):
+            raise ValueError("compression must be 'largest' or 'fri'")
+        if u_dev.numel() < len(tensors) or not u_dev.is_contiguous():
+            raise ValueError("u_dev must hold one contiguous uniform per tensor")
+        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
+        handles = (ctypes.c_void_p * len(tensors))(*[t._h.value for t in tensors])
+        _capi.check(_capi.lib().fricc_sparse_compress_multi_async(
+            handles, len(tensors), ctypes.c_void_p(x.data_ptr()), int(m), method, ctypes.c_void_p(u_dev.data_ptr()),
+            _capi.stream_ptr(stream)))
+        for t in tensors:
+            t._host = None
+            t._async = True
+        return list(tensors)
+
     def sync_status(self, stream=None):
         """Statistics and status of the last asynchronous compression (raises
         on non-finite input, p > 1 or a bad uniform)."""

@@ -99,6 +99,12 @@ int fricc_sparse_compress_async(fricc_sparse* sp, const double* x, int64_t m, in
 int fricc_sparse_compress_pair_async(fricc_sparse* sp, fricc_sparse* sp2, const double* x, int64_t m, int compression,
                                      const double* u_dev, const double* u_dev2, void* stream);
 
+/* The same for count samples: sps[k] with the uniform at u_dev[k] (count
+ * consecutive device doubles).  Replaces the per-term constructions of the
+ * same T2 of FRILCCSD's "Independent Compressions" (fricc/frilccsd.py:233-457). */
+int fricc_sparse_compress_multi_async(fricc_sparse* const* sps, int count, const double* x, int64_t m,
+                                      int compression, const double* u_dev, void* stream);
+
 /* Waits for work on `stream`, reads the scalars the last asynchronous
  * compression of sp left on the device (rewritten by every replay of a
  * recorded graph), makes its statistics visible to fricc_sparse_info and

@@ -197,3 +197,27 @@ def test_pair_equals_two_compressions(mods, comp):
             assert all(info[q] == sinfo[q] for q in keys), (info, sinfo)
             oi, ov, _ = lo.compress(x, m, method, u)
             assert np.array_equal(T.indices(), oi) and np.array_equal(T.values().view(np.int64), ov.view(np.int64))
+
+
+def test_multi_equals_separate_compressions(mods):
+    """compress_multi_async: six samples of one vector (uniforms in order,
+    u = 1 among them), each bit for bit its own compression and the oracle's."""
+    py_rccsd, lo, torch = mods
+    x = _vec(200_000, 9)
+    m = 15_000
+    xd = torch.from_numpy(x).cuda()
+    dims = (1, 1, 1, x.size)
+    us = [0.31, 1.0, 0.0, 0.77, 0.5, 0.123]
+    ud = torch.tensor(us, dtype=torch.float64, device="cuda")
+    ts = [py_rccsd.SparseTensor4d.empty(dims) for _ in us]
+    py_rccsd.SparseTensor4d.compress_multi_async(ts, xd, m, "fri", ud)
+    for k, (T, u) in enumerate(zip(ts, us)):
+        info = T.sync_status()
+        S = py_rccsd.SparseTensor4d.empty(dims)
+        S.recompress_async(xd, m, "fri", ud[k:k + 1])
+        sinfo = S.sync_status()
+        assert np.array_equal(T.indices(), S.indices()), k
+        assert np.array_equal(T.values().view(np.int64), S.values().view(np.int64)), k
+        assert info["n_notfound"] == sinfo["n_notfound"] and info["p_total"] == sinfo["p_total"]
+        oi, ov, _ = lo.compress(x, m, lo.FRI_SYSTEMATIC, u)
+        assert np.array_equal(T.indices(), oi) and np.array_equal(T.values().view(np.int64), ov.view(np.int64))
