@@ -221,15 +221,15 @@ __device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* total, uint
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------------------
-// Device workspace: a grow-only arena per named slot.  One per device.
-struct Workspace {
-  void* ptr[64] = {nullptr};
-  size_t cap[64] = {0};
-};
-
+// Device workspace: grow-only slots of the calling thread's selected arena on
+// the current device (arena.h holds the rules).
 int ws_get(int slot, size_t bytes, void** out);
 
-// Exclusive scan of n uint32 values (in place allowed), via tile sums.
-int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, hipStream_t s);
+// Exclusive scan of n uint32 values (in place allowed), via tile sums.  sums:
+// scan_u32_sums_words(n) words of device scratch, or nullptr to take them from
+// the workspace.
+int64_t scan_u32_sums_words(int64_t n);
+int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, hipStream_t s,
+             uint32_t* sums = nullptr);
 
 }  // namespace fricc

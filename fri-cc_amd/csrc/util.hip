@@ -1,4 +1,5 @@
 // Error state, device workspace arena and a generic exclusive u32 scan.
+#include "arena.h"
 #include "common.h"
 
 #include <mutex>
@@ -9,17 +10,38 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
-// Workspace arenas: per device, NARENA independent sets of grow-only slots.
-// Work that may run concurrently (solves on different streams) selects
-// different arenas; everything recorded or replayed for one solve uses that
-// solve's arena, so two solves never share scratch.
-constexpr int NARENA = 64;
-static Workspace g_ws[16][NARENA];
+// Workspace arenas: per device, NARENA independent sets of grow-only slots
+// (arena.h).  Work that may run concurrently (solves on different streams)
+// selects different arenas; everything recorded or replayed for one solve
+// uses that solve's arena, so two solves never share scratch.  The selection
+// is per host thread.
+struct HipAlloc {
+  int alloc(void** p, size_t b) { return (int)hipMalloc(p, b); }
+  int free(void* p) { return (int)hipFree(p); }
+  int sync() { return (int)hipDeviceSynchronize(); }
+  const char* error_string(int e) { return hipGetErrorString((hipError_t)e); }
+};
+
+constexpr int MAXDEV = 16;
+static ArenaBook<HipAlloc> g_book[MAXDEV];
 static std::mutex g_ws_mu;
-static int g_arena = 0;
+static thread_local int g_arena = 0;
+
+static int arena_status(int st) {
+  return st == ARENA_OK ? OK : st == ARENA_EINVAL ? EINVAL_ : st == ARENA_ENOMEM ? ENOMEM_ : EDEVICE;
+}
+
+static int current_device(int* dev) {
+  FRICC_HIP(hipGetDevice(dev));
+  if (*dev < 0 || *dev >= MAXDEV) {
+    set_error("workspace: device index out of range");
+    return EINVAL_;
+  }
+  return OK;
+}
 
 int ws_select(int arena) {
-  if (arena < 0 || arena >= NARENA) {
+  if (!ArenaBook<HipAlloc>::valid_arena(arena)) {
     set_error("workspace arena out of range (0..63)");
     return EINVAL_;
   }
@@ -28,53 +50,36 @@ int ws_select(int arena) {
 }
 
 int ws_release(int arena) {
-  if (arena < 0 || arena >= NARENA) {
-    set_error("workspace arena out of range (0..63)");
-    return EINVAL_;
-  }
   int dev = 0;
-  FRICC_HIP(hipGetDevice(&dev));
+  FRICC_TRY(current_device(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace& w = g_ws[dev][arena];
-  bool any = false;
-  for (int s = 0; s < 64; s++) any = any || w.ptr[s];
-  if (any) FRICC_HIP(hipDeviceSynchronize());
-  for (int s = 0; s < 64; s++) {
-    if (w.ptr[s]) FRICC_HIP(hipFree(w.ptr[s]));
-    w.ptr[s] = nullptr;
-    w.cap[s] = 0;
-  }
-  return OK;
+  HipAlloc al;
+  std::string err;
+  const int st = g_book[dev].release(al, arena, &err);
+  if (st != ARENA_OK) set_error(err);
+  return arena_status(st);
+}
+
+int ws_pin(int arena, int delta) {
+  int dev = 0;
+  FRICC_TRY(current_device(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  std::string err;
+  const int st = g_book[dev].pin(arena, delta, &err);
+  if (st != ARENA_OK) set_error(err);
+  return arena_status(st);
 }
 
 // Grow-only slot of the selected arena.  Contents are not preserved on growth.
 int ws_get(int slot, size_t bytes, void** out) {
   int dev = 0;
-  FRICC_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 16 || slot < 0 || slot >= 64) {
-    set_error("workspace slot/device out of range");
-    return EINVAL_;
-  }
+  FRICC_TRY(current_device(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace& w = g_ws[dev][g_arena];
-  if (bytes == 0) bytes = 256;
-  if (w.cap[slot] < bytes) {
-    if (w.ptr[slot]) {
-      FRICC_HIP(hipDeviceSynchronize());
-      FRICC_HIP(hipFree(w.ptr[slot]));
-      w.ptr[slot] = nullptr;
-      w.cap[slot] = 0;
-    }
-    size_t b = bytes + bytes / 8 + 256;
-    hipError_t e = hipMalloc(&w.ptr[slot], b);
-    if (e != hipSuccess) {
-      set_error(std::string("workspace hipMalloc failed: ") + hipGetErrorString(e));
-      return ENOMEM_;
-    }
-    w.cap[slot] = b;
-  }
-  *out = w.ptr[slot];
-  return OK;
+  HipAlloc al;
+  std::string err;
+  const int st = g_book[dev].get(al, g_arena, slot, bytes, out, &err);
+  if (st != ARENA_OK) set_error(err);
+  return arena_status(st);
 }
 
 // --- exclusive scan of u32 ----------------------------------------------------
@@ -135,12 +140,16 @@ __global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restri
   }
 }
 
-int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, hipStream_t s) {
+int64_t scan_u32_sums_words(int64_t n) { return std::max<int64_t>(1, ceil_div(n, SCAN_TILE)); }
+
+int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, hipStream_t s, uint32_t* sums) {
   if (n <= 0) return OK;
   const int64_t nt = ceil_div(n, SCAN_TILE);
-  void* p = nullptr;
-  FRICC_TRY(ws_get(63, (size_t)nt * 4, &p));
-  uint32_t* sums = (uint32_t*)p;
+  if (!sums) {
+    void* p = nullptr;
+    FRICC_TRY(ws_get(63, (size_t)nt * 4, &p));
+    sums = (uint32_t*)p;
+  }
   k_scan_tile_sums<<<nt, SCAN_NT, 0, s>>>(in, n, sums);
   k_scan_sums<<<1, 1024, 0, s>>>(sums, nt, total_dev);
   k_scan_tiles<<<nt, SCAN_NT, 0, s>>>(in, out, n, sums);
@@ -175,6 +184,8 @@ extern "C" {
 int fricc_workspace_select(int arena) { return fricc::ws_select(arena); }
 // Frees an arena's buffers (waits for the device first).
 int fricc_workspace_release(int arena) { return fricc::ws_release(arena); }
+// +1: a recorded graph holds the arena's slots; -1: that graph was released.
+int fricc_workspace_pin(int arena, int delta) { return fricc::ws_pin(arena, delta); }
 
 // Device-to-device copy of `bytes` (a multiple of 16, 16-byte aligned
 // pointers) by a plain streaming kernel.  Bandwidth reference for the

@@ -23,7 +23,7 @@ EXPORTS = (
     "fricc_sparse_prepare", "fricc_split_bf16", "fricc_set_bf16_passes", "fricc_t2_finalize",
     "fricc_set_bf16_view_width", "fricc_stream_copy", "fricc_sparse_compress_async", "fricc_sparse_status",
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
-    "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async",
+    "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
 )
 
 
@@ -100,6 +100,7 @@ def lib():
     L.fricc_sparse_status.argtypes = [_vp, _vp]
     L.fricc_workspace_select.argtypes = [ctypes.c_int]
     L.fricc_workspace_release.argtypes = [ctypes.c_int]
+    L.fricc_workspace_pin.argtypes = [ctypes.c_int, ctypes.c_int]
     L.fricc_t2_finalize.argtypes = [ctypes.c_int, ctypes.c_int] + [_vp] * 3 + [ctypes.c_int] + [_vp] * 7 + [_vp]
     L.fricc_sparse_prepare.argtypes = [_vp, ctypes.c_char_p, I64x4, I64x4, ctypes.c_int, _vp]
     for name in EXPORTS:

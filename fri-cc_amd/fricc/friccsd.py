@@ -549,6 +549,21 @@ def arena_stream(arena):
     return st
 
 
+def _unpin(arena, device):
+    torch = _torch()
+    with torch.cuda.device(device):
+        _capi.check(_capi.lib().fricc_workspace_pin(int(arena), -1))
+
+
+def _pin_arena(arena, owner):
+    """Pin the library arena ``arena`` on the current device for as long as
+    ``owner`` (a recorded graph's holder) lives."""
+    import weakref
+
+    _capi.check(_capi.lib().fricc_workspace_pin(int(arena), 1))
+    weakref.finalize(owner, _unpin, int(arena), _torch().cuda.current_device())
+
+
 class GraphStep:
     """``update_amps`` recorded once as a graph for this (nocc, nvir, m) shape
     and replayed every iteration: one launch of the whole step instead of a
@@ -583,6 +598,10 @@ class GraphStep:
         with torch.cuda.graph(self.graph, stream=st):
             self.out = cc._device_step(self.t1, self.t2, eris)
         torch.cuda.synchronize()
+        # the graph holds the addresses of the arena's slots: while it lives,
+        # growing one of them (an eager call on a larger shape on this arena)
+        # is an error instead of freeing memory the graph uses
+        _pin_arena(cc.arena, self)
 
     def issue(self):
         """Queue one iteration on the current stream (uniforms, replay); no wait."""

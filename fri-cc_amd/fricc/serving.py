@@ -134,6 +134,27 @@ def device_free(need):
     return free
 
 
+def _solve_failure(err):
+    """True when ``err`` is a failure of one solve that the server answers and
+    goes on from: a refused request (ValueError, e.g. m > |T2|) or a status a
+    step reported about its own data (non-finite amplitudes, p > 1).  A device
+    error or any other runtime error is not: the context may be unusable, and
+    it must reach the caller."""
+    from . import _capi
+
+    if isinstance(err, _capi.FriccError):
+        return isinstance(err, ValueError) or err.status in (_capi.FRICC_EPROB, _capi.FRICC_ENONFINITE)
+    return isinstance(err, ValueError)
+
+
+def _release_arenas(n):
+    """Free the library scratch of arenas 1..n (waits for the device)."""
+    from . import _capi
+
+    for a in range(1, n + 1):
+        _capi.check(_capi.lib().fricc_workspace_release(a))
+
+
 def system_key(req):
     """(nocc, nvir, naux, sys_seed): the synthetic system a request names."""
     return (req.nocc, req.nvir, req.naux or 3 * (req.nocc + req.nvir), req.sys_seed)
@@ -212,6 +233,10 @@ class Solve:
             self.cc._iter = len(self.energies)
             self.runner.issue()
 
+    def close(self):
+        """Drop the recorded step (which unpins the solve's scratch arena)."""
+        self.runner = None
+
     def ready(self):
         return self.stream.query()
 
@@ -230,6 +255,11 @@ class Solve:
             self.converged = True
         self.done = self.converged or len(self.energies) >= self.req.max_cycle or not math.isfinite(e)
         return self.done
+
+
+# Whether solves' device work may run side by side: every kernel a step
+# launches must finish without waiting for other workgroups of its grid.
+OVERLAP_SAFE = False
 
 
 class DeviceOrder:
@@ -291,6 +321,11 @@ class Server:
                  policy="srpt", record_timeline=False):
         if policy not in ("srpt", "rr"):
             raise ValueError(f"unknown policy {policy!r} (srpt, rr)")
+        if overlap and not OVERLAP_SAFE:
+            raise ValueError("Server(overlap=True) is disabled: the dense remainder's vendor GEMMs (stream-K "
+                             "kernels whose workgroups wait for each other) stopped making progress when solves "
+                             "ran side by side; one solve's device work at a time (overlap=False) is the safe "
+                             "order")
         self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision
@@ -442,6 +477,8 @@ class Server:
                         self._dirty.add(arena)
                         s = self._try_solve(req, arena, system)
                 except (ValueError, RuntimeError) as err:
+                    if not _solve_failure(err):
+                        raise
                     # the solver refused the request (e.g. m > |T2|): answer it, go on
                     pending.pop(0)
                     free_arenas.insert(0, arena)
@@ -486,7 +523,10 @@ class Server:
                 error = ""
                 try:
                     done = s.complete(order)
-                except RuntimeError as err:  # a status the step reported (non-finite amplitudes, p > 1)
+                except (ValueError, RuntimeError) as err:
+                    if not _solve_failure(err):
+                        raise
+                    # a status the step reported about its own data (non-finite amplitudes, p > 1)
                     done, error = True, f"{type(err).__name__}: {err}"
                 if order is not None:
                     order.end(s.stream)  # the copies of the outputs into the inputs
@@ -495,6 +535,7 @@ class Server:
                     results.append(Result(s.req.id, s.req.nocc, s.req.nvir, s.req.m, s.req.arrival, s.admitted, fin,
                                           len(s.energies), s.converged, list(s.energies), error=error))
                     active.remove(s)
+                    s.close()
                     used -= s.need
                     self._note((round(fin, 3), "retire", s.req.id, s.req.nocc, s.req.nvir, s.need, used, None))
                     free_arenas.append(s.cc.arena)
@@ -522,8 +563,7 @@ class Server:
             if not progressed:
                 time.sleep(5e-5)
         torch.cuda.synchronize()
-        for a in range(1, self.max_active + 1):
-            _capi.check(_capi.lib().fricc_workspace_release(a))
+        _release_arenas(self.max_active)
         self.makespan = clock() - t0
         return sorted(results, key=lambda r: r.id)
 

@@ -10,8 +10,9 @@
  * where the reference terminates the process (bad compression/term string,
  * p > 1, out-of-range index) return an error status instead.
  *
- * Threading: calls on one device are serialised by the caller; the library
- * keeps a per-device workspace arena.
+ * Threading: calls that share a workspace arena on one device are serialised
+ * by the caller; the library keeps per-device scratch arenas, selected per
+ * host thread (fricc_workspace_select).
  */
 #ifndef FRICC_CAPI_H
 #define FRICC_CAPI_H
@@ -207,19 +208,25 @@ int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void
                       const double* lovov, const double* t1new, double* sums, void* stream);
 
 /* Column-tile width of the gathered views of FRICC_BF16 contractions: 0 =
- * automatic (512 for operands too large for the caches, else 128), or a
+ * automatic (256 for operands of more than 1 GB as FP32, else 128), or a
  * multiple of 128.  Tuning knob; no counterpart in the reference. */
 int fricc_set_bf16_view_width(int width);
 
 /* Scratch arenas.  The library keeps grow-only scratch (compression buffers,
  * view slot tables, partial sums) per device in 64 numbered arenas; calls use
- * the arena selected last (0 by default).  Work that can run concurrently on
- * different streams -- solves served side by side -- must use different
- * arenas; a recorded graph keeps the arena it was recorded with.
+ * the arena the calling thread selected last (0 by default; the selection is
+ * per host thread).  Work that can run concurrently on different streams --
+ * solves served side by side -- must use different arenas; a recorded graph
+ * keeps the addresses of the arena it was recorded with.
  * fricc_workspace_release frees an arena's buffers (it waits for the device).
+ * fricc_workspace_pin(arena, +1) declares that a recorded graph holds the
+ * arena, (arena, -1) that it was released: while an arena is pinned, a call
+ * that would have to grow one of its slots fails with FRICC_EINVAL instead of
+ * freeing memory the graph still uses, and release fails likewise.
  * No counterpart in the reference. */
 int fricc_workspace_select(int arena);
 int fricc_workspace_release(int arena);
+int fricc_workspace_pin(int arena, int delta);
 
 /* Device-to-device copy of `bytes` (a multiple of 16; 16-byte aligned device
  * pointers) by a plain streaming kernel: the reference point for achievable

@@ -283,10 +283,14 @@ def default_settings(**kw):
     return s
 
 
-def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False):
+def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False, state=None):
     """fricc/friccsd.py:29-301.  ``uniforms`` is an iterator of floats in [0,1],
     one consumed per FRI compression (lin_space_add_const, src/fri_utils.cpp:47),
-    in the order t2_sparse, t2_sparse_alt."""
+    in the order t2_sparse, t2_sparse_alt.
+
+    ``state`` (a dict, kept across calls by the caller) carries what the
+    reference keeps on the solver object for the unbiased estimator: "iter"
+    (``cc._iter``, the iteration index) and "t2_avg" (``cc._t2_avg``)."""
     o, v = t1.shape
     m = int(settings["m_keep"])
     if m > t2.size or m == 0:
@@ -306,7 +310,20 @@ def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False):
     ts = SparseT(t2, m, comp, draw(), mode)
     ts_alt = None
     if "O^3V^3" in cc or "O^4V^2" in cc:
-        ts_alt = SparseT(t2, m, comp, draw(), mode)
+        it_now = 0 if state is None else state.get("iter", 0)
+        if not settings["unbiased"] or it_now < settings["burn_in"]:
+            ts_alt = SparseT(t2, m, comp, draw(), mode)
+        else:
+            # fricc/friccsd.py:117-136: after the burn-in the second sample is
+            # taken of the running average Y(t+1) = (1 - 1/t) Y(t) + X(t+1)/t,
+            # with the reference's operation order t2/it + (it-1)/it * avg;
+            # at it == 0 the average starts as t2 itself
+            it = it_now - settings["burn_in"]
+            if it == 0:
+                state["t2_avg"] = t2.copy()
+            else:
+                state["t2_avg"] = t2 / it + (it - 1) / it * state["t2_avg"]
+            ts_alt = SparseT(np.ascontiguousarray(state["t2_avg"]), m, comp, draw(), mode)
 
     Foo = cc_Foo(t1, t2, eris)
     Fvv = cc_Fvv(t1, t2, eris)
@@ -433,8 +450,10 @@ def kernel(eris, settings, uniforms, max_cycle=50, tol=1e-8, tolnormt=1e-6, t1=N
     ecc = energy(t1, t2, eris)
     energies = []
     conv = False
+    state = {}
     for istep in range(max_cycle):
-        t1new, t2new = update_amps(t1, t2, eris, settings, uniforms)
+        state["iter"] = istep  # cc._iter (the reference never sets it, quirk A.3)
+        t1new, t2new = update_amps(t1, t2, eris, settings, uniforms, state=state)
         normt = np.linalg.norm(amplitudes_to_vector(t1new, t2new) - amplitudes_to_vector(t1, t2))
         t1, t2 = t1new, t2new
         eold, ecc = ecc, energy(t1, t2, eris)

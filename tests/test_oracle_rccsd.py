@@ -139,3 +139,56 @@ def test_fri_kernel_runs_and_is_near_ccsd():
                                            tol=0, tolnormt=0)
     assert len(energies) == 120
     assert abs(np.mean(energies[40:]) - ecc) < 0.02 * abs(ecc)
+
+
+def _unbiased_run(eris, t1, t2, burn_in, n_iter, us, planted=None):
+    """The oracle's loop body with the unbiased branch on: returns the t2
+    inputs of every iteration and the running average the branch kept."""
+    s = rccsd.default_settings(m_keep=t2.size // 3, unbiased=True, burn_in=burn_in)
+    state, xs, avgs = {}, [], []
+    uit = iter(us)
+    for istep in range(n_iter):
+        state["iter"] = istep
+        xs.append(t2.copy())
+        t1, t2 = rccsd.update_amps(t1, t2, eris, s, uit, state=state)
+        avgs.append(state.get("t2_avg"))
+    return xs, avgs
+
+
+def test_unbiased_average_is_mean_of_inputs_after_burn_in():
+    """fricc/friccsd.py:117-136: after the burn-in the running average
+    Y(it) = t2/it + (it-1)/it * Y(it-1), started at Y(0) = t2, equals the
+    arithmetic mean of the t2 inputs of iterations burn_in+1 .. burn_in+it
+    (the input at it == 0 drops out with weight 0 at it == 1); before the
+    burn-in no average exists."""
+    eris, _ = _system()
+    t1, t2 = _amps(eris)
+    burn_in, n_iter = 2, 7
+    xs, avgs = _unbiased_run(eris, t1, t2, burn_in, n_iter, np.random.default_rng(5).random(2 * n_iter))
+    assert avgs[0] is None and avgs[1] is None
+    np.testing.assert_array_equal(avgs[burn_in], xs[burn_in])       # it == 0: t2 itself
+    np.testing.assert_array_equal(avgs[burn_in + 1], xs[burn_in + 1])  # it == 1: 0 * Y + t2/1
+    for istep in range(burn_in + 2, n_iter):
+        it = istep - burn_in
+        mean = np.mean(xs[burn_in + 1:istep + 1], axis=0)
+        np.testing.assert_allclose(avgs[istep], mean, rtol=1e-13, atol=1e-16)
+        # a plausible slip (weights it/(it+1) instead of (it-1)/it) is far off
+        wrong = xs[istep] / (it + 1) + it / (it + 1) * avgs[istep - 1]
+        assert np.abs(wrong - mean).max() > 1e-6 * np.abs(mean).max()
+
+
+def test_unbiased_first_two_averaged_steps_equal_plain_steps():
+    """At it == 0 and it == 1 the average equals the current t2 exactly, so
+    the update with the unbiased branch on equals the plain update with the
+    same uniforms."""
+    eris, _ = _system()
+    t1, t2 = _amps(eris)
+    m = t2.size // 3
+    us = [0.11, 0.62]
+    for it in (0, 1):
+        plain = rccsd.update_amps(t1, t2, eris, rccsd.default_settings(m_keep=m), iter(us))
+        st = {"iter": 3 + it, "t2_avg": np.full_like(t2, 7.0)}
+        ub = rccsd.update_amps(t1, t2, eris, rccsd.default_settings(m_keep=m, unbiased=True, burn_in=3), iter(us),
+                               state=st)
+        np.testing.assert_array_equal(plain[0], ub[0])
+        np.testing.assert_array_equal(plain[1], ub[1])

@@ -99,3 +99,116 @@ def test_srpt_second_slot_never_delays_the_shorter_solve():
     assert server._may_follow(long_, [solve(49, 2.0)])
     assert server._may_follow(long_, [solve(0, None)])
     assert sv.Server(policy="rr")._may_follow(long_, [short])
+
+
+def test_overlap_refused_while_unsafe():
+    """Side-by-side replay stays off while a step's kernels may wait for
+    workgroups of their own grid (the vendor stream-K GEMMs)."""
+    import pytest
+
+    if not sv.OVERLAP_SAFE:
+        with pytest.raises(ValueError, match="overlap"):
+            sv.Server(overlap=True)
+
+
+def test_solve_failure_classification():
+    """Only a refused request or a step's own data error is answered as one
+    solve's failure; device errors and other runtime errors are not."""
+    from fricc import _capi
+
+    assert sv._solve_failure(ValueError("m_keep"))
+    assert sv._solve_failure(_capi.FriccValueError(_capi.FRICC_EINVAL, "bad"))
+    assert sv._solve_failure(_capi.FriccError(_capi.FRICC_EPROB, "p > 1"))
+    assert sv._solve_failure(_capi.FriccError(_capi.FRICC_ENONFINITE, "nan"))
+    assert not sv._solve_failure(_capi.FriccError(_capi.FRICC_EDEVICE, "illegal address"))
+    assert not sv._solve_failure(_capi.FriccError(_capi.FRICC_ENOMEM, "oom"))
+    assert not sv._solve_failure(RuntimeError("HIP error: an illegal memory access was encountered"))
+
+
+def _fake_device(monkeypatch, solve_cls):
+    """Replace the device layer the scheduler touches (torch.cuda, streams)
+    with host stand-ins, and the solve with ``solve_cls``."""
+    from types import SimpleNamespace as NS
+
+    class Event:
+        def __init__(self, **kw):
+            self.t = 0.0
+
+        def record(self, stream=None):
+            pass
+
+        def elapsed_time(self, other):
+            return 1.0
+
+    class Stream:
+        def wait_event(self, ev):
+            pass
+
+        def synchronize(self):
+            pass
+
+        def query(self):
+            return True
+
+    cuda = NS(synchronize=lambda: None, mem_get_info=lambda: (1 << 40, 1 << 40), memory_reserved=lambda: 0,
+              memory_allocated=lambda: 0, empty_cache=lambda: None, Event=Event, Stream=Stream,
+              OutOfMemoryError=MemoryError)
+    fake = NS(cuda=cuda, OutOfMemoryError=MemoryError)
+    monkeypatch.setattr(sv, "_torch", lambda: fake)
+    monkeypatch.setattr(sv, "arena_stream", lambda arena: Stream())
+    monkeypatch.setattr(sv, "Solve", solve_cls)
+
+
+def _fake_solve(fail_status):
+    from fricc import _capi
+
+    class FakeSolve:
+        def __init__(self, req, arena, precision, stream=None, system=None):
+            self.req, self.stream, self.energies, self.converged = req, stream, [], False
+            self.cc = type("CC", (), {"arena": arena})()
+            self.t_iter = None
+
+        def issue(self):
+            pass
+
+        def ready(self):
+            return True
+
+        def complete(self, order=None):
+            raise _capi.FriccError(fail_status, "mocked status")
+
+        def close(self):
+            pass
+
+    return FakeSolve
+
+
+def test_device_error_reaches_the_caller(monkeypatch):
+    """A device error while a solve runs (mocked FRICC_EDEVICE status) stops
+    the server and reaches the caller instead of being recorded as one
+    solve's error."""
+    import pytest
+
+    from fricc import _capi
+
+    _fake_device(monkeypatch, _fake_solve(_capi.FRICC_EDEVICE))
+    server = sv.Server(hbm_budget=1e12, precision="mixed", max_active=2)
+    monkeypatch.setattr(server.systems, "request", lambda req: type("F", (), {"done": lambda s: True,
+                                                                            "result": lambda s: None})())
+    with pytest.raises(_capi.FriccError) as ei:
+        server.run([sv.Request(0, 0.0, 3, 10, m=50, max_cycle=2)])
+    assert ei.value.status == _capi.FRICC_EDEVICE
+
+
+def test_data_error_is_one_solves_failure(monkeypatch):
+    """A step's own data error (mocked FRICC_ENONFINITE) retires that solve
+    with the error recorded; the server goes on."""
+    from fricc import _capi
+
+    _fake_device(monkeypatch, _fake_solve(_capi.FRICC_ENONFINITE))
+    server = sv.Server(hbm_budget=1e12, precision="mixed", max_active=2)
+    monkeypatch.setattr(server.systems, "request", lambda req: type("F", (), {"done": lambda s: True,
+                                                                            "result": lambda s: None})())
+    monkeypatch.setattr(sv, "_release_arenas", lambda n: None)
+    res = server.run([sv.Request(0, 0.0, 3, 10, m=50, max_cycle=2), sv.Request(1, 0.0, 3, 10, m=50, max_cycle=2)])
+    assert len(res) == 2 and all("mocked status" in r.error for r in res)

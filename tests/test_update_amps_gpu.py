@@ -121,6 +121,31 @@ def test_kernel_trajectory_matches_oracle(env):
     np.testing.assert_allclose(cc.energies, en, rtol=1e-9)
 
 
+def test_unbiased_branch_matches_oracle(env):
+    """The unbiased estimator (fricc/friccsd.py:108-136): after the burn-in,
+    t2_sparse_alt samples the running average of t2 instead of t2.  Seven
+    iterations with burn_in = 2 (so the average runs over five, past the
+    it == 0 and it == 1 special cases) agree with the oracle loop on the same
+    uniforms: energies to 1e-9 relative, final t2 likewise."""
+    torch, FRICCSD, SyntheticSystem, rccsd = env
+    o, v, naux = 4, 14, 40
+    us = list(np.random.default_rng(13).random(40))
+    settings = {"m_keep": 1500, "compression": "fri", "sampling_method": "systematic", "level_shift": 2.0,
+                "unbiased": True, "burn_in": 2}
+    cc, eris, oeris, s = _pair(env, o, v, naux, 8, settings, us)
+    s.update(unbiased=True, burn_in=2)
+    cc.max_cycle = 7
+    cc.conv_tol = cc.conv_tol_normt = 0
+    cc.kernel()
+    _, _, _, ot2, en = rccsd.kernel(oeris, s, iter(us), max_cycle=7, tol=0, tolnormt=0)
+    assert len(cc.energies) == 7
+    np.testing.assert_allclose(cc.energies, en, rtol=1e-9)
+    gt2 = cc.t2.cpu().numpy()
+    assert np.abs(gt2 - ot2).max() <= 1e-9 * np.abs(ot2).max()
+    # the averaged samples were taken: the branch ran past the burn-in
+    assert getattr(cc, "_t2_avg", None) is not None
+
+
 def test_nan_stops_the_loop(env):
     torch, FRICCSD, SyntheticSystem, rccsd = env
     cc = FRICCSD(SyntheticSystem(3, 8, 20, 1), {"m_keep": 300, "compressed_contractions": []},
