@@ -40,6 +40,9 @@
 #include <stdlib.h>
 #include <math.h>
 #include <omp.h>
+#include <immintrin.h>
+
+#include <vector>
 
 #include <algorithm>
 #include <numeric>
@@ -47,6 +50,74 @@
 #include <string>
 
 typedef unsigned __int128 u128;
+
+// (pq|rs) = fma chain over Q, tiled like a GEMM: rows (p,q) with their L
+// values packed [row][Q], columns (r,s) in chunks packed [Q][col]; each output
+// element is still acc = 0, acc = fma(a_Q, b_Q, acc) for Q = 0..naux-1 in
+// order (vector lanes hold different elements), so every value is bitwise
+// that of the plain loop.
+__attribute__((target("avx2,fma"))) static void eri_block_packed(const double* L, int64_t naux, int64_t nmo,
+                                                                  int64_t p0, int64_t p1, int64_t q0, int64_t q1,
+                                                                  int64_t r0, int64_t r1, int64_t s0, int64_t s1,
+                                                                  double* out) {
+  const int64_t nq = q1 - q0, nr = r1 - r0, ns = s1 - s0;
+  const int64_t nrow = (p1 - p0) * nq, ncol = nr * ns, n2 = nmo * nmo;
+  std::vector<double> At((size_t)nrow * naux);
+#pragma omp parallel for schedule(static)
+  for (int64_t row = 0; row < nrow; row++) {
+    const int64_t off = (row / nq + p0) * nmo + (row % nq + q0);
+    for (int64_t Q = 0; Q < naux; Q++) At[(size_t)row * naux + Q] = L[Q * n2 + off];
+  }
+  constexpr int64_t CB = 240;  // columns per chunk: 20 groups of 12
+  const int64_t nchunk = (ncol + CB - 1) / CB;
+#pragma omp parallel
+  {
+    std::vector<double> Bp((size_t)naux * CB);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < nchunk; ch++) {
+      const int64_t c0 = ch * CB, nc = std::min(CB, ncol - c0);
+      for (int64_t Q = 0; Q < naux; Q++)
+        for (int64_t c = 0; c < CB; c++) {
+          const int64_t col = c0 + c;
+          Bp[(size_t)Q * CB + c] =
+              c < nc ? L[Q * n2 + (col / ns + r0) * nmo + (col % ns + s0)] : 0.0;
+        }
+      int64_t row = 0;
+      for (; row + 4 <= nrow; row += 4) {
+        const double* a0 = &At[(size_t)row * naux];
+        for (int64_t j = 0; j < nc; j += 12) {
+          __m256d acc[4][3];
+          for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 3; k++) acc[i][k] = _mm256_setzero_pd();
+          for (int64_t Q = 0; Q < naux; Q++) {
+            const double* b = &Bp[(size_t)Q * CB + j];
+            const __m256d b0 = _mm256_loadu_pd(b), b1 = _mm256_loadu_pd(b + 4), b2 = _mm256_loadu_pd(b + 8);
+            for (int i = 0; i < 4; i++) {
+              const __m256d a = _mm256_broadcast_sd(a0 + (size_t)i * naux + Q);
+              acc[i][0] = _mm256_fmadd_pd(a, b0, acc[i][0]);
+              acc[i][1] = _mm256_fmadd_pd(a, b1, acc[i][1]);
+              acc[i][2] = _mm256_fmadd_pd(a, b2, acc[i][2]);
+            }
+          }
+          for (int i = 0; i < 4; i++) {
+            alignas(32) double t[12];
+            for (int k = 0; k < 3; k++) _mm256_store_pd(t + 4 * k, acc[i][k]);
+            const int64_t w = std::min<int64_t>(12, nc - j);
+            for (int64_t c = 0; c < w; c++) out[(size_t)(row + i) * ncol + c0 + j + c] = t[c];
+          }
+        }
+      }
+      for (; row < nrow; row++) {
+        const double* a = &At[(size_t)row * naux];
+        for (int64_t c = 0; c < nc; c++) {
+          double acc = 0.0;
+          for (int64_t Q = 0; Q < naux; Q++) acc = __builtin_fma(a[Q], Bp[(size_t)Q * CB + c], acc);
+          out[(size_t)row * ncol + c0 + c] = acc;
+        }
+      }
+    }
+  }
+}
 
 extern "C" {
 
@@ -406,9 +477,10 @@ int orc_contract_ref(const char* term, const double* W, const int64_t* wshape, c
 // of an 8-fold orbit gets bitwise the same value.  The GPU builder uses the
 // same chain, so the integrals are bit-identical between the two.
 // Writes out[(p-p0), (q-q0), (r-r0), (s-s0)] row-major.
-void orc_eri_block(const double* L, int64_t naux, int64_t nmo, int64_t p0, int64_t p1,
-                   int64_t q0, int64_t q1, int64_t r0, int64_t r1, int64_t s0, int64_t s1,
-                   double* out) {
+// The plain loop: each element's fma chain over Q, L read with stride nmo^2.
+void orc_eri_block_plain(const double* L, int64_t naux, int64_t nmo, int64_t p0, int64_t p1,
+                         int64_t q0, int64_t q1, int64_t r0, int64_t r1, int64_t s0, int64_t s1,
+                         double* out) {
   const int64_t np = p1 - p0, nq = q1 - q0, nr = r1 - r0, ns = s1 - s0;
   const int64_t n2 = nmo * nmo;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -422,6 +494,15 @@ void orc_eri_block(const double* L, int64_t naux, int64_t nmo, int64_t p0, int64
           for (int64_t Q = 0; Q < naux; Q++) acc = fma(a[Q * n2], b[Q * n2], acc);
           out[((p * nq + q) * nr + r) * ns + s] = acc;
         }
+}
+
+void orc_eri_block(const double* L, int64_t naux, int64_t nmo, int64_t p0, int64_t p1,
+                   int64_t q0, int64_t q1, int64_t r0, int64_t r1, int64_t s0, int64_t s1,
+                   double* out) {
+  if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma"))
+    eri_block_packed(L, naux, nmo, p0, p1, q0, q1, r0, r1, s0, s1, out);
+  else
+    orc_eri_block_plain(L, naux, nmo, p0, p1, q0, q1, r0, r1, s0, s1, out);
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -54,6 +54,8 @@ def lib():
         L.orc_contract_ref.restype = ctypes.c_int
         L.orc_eri_block.argtypes = [dp, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_int64] * 8 + [dp]
         L.orc_eri_block.restype = None
+        L.orc_eri_block_plain.argtypes = L.orc_eri_block.argtypes
+        L.orc_eri_block_plain.restype = None
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -125,12 +127,15 @@ def contract_ref(term, W, idx, val, tdims, out, block_lo=0, block_hi=-1, idx4=No
     return out
 
 
-def eri_block(L, p, q, r, s):
-    """(pq|rs) as the fma chain over Q (bit-identical to the GPU builder)."""
+def eri_block(L, p, q, r, s, plain=False):
+    """(pq|rs) as the fma chain over Q (bit-identical to the GPU builder).
+    ``plain``: the unblocked loop (the packed, tiled one computes the same
+    chains; the two are compared in tests)."""
     L = np.ascontiguousarray(L, dtype=np.float64)
     naux, nmo, _ = L.shape
     out = np.empty((p[1] - p[0], q[1] - q[0], r[1] - r[0], s[1] - s[0]))
-    lib().orc_eri_block(_dp(L), naux, nmo, p[0], p[1], q[0], q[1], r[0], r[1], s[0], s[1], _dp(out))
+    f = lib().orc_eri_block_plain if plain else lib().orc_eri_block
+    f(_dp(L), naux, nmo, p[0], p[1], q[0], q[1], r[0], r[1], s[0], s[1], _dp(out))
     return out
 
 

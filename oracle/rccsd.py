@@ -33,6 +33,13 @@ from . import liboracle
 ALLOWED_CONTRACTIONS = ("O^2V^4", "O^3V^3", "O^4V^2")
 
 
+def einsum(spec, *ops):
+    """np.einsum with its contraction-order search (pairwise BLAS
+    contractions): the same sums as PySCF's lib.einsum, which the reference
+    calls, in a BLAS order; FP64 throughout."""
+    return np.einsum(spec, *ops, optimize=True)
+
+
 class ERIs:
     """Chemists'-notation MO integral blocks, as ``ccsd._ChemistsERIs`` holds them
     (fricc/friccsd.py:80-86,164-187,310-330): oooo=(ij|kl), ovoo=(ia|jk),
@@ -55,6 +62,26 @@ class ERIs:
     def get_ovvv(self):
         return self.ovvv
 
+    @classmethod
+    def from_factors(cls, L, mo_energy, nocc):
+        """The blocks straight from the factors (no full nmo^4 tensor): each
+        integral is the same fma chain as in the full build."""
+        e = cls.__new__(cls)
+        o, nmo = nocc, L.shape[1]
+        O, V = (0, o), (o, nmo)
+        e.fock = np.diag(mo_energy)
+        e.mo_energy = np.array(mo_energy, dtype=np.float64)
+        e.nocc = o
+        blk = lambda a, b, c, d: liboracle.eri_block(L, a, b, c, d)  # noqa: E731
+        e.oooo = blk(O, O, O, O)
+        e.ovoo = blk(O, V, O, O)
+        e.ovov = blk(O, V, O, V)
+        e.oovv = blk(O, O, V, V)
+        e.ovvo = blk(O, V, V, O)
+        e.ovvv = blk(O, V, V, V)
+        e.vvvv = blk(V, V, V, V)
+        return e
+
 
 def synthetic_L(nocc, nvir, naux, seed):
     """Synthetic density-fitting factors (the c2/c3 recipe): occupied energies
@@ -71,6 +98,12 @@ def synthetic_L(nocc, nvir, naux, seed):
     return np.ascontiguousarray(L), np.concatenate([eo, ev])
 
 
+def synthetic_eris_blocks(nocc, nvir, naux, seed):
+    """synthetic_eris without the full ERI tensor (for c2/c3-sized systems)."""
+    L, e = synthetic_L(nocc, nvir, naux, seed)
+    return ERIs.from_factors(L, e, nocc), L
+
+
 def synthetic_eris(nocc, nvir, naux, seed):
     L, e = synthetic_L(nocc, nvir, naux, seed)
     nmo = nocc + nvir
@@ -83,10 +116,10 @@ def synthetic_eris(nocc, nvir, naux, seed):
 def cc_Foo(t1, t2, eris):
     o = t1.shape[0]
     ovov = eris.ovov
-    Fki = 2 * np.einsum("kcld,ilcd->ki", ovov, t2)
-    Fki -= np.einsum("kdlc,ilcd->ki", ovov, t2)
-    Fki += 2 * np.einsum("kcld,ic,ld->ki", ovov, t1, t1)
-    Fki -= np.einsum("kdlc,ic,ld->ki", ovov, t1, t1)
+    Fki = 2 * einsum("kcld,ilcd->ki", ovov, t2)
+    Fki -= einsum("kdlc,ilcd->ki", ovov, t2)
+    Fki += 2 * einsum("kcld,ic,ld->ki", ovov, t1, t1)
+    Fki -= einsum("kdlc,ic,ld->ki", ovov, t1, t1)
     Fki += eris.fock[:o, :o]
     return Fki
 
@@ -94,10 +127,10 @@ def cc_Foo(t1, t2, eris):
 def cc_Fvv(t1, t2, eris):
     o = t1.shape[0]
     ovov = eris.ovov
-    Fac = -2 * np.einsum("kcld,klad->ac", ovov, t2)
-    Fac += np.einsum("kdlc,klad->ac", ovov, t2)
-    Fac -= 2 * np.einsum("kcld,ka,ld->ac", ovov, t1, t1)
-    Fac += np.einsum("kdlc,ka,ld->ac", ovov, t1, t1)
+    Fac = -2 * einsum("kcld,klad->ac", ovov, t2)
+    Fac += einsum("kdlc,klad->ac", ovov, t2)
+    Fac -= 2 * einsum("kcld,ka,ld->ac", ovov, t1, t1)
+    Fac += einsum("kdlc,ka,ld->ac", ovov, t1, t1)
     Fac += eris.fock[o:, o:]
     return Fac
 
@@ -105,8 +138,8 @@ def cc_Fvv(t1, t2, eris):
 def cc_Fov(t1, t2, eris):
     o = t1.shape[0]
     ovov = eris.ovov
-    Fkc = 2 * np.einsum("kcld,ld->kc", ovov, t1)
-    Fkc -= np.einsum("kdlc,ld->kc", ovov, t1)
+    Fkc = 2 * einsum("kcld,ld->kc", ovov, t1)
+    Fkc -= einsum("kdlc,ld->kc", ovov, t1)
     Fkc += eris.fock[:o, o:]
     return Fkc
 
@@ -114,64 +147,72 @@ def cc_Fov(t1, t2, eris):
 def Loo(t1, t2, eris):
     o = t1.shape[0]
     fov = eris.fock[:o, o:]
-    Lki = cc_Foo(t1, t2, eris) + np.einsum("kc,ic->ki", fov, t1)
-    Lki += 2 * np.einsum("lcki,lc->ki", eris.ovoo, t1)
-    Lki -= np.einsum("kcli,lc->ki", eris.ovoo, t1)
+    Lki = cc_Foo(t1, t2, eris) + einsum("kc,ic->ki", fov, t1)
+    Lki += 2 * einsum("lcki,lc->ki", eris.ovoo, t1)
+    Lki -= einsum("kcli,lc->ki", eris.ovoo, t1)
     return Lki
 
 
 def Lvv(t1, t2, eris):
     o = t1.shape[0]
     fov = eris.fock[:o, o:]
-    Lac = cc_Fvv(t1, t2, eris) - np.einsum("kc,ka->ac", fov, t1)
-    Lac += 2 * np.einsum("kdac,kd->ac", eris.ovvv, t1)
-    Lac -= np.einsum("kcad,kd->ac", eris.ovvv, t1)
+    Lac = cc_Fvv(t1, t2, eris) - einsum("kc,ka->ac", fov, t1)
+    Lac += 2 * einsum("kdac,kd->ac", eris.ovvv, t1)
+    Lac -= einsum("kcad,kd->ac", eris.ovvv, t1)
     return Lac
 
 
 def cc_Woooo(t1, t2, eris):
-    Wklij = np.einsum("lcki,jc->klij", eris.ovoo, t1)
-    Wklij += np.einsum("kclj,ic->klij", eris.ovoo, t1)
-    Wklij += np.einsum("kcld,ijcd->klij", eris.ovov, t2)
-    Wklij += np.einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
+    Wklij = einsum("lcki,jc->klij", eris.ovoo, t1)
+    Wklij += einsum("kclj,ic->klij", eris.ovoo, t1)
+    Wklij += einsum("kcld,ijcd->klij", eris.ovov, t2)
+    Wklij += einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
     Wklij += eris.oooo.transpose(0, 2, 1, 3)
     return Wklij
 
 
 def cc_Wvoov(t1, t2, eris):
-    Wakic = np.einsum("kcad,id->akic", eris.ovvv, t1)
-    Wakic -= np.einsum("kcli,la->akic", eris.ovoo, t1)
+    Wakic = einsum("kcad,id->akic", eris.ovvv, t1)
+    Wakic -= einsum("kcli,la->akic", eris.ovoo, t1)
     Wakic += eris.ovvo.transpose(2, 0, 3, 1)
-    Wakic -= 0.5 * np.einsum("ldkc,ilda->akic", eris.ovov, t2)
-    Wakic -= 0.5 * np.einsum("lckd,ilad->akic", eris.ovov, t2)
-    Wakic -= np.einsum("ldkc,id,la->akic", eris.ovov, t1, t1)
-    Wakic += np.einsum("ldkc,ilad->akic", eris.ovov, t2)
+    Wakic -= 0.5 * einsum("ldkc,ilda->akic", eris.ovov, t2)
+    Wakic -= 0.5 * einsum("lckd,ilad->akic", eris.ovov, t2)
+    Wakic -= einsum("ldkc,id,la->akic", eris.ovov, t1, t1)
+    Wakic += einsum("ldkc,ilad->akic", eris.ovov, t2)
     return Wakic
 
 
 def cc_Wvovo(t1, t2, eris):
-    Wakci = np.einsum("kdac,id->akci", eris.ovvv, t1)
-    Wakci -= np.einsum("lcki,la->akci", eris.ovoo, t1)
+    Wakci = einsum("kdac,id->akci", eris.ovvv, t1)
+    Wakci -= einsum("lcki,la->akci", eris.ovoo, t1)
     Wakci += eris.oovv.transpose(2, 0, 3, 1)
-    Wakci -= 0.5 * np.einsum("lckd,ilda->akci", eris.ovov, t2)
-    Wakci -= np.einsum("lckd,id,la->akci", eris.ovov, t1, t1)
+    Wakci -= 0.5 * einsum("lckd,ilda->akci", eris.ovov, t2)
+    Wakci -= einsum("lckd,id,la->akci", eris.ovov, t1, t1)
     return Wakci
 
 
-def cc_Wvvvv(t1, t2, eris):
-    """fricc/friccsd.py:325-331."""
-    Wabcd = np.einsum("kdac,kb->abcd", eris.ovvv, -t1)
-    Wabcd -= np.einsum("kcbd,ka->abcd", eris.ovvv, t1)
-    Wabcd += eris.vvvv.transpose(0, 2, 1, 3)
+def cc_Wvvvv(t1, t2, eris, chunk=16):
+    """fricc/friccsd.py:325-331: W = -sum_k (kd|ac) t_kb - sum_k (kc|bd) t_ka
+    + (ac|bd), in that order per element; built over blocks of a so that no
+    second v^4 temporary exists."""
+    v = t1.shape[1]
+    ovvv = eris.ovvv
+    Wabcd = np.empty((v, v, v, v))
+    for a0 in range(0, v, chunk):
+        a1 = min(v, a0 + chunk)
+        w = einsum("kdac,kb->abcd", ovvv[:, :, a0:a1, :], -t1)
+        w -= einsum("kcbd,ka->abcd", ovvv, t1[:, a0:a1])
+        w += eris.vvvv[a0:a1].transpose(0, 2, 1, 3)
+        Wabcd[a0:a1] = w
     return Wabcd
 
 
 def energy(t1, t2, eris):
     o = t1.shape[0]
-    e = 2 * np.einsum("ia,ia", eris.fock[:o, o:], t1)
-    tau = np.einsum("ia,jb->ijab", t1, t1) + t2
-    e += 2 * np.einsum("ijab,iajb", tau, eris.ovov)
-    e -= np.einsum("ijab,ibja", tau, eris.ovov)
+    e = 2 * einsum("ia,ia", eris.fock[:o, o:], t1)
+    tau = einsum("ia,jb->ijab", t1, t1) + t2
+    e += 2 * einsum("ijab,iajb", tau, eris.ovov)
+    e -= einsum("ijab,ibja", tau, eris.ovov)
     return float(e)
 
 
@@ -181,7 +222,7 @@ def init_amps(eris):
     eia = mo_e[:o, None] - mo_e[None, o:]
     t1 = eris.fock[:o, o:] / eia
     t2 = eris.ovov.transpose(0, 2, 1, 3) / (eia[:, None, :, None] + eia[None, :, None, :])
-    emp2 = 2 * np.einsum("ijab,iajb", t2, eris.ovov) - np.einsum("jiab,iajb", t2, eris.ovov)
+    emp2 = 2 * einsum("ijab,iajb", t2, eris.ovov) - einsum("jiab,iajb", t2, eris.ovov)
     return float(emp2), t1, np.ascontiguousarray(t2)
 
 
@@ -212,58 +253,58 @@ def contract(W, T, out, term):
     """contract_DTSpT as the einsum identity of fricc/tests/test_sparse_tensors.py."""
     t = T.dense
     if term == "0101":
-        out += np.einsum("klij,klab->ijab", W, t)
+        out += einsum("klij,klab->ijab", W, t)
     elif term == "2323":
-        out += np.einsum("abcd,ijcd->ijab", W, t)
+        out += einsum("abcd,ijcd->ijab", W, t)
     elif term == "1302":
-        out += 2 * np.einsum("akic,kjcb->ijab", W, t)
+        out += 2 * einsum("akic,kjcb->ijab", W, t)
     elif term == "1202":
-        out -= np.einsum("akci,kjcb->ijab", W, t)
+        out -= einsum("akci,kjcb->ijab", W, t)
     elif term == "1303":
-        out += np.einsum("akic,kjbc->ijab", W, t)
+        out += einsum("akic,kjbc->ijab", W, t)
     elif term == "1203":
-        out += np.einsum("bkci,kjac->ijab", W, t)
+        out += einsum("bkci,kjac->ijab", W, t)
     elif term == "1323":
-        out += np.einsum("kcld,ijcd->klij", W, t)
+        out += einsum("kcld,ijcd->klij", W, t)
     elif term == "0112":
-        out -= 0.5 * np.einsum("ldkc,ilda->akic", W, t)
+        out -= 0.5 * einsum("ldkc,ilda->akic", W, t)
     elif term == "0313":
-        out -= 0.5 * np.einsum("lckd,ilad->akic", W, t)
+        out -= 0.5 * einsum("lckd,ilad->akic", W, t)
     elif term == "0113":
-        out += np.einsum("ldkc,ilad->akic", W, t)
+        out += einsum("ldkc,ilad->akic", W, t)
     elif term == "0312":
-        out -= 0.5 * np.einsum("lckd,ilda->akci", W, t)
+        out -= 0.5 * einsum("lckd,ilda->akci", W, t)
     else:
         raise ValueError(term)
     return out
 
 
 def sparse_cc_Woooo(t1, ts, eris):
-    Wklij = np.einsum("lcki,jc->klij", eris.ovoo, t1)
-    Wklij += np.einsum("kclj,ic->klij", eris.ovoo, t1)
+    Wklij = einsum("lcki,jc->klij", eris.ovoo, t1)
+    Wklij += einsum("kclj,ic->klij", eris.ovoo, t1)
     contract(eris.ovov, ts, Wklij, "1323")
-    Wklij += np.einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
+    Wklij += einsum("kcld,ic,jd->klij", eris.ovov, t1, t1)
     Wklij += eris.oooo.transpose(0, 2, 1, 3)
     return Wklij
 
 
 def sparse_cc_Wvoov(t1, ts, eris):
-    Wakic = np.einsum("kcad,id->akic", eris.ovvv, t1)
-    Wakic -= np.einsum("kcli,la->akic", eris.ovoo, t1)
+    Wakic = einsum("kcad,id->akic", eris.ovvv, t1)
+    Wakic -= einsum("kcli,la->akic", eris.ovoo, t1)
     Wakic += eris.ovvo.transpose(2, 0, 3, 1)
     contract(eris.ovov, ts, Wakic, "0112")
     contract(eris.ovov, ts, Wakic, "0313")
     contract(eris.ovov, ts, Wakic, "0113")
-    Wakic -= np.einsum("ldkc,id,la->akic", eris.ovov, t1, t1)
+    Wakic -= einsum("ldkc,id,la->akic", eris.ovov, t1, t1)
     return Wakic
 
 
 def sparse_cc_Wvovo(t1, ts, eris):
-    Wakci = np.einsum("kdac,id->akci", eris.ovvv, t1)
-    Wakci -= np.einsum("lcki,la->akci", eris.ovoo, t1)
+    Wakci = einsum("kdac,id->akci", eris.ovvv, t1)
+    Wakci -= einsum("lcki,la->akci", eris.ovoo, t1)
     Wakci += eris.oovv.transpose(2, 0, 3, 1)
     contract(eris.ovov, ts, Wakci, "0312")
-    Wakci -= np.einsum("lckd,id,la->akci", eris.ovov, t1, t1)
+    Wakci -= einsum("lckd,id,la->akci", eris.ovov, t1, t1)
     return Wakci
 
 
@@ -331,33 +372,33 @@ def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False, state=Non
     Foo[np.diag_indices(o)] -= mo_e_o
     Fvv[np.diag_indices(v)] -= mo_e_v
 
-    t1new = -2 * np.einsum("kc,ka,ic->ia", fov, t1, t1)
-    t1new += np.einsum("ac,ic->ia", Fvv, t1)
-    t1new += -np.einsum("ki,ka->ia", Foo, t1)
-    t1new += 2 * np.einsum("kc,kica->ia", Fov, t2)
-    t1new += -np.einsum("kc,ikca->ia", Fov, t2)
-    t1new += np.einsum("kc,ic,ka->ia", Fov, t1, t1)
+    t1new = -2 * einsum("kc,ka,ic->ia", fov, t1, t1)
+    t1new += einsum("ac,ic->ia", Fvv, t1)
+    t1new += -einsum("ki,ka->ia", Foo, t1)
+    t1new += 2 * einsum("kc,kica->ia", Fov, t2)
+    t1new += -einsum("kc,ikca->ia", Fov, t2)
+    t1new += einsum("kc,ic,ka->ia", Fov, t1, t1)
     t1new += fov
-    t1new += 2 * np.einsum("kcai,kc->ia", eris.ovvo, t1)
-    t1new += -np.einsum("kiac,kc->ia", eris.oovv, t1)
+    t1new += 2 * einsum("kcai,kc->ia", eris.ovvo, t1)
+    t1new += -einsum("kiac,kc->ia", eris.oovv, t1)
     ovvv = eris.ovvv
-    t1new += 2 * np.einsum("kdac,ikcd->ia", ovvv, t2)
-    t1new += -np.einsum("kcad,ikcd->ia", ovvv, t2)
-    t1new += 2 * np.einsum("kdac,kd,ic->ia", ovvv, t1, t1)
-    t1new += -np.einsum("kcad,kd,ic->ia", ovvv, t1, t1)
+    t1new += 2 * einsum("kdac,ikcd->ia", ovvv, t2)
+    t1new += -einsum("kcad,ikcd->ia", ovvv, t2)
+    t1new += 2 * einsum("kdac,kd,ic->ia", ovvv, t1, t1)
+    t1new += -einsum("kcad,kd,ic->ia", ovvv, t1, t1)
     ovoo = eris.ovoo
-    t1new += -2 * np.einsum("lcki,klac->ia", ovoo, t2)
-    t1new += np.einsum("kcli,klac->ia", ovoo, t2)
-    t1new += -2 * np.einsum("lcki,lc,ka->ia", ovoo, t1, t1)
-    t1new += np.einsum("kcli,lc,ka->ia", ovoo, t1, t1)
+    t1new += -2 * einsum("lcki,klac->ia", ovoo, t2)
+    t1new += einsum("kcli,klac->ia", ovoo, t2)
+    t1new += -2 * einsum("lcki,lc,ka->ia", ovoo, t1, t1)
+    t1new += einsum("kcli,lc,ka->ia", ovoo, t1, t1)
 
-    tmp2 = np.einsum("kibc,ka->abic", eris.oovv, -t1)
+    tmp2 = einsum("kibc,ka->abic", eris.oovv, -t1)
     tmp2 += ovvv.transpose(1, 3, 0, 2)
-    tmp = np.einsum("abic,jc->ijab", tmp2, t1)
+    tmp = einsum("abic,jc->ijab", tmp2, t1)
     t2new = tmp + tmp.transpose(1, 0, 3, 2)
-    tmp2 = np.einsum("kcai,jc->akij", eris.ovvo, t1)
+    tmp2 = einsum("kcai,jc->akij", eris.ovvo, t1)
     tmp2 += ovoo.transpose(1, 3, 0, 2)
-    tmp = np.einsum("akij,kb->ijab", tmp2, t1)
+    tmp = einsum("akij,kb->ijab", tmp2, t1)
     t2new -= tmp + tmp.transpose(1, 0, 3, 2)
     t2new += eris.ovov.transpose(0, 2, 1, 3)
 
@@ -376,24 +417,26 @@ def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False, state=Non
     Wvvvv = cc_Wvvvv(t1, t2, eris)
 
     if "O^4V^2" in cc:
-        t2new += np.einsum("klij,ka,lb->ijab", Woooo, t1, t1)
+        t2new += einsum("klij,ka,lb->ijab", Woooo, t1, t1)
         contract(Woooo, ts, t2new, "0101")
     else:
-        tau = t2 + np.einsum("ia,jb->ijab", t1, t1)
-        t2new += np.einsum("klij,klab->ijab", Woooo, tau)
+        tau = t2 + einsum("ia,jb->ijab", t1, t1)
+        t2new += einsum("klij,klab->ijab", Woooo, tau)
 
     if settings["vvvv_t1t1"] == "reference":
-        t2new += np.einsum("abcd,ia,jb->ijab", Wvvvv, t1, t1)  # friccsd.py:222 (quirk A.1)
+        # friccsd.py:222 (quirk A.1): einsum("abcd,ia,jb->ijab") = t_ia t_jb sum_cd W_abcd
+        Wsum = Wvvvv.reshape(Wvvvv.shape[0], Wvvvv.shape[1], -1).sum(axis=2)
+        t2new += einsum("ab,ia,jb->ijab", Wsum, t1, t1)
     else:
-        t2new += np.einsum("abcd,ic,jd->ijab", Wvvvv, t1, t1)  # RCCSD tau form
+        t2new += einsum("abcd,ic,jd->ijab", Wvvvv, t1, t1)  # RCCSD tau form
     if "O^2V^4" in cc:
         contract(Wvvvv, ts, t2new, "2323")
     else:
-        t2new += np.einsum("abcd,ijcd->ijab", Wvvvv, t2)
+        t2new += einsum("abcd,ijcd->ijab", Wvvvv, t2)
 
-    tmp = np.einsum("ac,ijcb->ijab", Lvv_, t2)
+    tmp = einsum("ac,ijcb->ijab", Lvv_, t2)
     t2new += tmp + tmp.transpose(1, 0, 3, 2)
-    tmp = np.einsum("ki,kjab->ijab", Loo_, t2)
+    tmp = einsum("ki,kjab->ijab", Loo_, t2)
     t2new -= tmp + tmp.transpose(1, 0, 3, 2)
 
     if "O^3V^3" in cc:
@@ -408,12 +451,12 @@ def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False, state=Non
         contract(Wvovo, ts, tmp, "1203")
         t2new -= tmp + tmp.transpose(1, 0, 3, 2)
     else:
-        tmp = 2 * np.einsum("akic,kjcb->ijab", Wvoov, t2)
-        tmp -= np.einsum("akci,kjcb->ijab", Wvovo, t2)
+        tmp = 2 * einsum("akic,kjcb->ijab", Wvoov, t2)
+        tmp -= einsum("akci,kjcb->ijab", Wvovo, t2)
         t2new += tmp + tmp.transpose(1, 0, 3, 2)
-        tmp = np.einsum("akic,kjbc->ijab", Wvoov, t2)
+        tmp = einsum("akic,kjbc->ijab", Wvoov, t2)
         t2new -= tmp + tmp.transpose(1, 0, 3, 2)
-        tmp = np.einsum("bkci,kjac->ijab", Wvovo, t2)
+        tmp = einsum("bkci,kjac->ijab", Wvovo, t2)
         t2new -= tmp + tmp.transpose(1, 0, 3, 2)
 
     eia = mo_e_o[:, None] - mo_e_v

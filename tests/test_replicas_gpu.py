@@ -68,3 +68,34 @@ def test_batch_average_matches_cpu_serial_average(env):
     assert abs(g.std(ddof=1) - c.std(ddof=1)) <= 1e-6 * abs(c.std(ddof=1)) + 1e-12
     s = batch.summary()
     assert abs(s["mean"] - g.mean()) < 1e-15 and s["n_replicas"] == R
+
+
+def test_c4_64_replicas_on_the_c2_system(env):
+    """c4 at its size: 64 replicas of the c2 system (nocc=20, nvir=180,
+    naux=600; the c1 system cannot hold m up to 2e6), m log-uniform in
+    [2e5, 2e6], Philox seeds 0..63, mixed precision, one graph launch per
+    iteration for all.  Replicas sampled across the batch equal their runs
+    alone bit for bit; the replica mean and standard error are finite and
+    reported."""
+    torch, FRICCSD, ReplicaBatch, run_alone, SyntheticSystem, rccsd, log_uniform_m = env
+    from fricc.systems import DeviceERIs
+
+    sysm = SyntheticSystem.from_config("c2")
+    eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
+    n = sysm.nocc ** 2 * sysm.nvir ** 2
+    R, K = 64, 3
+    m_list = log_uniform_m(R, 2e5, 2e6, seed=1, cap=n)
+    assert min(m_list) >= 200_000 and max(m_list) <= 2_000_000
+    settings = {"compression": "fri", "sampling_method": "systematic", "precision": "mixed"}
+    batch = ReplicaBatch(eris, settings, m_list, list(range(R)), level_shift=2.0)
+    print("c4: 64 replicas recorded", flush=True)
+    batch.run(K)
+    for r in (0, 29, 63):
+        en, t2 = run_alone(eris, settings, m_list[r], r, K, level_shift=2.0)
+        assert [batch.energies[i][r] for i in range(K)] == en, r
+        assert torch.equal(batch.t2[r], t2), r
+    s = batch.summary()
+    print(f"c4 replica E_corr after {K} iterations: {s['mean']:.10f} +- {s['stderr']:.3g} (n={s['n_replicas']})")
+    assert s["n_replicas"] == R and np.isfinite(s["mean"]) and 0 < s["stderr"] < 1.0
+    del batch, eris
+    torch.cuda.empty_cache()

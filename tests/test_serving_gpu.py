@@ -131,3 +131,26 @@ def test_one_solves_work_at_a_time(serving):
     ids = [rid for _, _, rid in iv]
     runs = 1 + sum(a != b for a, b in zip(ids, ids[1:]))
     assert runs > len(set(ids))
+
+
+def test_c5_sizes_alone_equal_in_the_mix(serving):
+    """The four c5 sizes (10x90, 20x180, 32x256, 48x352; m/|T2| in [1%, 10%],
+    mixed precision, 240 GB budget) served together, a few iterations each:
+    every solve's energies are the same as when it runs alone."""
+    import torch
+
+    sv = serving
+    sizes = [(10, 90), (20, 180), (32, 256), (48, 352)]
+    fr = [0.01, 0.10, 0.04, 0.02]
+    reqs = [sv.Request(i, 0.02 * i, o, v, m=int(f * o * o * v * v), seed=500 + i, sys_seed=1, max_cycle=3, tol=0.0)
+            for i, ((o, v), f) in enumerate(zip(sizes, fr))]
+    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=8)
+    res = server.run(reqs)
+    assert [r.iterations for r in res] == [3] * 4 and not any(r.error for r in res)
+    torch.cuda.empty_cache()
+    for r, q in zip(res, reqs):
+        print(f"c5 {q.nocc}x{q.nvir}: in the mix {r.energies}", flush=True)
+        assert r.energies == sv.solve_alone(q, precision="mixed"), (q.nocc, q.nvir)
+        torch.cuda.empty_cache()
+    st = sv.latency_stats(res, server.makespan)
+    print(f"c5 sizes: makespan {server.makespan:.2f} s, p50 {st['p50_s']:.2f} s, p99 {st['p99_s']:.2f} s")
