@@ -176,6 +176,16 @@ int fricc_compress(const double* x, int64_t n, int64_t m, int compression, doubl
   return st;
 }
 
+int fricc_compress_noalloc(const double* x, int64_t n, int64_t m, int compression, double u, uint32_t* out_idx,
+                           double* out_val, fricc_compress_info* info, void* stream) {
+  CompressInfo ci;
+  int st = compress_noalloc(x, n, m, compression, u, out_idx, out_val, &ci, (hipStream_t)stream);
+  if (info) memcpy(info, &ci, sizeof(ci));
+  return st;
+}
+
+int64_t fricc_compress_noalloc_max_n(void) { return compress_noalloc_max_n(); }
+
 int fricc_sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
                             int64_t* n_notfound, void* stream) {
   return sample_systematic(p, n, n_sample, u, out_idx, n_notfound, (hipStream_t)stream);

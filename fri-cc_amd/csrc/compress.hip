@@ -29,6 +29,8 @@
 #include "common.h"
 #include "fricc_internal.h"
 
+#include <mutex>
+
 namespace fricc {
 
 constexpr int NT = 256;
@@ -650,8 +652,6 @@ __global__ void k_export_scalars(const CompScalars* __restrict__ sc, CompScalars
 }
 
 // ---------------------------------------------------------------- driver
-enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
-       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11, WS_SC2 = 12 };
 
 size_t comp_scalars_size() { return sizeof(CompScalars); }
 
@@ -686,22 +686,82 @@ int comp_scalars_status(const void* host_sc, CompressInfo* info) {
   return OK;
 }
 
-// The whole compression on stream s with no host round trip: every size the
-// host does not know (candidate count, |D|, samples) stays on the device and
-// the kernels read it there; buffers are sized for the worst case.  The
-// call's scalars end in the workspace (*sc_out) and, when export_sc is given,
-// in that device buffer.  u comes from u_dev when given, else u_val.
-int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
-                    uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s, void** sc_out) {
-  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "compress: vector length must be in [1, 2^31)");
-  FRICC_CHECK(m > 0 && m <= n, "compress: need 0 < m <= n (fricc/friccsd.py:68-72)");
-  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC, "compress: unknown compression method");
-  FRICC_CHECK(u_dev || (u_val >= 0.0 && u_val <= 1.0), "compress: uniform must lie in [0, 1]");
-  void *p_sc, *p_hist, *p_tcnt, *p_k0, *p_v0, *p_k1, *p_v1, *p_sh, *p_tot, *p_ts, *p_ths;
+// Scratch of one compression of n elements.
+struct CompScratch {
+  CompScalars* sc;
+  uint32_t* hist;   // 4096 top-bit buckets, then 256 digit totals
+  uint32_t* tcnt;   // nt + 1 per-tile candidate counts / offsets
+  uint32_t* ssum;   // tile sums of the u32 scan of tcnt
+  u128* tot;        // total of the probability prefix
+  uint64_t* kA;     // candidate keys and indices, two buffers each (radix ping-pong)
+  uint32_t* vA;
+  uint64_t* kB;
+  uint32_t* vB;
+  uint32_t* bh;     // per-tile digit histograms [256][nt]
+  u128* ts;         // nt + 1 tile sums
+  u128* ths;        // nt * NT per-thread sums
+};
+
+// Byte offsets of the parts of CompScratch inside one buffer (256-byte
+// aligned parts), and the total.
+struct CompLayout {
+  size_t off[12];
+  size_t total;
+};
+
+static CompLayout comp_layout(int64_t n) {
   const int64_t nt = ceil_div(n, TILE);
+  const size_t sz[12] = {sizeof(CompScalars),
+                         (size_t)(4096 + 256) * 4,
+                         (size_t)(nt + 1) * 4,
+                         (size_t)scan_u32_sums_words(nt + 1) * 4,
+                         64,
+                         (size_t)n * 8,
+                         (size_t)n * 4,
+                         (size_t)n * 8,
+                         (size_t)n * 4,
+                         (size_t)nt * 256 * 4,
+                         (size_t)(nt + 1) * sizeof(u128),
+                         (size_t)nt * NT * sizeof(u128)};
+  CompLayout L;
+  size_t o = 0;
+  for (int i = 0; i < 12; i++) {
+    L.off[i] = o;
+    o += (sz[i] + 255) / 256 * 256;
+  }
+  L.total = o;
+  return L;
+}
+
+static CompScratch comp_scratch_at(uint8_t* base, const CompLayout& L) {
+  CompScratch w;
+  w.sc = (CompScalars*)(base + L.off[0]);
+  w.hist = (uint32_t*)(base + L.off[1]);
+  w.tcnt = (uint32_t*)(base + L.off[2]);
+  w.ssum = (uint32_t*)(base + L.off[3]);
+  w.tot = (u128*)(base + L.off[4]);
+  w.kA = (uint64_t*)(base + L.off[5]);
+  w.vA = (uint32_t*)(base + L.off[6]);
+  w.kB = (uint64_t*)(base + L.off[7]);
+  w.vB = (uint32_t*)(base + L.off[8]);
+  w.bh = (uint32_t*)(base + L.off[9]);
+  w.ts = (u128*)(base + L.off[10]);
+  w.ths = (u128*)(base + L.off[11]);
+  return w;
+}
+
+// Workspace slots of a compression (the resample reads the first sample's
+// scalars and sums back from the same slots).
+enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
+       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11, WS_SC2 = 12, WS_SSUM = 13 };
+
+static int comp_scratch_ws(int64_t n, CompScratch* w) {
+  const int64_t nt = ceil_div(n, TILE);
+  void *p_sc, *p_hist, *p_tcnt, *p_ssum, *p_k0, *p_v0, *p_k1, *p_v1, *p_sh, *p_tot, *p_ts, *p_ths;
   FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
   FRICC_TRY(ws_get(WS_HIST, (4096 + 256) * 4, &p_hist));
   FRICC_TRY(ws_get(WS_TCNT, (size_t)(nt + 1) * 4, &p_tcnt));
+  FRICC_TRY(ws_get(WS_SSUM, (size_t)scan_u32_sums_words(nt + 1) * 4, &p_ssum));
   FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
   FRICC_TRY(ws_get(WS_K0, (size_t)n * 8, &p_k0));
   FRICC_TRY(ws_get(WS_V0, (size_t)n * 4, &p_v0));
@@ -710,24 +770,55 @@ int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_
   FRICC_TRY(ws_get(WS_SH, (size_t)nt * 256 * 4, &p_sh));
   FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
   FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
-  CompScalars* sc = (CompScalars*)p_sc;
-  uint32_t* hist = (uint32_t*)p_hist;
+  w->sc = (CompScalars*)p_sc;
+  w->hist = (uint32_t*)p_hist;
+  w->tcnt = (uint32_t*)p_tcnt;
+  w->ssum = (uint32_t*)p_ssum;
+  w->tot = (u128*)p_tot;
+  w->kA = (uint64_t*)p_k0;
+  w->vA = (uint32_t*)p_v0;
+  w->kB = (uint64_t*)p_k1;
+  w->vB = (uint32_t*)p_v1;
+  w->bh = (uint32_t*)p_sh;
+  w->ts = (u128*)p_ts;
+  w->ths = (u128*)p_ths;
+  return OK;
+}
+
+static int check_compress_args(int64_t n, int64_t m, int method, double u_val, const double* u_dev) {
+  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "compress: vector length must be in [1, 2^31)");
+  FRICC_CHECK(m > 0 && m <= n, "compress: need 0 < m <= n (fricc/friccsd.py:68-72)");
+  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC, "compress: unknown compression method");
+  FRICC_CHECK(u_dev || (u_val >= 0.0 && u_val <= 1.0), "compress: uniform must lie in [0, 1]");
+  return OK;
+}
+
+// The whole compression on stream s with no host round trip, in the given
+// scratch: every size the host does not know (candidate count, |D|, samples)
+// stays on the device and the kernels read it there; buffers are sized for
+// the worst case.  The call's scalars end in w.sc and, when export_sc is
+// given, in that device buffer.  u comes from u_dev when given, else u_val.
+static int compress_pipeline(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
+                             uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s,
+                             const CompScratch& w) {
+  const int64_t nt = ceil_div(n, TILE);
+  CompScalars* sc = w.sc;
+  uint32_t* hist = w.hist;
   uint32_t* dtot = hist + 4096;
-  uint32_t* tcnt = (uint32_t*)p_tcnt;
-  uint64_t* kA = (uint64_t*)p_k0;
-  uint32_t* vA = (uint32_t*)p_v0;
-  uint64_t* kB = (uint64_t*)p_k1;
-  uint32_t* vB = (uint32_t*)p_v1;
-  uint32_t* bh = (uint32_t*)p_sh;
-  u128* ts = (u128*)p_ts;
-  *sc_out = sc;
+  uint32_t* tcnt = w.tcnt;
+  uint64_t* kA = w.kA;
+  uint32_t* vA = w.vA;
+  uint64_t* kB = w.kB;
+  uint32_t* vB = w.vB;
+  uint32_t* bh = w.bh;
+  u128* ts = w.ts;
 
   k_init_scalars<<<1, NT, 0, s>>>(sc, m, 0, u_val, u_dev, hist, 4096);
   const int gA = (int)std::min<int64_t>(2048, ceil_div(n, NT * 8));
   k_hist_top<<<gA, NT, 0, s>>>(x, n, hist);
   k_select_top<<<1, NT, 0, s>>>(hist, m, sc);
   k_norm_count<<<nt, NT, 0, s>>>(x, n, sc, tcnt);
-  FRICC_TRY(scan_u32(tcnt, tcnt, nt, nullptr, s));
+  FRICC_TRY(scan_u32(tcnt, tcnt, nt, nullptr, s, w.ssum));
   k_compact<<<nt, NT, 0, s>>>(x, n, sc, tcnt, kA, vA);
   // --- sort candidates (stable LSD, 8 byte digits)
   const int gS = (int)std::min<int64_t>(nt, 2048);
@@ -751,15 +842,24 @@ int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_
     k_dfinal<<<1, NT, 0, s>>>(kA, vA, m, sc, ts);
     k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, -1, out_idx, out_val);
     // --- systematic sampling of the remainder
-    k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (u128*)p_ths);
-    k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-    k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, (const u128*)p_ths, out_idx, out_val);
-    k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
+    k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, w.ths);
+    k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, w.tot);
+    k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, w.ths, out_idx, out_val);
+    k_samp_final<<<1, 1, 0, s>>>(sc, w.tot);
     k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
   }
   if (export_sc) k_export_scalars<<<1, 64, 0, s>>>(sc, (CompScalars*)export_sc);
   FRICC_HIP(hipGetLastError());
   return OK;
+}
+
+int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
+                    uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s, void** sc_out) {
+  FRICC_TRY(check_compress_args(n, m, method, u_val, u_dev));
+  CompScratch w;
+  FRICC_TRY(comp_scratch_ws(n, &w));
+  *sc_out = w.sc;
+  return compress_pipeline(x, n, m, method, u_val, u_dev, out_idx, out_val, export_sc, s, w);
 }
 
 // Scalars of a second sample of the same compression: everything but the
@@ -829,6 +929,64 @@ int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32
   FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
   FRICC_HIP(hipStreamSynchronize(s));
   if (dup) *dup = h.dup != 0;
+  return comp_scalars_status(&h, info);
+}
+
+// ---------------------------------------------------------------- allocation-free entry
+// fricc_compress without any allocation, for callers that cannot allocate
+// (graph capture, tight serving loops): the whole scratch of the pipeline
+// lives in this file-scope device array, of fixed size for vectors of up to
+// NOALLOC_MAX_N elements.  Each GPU the library runs on holds its own copy of
+// the array until the process ends; its address on the current device is
+// looked up once (hipGetSymbolAddress) and kept per device -- in host code the
+// array's name is not a device address.  Calls on one GPU share the array, so
+// calls that overlap on one GPU race; every call initialises what it reads
+// (scalars and histograms by kernels), so nothing carries over between calls.
+constexpr int64_t NOALLOC_MAX_N = (int64_t)1 << 22;
+// comp_layout(NOALLOC_MAX_N).total is 25.25 bytes per element plus a few
+// hundred KB of small parts; the array has 26 per element plus 1 MB
+constexpr size_t NOALLOC_BYTES = (size_t)26 * NOALLOC_MAX_N + ((size_t)1 << 20);
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAXDEV = 16;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAXDEV] = {nullptr};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device, with that device current).
+static int scratch_address(void** out) {
+  int dev = 0;
+  FRICC_HIP(hipGetDevice(&dev));
+  FRICC_CHECK(dev >= 0 && dev < NOALLOC_MAXDEV, "compress_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FRICC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FRICC_CHECK(p != nullptr, "compress_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return OK;
+}
+
+int64_t compress_noalloc_max_n() { return NOALLOC_MAX_N; }
+
+int compress_noalloc(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
+                     double* out_val, CompressInfo* info, hipStream_t s) {
+  memset(info, 0, sizeof(*info));
+  FRICC_TRY(check_compress_args(n, m, method, u, nullptr));
+  FRICC_CHECK(n <= NOALLOC_MAX_N,
+              "compress_noalloc: vector longer than the fixed scratch allows (at most 4194304 elements; "
+              "use fricc_compress)");
+  const CompLayout L = comp_layout(n);
+  FRICC_CHECK(L.total <= NOALLOC_BYTES, "compress_noalloc: scratch layout exceeds the fixed array");
+  void* base = nullptr;
+  FRICC_TRY(scratch_address(&base));
+  const CompScratch w = comp_scratch_at((uint8_t*)base, L);
+  FRICC_TRY(compress_pipeline(x, n, m, method, u, nullptr, out_idx, out_val, nullptr, s, w));
+  CompScalars h;
+  FRICC_HIP(hipMemcpyAsync(&h, w.sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
   return comp_scalars_status(&h, info);
 }
 

@@ -29,6 +29,11 @@ int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_
 // just compressed into idx/val (same x, n, m, method, stream and arena).
 int compress_resample(const double* x, int64_t n, int64_t m, int method, const double* u_dev, const uint32_t* idx,
                       const double* val, uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s);
+// The same as compress(), with all scratch in a fixed device array (no
+// allocation); n <= compress_noalloc_max_n().
+int compress_noalloc(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,
+                     double* out_val, CompressInfo* info, hipStream_t s);
+int64_t compress_noalloc_max_n();
 size_t comp_scalars_size();
 // device address of the "output may repeat an index" word of a call's scalars
 const uint32_t* comp_scalars_dup_flag(const void* dev_sc);

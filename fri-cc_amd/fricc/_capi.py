@@ -24,6 +24,7 @@ EXPORTS = (
     "fricc_set_bf16_view_width", "fricc_stream_copy", "fricc_sparse_compress_async", "fricc_sparse_status",
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
     "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
+    "fricc_compress_noalloc", "fricc_compress_noalloc_max_n",
 )
 
 
@@ -74,6 +75,10 @@ def lib():
     L.fricc_last_error.restype = ctypes.c_char_p
     L.fricc_compress.argtypes = [_vp, _i64, _i64, ctypes.c_int, ctypes.c_double, _vp, _vp,
                                  ctypes.POINTER(CompressInfo), _vp]
+    L.fricc_compress_noalloc.argtypes = [_vp, _i64, _i64, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                                         ctypes.POINTER(CompressInfo), _vp]
+    L.fricc_compress_noalloc_max_n.restype = _i64
+    L.fricc_compress_noalloc_max_n.argtypes = []
     L.fricc_sample_systematic.argtypes = [_vp, _i64, _i64, ctypes.c_double, _vp, ctypes.POINTER(_i64), _vp]
     L.fricc_sparse_create.argtypes = [I64x4, ctypes.POINTER(_vp)]
     L.fricc_sparse_destroy.argtypes = [_vp]

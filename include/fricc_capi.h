@@ -65,6 +65,20 @@ const char* fricc_last_error(void);
 int fricc_compress(const double* x, int64_t n, int64_t m, int compression, double u, uint32_t* out_idx,
                    double* out_val, fricc_compress_info* info, void* stream);
 
+/* fricc_compress with no allocation of any kind, for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments, results and errors as fricc_compress; the scratch is a fixed
+ * device array of the library (about 110 MB per GPU, held from the first
+ * launch of any library kernel on that GPU until the process ends), so n is
+ * limited to fricc_compress_noalloc_max_n() = 4194304 elements (FRICC_EINVAL
+ * beyond).  All calls on one GPU share that array: calls that overlap on one
+ * GPU (different streams or threads) race and must be serialised by the
+ * caller.  It waits for the stream at the end, like fricc_compress. */
+int fricc_compress_noalloc(const double* x, int64_t n, int64_t m, int compression, double u, uint32_t* out_idx,
+                           double* out_val, fricc_compress_info* info, void* stream);
+int64_t fricc_compress_noalloc_max_n(void);
+
 /* sample_systematic(n_sample, probs) (src/fri_utils.cpp:129-159) on device
  * probabilities p[n] (each in [0,1]): S_k = j with P_j <= fl(k+u) < P_{j+1},
  * P the canonical (exact, once-rounded) interval ends; k past the last
