@@ -9,9 +9,19 @@ fricc.systems), MP2 guess, systematic FRI with m = 4e6 uniforms from
 srand(1)/rand(), level shift 2.0 (plain Jacobi diverges on this system), all
 state resident in HBM before the timed region.
 
-The headline value is the precision mode ("mixed": FP32 sparse contractions
-and dense GEMMs; FP64 compression, energy, denominators); the FP64 plain form
-is timed beside it.  Prints one JSON line.
+The headline value is the "mixed" precision mode (every sparse contraction
+on BF16 MFMA with hi/lo split operands -- three passes, FP32 accumulate --
+the dense products in FP32; compression, energy and denominators in FP64),
+over c3's 40 iterations: W warmup iterations, then K timed ones (bracketed by
+synchronisation), then the rest of the 40, every iteration's latency
+reported.  Timed beside it: c3's mixture as written ("bf16-ladder": the
+ladder on single-pass BF16, everything else FP32), FP32 and the FP64 plain
+form, each with the per-term residual error against the FP64 step on the
+same sampled T2 (the single-pass ladder misses the |dE| <= 1e-6 Ha target;
+the headline mode meets it).  Prints one JSON line.
+
+``--summarize-profile DIR`` (no GPU) turns a rocprofv3 --kernel-trace
+--stats output directory into the markdown tables kept under profiles/.
 """
 import argparse
 import json
@@ -24,6 +34,9 @@ sys.path.insert(0, os.path.join(ROOT, "fri-cc_amd"))
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 HBM_GBS = 8000.0
+# precision mode -> (sparse contractions, ladder); mirrors fricc.friccsd.PRECISION_MODES
+MODE_PREC = {"fp64": ("fp64", "fp64"), "fp32": ("fp32", "fp32"), "bf16-ladder": ("fp32", "bf16x1"),
+             "mixed": ("bf16", "bf16")}
 
 
 def _pct(a, q):
@@ -32,73 +45,137 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
 
-def run_mode(eris, cfg, mode, steps, warmup, seed, solver=None):
+def run_mode(eris, cfg, mode, steps, warmup, seed, solver=None, total=None):
+    """``total`` (>= warmup + steps) iterations from the MP2 guess: warmup
+    iterations, the timed window of ``steps`` (bracketed by synchronisation:
+    ``it_per_s``), then the rest; every iteration's host-clock latency (each
+    includes the step's one read of energy and norm) and energy recorded, and
+    per-section HIP-event times averaged over the window."""
     import torch
 
     from fricc.friccsd import FRICCSD, SectionTimer
     from fricc.uniforms import GlibcUniforms
 
+    total = max(total or 0, warmup + steps)
     settings = {"m_keep": cfg["m_keep"], "compression": "fri", "sampling_method": "systematic",
                 "precision": mode}
     cc = (solver or FRICCSD)(eris, settings, uniforms=GlibcUniforms(seed))
     cc.level_shift = 2.0
     t1, t2 = cc.get_init_guess(eris)
-    e = cc.energy(t1, t2, eris)
-
-    def step(t1, t2, e):
-        # as the kernel loop: the energy and the amplitude-change norm come
-        # from the fused finalize pass of update_amps (read back once)
-        t1n, t2n = cc.update_amps(t1, t2, eris)
-        _, _, en, normt = cc._last_step
-        return t1n, t2n, en, normt
-
-    for _ in range(warmup):
-        t1, t2, e, _ = step(t1, t2, e)
-    torch.cuda.synchronize()
-    cc.timer = SectionTimer()
-    lat = []
-    energies = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    lat, energies, window = [], [], None
+    for it in range(total):
+        if it == warmup:
+            torch.cuda.synchronize()
+            cc.timer = SectionTimer()
+            t_win = time.perf_counter()
         ts = time.perf_counter()
-        t1, t2, e, normt = step(t1, t2, e)  # energy read back each cycle, as the loop does
-        lat.append(time.perf_counter() - ts)
-        energies.append(e)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    sec = cc.timer.ms()
+        cc._iter = it
+        t1, t2 = cc.update_amps(t1, t2, eris)  # energy and norm read back each cycle, as the loop does
+        lat.append(1e3 * (time.perf_counter() - ts))
+        energies.append(cc._last_step[2])
+        if it == warmup + steps - 1:
+            torch.cuda.synchronize()
+            window = time.perf_counter() - t_win
+            sec = cc.timer.ms()
+            cc.timer = None
     info = cc.sparse_info[0]
-    cc.final_amplitudes = (t1, t2)
-    return {"state": (t1, t2), "slots": cc._slots, "it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "p50_ms": 1e3 * _pct(lat, 50),
-            "p99_ms": 1e3 * _pct(lat, 99), "sections_ms": {k: sum(v) / len(v) for k, v in sec.items() if v},
-            "energies": energies, "n_D": info["n_D"], "n_sampled": info["n_sampled"]}
+    wl = lat[warmup:warmup + steps]
+    return {"state": (t1, t2), "slots": cc._slots, "it_per_s": steps / window, "ms_per_step": 1e3 * window / steps,
+            "p50_ms": _pct(wl, 50), "p99_ms": _pct(wl, 99),
+            "sections_ms": {k: sum(v) / len(v) for k, v in sec.items() if v},
+            "latency_ms": lat, "energies": energies, "n_D": info["n_D"], "n_sampled": info["n_sampled"]}
 
 
-def precision_error(eris, cfg, mode, t1, t2, seed):
+def per_term_errors(eris, cfg, mode, t1, t2):
     """One step from the same (t1, t2) with the same uniforms in FP64 and in
     ``mode``: both compress the same T2 (compression is FP64 in every mode),
-    so the difference is the arithmetic alone."""
+    so the differences are the arithmetic alone.  Each compressed contraction
+    is also evaluated into its own zeroed FP64 buffer (friccsd.TERM_PROBE);
+    reported per term: max |d| / max |ref| and ||d||_F / ||ref||_F; then the
+    same for t2new, t1new and the correlation energy."""
     import torch
 
+    from fricc import friccsd
     from fricc.friccsd import FRICCSD
     from fricc.uniforms import FixedUniforms
 
     us = [0.3141592653589793, 0.2718281828459045]
-    out = {}
     settings = {"m_keep": cfg["m_keep"], "compression": "fri", "sampling_method": "systematic"}
-    c64 = FRICCSD(eris, dict(settings, precision="fp64"), uniforms=FixedUniforms(us))
-    cm = FRICCSD(eris, dict(settings, precision=mode), uniforms=FixedUniforms(us))
-    c64.level_shift = cm.level_shift = 2.0
-    a1, a2 = c64.update_amps(t1, t2, eris)
-    b1, b2 = cm.update_amps(t1, t2, eris)
-    out["t2_max_rel"] = (b2 - a2).abs().max().item() / a2.abs().max().item()
-    out["t1_max_rel"] = (b1 - a1).abs().max().item() / max(a1.abs().max().item(), 1e-300)
-    out["dE_corr"] = abs(c64.energy(a1, a2, eris) - cm.energy(b1, b2, eris))
-    out["E_corr_fp64"] = c64.energy(a1, a2, eris)
-    del a1, a2, b1, b2
+    probes, outs = {}, {}
+    for m in ("fp64", mode):
+        cc = FRICCSD(eris, dict(settings, precision=m), uniforms=FixedUniforms(us))
+        cc.level_shift = 2.0
+        friccsd.TERM_PROBE = probes.setdefault(m, {})
+        try:
+            a1, a2 = cc.update_amps(t1, t2, eris)
+        finally:
+            friccsd.TERM_PROBE = None
+        outs[m] = (a1, a2, cc.energy(a1, a2, eris), cc._slots[0].indices())
+    r1, r2, e_ref, i_ref = outs["fp64"]
+    b1, b2, e_m, i_m = outs[mode]
+    terms = {}
+    for spec, ref in probes["fp64"].items():
+        d = probes[mode][spec] - ref
+        terms[spec] = {"max_rel": d.abs().max().item() / max(ref.abs().max().item(), 1e-300),
+                       "fro_rel": (d.norm() / max(ref.norm().item(), 1e-300)).item()}
+    res = {"same_sampled_T2": bool((i_ref == i_m).all()), "terms": terms,
+           "t2_max_rel": (b2 - r2).abs().max().item() / r2.abs().max().item(),
+           "t2_fro_rel": ((b2 - r2).norm() / r2.norm()).item(),
+           "t1_max_rel": (b1 - r1).abs().max().item() / max(r1.abs().max().item(), 1e-300),
+           "dE_corr": abs(e_m - e_ref), "E_corr_fp64": e_ref}
+    del probes, outs
     torch.cuda.empty_cache()
-    return out
+    return res
+
+
+def dense_work(eris, cfg, mode, t1, t2):
+    """Algorithmic flops and bytes of one step's dense products, counted per
+    product by fricc.dense (one step from (t1, t2))."""
+    from fricc import dense
+    from fricc.friccsd import FRICCSD
+    from fricc.uniforms import FixedUniforms
+
+    cc = FRICCSD(eris, {"m_keep": cfg["m_keep"], "compression": "fri", "sampling_method": "systematic",
+                        "precision": mode}, uniforms=FixedUniforms([0.5, 0.5]))
+    cc.level_shift = 2.0
+    dense.COUNTER = {}
+    try:
+        cc.update_amps(t1, t2, eris)
+        return dict(dense.COUNTER)
+    finally:
+        dense.COUNTER = None
+
+
+def iteration_roofline(o, v, m, mode, dense_rec, ms_measured):
+    """T_roof / T_measured of one iteration: T_roof sums max(F / P, B / BW)
+    over the parts, with the algorithmic work of each (compression: two
+    streaming passes of x and the COO output per sample; contractions:
+    2 m (v^2 + 2 o^2 + 8 o v) flops in the mode's precision, the ladder's
+    operand bytes v^4 s_W plus the FP64 output read and written; the dense
+    products as counted)."""
+    N = o * o * v * v
+    sp, lp = MODE_PREC[mode]
+    peak = lambda p: 1e12 * PEAK_TFLOPS[p]  # noqa: E731
+    bw = 1e9 * HBM_GBS
+    npass = lambda p: 3 if p == "bf16" else 1  # noqa: E731
+    w_bytes = {"fp64": 8, "fp32": 4, "bf16": 4, "bf16x1": 2}
+    parts = {}
+    parts["compression x2"] = 2 * (2 * 8 * N + 12 * m) / bw
+    lf = 2.0 * m * v * v * npass(lp)
+    parts["ladder 2323"] = max(lf / peak(lp.replace("x1", "")), (float(v) ** 4 * w_bytes[lp] + 2.0 * N * 8) / bw)
+    of = 16.0 * m * o * v * npass(sp)
+    parts["O3V3 (8 terms)"] = max(of / peak(sp), 8 * 3 * N * w_bytes[sp] / bw)
+    parts["O4V2 (2 terms)"] = 4.0 * m * o * o * npass(sp) / peak(sp)
+    tdense = 0.0
+    for rec in dense_rec.values():
+        tdense += max(rec["flops"] / peak("fp64" if rec["fp64"] else "fp32"), rec["bytes"] / bw)
+    parts["dense remainder"] = tdense
+    parts["finalize"] = (6 * 8 * N) / bw  # five FP64 o^2 v^2 inputs read, t2new written
+    t_roof = sum(parts.values())
+    return {"T_roof_ms": 1e3 * t_roof, "T_measured_ms": ms_measured, "frac": 1e3 * t_roof / ms_measured,
+            "parts_ms": {k: 1e3 * x for k, x in parts.items()},
+            "dense_flops": sum(r["flops"] for r in dense_rec.values()),
+            "dense_bytes": sum(r["bytes"] for r in dense_rec.values())}
 
 
 # (W shape, output shape, number of output blocks the reference loops over)
@@ -242,8 +319,8 @@ def replica_bench(args):
 
 def traffic_child(args):
     """Run under rocprofv3 --pmc by measure_traffic: one calibration copy of
-    COPY_BYTES, then the warm mixed-precision iteration whose ladder launch is
-    read."""
+    COPY_BYTES, then warm iterations of the headline mode, whose ladder
+    launch is read."""
     import ctypes
 
     import torch
@@ -261,11 +338,11 @@ def traffic_child(args):
     cfg = CONFIGS[args.config]
     sysm = SyntheticSystem.from_config(args.config)
     eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
-    run_mode(eris, cfg, "mixed", 1, 2, args.seed)
+    run_mode(eris, cfg, args.headline, 1, 2, args.seed)
     torch.cuda.synchronize()
 
 
-def measure_traffic(args):
+def measure_traffic(args, head):
     """HBM bytes of one ladder launch from two rocprofv3 --pmc runs of
     traffic_child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
     by what the same counter reads for the COPY_BYTES copy (which moves
@@ -285,7 +362,8 @@ def measure_traffic(args):
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = os.path.join(base, counter)
         cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
-               os.path.abspath(__file__), "--traffic-child", "--config", args.config, "--seed", str(args.seed)]
+               os.path.abspath(__file__), "--traffic-child", "--config", args.config, "--seed", str(args.seed),
+               "--headline", head]
         try:
             subprocess.run(cmd, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env,
                            cwd="/tmp", check=False)
@@ -329,7 +407,7 @@ def serve_bench(args):
     # up to 8 solves admitted, their iterations interleaved (shortest remaining
     # work first by default), one solve's work at a time on the device
     server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=args.serve_active, progress=progress,
-                       policy=args.serve_policy)
+                       policy=args.serve_policy, overlap=args.serve_overlap)
     torch.cuda.empty_cache()
     at_start = {"allocated": torch.cuda.memory_allocated(), "device_free": torch.cuda.mem_get_info()[0]}
     print(f"# serving: memory at start {at_start}", file=sys.stderr, flush=True)
@@ -337,7 +415,8 @@ def serve_bench(args):
     st = sv.latency_stats(res, server.makespan)
     st.update({"sizes": [f"{o}x{v}" for o, v in sizes], "rate_per_s": args.serve_rate,
                "max_active": args.serve_active, "policy": args.serve_policy,
-               "device_order": "one solve at a time (interleaved iterations)",
+               "device_order": ("side by side (overlap)" if args.serve_overlap
+                                else "one solve at a time (interleaved iterations)"),
                "max_concurrent": server.max_concurrent, "hbm_budget_gb": 240,
                "footprint_gb": {f"{o}x{v}": round(b / 1e9, 2) for (o, v), b in server.measured.items()},
                "memory_at_start": at_start, "admission_log": server.log[:64],
@@ -349,16 +428,55 @@ def serve_bench(args):
     return st
 
 
+def cpu_model():
+    """The host CPU's model name and logical core count (lscpu / os)."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": name, "logical_cpus": os.cpu_count()}
+
+
+def summarize_profile(path):
+    """Markdown tables from a rocprofv3 --kernel-trace --stats output
+    directory (its *kernel_stats.csv): per kernel the calls, total and average
+    duration, sorted by total time."""
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(path, "**", "*kernel_stats.csv"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *kernel_stats.csv under {path}")
+    rows = list(csv.DictReader(open(files[0])))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        name = r["Name"]
+        name = name if len(name) <= 90 else name[:87] + "..."
+        out.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                   f"{float(r['AverageNs']) / 1e3:.1f} | {100 * float(r['TotalDurationNs']) / tot:.1f} |")
+    out.append(f"\nTotal kernel time {tot / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches "
+               f"({os.path.basename(files[0])}).")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--iterations", type=int, default=40, help="iterations of the headline trajectory (c3: 40)")
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--modes", default="mixed,fp32,fp64")
+    ap.add_argument("--modes", default="mixed,bf16-ladder,fp32,fp64")
+    ap.add_argument("--headline", default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
+    ap.add_argument("--no-errors", action="store_true", help="skip the per-term precision errors")
     ap.add_argument("--replicas", type=int, default=0, help="also run R batched replicas of the o=20 v=180 system")
     ap.add_argument("--replica-iters", type=int, default=3)
     ap.add_argument("--replica-precision", default="mixed")
@@ -368,10 +486,15 @@ def main():
     ap.add_argument("--serve-policy", default="srpt", choices=("srpt", "rr"),
                     help="next iteration: shortest remaining work first, or round robin")
     ap.add_argument("--serve-sizes", default="10x90,20x180,32x256")
+    ap.add_argument("--serve-overlap", action="store_true", help="solves' iterations side by side on the device")
     ap.add_argument("--lccsd-modes", default="mixed,fp64", help="FRI-LCCSD iterations timed beside ('' for none)")
     ap.add_argument("--lccsd-steps", type=int, default=3)
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--summarize-profile", default=None, help="markdown summary of a rocprofv3 output dir; no GPU")
     args = ap.parse_args()
+    if args.summarize_profile:
+        print(summarize_profile(args.summarize_profile))
+        return
     if args.traffic_child:
         traffic_child(args)
         return
@@ -388,15 +511,20 @@ def main():
     cfg = CONFIGS[args.config]
     o, v = cfg["nocc"], cfg["nvir"]
     m = cfg["m_keep"]
+    N = o * o * v * v
     t0 = time.perf_counter()
     sysm = SyntheticSystem.from_config(args.config)
     eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
+    modes = args.modes.split(",")
+    head = args.headline if args.headline in modes else modes[0]
     res = {}
-    for mode in args.modes.split(","):
-        res[mode] = run_mode(eris, cfg, mode, args.steps, args.warmup, args.seed)
-        print(f"# {mode}: {json.dumps({k: v for k, v in res[mode].items() if k not in ('energies', 'state', 'slots')})}",
+    for mode in [head] + [x for x in modes if x != head]:
+        res[mode] = run_mode(eris, cfg, mode, args.steps, args.warmup, args.seed,
+                             total=args.iterations if mode == head else None)
+        print(f"# {mode}: " + json.dumps({k: v_ for k, v_ in res[mode].items()
+                                          if k not in ("energies", "state", "slots", "latency_ms")}),
               file=sys.stderr, flush=True)
     lccsd = {}
     if args.lccsd_modes:
@@ -408,23 +536,26 @@ def main():
                            "E_corr_last": rl["energies"][-1]}
             del rl
             print(f"# FRI-LCCSD {mode}: {lccsd[mode]}", file=sys.stderr, flush=True)
+    r = res[head]
+    t1s, t2s = r["state"]
+    # per-term errors of every reduced mode against FP64 on the same sampled T2
     errs = {}
-    t1s = t2s = None
-    if "fp64" in res:
-        t1s, t2s = res[list(res)[0]]["state"]
+    if not args.no_errors:
         for mode in res:
             if mode != "fp64":
-                errs[mode] = precision_error(eris, cfg, mode, t1s, t2s, args.seed)
-                print(f"# error {mode}: {errs[mode]}", file=sys.stderr, flush=True)
-    head = "mixed" if "mixed" in res else list(res)[0]
-    r = res[head]
-    N = o * o * v * v
+                errs[mode] = per_term_errors(eris, cfg, mode, t1s, t2s)
+                print(f"# error {mode}: {json.dumps(errs[mode])}", file=sys.stderr, flush=True)
+    # whole-iteration roofline of every mode (dense work counted in one step)
+    whole = {}
+    for mode in res:
+        rec = dense_work(eris, cfg, mode, t1s, t2s)
+        whole[mode] = iteration_roofline(o, v, m, mode, rec, res[mode]["ms_per_step"])
+    sp, lp = MODE_PREC[head]
     lad_ms = r["sections_ms"].get("ladder")
-    prec = {"mixed": "bf16", "fp32": "fp32", "fp64": "fp64"}[head]
-    # ladder 2323, one launch: 2 m v^2 flops; v^4 W elements + the FP64 output
-    # read and written (bf16 W: HBM-bound streaming W; FP32/FP64: MFMA-bound)
+    # ladder 2323, one launch: 2 m v^2 flops; the v^4 operand in its stored
+    # form plus the FP64 output read and written
+    w_bytes = {"fp64": 8, "fp32": 4, "bf16": 4, "bf16x1": 2}[lp]
     lad_flops = 2.0 * m * v * v
-    w_bytes = {"bf16": 2, "fp32": 4, "fp64": 8}[prec]
     lad_bytes = float(v) ** 4 * w_bytes + 2.0 * N * 8
     # achievable HBM bandwidth: the streaming copy kernel, HIP events on its stream
     import ctypes
@@ -445,19 +576,21 @@ def main():
     torch.cuda.synchronize()
     copy_gbs = 2.0 * COPY_BYTES * 10 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
     del ca, cb
-    if prec == "bf16":
+    if lp.startswith("bf16"):
         roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm_bf16)", "bound": "hbm",
                 "achieved": lad_bytes / (lad_ms * 1e-3) / 1e9 if lad_ms else None, "peak": HBM_GBS, "unit": "GB/s"}
     else:
         roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm)", "bound": "mfma",
-                "achieved": lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None, "peak": PEAK_TFLOPS[prec],
+                "achieved": lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None, "peak": PEAK_TFLOPS[lp],
                 "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
     roof["traffic"] = None
     roof.update({"launch_ms": lad_ms, "algorithmic_bytes_per_launch": lad_bytes,
                  "algorithmic_flops_per_launch": lad_flops,
                  "algorithmic_tflops": lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None,
-                 "achievable_hbm_gbs_copy": copy_gbs})
+                 "achievable_hbm_gbs_copy": copy_gbs, "timed_window": f"iterations {args.warmup + 1}.."
+                                                                     f"{args.warmup + args.steps}"})
+    lat = r["latency_ms"]
     out = {
         "metric": "FRI-CCSD iter/s at o=32,v=256,m=4e6 (fp32/bf16 vs fp64); p50/p99 solve latency",
         "value": r["it_per_s"],
@@ -469,14 +602,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "bf16/fp32", "fp32": "fp32", "fp64": "fp64"}[head],
+        "dtype": {"bf16-ladder": "bf16/fp32", "mixed": "bf16x3/fp32", "fp32": "fp32", "fp64": "fp64"}[head],
         "data": "synthetic",
         "config": {"workload": f"FRI-CCSD iteration, synthetic DF system nocc={o} nvir={v} naux={cfg['naux']} "
                                f"seed={cfg['seed']}, m_keep={m}, systematic FRI, level_shift=2.0",
                    "nocc": o, "nvir": v, "m_keep": m, "precision_mode": head},
         "iteration_latency_ms": {"p50": r["p50_ms"], "p99": r["p99_ms"]},
+        "trajectory": {"iterations": len(lat), "latency_ms": [round(x, 3) for x in lat],
+                       "mean_ms": sum(lat) / len(lat), "it_per_s": 1e3 * len(lat) / sum(lat),
+                       "E_corr": r["energies"]},
         "sections_ms": r["sections_ms"],
         "roofline": roof,
+        "iteration_roofline": whole,
         "setup_s": t_setup,
         "n_D": r["n_D"],
         "n_sampled": r["n_sampled"],
@@ -492,8 +629,8 @@ def main():
                            "workload": "FRI-LCCSD iteration on the same system, m and uniforms"}
     out["cpu_baseline"] = None
     # host copies for the CPU leg, then free the GPU for the profiling passes
-    t2_host = res[head]["state"][1].cpu().numpy().ravel()
-    ts = res[head]["slots"][0]
+    t2_host = t2s.cpu().numpy().ravel()
+    ts = r["slots"][0]
     ts_idx, ts_val = ts.indices(), ts.values()
     for k in list(res):
         res[k].pop("state", None)
@@ -506,8 +643,8 @@ def main():
     torch.cuda.empty_cache()
     print(f"# memory after the timed modes: allocated {torch.cuda.memory_allocated() / 1e9:.2f} GB, "
           f"device free {torch.cuda.mem_get_info()[0] / 1e9:.2f} GB", file=sys.stderr, flush=True)
-    if not args.no_traffic and head == "mixed":
-        tr, note = measure_traffic(args)
+    if not args.no_traffic:
+        tr, note = measure_traffic(args, head)
         print(f"# traffic: {tr} ({note})", file=sys.stderr, flush=True)
         if tr is not None:
             roof["traffic"] = tr["bytes"]
@@ -529,8 +666,10 @@ def main():
     if not args.no_cpu_baseline:
         t_cpu, thr, desc = cpu_baseline_sample(
             t2_host, ts_idx, ts_val, o, v, m, log=lambda s: print(f"# cpu: {s}", file=sys.stderr, flush=True))
+        cpu = cpu_model()
         out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "iter/s", "cores": thr, "kind": "port",
-                               "s_per_iter": t_cpu, "sample": desc}
+                               "s_per_iter": t_cpu, "cpu": cpu["model"], "logical_cpus": cpu["logical_cpus"],
+                               "sample": desc}
         out["speedup_vs_cpu"] = r["it_per_s"] * t_cpu
     print(json.dumps(out), flush=True)
 

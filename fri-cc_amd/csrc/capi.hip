@@ -115,6 +115,7 @@ static int dtype_of(int precision, int* dt) {
   if (precision == FRICC_FP64) *dt = DT_F64;
   else if (precision == FRICC_FP32) *dt = DT_F32;
   else if (precision == FRICC_BF16) *dt = DT_BF16;
+  else if (precision == FRICC_BF16X1) *dt = DT_BF16X1;
   else FRICC_CHECK(false, "unknown precision");
   return OK;
 }
@@ -122,7 +123,7 @@ static int dtype_of(int precision, int* dt) {
 static int g_npass = 3;
 
 static int view_width(const Plan& p, int dt) {
-  return dt == DT_BF16 ? bf16_view_width(p.Rh * p.Rl, p.Xh * p.Xl) : 64;
+  return (dt == DT_BF16 || dt == DT_BF16X1) ? bf16_view_width(p.Rh * p.Rl, p.Xh * p.Xl) : 64;
 }
 
 static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, void* out, const int64_t os[4],
@@ -139,12 +140,15 @@ static int run_contract(const Plan& p, const void* A, int dt, Sparse* T, void* o
   om.oyh = os[p.opos[2]];
   om.oyl = os[p.opos[3]];
   if (dt == DT_BF16) return gathered_gemm_bf16(v, (const uint16_t*)A, p.Rh * p.Rl, p.Xh * p.Xl, om, p.alpha, g_npass, s);
+  if (dt == DT_BF16X1)
+    return gathered_gemm_bf16(v, (const uint16_t*)A, p.Rh * p.Rl, p.Xh * p.Xl, om, p.alpha, 1, s, /*hilo=*/0);
   return gathered_gemm(v, A, dt, p.Xh * p.Xl, om, p.alpha, s);
 }
 
-// FP64/FP32 (or a permuted FP64 W) -> the BF16 [hi | lo] operand in workspace slot 31
+// FP64/FP32 (or a permuted FP64 W) -> the BF16 operand (hi | lo records, or
+// hi-only ones for DT_BF16X1) in workspace slot 31
 static int bf16_operand_from_w(const Plan& p, const double* W, const int64_t W_strides[4], hipStream_t s,
-                               const uint16_t** out) {
+                               const uint16_t** out, int dt) {
   const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
   void* A = nullptr;
   void* A2 = nullptr;
@@ -152,7 +156,7 @@ static int bf16_operand_from_w(const Plan& p, const double* W, const int64_t W_s
   FRICC_TRY(ws_get(31, (size_t)R * bf16_xblocks(X) * 512 * 2, &A2));
   const int64_t st[4] = {W_strides[p.wpos[0]], W_strides[p.wpos[1]], W_strides[p.wpos[2]], W_strides[p.wpos[3]]};
   FRICC_TRY(permute_operand(W, p.Rh, p.Rl, p.Xh, p.Xl, st, A, DT_F32, s));
-  FRICC_TRY(split_bf16(A, DT_F32, R, X, (uint16_t*)A2, s));
+  FRICC_TRY(split_bf16(A, DT_F32, R, X, (uint16_t*)A2, s, nullptr, 1.0, 0.0, dt == DT_BF16 ? 1 : 0));
   *out = (const uint16_t*)A2;
   return OK;
 }
@@ -362,9 +366,9 @@ int fricc_contract(const char* term, const double* W, const int64_t W_shape[4], 
   int dt;
   FRICC_TRY(dtype_of(precision, &dt));
   hipStream_t s = (hipStream_t)stream;
-  if (dt == DT_BF16) {
+  if (dt == DT_BF16 || dt == DT_BF16X1) {
     const uint16_t* A2 = nullptr;
-    FRICC_TRY(bf16_operand_from_w(p, W, W_strides, s, &A2));
+    FRICC_TRY(bf16_operand_from_w(p, W, W_strides, s, &A2, dt));
     return run_contract(p, A2, dt, T, out, out_strides, s);
   }
   const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
@@ -399,9 +403,9 @@ int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int 
   const int of32 = out_precision == FRICC_FP32;
   hipStream_t s = (hipStream_t)stream;
   if (prepared) return run_contract(p, W_or_A, dt, T, out, out_strides, s, of32);
-  if (dt == DT_BF16) {
+  if (dt == DT_BF16 || dt == DT_BF16X1) {
     const uint16_t* A2 = nullptr;
-    FRICC_TRY(bf16_operand_from_w(p, (const double*)W_or_A, W_strides, s, &A2));
+    FRICC_TRY(bf16_operand_from_w(p, (const double*)W_or_A, W_strides, s, &A2, dt));
     return run_contract(p, A2, dt, T, out, out_strides, s, of32);
   }
   const int64_t R = p.Rh * p.Rl, X = p.Xh * p.Xl;
@@ -433,6 +437,12 @@ int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, voi
   int dt;
   FRICC_TRY(dtype_of(src_precision, &dt));
   return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream);
+}
+
+int fricc_split_bf16_hi(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream) {
+  int dt;
+  FRICC_TRY(dtype_of(src_precision, &dt));
+  return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream, nullptr, 1.0, 0.0, 0);
 }
 
 int fricc_split_bf16_lincomb(const void* A, const void* B, int src_precision, double alpha, double beta, int64_t R,

@@ -12,13 +12,11 @@
 // loads one A value per 16-row x block (16 consecutive x of one gathered row:
 // 128 B segments) and one B value per 16-column block.
 #include "fricc_internal.h"
+#include "mfma.h"
 
 #include <utility>
 
 namespace fricc {
-
-typedef __attribute__((ext_vector_type(4))) double d4;
-typedef __attribute__((ext_vector_type(4))) float f4;
 
 struct OutDev {
   void* out;
@@ -54,27 +52,6 @@ __device__ inline void out_add_batch(TO* __restrict__ out, const int64_t (&off)[
   for (int n = 0; n < N; n++)
     if (ok[n]) out[off[n]] = cur[n] + (TO)(alpha * v[n]);
 }
-
-template <typename T>
-struct Acc;
-template <>
-struct Acc<double> {
-  typedef d4 type;
-  static __device__ inline d4 mfma(double a, double b, d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // D register i of lane group g holds row g + 4 i (measured on gfx950)
-  static __device__ inline int row(int g, int i) { return g + 4 * i; }
-};
-template <>
-struct Acc<float> {
-  typedef f4 type;
-  static __device__ inline f4 mfma(float a, float b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  // D register i of lane group g holds row 4 g + i
-  static __device__ inline int row(int g, int i) { return 4 * g + i; }
-};
 
 constexpr int GT_X = 256;
 constexpr int GT_Y = 64;
@@ -289,18 +266,26 @@ __device__ inline bf8 cat_bf8(v4s a, v4s b) {
 constexpr int BX = 256;
 constexpr int KG = 4;                  // 8-row groups of B per K step
 constexpr int KR = 32;                 // gathered rows per K step (one MFMA K)
-constexpr int A_ROW = 2 * BX * 2;      // staged bytes of one A row: hi | lo
-constexpr int A_PITCH = A_ROW + 32;    // LDS pitch of a staged A row
-// byte offset of staged row k: rows k and k+8 would meet the same banks at
-// this pitch, so each group of 8 rows starts 128 bytes (half the banks)
-// further on than the pitch alone would place it
-constexpr int A_GROUP = 8 * A_PITCH + 128;
-__device__ constexpr uint32_t a_row_off(int k) { return (uint32_t)((k >> 3) * A_GROUP + (k & 7) * A_PITCH); }
-constexpr int A_STAGE = (KR / 8) * A_GROUP;
-static_assert(7 * A_PITCH + A_ROW <= A_GROUP, "staged rows must not overlap");
+
+// Geometry of the staged A rows.  HILO: the operand's records carry the lo
+// plane (1 KB: hi 256 | lo 256); else only the hi plane (512 B), the operand
+// of single-pass products.
+template <bool HILO>
+struct AGeom {
+  static constexpr int REC = HILO ? 2 * BX : BX;  // bf16 per record
+  static constexpr int ROW = REC * 2;             // staged bytes of one A row
+  static constexpr int PITCH = ROW + 32;          // LDS pitch of a staged A row
+  // byte offset of staged row k: rows k and k+8 would meet the same banks at
+  // this pitch, so each group of 8 rows starts 128 bytes (half the banks)
+  // further on than the pitch alone would place it
+  static constexpr int GROUP = 8 * PITCH + 128;
+  static constexpr int STAGE = (KR / 8) * GROUP;
+  static __device__ constexpr uint32_t off(int k) { return (uint32_t)((k >> 3) * GROUP + (k & 7) * PITCH); }
+  static_assert(7 * PITCH + ROW <= GROUP, "staged rows must not overlap");
+};
 
 struct B16Args {
-  const uint16_t* A;  // [X/256][R][hi 256 | lo 256]
+  const uint16_t* A;  // [X/256][R][hi 256 | lo 256], or [X/256][R][hi 256] for hi-only operands
   int64_t R, X, Y;
   const uint32_t* tile_off;
   const uint32_t* tile_order;  // tiles by decreasing row count
@@ -339,10 +324,10 @@ __device__ inline void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-template <int BYB>
+template <int BYB, bool HILO>
 struct StageMem {
-  uint8_t A[A_STAGE];   // staged rows [k]: hi BX | lo BX (bf16), at a_row_off(k)
-  uint4 B[2][KG][BYB];  // [hi|lo][group][y]
+  uint8_t A[AGeom<HILO>::STAGE];  // staged rows [k]: hi BX (| lo BX) bf16, at AGeom::off(k)
+  uint4 B[2][KG][BYB];            // [hi|lo][group][y]
 };
 
 // Operands of the pipeline, copied out of the kernel arguments once.  The
@@ -362,16 +347,20 @@ struct StageSrc {
 // Wave w stages A rows 8w..8w+7 (one 1 KB record per DMA: lanes 0-31 the hi
 // half, 32-63 the lo half) and a quarter of the B chunks (64 columns of one
 // 8-row group and plane per DMA).
-template <int NPASS, int BYB>
-__device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t bcol, StageMem<BYB>& sm, int w, int lane) {
+template <int NPASS, int BYB, bool HILO>
+__device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t bcol, StageMem<BYB, HILO>& sm, int w,
+                                   int lane) {
   static_assert(BX == 256 && (BYB == 128 || BYB == 256) && KG == 4 && KR == 32,
                 "wave assignment assumes 4 waves, BX 256, BYB 128 or 256");
+  typedef AGeom<HILO> G;
   const uint32_t la = lds_addr(&sm.A[0]), lb = lds_addr(&sm.B[0][0][0]);
   cu32* rl = p.glist + k + 8 * w;
   const uint32_t r0 = rl[0], r1 = rl[1], r2 = rl[2], r3 = rl[3], r4 = rl[4], r5 = rl[5], r6 = rl[6], r7 = rl[7];
   const uint32_t rr[8] = {r0, r1, r2, r3, r4, r5, r6, r7};
+  // one record per DMA: 64 lanes for a hi | lo record, lanes 0-31 for a hi one
 #pragma unroll
-  for (int i = 0; i < 8; i++) dma16(p.A + (int64_t)rr[i] * (2 * BX), la + a_row_off(8 * w + i));
+  for (int i = 0; i < 8; i++)
+    if (HILO || lane < 32) dma16(p.A + (int64_t)rr[i] * G::REC, la + G::off(8 * w + i));
   constexpr int NCH = BYB / 64;          // 64-column chunks per group
   constexpr int PER = KG * NCH / 4;      // chunks per wave
   const uint32_t kg = k >> 3;
@@ -386,14 +375,17 @@ __device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t bcol, 
   }
 }
 
-template <int BYB>
-__device__ inline v4s tr_read(const StageMem<BYB>& sm, uint32_t off) {
+template <int BYB, bool HILO>
+__device__ inline v4s tr_read(const StageMem<BYB, HILO>& sm, uint32_t off) {
   return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s*)((lds_u8*)(lds_void*)&sm.A[0] + off));
 }
 
 // MFMAs of one stage: wave (wx, wy) owns x [wx*128, +128) and y [wy*BYB/2, +BYB/2)
-template <int NPASS, int BYB>
-__device__ inline void stage_compute(const StageMem<BYB>& sm, f4 (&acc)[8][BYB / 32], int wx, int wy, int lr, int lk) {
+template <int NPASS, int BYB, bool HILO>
+__device__ inline void stage_compute(const StageMem<BYB, HILO>& sm, f4 (&acc)[8][BYB / 32], int wx, int wy, int lr,
+                                     int lk) {
+  static_assert(HILO || NPASS == 1, "the lo plane of A is needed for three passes");
+  typedef AGeom<HILO> G;
   constexpr int NQ = BYB / 32;
   uint4 bh[NQ], bl[NQ];
 #pragma unroll
@@ -403,8 +395,8 @@ __device__ inline void stage_compute(const StageMem<BYB>& sm, f4 (&acc)[8][BYB /
   }
   // this lane's transposed-read address: row 8*lk + lr/4 (+4 for the second
   // read), columns 4*(lr%4).. of each 16-column block
-  const uint32_t r0 = a_row_off(8 * lk + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
-  const uint32_t r1 = a_row_off(8 * lk + 4 + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
+  const uint32_t r0 = G::off(8 * lk + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
+  const uint32_t r1 = G::off(8 * lk + 4 + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
 #pragma unroll
   for (int s = 0; s < 8; s++) {
     const uint32_t c = (uint32_t)((wx * 128 + 16 * s) * 2);
@@ -480,13 +472,13 @@ __device__ inline void bf16_epilogue(std::index_sequence<Qs...>, const B16Args& 
   (bf16_epilogue_q<(int)Qs, NQ>(a, acc, x0, y0, wx, wy, lr, lk, sp, NQ * 32), ...);
 }
 
-template <int NPASS, int BYB, int S>
+template <int NPASS, int BYB, int S, bool HILO>
 __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   constexpr int NQ = BYB / 32;
   constexpr int PER = KG * (BYB / 64) / 4;          // B chunks per wave per stage
   constexpr int LW = 8 + 2 * PER;                   // DMA instructions per wave per stage
   constexpr int LWP = NPASS > 1 ? LW : LW - PER;    // the lo planes of B are skipped with one pass
-  __shared__ StageMem<BYB> st[S];
+  __shared__ StageMem<BYB, HILO> st[S];
   static_assert(S == 2 || S == 3, "2 or 3 stages");
   int64_t bx, yb;
   int sp;
@@ -523,7 +515,7 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   const int wx = w & 1, wy = w >> 1;
   const int64_t x0 = bx * BX;
   StageSrc ps;
-  ps.A = a.A + (bx * a.R) * (2 * BX) + 8 * lane;
+  ps.A = a.A + (bx * a.R) * AGeom<HILO>::REC + 8 * lane;
   ps.Bhi = a.Bhi;
   ps.Blo = a.Blo;
   ps.glist = (cu32*)a.glist;
@@ -544,7 +536,7 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   if (k < g_end) {
 #pragma unroll
     for (int s = 0; s < S - 1; s++)
-      if (k + s * KR < g_end) stage_issue<NPASS, BYB>(ps, k + s * KR, bcol, st[s], w, lane);
+      if (k + s * KR < g_end) stage_issue<NPASS, BYB, HILO>(ps, k + s * KR, bcol, st[s], w, lane);
     int cur = 0;
     for (; k < g_end; k += KR) {
       if (S > 2 && k + KR < g_end)
@@ -555,8 +547,8 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
       __builtin_amdgcn_s_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
       const int nxt = cur == 0 ? S - 1 : cur - 1;  // (cur + S - 1) % S
-      if (k + (S - 1) * KR < g_end) stage_issue<NPASS, BYB>(ps, k + (S - 1) * KR, bcol, st[nxt], w, lane);
-      stage_compute<NPASS, BYB>(st[cur], acc, wx, wy, lr, lk);
+      if (k + (S - 1) * KR < g_end) stage_issue<NPASS, BYB, HILO>(ps, k + (S - 1) * KR, bcol, st[nxt], w, lane);
+      stage_compute<NPASS, BYB, HILO>(st[cur], acc, wx, wy, lr, lk);
       cur = cur == S - 1 ? 0 : cur + 1;
     }
   }
@@ -590,10 +582,13 @@ int bf16_view_width(int64_t R, int64_t X) {
 }
 
 int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& m, double alpha,
-                       int npass, hipStream_t s) {
-  FRICC_CHECK(v->dtype == DT_BF16 && v->TY % 128 == 0, "gathered_gemm_bf16: needs a BF16 view with tile width k*128");
+                       int npass, hipStream_t s, int hilo) {
+  FRICC_CHECK((v->dtype == DT_BF16 || v->dtype == DT_BF16X1) && v->TY % 128 == 0,
+              "gathered_gemm_bf16: needs a BF16 view with tile width k*128");
   FRICC_CHECK(X > 0 && X < ((int64_t)1 << 31), "gathered_gemm_bf16: bad X");
   FRICC_CHECK(npass == 1 || npass == 3, "gathered_gemm_bf16: npass must be 1 or 3");
+  FRICC_CHECK(hilo || npass == 1, "gathered_gemm_bf16: three passes need the lo plane of A");
+  FRICC_CHECK(npass == 1 || v->Blo != nullptr, "gathered_gemm_bf16: three passes need the lo plane of the view");
   if (v->ntiles == 0) return OK;
   B16Args a;
   a.om.out = m.out;
@@ -638,15 +633,19 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   }
   const int64_t nb = nb0 * nsplit;
   if (byb == 256) {
-    if (npass == 3)
-      k_ggemm_bf16<3, 256, 2><<<nb, 256, 0, s>>>(a);
+    if (!hilo)
+      k_ggemm_bf16<1, 256, 2, false><<<nb, 256, 0, s>>>(a);
+    else if (npass == 3)
+      k_ggemm_bf16<3, 256, 2, true><<<nb, 256, 0, s>>>(a);
     else
-      k_ggemm_bf16<1, 256, 2><<<nb, 256, 0, s>>>(a);
+      k_ggemm_bf16<1, 256, 2, true><<<nb, 256, 0, s>>>(a);
   } else {
-    if (npass == 3)
-      k_ggemm_bf16<3, 128, 3><<<nb, 256, 0, s>>>(a);
+    if (!hilo)
+      k_ggemm_bf16<1, 128, 3, false><<<nb, 256, 0, s>>>(a);
+    else if (npass == 3)
+      k_ggemm_bf16<3, 128, 3, true><<<nb, 256, 0, s>>>(a);
     else
-      k_ggemm_bf16<1, 128, 3><<<nb, 256, 0, s>>>(a);
+      k_ggemm_bf16<1, 128, 3, true><<<nb, 256, 0, s>>>(a);
   }
   FRICC_HIP(hipGetLastError());
   if (nsplit > 1) k_splitk_reduce<<<ceil_div(X * v->Y, 256), 256, 0, s>>>(a.partial, nsplit, X, v->Y, a.om, alpha);
@@ -659,10 +658,12 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
 // lo = bf16(a - hi)
 // With B: the split of alpha*A + beta*B, formed in the source precision (one
 // rounding of the sum of the two products; both exact for alpha, beta = 2, -1).
+// hilo 0: hi-only records [X/256][R][hi 256]
 template <typename T>
 __global__ void k_split_bf16(const T* __restrict__ A, const T* __restrict__ B, T alpha, T beta, int64_t R,
-                             int64_t X, int64_t n, __hip_bfloat16* __restrict__ out) {
+                             int64_t X, int64_t n, int hilo, __hip_bfloat16* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t rec_len = hilo ? 512 : 256;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
     // e = (xb * R + r) * 256 + xi
     const int64_t xi = e & 255, rec = e >> 8;
@@ -670,22 +671,22 @@ __global__ void k_split_bf16(const T* __restrict__ A, const T* __restrict__ B, T
     float v = 0.0f;
     if (x < X) v = B ? (float)(alpha * A[r * X + x] + beta * B[r * X + x]) : (float)A[r * X + x];
     const __hip_bfloat16 h = __float2bfloat16(v);
-    out[rec * 512 + xi] = h;
-    out[rec * 512 + 256 + xi] = __float2bfloat16(v - __bfloat162float(h));
+    out[rec * rec_len + xi] = h;
+    if (hilo) out[rec * rec_len + 256 + xi] = __float2bfloat16(v - __bfloat162float(h));
   }
 }
 
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s, const void* B,
-               double alpha, double beta) {
+               double alpha, double beta, int hilo) {
   const int64_t n = R * bf16_xblocks(X) * 256;
   if (n == 0) return OK;
   const int64_t grid = std::min<int64_t>(ceil_div(n, 256), 256 * 64);
   if (src_dtype == DT_F64)
-    k_split_bf16<double><<<grid, 256, 0, s>>>((const double*)A, (const double*)B, alpha, beta, R, X, n,
+    k_split_bf16<double><<<grid, 256, 0, s>>>((const double*)A, (const double*)B, alpha, beta, R, X, n, hilo,
                                               (__hip_bfloat16*)A2);
   else if (src_dtype == DT_F32)
     k_split_bf16<float><<<grid, 256, 0, s>>>((const float*)A, (const float*)B, (float)alpha, (float)beta, R, X,
-                                             n, (__hip_bfloat16*)A2);
+                                             n, hilo, (__hip_bfloat16*)A2);
   else
     FRICC_CHECK(false, "split_bf16: source must be FP64 or FP32");
   FRICC_HIP(hipGetLastError());

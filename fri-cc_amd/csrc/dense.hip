@@ -1,0 +1,490 @@
+// Dense contractions of two strided tensors on MFMA: the O(o^2 v^3) products
+// of the amplitude update (PySCF's imd.cc_Foo / cc_Fvv / cc_Fov / Loo / Lvv,
+// cc_Woooo / cc_Wvoov / cc_Wvovo and the direct t1/t2 terms,
+// fricc/friccsd.py:144-207, 309-376) written as one batched GEMM each.
+//
+//   C[b][m][n] = alpha * sum_k A[b][m][k] B[b][k][n] + beta * C[b][m][n]
+//
+// Every index (b, m, n, k) stands for up to three letters, each with its own
+// stride, so permuted views, broadcasts (stride 0) and products summed over
+// several letters need no copies.  An einsum-style planner (dense_contract)
+// groups the letters of a spec into b/m/n/k and merges adjacent letters whose
+// strides compose.
+//
+// Tiling: 256 threads (4 waves as 2 x 2), a BM x BN output tile per block,
+// K advancing 16 per step.  Each step's A (BK x BM) and B (BK x BN) tiles are
+// loaded from global memory into registers while the MFMAs of the previous
+// step run from LDS, then written to the other LDS buffer (one barrier per
+// step).  Consecutive threads load consecutive elements along whichever of
+// the tile's two indices has unit stride, so the loads coalesce.  Elements
+// are converted to the compute type on load (FP64 operands of an FP32
+// product need no converted copy).  Compute types: FP32 on
+// v_mfma_f32_16x16x4f32, FP64 on v_mfma_f64_16x16x4f64; the accumulation of
+// each output is a fixed sequence (no atomics), so results are reproducible.
+// Small output grids split K; the partial tiles are added in split order by a
+// second kernel.
+#include "fricc_internal.h"
+#include "mfma.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace fricc {
+
+// An index i of a group of up to three letters: i = (i2 * L1 + i1) * L0 + i0,
+// addressed as i2 * s2 + i1 * s1 + i0 * s0 (i < 2^31).
+struct Lvl {
+  int64_t L0, L1;
+  int64_t s0, s1, s2;
+  double r0, r1;  // 1 / L0, 1 / L1
+  int nl;         // levels in use (0..3)
+};
+
+// q = i / L, r = i % L for 0 <= i < 2^31 via the FP64 reciprocal (the product
+// is within one of the quotient; one correction step)
+__device__ inline void qr(int64_t i, int64_t L, double rcp, int64_t& q, int64_t& r) {
+  q = (int64_t)((double)i * rcp);
+  r = i - q * L;
+  if (r < 0) {
+    q--;
+    r += L;
+  } else if (r >= L) {
+    q++;
+    r -= L;
+  }
+}
+
+__device__ inline int64_t lvl_off(const Lvl& d, int64_t i) {
+  if (d.nl <= 1) return i * d.s0;
+  int64_t q, r0;
+  qr(i, d.L0, d.r0, q, r0);
+  if (d.nl == 2) return q * d.s1 + r0 * d.s0;
+  int64_t q2, r1;
+  qr(q, d.L1, d.r1, q2, r1);
+  return q2 * d.s2 + r1 * d.s1 + r0 * d.s0;
+}
+
+struct DArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  void* partial;
+  int a64, b64, c64;
+  int a_kfast, b_kfast;  // the tile load walks k fastest (else m / n fastest)
+  Lvl Ab, Am, Ak, Bb, Bk, Bn, Cb, Cm, Cn;
+  int64_t M, N, K, batch;
+  int64_t ntm, ntn;
+  int nsplit;
+  int64_t kchunk;  // K range per split, a multiple of DBK
+  double alpha, beta;
+};
+
+constexpr int DBK = 16;
+
+template <typename T>
+__device__ inline T ld_as(const void* p, int f64, int64_t off) {
+  return f64 ? (T)((const double*)p)[off] : (T)((const float*)p)[off];
+}
+
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
+  constexpr int PA = BM + (sizeof(T) == 4 ? 16 : 0);  // LDS pitches: the two k rows one fragment read
+  constexpr int PB = BN + (sizeof(T) == 4 ? 16 : 0);  // touches fall on different banks
+  constexpr int EA = BM * DBK / 256, EB = BN * DBK / 256;
+  constexpr int MB = BM / 32, NB = BN / 32;  // 16 x 16 blocks per wave
+  typedef typename Acc<T>::type AT;
+  __shared__ T sA[2][DBK][PA];
+  __shared__ T sB[2][DBK][PB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int64_t tm = (int64_t)blockIdx.x % a.ntm, tn = (int64_t)blockIdx.x / a.ntm;
+  const int64_t b = (int64_t)blockIdx.y / a.nsplit;
+  const int sp = (int)((int64_t)blockIdx.y % a.nsplit);
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = (int64_t)sp * a.kchunk, kend = min(a.K, kbeg + a.kchunk);
+  const int64_t abase = lvl_off(a.Ab, b), bbase = lvl_off(a.Bb, b);
+
+  // staging coordinates of this thread's elements
+  int amm[EA], akk[EA], bnn[EB], bkk[EB];
+  int64_t amoff[EA], bnoff[EB];
+  bool amok[EA], bnok[EB];
+#pragma unroll
+  for (int j = 0; j < EA; j++) {
+    const int e = tid + 256 * j;
+    amm[j] = a.a_kfast ? e / DBK : e % BM;
+    akk[j] = a.a_kfast ? e % DBK : e / BM;
+    const int64_t m = m0 + amm[j];
+    amok[j] = m < a.M;
+    amoff[j] = amok[j] ? abase + lvl_off(a.Am, m) : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < EB; j++) {
+    const int e = tid + 256 * j;
+    bnn[j] = a.b_kfast ? e / DBK : e % BN;
+    bkk[j] = a.b_kfast ? e % DBK : e / BN;
+    const int64_t n = n0 + bnn[j];
+    bnok[j] = n < a.N;
+    bnoff[j] = bnok[j] ? bbase + lvl_off(a.Bn, n) : 0;
+  }
+  T ra[EA], rb[EB];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < EA; j++) {
+      const int64_t k = k0 + akk[j];
+      ra[j] = (amok[j] && k < kend) ? ld_as<T>(a.A, a.a64, amoff[j] + lvl_off(a.Ak, k)) : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < EB; j++) {
+      const int64_t k = k0 + bkk[j];
+      rb[j] = (bnok[j] && k < kend) ? ld_as<T>(a.B, a.b64, bnoff[j] + lvl_off(a.Bk, k)) : (T)0;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < EA; j++) sA[buf][akk[j]][amm[j]] = ra[j];
+#pragma unroll
+    for (int j = 0; j < EB; j++) sB[buf][bkk[j]][bnn[j]] = rb[j];
+  };
+
+  AT acc[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; i++)
+#pragma unroll
+    for (int j = 0; j < NB; j++) acc[i][j] = AT{0, 0, 0, 0};
+
+  const int lr = lane & 15, lk = lane >> 4;
+  const int mw = wm * (BM / 2), nw = wn * (BN / 2);
+  if (kbeg < kend) {
+    gload(kbeg);
+    lstore(0);
+    __syncthreads();
+    int cur = 0;
+    for (int64_t k0 = kbeg; k0 < kend; k0 += DBK) {
+      const bool more = k0 + DBK < kend;
+      if (more) gload(k0 + DBK);
+#pragma unroll
+      for (int ks = 0; ks < DBK / 4; ks++) {
+        T af[MB], bf[NB];
+#pragma unroll
+        for (int i = 0; i < MB; i++) af[i] = sA[cur][4 * ks + lk][mw + 16 * i + lr];
+#pragma unroll
+        for (int j = 0; j < NB; j++) bf[j] = sB[cur][4 * ks + lk][nw + 16 * j + lr];
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+          for (int j = 0; j < NB; j++) acc[i][j] = Acc<T>::mfma(af[i], bf[j], acc[i][j]);
+      }
+      if (more) lstore(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+  // epilogue: lane holds column lr of each 16 x 16 block, rows Acc<T>::row(lk, i)
+  const int64_t cbase = lvl_off(a.Cb, b);
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const int64_t n = n0 + nw + 16 * j + lr;
+    if (n >= a.N) continue;
+    const int64_t cn = a.nsplit > 1 ? 0 : lvl_off(a.Cn, n);
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t m = m0 + mw + 16 * i + Acc<T>::row(lk, r);
+        if (m >= a.M) continue;
+        const T v = acc[i][j][r];
+        if (a.nsplit > 1) {
+          ((T*)a.partial)[(((int64_t)sp * a.batch + b) * a.M + m) * a.N + n] = v;
+        } else {
+          const int64_t off = cbase + lvl_off(a.Cm, m) + cn;
+          double o = a.alpha * (double)v;
+          if (a.beta != 0.0) o += a.beta * (a.c64 ? ((const double*)a.C)[off] : (double)((const float*)a.C)[off]);
+          if (a.c64)
+            ((double*)a.C)[off] = o;
+          else
+            ((float*)a.C)[off] = (float)o;
+        }
+      }
+  }
+}
+
+// C = alpha * sum over splits (in split order) + beta * C
+template <typename T>
+__global__ void k_dgemm_reduce(DArgs a) {
+  const int64_t total = a.batch * a.M * a.N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    T s = 0;
+    for (int q = 0; q < a.nsplit; q++) s += ((const T*)a.partial)[(int64_t)q * total + e];
+    const int64_t n = e % a.N, m = (e / a.N) % a.M, b = e / (a.N * a.M);
+    const int64_t off = lvl_off(a.Cb, b) + lvl_off(a.Cm, m) + lvl_off(a.Cn, n);
+    double o = a.alpha * (double)s;
+    if (a.beta != 0.0) o += a.beta * (a.c64 ? ((const double*)a.C)[off] : (double)((const float*)a.C)[off]);
+    if (a.c64)
+      ((double*)a.C)[off] = o;
+    else
+      ((float*)a.C)[off] = (float)o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Planner: einsum spec of two operands -> one strided batched GEMM.
+
+struct Letter {
+  char c;
+  int64_t n;
+  int64_t s[3];  // stride in A, B, C (0 where absent)
+  bool in[3];
+};
+
+// Letters of a group, outer to inner, merged where strides compose in every
+// operand that holds the group; at most three levels remain.  The order tried
+// first is that of the larger operand (its innermost level then has the unit
+// stride the tile loads walk), then the others.
+static int group_levels(const std::vector<Letter>& g, const std::vector<int>& ops, std::vector<Letter>* out,
+                        const char* what) {
+  std::vector<Letter> v;
+  for (auto& l : g)
+    if (l.n > 1) v.push_back(l);
+  std::vector<Letter> best;
+  bool have = false;
+  for (int op : ops) {
+    std::vector<Letter> o = v;
+    std::stable_sort(o.begin(), o.end(), [&](const Letter& x, const Letter& y) { return x.s[op] > y.s[op]; });
+    std::vector<Letter> r;
+    for (auto& l : o) {
+      if (!r.empty()) {
+        Letter& p = r.back();
+        bool ok = true;
+        for (int q : ops) ok = ok && p.s[q] == l.n * l.s[q];
+        if (ok) {
+          p.n *= l.n;
+          for (int q = 0; q < 3; q++) p.s[q] = l.s[q];
+          continue;
+        }
+      }
+      r.push_back(l);
+    }
+    if (!have || r.size() < best.size()) {
+      best = r;
+      have = true;
+    }
+    if (best.size() <= 2) break;
+  }
+  FRICC_CHECK(best.size() <= 3, std::string("dense_contract: the ") + what +
+                                    " letters do not reduce to three strided levels (copy an operand first)");
+  *out = best;
+  return OK;
+}
+
+// levels [outer .. inner] -> Lvl for operand op, and the group's extent
+static Lvl make_lvl(const std::vector<Letter>& lv, int op, int64_t* extent) {
+  Lvl d;
+  d.nl = (int)lv.size();
+  d.L0 = d.L1 = 1;
+  d.s0 = d.s1 = d.s2 = 0;
+  int64_t n = 1;
+  const int k = (int)lv.size();
+  if (k >= 1) {
+    d.L0 = lv[k - 1].n;
+    d.s0 = lv[k - 1].s[op];
+  }
+  if (k >= 2) {
+    d.L1 = lv[k - 2].n;
+    d.s1 = lv[k - 2].s[op];
+  }
+  if (k >= 3) d.s2 = lv[0].s[op];
+  for (auto& l : lv) n *= l.n;
+  d.r0 = 1.0 / (double)d.L0;
+  d.r1 = 1.0 / (double)d.L1;
+  *extent = n;
+  return d;
+}
+
+template <typename T, int BM, int BN>
+static void launch_tile(const DArgs& a, hipStream_t s) {
+  dim3 grid((unsigned)(a.ntm * a.ntn), (unsigned)(a.batch * a.nsplit));
+  k_dgemm<T, BM, BN><<<grid, 256, 0, s>>>(a);
+}
+
+template <typename T>
+static int launch_dgemm(DArgs a, int bm, int bn, hipStream_t s) {
+#define FRICC_TILE(X, Y)                \
+  if (bm == X && bn == Y) {             \
+    launch_tile<T, X, Y>(a, s);         \
+    return OK;                          \
+  }
+  FRICC_TILE(128, 128)
+  FRICC_TILE(128, 64)
+  FRICC_TILE(64, 128)
+  FRICC_TILE(64, 64)
+  FRICC_TILE(128, 32)
+  FRICC_TILE(32, 128)
+  FRICC_TILE(64, 32)
+  FRICC_TILE(32, 64)
+  FRICC_TILE(32, 32)
+#undef FRICC_TILE
+  set_error("dense_contract: no kernel for this tile shape");
+  return EINVAL_;
+}
+
+static int pick_tile(int64_t n) { return n <= 32 ? 32 : n <= 64 ? 64 : 128; }
+
+int dense_contract(const char* spec, double alpha, const void* A, int a64, int an, const int64_t* ash,
+                   const int64_t* ast, const void* B, int b64, int bn, const int64_t* bsh, const int64_t* bst,
+                   double beta, void* C, int c64, int cn, const int64_t* csh, const int64_t* cst, int compute64,
+                   hipStream_t s) {
+  FRICC_CHECK(spec && A && B && C, "dense_contract: null argument");
+  FRICC_CHECK(an >= 0 && an <= 8 && bn >= 0 && bn <= 8 && cn >= 0 && cn <= 8, "dense_contract: at most 8 axes");
+  const char* comma = strchr(spec, ',');
+  const char* arrow = strstr(spec, "->");
+  FRICC_CHECK(comma && arrow && comma < arrow, "dense_contract: spec must look like 'ab,bc->ac'");
+  const std::string sa(spec, comma), sb(comma + 1, arrow), sc(arrow + 2);
+  FRICC_CHECK((int)sa.size() == an && (int)sb.size() == bn && (int)sc.size() == cn,
+              "dense_contract: spec and tensor ranks differ");
+  std::vector<Letter> lets;
+  auto add = [&](const std::string& str, const int64_t* sh, const int64_t* st, int op) -> int {
+    for (size_t i = 0; i < str.size(); i++) {
+      const char c = str[i];
+      FRICC_CHECK(c >= 'a' && c <= 'z', "dense_contract: letters must be a-z");
+      for (size_t j = 0; j < i; j++) FRICC_CHECK(str[j] != c, "dense_contract: repeated letter in one operand");
+      Letter* l = nullptr;
+      for (auto& x : lets)
+        if (x.c == c) l = &x;
+      if (!l) {
+        Letter nl;
+        nl.c = c;
+        nl.n = sh[i];
+        for (int q = 0; q < 3; q++) {
+          nl.s[q] = 0;
+          nl.in[q] = false;
+        }
+        lets.push_back(nl);
+        l = &lets.back();
+      }
+      FRICC_CHECK(sh[i] >= 1, "dense_contract: empty axis");
+      FRICC_CHECK(l->n == sh[i], std::string("dense_contract: extents of letter ") + c + " differ");
+      l->s[op] = st[i];
+      l->in[op] = true;
+    }
+    return OK;
+  };
+  FRICC_TRY(add(sa, ash, ast, 0));
+  FRICC_TRY(add(sb, bsh, bst, 1));
+  FRICC_TRY(add(sc, csh, cst, 2));
+  std::vector<Letter> gb, gm, gn, gk;
+  for (auto& l : lets) {
+    const bool ia = l.in[0], ib = l.in[1], ic = l.in[2];
+    if (ia && ib && ic)
+      gb.push_back(l);
+    else if (ia && ic)
+      gm.push_back(l);
+    else if (ib && ic)
+      gn.push_back(l);
+    else if (ia && ib)
+      gk.push_back(l);
+    else
+      FRICC_CHECK(l.n == 1, std::string("dense_contract: letter ") + l.c + " appears in only one operand");
+  }
+  std::vector<Letter> lb, lm, ln, lk;
+  int64_t sizes[3] = {1, 1, 1};
+  for (int i = 0; i < an; i++) sizes[0] *= ash[i];
+  for (int i = 0; i < bn; i++) sizes[1] *= bsh[i];
+  for (int i = 0; i < cn; i++) sizes[2] *= csh[i];
+  auto by_size = [&](std::vector<int> ops) {
+    std::stable_sort(ops.begin(), ops.end(), [&](int x, int y) { return sizes[x] > sizes[y]; });
+    return ops;
+  };
+  FRICC_TRY(group_levels(gb, by_size({0, 1, 2}), &lb, "batch"));
+  FRICC_TRY(group_levels(gm, by_size({0, 2}), &lm, "m"));
+  FRICC_TRY(group_levels(gn, by_size({1, 2}), &ln, "n"));
+  FRICC_TRY(group_levels(gk, by_size({0, 1}), &lk, "k"));
+  DArgs a;
+  int64_t nb, M, N, K, t;
+  a.Ab = make_lvl(lb, 0, &nb);
+  a.Bb = make_lvl(lb, 1, &t);
+  a.Cb = make_lvl(lb, 2, &t);
+  a.Am = make_lvl(lm, 0, &M);
+  a.Cm = make_lvl(lm, 2, &t);
+  a.Bn = make_lvl(ln, 1, &N);
+  a.Cn = make_lvl(ln, 2, &t);
+  a.Ak = make_lvl(lk, 0, &K);
+  a.Bk = make_lvl(lk, 1, &t);
+  FRICC_CHECK(nb < ((int64_t)1 << 31) && M < ((int64_t)1 << 31) && N < ((int64_t)1 << 31) && K < ((int64_t)1 << 31),
+              "dense_contract: an index group exceeds 2^31");
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.a64 = a64;
+  a.b64 = b64;
+  a.c64 = c64;
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.batch = nb;
+  a.alpha = alpha;
+  a.beta = beta;
+  // coalescing: walk the unit-stride index of each tile
+  a.a_kfast = (a.Ak.s0 == 1 && a.Am.s0 != 1) ? 1 : 0;
+  a.b_kfast = (a.Bk.s0 == 1 && a.Bn.s0 != 1) ? 1 : 0;
+  const int bm = pick_tile(M), bnt = pick_tile(N);
+  a.ntm = ceil_div(M, bm);
+  a.ntn = ceil_div(N, bnt);
+  // split K when the output grid cannot fill the device
+  const int64_t tiles = a.ntm * a.ntn * nb;
+  int nsplit = 1;
+  if (tiles < 512 && K >= 8 * DBK) {
+    nsplit = (int)std::min<int64_t>(std::min<int64_t>(ceil_div(1024, tiles), K / (4 * DBK)), 256);
+    nsplit = std::max(nsplit, 1);
+  }
+  a.kchunk = ceil_div(ceil_div(K, nsplit), DBK) * DBK;
+  nsplit = (int)ceil_div(K, a.kchunk);
+  a.nsplit = std::max(nsplit, 1);
+  FRICC_CHECK(nb * a.nsplit < 65536 && a.ntm * a.ntn < ((int64_t)1 << 31), "dense_contract: grid too large");
+  a.partial = nullptr;
+  if (a.nsplit > 1) {
+    void* p = nullptr;
+    FRICC_TRY(ws_get(50, (size_t)a.nsplit * nb * M * N * (compute64 ? 8 : 4), &p));
+    a.partial = p;
+  }
+  if (M == 0 || N == 0 || nb == 0) return OK;
+  if (compute64)
+    FRICC_TRY(launch_dgemm<double>(a, bm, bnt, s));
+  else
+    FRICC_TRY(launch_dgemm<float>(a, bm, bnt, s));
+  if (a.nsplit > 1) {
+    const int64_t total = nb * M * N;
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(total, 256), 4096);
+    if (compute64)
+      k_dgemm_reduce<double><<<g, 256, 0, s>>>(a);
+    else
+      k_dgemm_reduce<float><<<g, 256, 0, s>>>(a);
+  }
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace fricc
+
+extern "C" {
+// C = alpha * einsum(spec, A, B) + beta * C for two strided device tensors
+// (see the top of this file).  *_precision: 0 FP64, 1 FP32 element types;
+// compute_precision: 0 FP64 MFMA, 1 FP32 MFMA.  No counterpart in the
+// reference (its dense products are PySCF/NumPy einsums,
+// fricc/friccsd.py:144-207, 309-376).
+int fricc_dense_contract(const char* spec, double alpha, const void* A, int a_precision, int a_ndim,
+                         const int64_t* a_shape, const int64_t* a_strides, const void* B, int b_precision, int b_ndim,
+                         const int64_t* b_shape, const int64_t* b_strides, double beta, void* C, int c_precision,
+                         int c_ndim, const int64_t* c_shape, const int64_t* c_strides, int compute_precision,
+                         void* stream) {
+  FRICC_CHECK((a_precision == 0 || a_precision == 1) && (b_precision == 0 || b_precision == 1) &&
+                  (c_precision == 0 || c_precision == 1) && (compute_precision == 0 || compute_precision == 1),
+              "dense_contract: precisions must be 0 (FP64) or 1 (FP32)");
+  return fricc::dense_contract(spec, alpha, A, a_precision == 0, a_ndim, a_shape, a_strides, B, b_precision == 0,
+                               b_ndim, b_shape, b_strides, beta, C, c_precision == 0, c_ndim, c_shape, c_strides,
+                               compute_precision == 0, (hipStream_t)stream);
+}
+}

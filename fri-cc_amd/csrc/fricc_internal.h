@@ -10,7 +10,9 @@
 namespace fricc {
 
 enum { COMP_LARGEST = 0, COMP_FRI_SYSTEMATIC = 1 };
-enum { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
+// DT_BF16: hi | lo bf16 planes (three-pass products); DT_BF16X1: the hi plane
+// only (single-pass products)
+enum { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_BF16X1 = 3 };
 
 struct CompressInfo {
   int64_t n_D, n_sampled, n_notfound, n_candidates;
@@ -55,7 +57,7 @@ struct View {
   uint32_t* tile_order = nullptr;  // [ntiles] tiles by decreasing row count (device)
   uint32_t* rowlist = nullptr;   // [cap_rows] r of each gathered row
   void* B = nullptr;             // [rows x TY] values (dtype); BF16: hi plane, bf16 [rows/8][TY][8]
-  void* Blo = nullptr;           // BF16 only: lo plane (value - hi), same layout
+  void* Blo = nullptr;           // DT_BF16 only: lo plane (value - hi), same layout
   size_t cap_rows = 0;           // worst-case rows the buffers hold
   uint64_t gen = 0;
   ~View();
@@ -97,14 +99,16 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
 // gathered rows per column.  A is given as 1 KB records
 // [bf16_xblocks(X)][R][hi 256 | lo 256] bf16.  npass 3 = hi*hi + hi*lo +
 // lo*hi, 1 = hi*hi.
+// hilo 0: A holds hi-only records [bf16_xblocks(X)][R][hi 256] (npass 1)
 int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& om, double alpha,
-                       int npass, hipStream_t s);
+                       int npass, hipStream_t s, int hilo = 1);
 // tile width of BF16 views for an operand of R x X
 int bf16_view_width(int64_t R, int64_t X);
 // FP64/FP32 row-major A[R][X] -> [hi | lo] bf16 planes (layout above)
 // B (optional): split alpha*A + beta*B instead of A
+// hilo 0: hi-only records [bf16_xblocks(X)][R][hi 256]
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s,
-               const void* B = nullptr, double alpha = 1.0, double beta = 0.0);
+               const void* B = nullptr, double alpha = 1.0, double beta = 0.0, int hilo = 1);
 // 256-column blocks of a BF16 operand (its size is R * blocks * 512 bf16)
 inline int64_t bf16_xblocks(int64_t X) { return (X + 255) / 256; }
 

@@ -189,7 +189,7 @@ __global__ void k_split_rows(const uint32_t* __restrict__ rows, int64_t w, const
     const float f = in[i];
     const __hip_bfloat16 h = __float2bfloat16(f);
     hi[i] = h;
-    lo[i] = __float2bfloat16(f - __bfloat162float(h));
+    if (lo) lo[i] = __float2bfloat16(f - __bfloat162float(h));  // no lo plane for single-pass views
   }
 }
 
@@ -224,7 +224,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   g.Rl = (uint32_t)v->Rl;
   g.Yl = (uint32_t)v->Yl;
   g.TY = (uint32_t)v->TY;
-  const bool grouped = v->dtype == DT_BF16;
+  const bool grouped = v->dtype == DT_BF16 || v->dtype == DT_BF16X1;
   g.R = (uint32_t)v->R;
   void *p_slot, *p_cnt;
   const size_t nslot = (size_t)v->ntiles * g.R;
@@ -237,6 +237,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   const int64_t nnz = sp->nnz;
   // bytes per gathered row (BF16 views: per plane)
   const size_t rowbytes = (size_t)v->TY * (v->dtype == DT_F64 ? 8 : v->dtype == DT_F32 ? 4 : 2);
+  const bool lo_plane = v->dtype == DT_BF16;
   const size_t cap = view_capacity(v->ntiles, g.R, nnz, v->kstep);
   if (v->cap_rows < cap || !v->rowlist) {
     if (v->rowlist) FRICC_HIP(hipFree(v->rowlist));
@@ -247,7 +248,7 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
     v->Blo = nullptr;
     FRICC_HIP(hipMalloc(&v->rowlist, cap * 4));
     FRICC_HIP(hipMalloc(&v->B, cap * rowbytes));
-    if (grouped) FRICC_HIP(hipMalloc(&v->Blo, cap * rowbytes));
+    if (lo_plane) FRICC_HIP(hipMalloc(&v->Blo, cap * rowbytes));
     v->cap_rows = cap;
   }
   k_zero_u32<<<(unsigned)std::min<size_t>(2048, (nslot + VNT - 1) / VNT), VNT, 0, s>>>(slot, (int64_t)nslot);
@@ -296,7 +297,7 @@ int get_view(Sparse* sp, const int pos[4], int TY, int dtype, hipStream_t s, Vie
     std::unique_ptr<View> nv(new View());
     for (int q = 0; q < 4; q++) nv->pos[q] = pos[q];
     nv->TY = TY;
-    nv->kstep = dtype == DT_BF16 ? 32 : 16;  // rows per MFMA K step
+    nv->kstep = (dtype == DT_BF16 || dtype == DT_BF16X1) ? 32 : 16;  // rows per MFMA K step
     nv->dtype = dtype;
     v = nv.get();
     sp->views[key] = std::move(nv);

@@ -12,8 +12,8 @@ LIB_PATH = os.path.join(_HERE, "libfricc_hip.so")
 
 FRICC_OK, FRICC_EINVAL, FRICC_EDEVICE, FRICC_ENOMEM, FRICC_EPROB, FRICC_ENONFINITE = range(6)
 LARGEST, FRI_SYSTEMATIC = 0, 1
-FP64, FP32, BF16 = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16}
+FP64, FP32, BF16, BF16X1 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16x1": BF16X1}
 
 # every symbol include/fricc_capi.h declares
 EXPORTS = (
@@ -24,7 +24,8 @@ EXPORTS = (
     "fricc_set_bf16_view_width", "fricc_stream_copy", "fricc_sparse_compress_async", "fricc_sparse_status",
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
     "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
-    "fricc_compress_noalloc", "fricc_compress_noalloc_max_n",
+    "fricc_compress_noalloc", "fricc_compress_noalloc_max_n", "fricc_dense_contract",
+    "fricc_split_bf16_hi",
 )
 
 
@@ -91,9 +92,14 @@ def lib():
     L.fricc_contract_prepared.argtypes = [ctypes.c_char_p, _vp, I64x4, _vp, _vp, I64x4, I64x4, ctypes.c_int, _vp]
     L.fricc_contract_spec.argtypes = [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.c_int, I64x4, I64x4, _vp, _vp,
                                       ctypes.c_int, I64x4, I64x4, ctypes.c_int, _vp]
+    _pi = ctypes.POINTER(_i64)
+    L.fricc_dense_contract.argtypes = [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_int, _pi, _pi,
+                                       _vp, ctypes.c_int, ctypes.c_int, _pi, _pi, ctypes.c_double, _vp, ctypes.c_int,
+                                       ctypes.c_int, _pi, _pi, ctypes.c_int, _vp]
     L.fricc_eri_build.argtypes = [_vp, _i64, _i64, I64x8, I64x4, _vp, _vp]
     L.fricc_set_gemm_order.argtypes = [ctypes.c_int]
     L.fricc_split_bf16.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
+    L.fricc_split_bf16_hi.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_split_bf16_lincomb.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i64, _i64, _vp,
                                            _vp]
     L.fricc_set_bf16_passes.argtypes = [ctypes.c_int]

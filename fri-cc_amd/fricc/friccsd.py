@@ -4,19 +4,23 @@ Mirrors ``fricc/friccsd.py`` of the reference (same settings dict, same call
 order, same compressed/uncompressed branches) with every array a torch CUDA
 tensor.  The compressions and the 11 sparse contractions run in the gfx950
 kernels of libfricc_hip.so (``py_rccsd``); the dense remainder (PySCF's
-``imd.*`` formulas, fricc/friccsd.py:144-192) is written as GEMMs over
-operands pre-arranged once per system (``DeviceERIs.prepared``).
+``imd.*`` formulas, fricc/friccsd.py:144-192) is one MFMA contraction per
+product (``fricc.dense``, libfricc_hip's own kernel, no vendor BLAS) over
+operands pre-arranged once per system (``DeviceERIs.prepared``) or read in
+place through their strides.
 
 Differences from the reference, each deliberate:
 * ``cc_Wvvvv`` (fricc/friccsd.py:325-331) is never materialised: its bare part
   (ac|bd) is the static ladder operand and its t1 dressing is contracted with
   the sparse T2 directly (two extra gathered contractions), which is the same
   sum in a different order.
-* precision (setting ``"precision"``): "fp64" (plain, the reference's
-  arithmetic); "fp32": sparse contractions on FP32 MFMA and the O(o^2 v^3)
-  dense GEMMs in FP32; "mixed": sparse contractions on BF16 MFMA with hi/lo
-  split operands (hi*hi + hi*lo + lo*hi, FP32 accumulate) and FP32 dense
-  GEMMs.  Compression, the energy and the denominators stay FP64 in all modes.
+* precision (setting ``"precision"``, PRECISION_MODES): "fp64" (plain, the
+  reference's arithmetic); "fp32": sparse contractions on FP32 MFMA and the
+  O(o^2 v^3) dense products in FP32; "bf16-ladder": the ladder on BF16 MFMA
+  (one pass, FP32 accumulate), every other contraction FP32; "mixed": sparse
+  contractions on BF16 MFMA with hi/lo split operands (hi*hi + hi*lo +
+  lo*hi, FP32 accumulate) and FP32 dense products.  Compression, the energy
+  and the denominators stay FP64 in all modes.
 * ``cc._iter`` is set by ``kernel`` (the reference never sets it, so
   ``unbiased=True`` raised AttributeError; quirk A.3), and both the
   ``"unbias"`` and ``"unbiased"`` keys are accepted (quirk A.2).
@@ -49,6 +53,17 @@ default_fri_settings = {
 }
 
 ALLOWED_CONTRACTIONS = ["O^2V^4", "O^3V^3", "O^4V^2"]
+
+# precision mode -> (precision of the sparse contractions, precision of the
+# ladder 2323); the dense products run in FP64 for "fp64", else FP32, and the
+# compressions, denominators, energy and norm are FP64 in every mode.
+#   "fp64": plain; "fp32": every contraction FP32;
+#   "bf16-ladder": the ladder on single-pass BF16 MFMA (FP32 accumulate),
+#       every other contraction FP32;
+#   "mixed": every sparse contraction on three-pass BF16 (hi*hi + hi*lo +
+#       lo*hi, about 16 significant bits per product), FP32 accumulate.
+PRECISION_MODES = {"fp64": ("fp64", "fp64"), "fp32": ("fp32", "fp32"), "bf16-ladder": ("fp32", "bf16x1"),
+                   "mixed": ("bf16", "bf16")}
 
 
 def _torch():
@@ -120,32 +135,6 @@ def _prepare(T, spec, wshape, oshape, precision):
                                                  _capi.stream_ptr()))
 
 
-def _mm_splitk(A, B, chunk=8192):
-    """A (M x K) @ B (K x N), both contiguous, for large K and small M*N: K is cut
-    into S chunks run as one batched GEMM, then summed (a plain GEMM of this
-    shape leaves most of the GPU idle)."""
-    torch = _torch()
-    M, K = A.shape
-    N = B.shape[1]
-    S = max(1, K // chunk)
-    while K % S:
-        S -= 1
-    Kc = K // S
-    return torch.bmm(A.reshape(M, S, Kc).transpose(0, 1), B.reshape(S, Kc, N)).sum(0)
-
-
-def _mm_splitk_nt(A, B, chunk=8192):
-    """A (M x K) @ B^T for B (N x K), both contiguous row-major."""
-    torch = _torch()
-    M, K = A.shape
-    N = B.shape[0]
-    S = max(1, K // chunk)
-    while K % S:
-        S -= 1
-    Kc = K // S
-    return torch.bmm(A.reshape(M, S, Kc).transpose(0, 1), B.reshape(N, S, Kc).permute(1, 2, 0)).sum(0)
-
-
 def _dyn_operand(A, R, X, precision):
     """Per-iteration A operand (Woooo, Wvoov, Wvovo) in the kernel precision."""
     torch = _torch()
@@ -174,12 +163,25 @@ def _out_prec(t):
     return _capi.FP32 if t.dtype == _torch().float32 else _capi.FP64
 
 
+# When a dict: every compressed contraction of a step is also evaluated alone
+# into a zeroed FP64 buffer stored under its spec, so the contributions of
+# two steps (e.g. a reduced precision mode against FP64 on the same sampled
+# T2) can be compared term by term.  Off in normal runs.
+TERM_PROBE = None
+
+
 def _contract_into(spec, alpha, A, wshape, T, out, precision):
     """out += alpha * einsum(spec, W, T) with A the prepared operand of W."""
-    _capi.check(_capi.lib().fricc_contract_spec(
-        spec.encode(), float(alpha), _vp(A), 1, _capi.i64x4(wshape), _capi.i64x4([0, 0, 0, 0]), T.handle(),
-        _vp(out), _out_prec(out), _capi.i64x4(out.shape), _capi.i64x4(out.stride()), _capi.PRECISIONS[precision],
-        _capi.stream_ptr()))
+    outs = [out]
+    if TERM_PROBE is not None:
+        probe = _torch().zeros(out.shape, dtype=_torch().float64, device=out.device)
+        TERM_PROBE[spec] = probe
+        outs.append(probe)
+    for o_ in outs:
+        _capi.check(_capi.lib().fricc_contract_spec(
+            spec.encode(), float(alpha), _vp(A), 1, _capi.i64x4(wshape), _capi.i64x4([0, 0, 0, 0]), T.handle(),
+            _vp(o_), _out_prec(o_), _capi.i64x4(o_.shape), _capi.i64x4(o_.stride()), _capi.PRECISIONS[precision],
+            _capi.stream_ptr()))
 
 
 def _check_m_keep(cc, t2):
@@ -270,7 +272,6 @@ def _update_amps_device(cc, t1, t2, eris):
     fvv = fock[o:, o:]
     t1 = t1.contiguous()
     t2 = t2.contiguous()
-    t1c = t1.to(cdt)
     t2c = t2.to(cdt) if cdt != f64 else t2
     C = lambda a: a.to(f64) if cdt != f64 else a  # noqa: E731
 
@@ -303,12 +304,25 @@ def _update_amps_device(cc, t1, t2, eris):
     tm.stop("compress")
     timings["compress"] = time.perf_counter() - t0
 
+    # every dense product below is one MFMA contraction (fricc.dense): E in
+    # the dense precision (FP32 in the reduced modes), E64 in FP64; operands
+    # are read in place through their strides, FP64 ones converted on load
+    from .dense import einsum as _de
+
+    def E(spec, A, B, out=None, alpha=1.0, beta=0.0):
+        return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=cdt)
+
+    def E64(spec, A, B, out=None, alpha=1.0, beta=0.0):
+        return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=f64)
+
     # ---- Foo, Fvv, Fov (imd.cc_Foo / cc_Fvv / cc_Fov)
-    # Fkc = sum_ld [2 (kc|ld) - (kd|lc)] t1[l,d] = sum_l Lov2[k,l] (c x d) @ t1[l]
-    Fov_ = torch.matmul(P["Lov2"], t1c.unsqueeze(-1)).sum(1).reshape(o, v)
-    # Fki = sum_lcd Lov2[k,(lcd)] t2[i,(lcd)];  Fac = -sum_kl t2[kl] @ Lov[kl]
-    Fki = _mm_splitk_nt(P["Lov2"].reshape(o, -1), t2c.reshape(o, -1)) + Fov_ @ t1c.T
-    Fac = -torch.bmm(t2c.reshape(o * o, v, v), P["Lov"].reshape(o * o, v, v)).sum(0) - t1c.T @ Fov_
+    # Fkc = sum_ld [2 (kc|ld) - (kd|lc)] t1[l,d]
+    Fov_ = E("klcd,ld->kc", P["Lov2"], t1)
+    # Fki = sum_lcd Lov2[k,l,c,d] t2[i,l,c,d] + Fkc t1[i,c];  Fac = -sum_kld t2[k,l,a,d] Lov[k,l,d,c] - t1[k,a] Fkc
+    Fki = E("klcd,ilcd->ki", P["Lov2"], t2c)
+    E("kc,ic->ki", Fov_, t1, out=Fki, beta=1.0)
+    Fac = E("klad,kldc->ac", t2c, P["Lov"], alpha=-1.0)
+    E("ka,kc->ac", t1, Fov_, out=Fac, alpha=-1.0, beta=1.0)
     Foo_u = C(Fki) + foo
     Fvv_u = C(Fac) + fvv
     Fov = C(Fov_) + fov
@@ -318,35 +332,35 @@ def _update_amps_device(cc, t1, t2, eris):
     Fvv.diagonal().sub_(mo_e_v)
 
     # ---- T1 equation (fricc/friccsd.py:152-176)
-    t1new = -2 * ((t1 @ fov.T) @ t1)
-    t1new += t1 @ Fvv.T
-    t1new -= Foo.T @ t1
-    # 2 sum_kc Fov[k,c] t2[k,i,c,a] - sum_kc Fov[k,c] t2[i,k,c,a]  (batched matrix-vector, t2 read once each)
-    Fovc = Fov.to(cdt)
-    t1new += C(2 * torch.matmul(Fovc.view(o, 1, 1, v), t2c).sum(0).reshape(o, v)
-               - torch.matmul(Fovc.reshape(1, ov), t2c.reshape(o, ov, v)).reshape(o, v))
-    t1new += (t1 @ Fov.T) @ t1
+    t1new = E64("ik,ka->ia", E64("ic,kc->ik", t1, fov), t1, alpha=-2.0)
+    E64("ic,ac->ia", t1, Fvv, out=t1new, beta=1.0)
+    E64("ki,ka->ia", Foo, t1, out=t1new, alpha=-1.0, beta=1.0)
+    # 2 sum_kc Fov[k,c] t2[k,i,c,a] - sum_kc Fov[k,c] t2[i,k,c,a]
+    E("kc,kica->ia", Fov, t2c, out=t1new, alpha=2.0, beta=1.0)
+    E("kc,ikca->ia", Fov, t2c, out=t1new, alpha=-1.0, beta=1.0)
+    E64("ik,ka->ia", E64("ic,kc->ik", t1, Fov), t1, out=t1new, beta=1.0)
     t1new += fov
     # 2 sum_kc (kc|ai) t1[k,c] - sum_kc (ki|ac) t1[k,c]
-    t1new += C(2 * (t1c.reshape(1, ov) @ P["ovvo"].reshape(ov, v * o)).reshape(v, o).T
-               - torch.matmul(P["oovv"].reshape(o, ov, v), t1c.view(o, v, 1)).sum(0).reshape(o, v))
-    # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] @ Z[(kcd),a]
-    t1new += C(_mm_splitk(t2c.reshape(o, -1), P["Z"].reshape(-1, v)))
-    # sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac)
-    LvvT1 = C(torch.matmul(t1c.reshape(o, 1, 1, v), P["Z"]).sum(0).reshape(v, v).T)
-    t1new += t1 @ LvvT1.T
-    # -2 lcki,klac + kcli,klac  ==  sum_kl Aoo[kl] (i x c) @ t2[kl]^T (c x a)
-    t1new += C(torch.bmm(P["Aoo"].reshape(o * o, o, v), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
+    E("kcai,kc->ia", P["ovvo"], t1, out=t1new, alpha=2.0, beta=1.0)
+    E("kiac,kc->ia", P["oovv"], t1, out=t1new, alpha=-1.0, beta=1.0)
+    # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] Z[(kcd),a]
+    E("ikcd,kcda->ia", t2c, P["Z"], out=t1new, beta=1.0)
+    # LvvT1[a,c] = sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac)
+    LvvT1 = C(E("kd,kcda->ac", t1, P["Z"]))
+    E64("ic,ac->ia", t1, LvvT1, out=t1new, beta=1.0)
+    # -2 lcki,klac + kcli,klac  ==  sum_klc Aoo[k,l,i,c] t2[k,l,a,c]
+    E("klic,klac->ia", P["Aoo"], t2c, out=t1new, beta=1.0)
     # Pk[k,i] = sum_lc (lc|ki) t1[l,c] and Qk[k,i] = sum_lc (kc|li) t1[l,c], shared
-    # by the t1 equation and Loo: matrix-vector products over row-major copies
-    # of ovoo, one output element per row
-    Pk = torch.mv(P64["ovoo_P"].reshape(o * o, ov), t1.reshape(ov)).reshape(o, o)
-    Qk = torch.mv(P64["ovoo_Q"].reshape(o * o, ov), t1.T.contiguous().reshape(ov)).reshape(o, o)
-    t1new += (Qk - 2 * Pk).T @ t1  # -2 lcki,lc,ka + kcli,lc,ka
+    # by the t1 equation and Loo
+    Pk = E64("kilc,lc->ki", P64["ovoo_P"], t1)
+    Qk = E64("kicl,lc->ki", P64["ovoo_Q"], t1)
+    E64("ki,ka->ia", Qk - 2 * Pk, t1, out=t1new, beta=1.0)  # -2 lcki,lc,ka + kcli,lc,ka
 
     # ---- Loo, Lvv (imd.Loo / imd.Lvv)
-    Loo = Foo_u + fov @ t1.T + 2 * Pk - Qk
-    Lvv = Fvv_u - t1.T @ fov + LvvT1
+    Loo = Foo_u + 2 * Pk - Qk
+    E64("kc,ic->ki", fov, t1, out=Loo, beta=1.0)
+    Lvv = Fvv_u + LvvT1
+    E64("ka,kc->ac", t1, fov, out=Lvv, alpha=-1.0, beta=1.0)
     Loo.diagonal().sub_(mo_e_o)
     Lvv.diagonal().sub_(mo_e_v)
 
@@ -354,29 +368,27 @@ def _update_amps_device(cc, t1, t2, eris):
     t0 = time.perf_counter()
     # U[k,c,l,i] = sum_d (kc|ld) t1[i,d]: with (ld|kc) = (kc|ld) and (kc|ai) = (ia|kc)
     # it gives the t1 pieces of Woooo, of both A_vo intermediates and of T2p
-    U = (P["ovov"].reshape(ov * o, v) @ t1c.T).reshape(o, v, o, o)
-    # lcki,jc->klij + kclj,ic->klij from one product G[p,q,r,x] = sum_c (pc|qr) t1[x,c]:
-    # the first term is G[l,k,i,j], the second G[k,l,j,i]
-    G = (P64["ovoo_Y"].reshape(o * o * o, v) @ t1.T).reshape(o, o, o, o)
-    Woooo = G.permute(1, 0, 2, 3).contiguous()
-    Woooo += G.permute(0, 1, 3, 2)
+    U = E("kcld,id->kcli", P["ovov"], t1)
+    # lcki,jc->klij + kclj,ic->klij: with Y[l,k,i,c] = (lc|ki) the first is
+    # sum_c Y[l,k,i,c] t1[j,c], the second sum_c Y[k,l,j,c] t1[i,c]
+    Woooo = E64("lkic,jc->klij", P64["ovoo_Y"], t1)
+    E64("kljc,ic->klij", P64["ovoo_Y"], t1, out=Woooo, beta=1.0)
     if "O^4V^2" in ccs:
         _contract_into("kcld,ijcd->klij", 1.0, eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
     else:
-        Woooo += torch.einsum("kcld,ijcd->klij", eris.ovov, t2)
-    Woooo += C(torch.einsum("ic,kclj->klij", t1c, U))  # sum_cd (kc|ld) t1[i,c] t1[j,d]
+        E64("kcld,ijcd->klij", eris.ovov, t2, out=Woooo, beta=1.0)
+    E("ic,kclj->klij", t1, U, out=Woooo, beta=1.0)  # sum_cd (kc|ld) t1[i,c] t1[j,d]
     Woooo += eris.oooo.permute(0, 2, 1, 3)
     # Wvoov as A_voov[k,c,i,a] = Wvoov[a,k,i,c];  Wvovo as A_vovo[k,c,i,a] = Wvovo[a,k,c,i]
-    t1b = t1c.expand(ov, o, v)
-    A_voov = torch.baddbmm(P["ovvo_p"].reshape(ov, o, v), t1b, P["P1"].reshape(ov, v, v))
-    A_vovo = torch.baddbmm(P["oovv_p"].reshape(ov, o, v), t1b, P["P2"].reshape(ov, v, v))
+    A_voov = P["ovvo_p"].clone()
+    E("id,kcda->kcia", t1, P["P1"], out=A_voov, beta=1.0)
+    A_vovo = P["oovv_p"].clone()
+    E("id,kcda->kcia", t1, P["P2"], out=A_vovo, beta=1.0)
     # - sum_l (kcli + sum_d ldkc t1[i,d]) t1[l,a];  - sum_l (lcki + sum_d lckd t1[i,d]) t1[l,a]
-    N1 = P["ovoo"] + U                                   # kcli + sum_d (ld|kc) t1[i,d]
-    N2 = P["ovoo"].permute(2, 1, 0, 3) + U.permute(2, 1, 0, 3)  # lcki + sum_d (lc|kd) t1[i,d]
-    A_voov.baddbmm_(N1.reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
-    A_vovo.baddbmm_(N2.reshape(ov, o, o).transpose(1, 2), t1c.expand(ov, o, v), alpha=-1)
-    A_voov = A_voov.reshape(o, v, o, v)
-    A_vovo = A_vovo.reshape(o, v, o, v)
+    E("kcli,la->kcia", P["ovoo"], t1, out=A_voov, alpha=-1.0, beta=1.0)
+    E("kcli,la->kcia", U, t1, out=A_voov, alpha=-1.0, beta=1.0)
+    E("lcki,la->kcia", P["ovoo"], t1, out=A_vovo, alpha=-1.0, beta=1.0)
+    E("lcki,la->kcia", U, t1, out=A_vovo, alpha=-1.0, beta=1.0)
     Wvoov = A_voov.permute(3, 0, 2, 1)  # [a,k,i,c] view
     Wvovo = A_vovo.permute(3, 0, 1, 2)  # [a,k,c,i] view
     if "O^3V^3" in ccs:
@@ -388,72 +400,72 @@ def _update_amps_device(cc, t1, t2, eris):
         _contract_into("ldkc,ilad->akic", 1.0, eris.sparse_operand("A0113c", sp), (o, v, o, v), ts_alt, Wvoov, sp)
         _contract_into("lckd,ilda->akci", -0.5, A0313, (o, v, o, v), ts_alt, Wvovo, sp)
     else:
-        ovov64 = eris.ovov.to(cdt)
-        t2w = t2c
-        Wvoov -= 0.5 * torch.einsum("ldkc,ilda->akic", ovov64, t2w)
-        Wvoov -= 0.5 * torch.einsum("lckd,ilad->akic", ovov64, t2w)
-        Wvoov += torch.einsum("ldkc,ilad->akic", ovov64, t2w)
-        Wvovo -= 0.5 * torch.einsum("lckd,ilda->akci", ovov64, t2w)
+        ovov_c = P["ovov"]
+        E("ldkc,ilda->akic", ovov_c, t2c, out=Wvoov, alpha=-0.5, beta=1.0)
+        E("lckd,ilad->akic", ovov_c, t2c, out=Wvoov, alpha=-0.5, beta=1.0)
+        E("ldkc,ilad->akic", ovov_c, t2c, out=Wvoov, beta=1.0)
+        E("lckd,ilda->akci", ovov_c, t2c, out=Wvovo, alpha=-0.5, beta=1.0)
     timings["W"] = time.perf_counter() - t0
 
     # ---- direct T2 terms into the output buffer acc (fricc/friccsd.py:210-235)
     t0 = time.perf_counter()
     # the ladder's t1 dressing (Gx/Gy contractions and their t1 products) in
-    # the dense-GEMM precision; added to the FP64 accumulator once
+    # the dense-GEMM precision
     X = torch.zeros((o, o, o, v), dtype=cdt, device=t2.device)  # sum_cd (kd|ac) T[ij,cd]
     Y = torch.zeros((o, o, o, v), dtype=cdt, device=t2.device)  # sum_cd (kc|bd) T[ij,cd]
     if "O^2V^4" in ccs:
         _contract_into("kdac,ijcd->ijka", 1.0, eris.sparse_operand("Gx", sp), (o, v, v, v), ts, X, sp)
         _contract_into("kcbd,ijcd->ijkb", 1.0, eris.sparse_operand("Gy", sp), (o, v, v, v), ts, Y, sp)
     else:
-        X += torch.einsum("kdac,ijcd->ijka", eris.ovvv, t2)
-        Y += torch.einsum("kcbd,ijcd->ijkb", eris.ovvv, t2)
+        E("kdac,ijcd->ijka", eris.ovvv, t2c, out=X, beta=1.0)
+        E("kcbd,ijcd->ijkb", eris.ovvv, t2c, out=Y, beta=1.0)
     if "O^4V^2" in ccs:
         # klij,ka,lb -> ijab  ==  sum_l Q[ij,l,a] t1[l,b],  Q[ij,l,a] = sum_k Woooo[k,l,i,j] t1[k,a]
-        X -= torch.einsum("klij,ka->ijla", Woooo.to(cdt), t1c)
+        E("klij,ka->ijla", Woooo, t1, out=X, alpha=-1.0, beta=1.0)
     # acc[ij,a,b] = -sum_k X[ij,k,a] t1[k,b] - sum_k t1[k,a] Y[ij,k,b]
-    acc = torch.bmm(X.reshape(o * o, o, v).transpose(1, 2), t1c.expand(o * o, o, v)).neg_()
-    acc.baddbmm_(t1c.T.expand(o * o, v, o), Y.reshape(o * o, o, v), alpha=-1)
-    acc = C(acc).reshape(o, o, v, v)
+    acc = torch.empty((o, o, v, v), dtype=f64, device=t2.device)
+    E("ijka,kb->ijab", X, t1, out=acc, alpha=-1.0)
+    E("ka,ijkb->ijab", t1, Y, out=acc, alpha=-1.0, beta=1.0)
     if "O^4V^2" in ccs:
         _contract_into("klij,klab->ijab", 1.0, _dyn_operand(Woooo, o * o, o * o, sp), (o, o, o, o), ts, acc, sp)
     else:
         tau = t2 + t1[:, None, :, None] * t1[None, :, None, :]
-        acc.reshape(o * o, v * v).addmm_(Woooo.reshape(o * o, o * o).T, tau.reshape(o * o, v * v))
+        E64("klij,klab->ijab", Woooo, tau, out=acc, beta=1.0)
     if cc.vvvv_t1t1 == "reference":
         # t_ia t_jb sum_cd W_abcd (fricc/friccsd.py:222, quirk A.1), added in the finalize kernel
-        G1 = P64["g_ka"].T @ t1
+        G1 = E64("ka,kb->ab", P64["g_ka"], t1)
         wsum = (P64["s_ab"] - G1 - G1.T).contiguous()
     else:
         wsum = None
-        tt = (t1[:, None, :, None] * t1[None, :, None, :]).reshape(o * o, v * v)
-        acc.reshape(o * o, v * v).addmm_(tt, P64["ladder"].reshape(v * v, v * v))
-        Xd = torch.einsum("kdac,ic,jd->ijka", eris.ovvv, t1, t1)
-        Yd = torch.einsum("kcbd,ic,jd->ijkb", eris.ovvv, t1, t1)
-        acc -= torch.einsum("ijka,kb->ijab", Xd, t1) + torch.einsum("ijkb,ka->ijab", Yd, t1)
+        # sum_cd W_abcd t_ic t_jd with W = (ac|bd) - sum_k (kd|ac) t_kb - sum_k (kc|bd) t_ka
+        tt = t1[:, None, :, None] * t1[None, :, None, :]
+        E64("ijcd,cdab->ijab", tt, P64["ladder"], out=acc, beta=1.0)
+        Xd = E64("kdai,jd->ijka", E64("kdac,ic->kdai", eris.ovvv, t1), t1)
+        Yd = E64("kbdi,jd->ijkb", E64("kcbd,ic->kbdi", eris.ovvv, t1), t1)
+        E64("ijka,kb->ijab", Xd, t1, out=acc, alpha=-1.0, beta=1.0)
+        E64("ijkb,ka->ijab", Yd, t1, out=acc, alpha=-1.0, beta=1.0)
     if "O^2V^4" in ccs:
-        _prepare(ts, "2323", (v, v, v, v), (o, o, v, v), sp)
+        spl = cc.ladder_precision
+        _prepare(ts, "2323", (v, v, v, v), (o, o, v, v), spl)
         tm.start("ladder")
-        _contract_into("abcd,ijcd->ijab", 1.0, eris.sparse_operand("ladder", sp), (v, v, v, v), ts, acc, sp)
+        _contract_into("abcd,ijcd->ijab", 1.0, eris.sparse_operand("ladder", spl), (v, v, v, v), ts, acc, spl)
         tm.stop("ladder")
     else:
-        acc.reshape(o * o, v * v).addmm_(t2.reshape(o * o, v * v), P64["ladder"].reshape(v * v, v * v))
+        E64("ijcd,cdab->ijab", t2, P64["ladder"], out=acc, beta=1.0)
     timings["ladder"] = time.perf_counter() - t0
 
     # ---- symmetrised T2 terms into S (fricc/friccsd.py:178-186, 237-279)
     t0 = time.perf_counter()
     # tmp = (ia|cb) t1[j,c] - sum_k oovv[k,i,b,c] t1[j,c] t1[k,a]
-    S = torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v)).reshape(o, o, v, v)
-    S3 = S.reshape(o * o, v, v)
-    # M[i,j,k,b] = sum_c (ki|bc) t1[j,c]
-    M = (P["oovv"].reshape(o * o * v, v) @ t1c.T).reshape(o, o, v, o).permute(1, 3, 0, 2)
-    S3.baddbmm_(t1c.T.expand(o * o, v, o), M.reshape(o * o, o, v), alpha=-1)
-    # - (sum_c (kc|ai) t1[j,c] + ovoo[i,a,j,k]) t1[k,b];  sum_c (ia|kc) t1[j,c] = U[i,a,k,j]
-    T2p = U.permute(0, 3, 1, 2) + P["ovoo"].permute(0, 2, 1, 3)
-    S3.baddbmm_(T2p.reshape(o * o, v, o), t1c.expand(o * o, o, v), alpha=-1)
+    S = E("jc,icab->ijab", t1, P["Qv"])
+    Mt = E("kibc,jc->kijb", P["oovv"], t1)  # sum_c (ki|bc) t1[j,c]
+    E("ka,kijb->ijab", t1, Mt, out=S, alpha=-1.0, beta=1.0)
+    # - (sum_c (ia|kc) t1[j,c] + ovoo[i,a,j,k]) t1[k,b];  sum_c (ia|kc) t1[j,c] = U[i,a,k,j]
+    E("iakj,kb->ijab", U, t1, out=S, alpha=-1.0, beta=1.0)
+    E("iajk,kb->ijab", P["ovoo"], t1, out=S, alpha=-1.0, beta=1.0)
     # + Lvv[a,c] t2[i,j,c,b] - Loo[k,i] t2[k,j,a,b]
-    S3.baddbmm_(Lvv.to(cdt).expand(o * o, v, v), t2c.reshape(o * o, v, v))
-    S.reshape(o, o * v * v).addmm_(Loo.to(cdt).T, t2c.reshape(o, -1), alpha=-1)
+    E("ac,ijcb->ijab", Lvv, t2c, out=S, beta=1.0)
+    E("ki,kjab->ijab", Loo, t2c, out=S, alpha=-1.0, beta=1.0)
     if "O^3V^3" in ccs:
         _prepare(ts, "1302", (v, o, o, v), (o, o, v, v), sp)
         _prepare(ts, "1303", (v, o, o, v), (o, o, v, v), sp)
@@ -474,9 +486,10 @@ def _update_amps_device(cc, t1, t2, eris):
         _contract_into("bkci,kjac->ijab", -1.0, Avo, (v, o, v, o), ts, S, sp)   # 1203
         tm.stop("O3V3")
     else:
-        t2w = t2c
-        S += 2 * torch.einsum("akic,kjcb->ijab", Wvoov, t2w) - torch.einsum("akci,kjcb->ijab", Wvovo, t2w)
-        S -= torch.einsum("akic,kjbc->ijab", Wvoov, t2w) + torch.einsum("bkci,kjac->ijab", Wvovo, t2w)
+        E("akic,kjcb->ijab", Wvoov, t2c, out=S, alpha=2.0, beta=1.0)
+        E("akci,kjcb->ijab", Wvovo, t2c, out=S, alpha=-1.0, beta=1.0)
+        E("akic,kjbc->ijab", Wvoov, t2c, out=S, alpha=-1.0, beta=1.0)
+        E("bkci,kjac->ijab", Wvovo, t2c, out=S, alpha=-1.0, beta=1.0)
     timings["O3V3"] = time.perf_counter() - t0
 
     # ---- denominators (fricc/friccsd.py:281-284), energy and norm: FP64, fused
@@ -694,7 +707,7 @@ class FRICCSD:
     ``mf`` is a PySCF SCF object (when PySCF is importable), a
     ``systems.SyntheticSystem`` or a ready ``systems.DeviceERIs``.
     Extra settings keys (not in the reference): ``"precision"`` ("fp64" |
-    "fp32" | "mixed"), ``"seed"`` (srand seed for the uniform stream, default:
+    "fp32" | "bf16-ladder" | "mixed"), ``"seed"`` (srand seed for the uniform stream, default:
     none, as the reference), ``"vvvv_t1t1"`` ("reference" | "rccsd"),
     ``"graph"`` (True: ``kernel`` records the step once and replays it)."""
 
@@ -718,8 +731,8 @@ class FRICCSD:
             raise ValueError("compression must be 'fri' or 'largest'")
         if settings["compression"] == "fri" and settings["sampling_method"] != "systematic":
             raise NotImplementedError("only systematic sampling runs on the GPU path")
-        if settings["precision"] not in ("fp64", "fp32", "mixed"):
-            raise ValueError("precision must be 'fp64', 'fp32' or 'mixed'")
+        if settings["precision"] not in PRECISION_MODES:
+            raise ValueError(f"precision must be one of {sorted(PRECISION_MODES)}")
         self.frozen = frozen
         self.max_cycle = 50
         self.conv_tol = 1e-7
@@ -741,7 +754,7 @@ class FRICCSD:
         self.timer = None
         torch = _torch()
         prec = settings["precision"]
-        self.sparse_precision = {"fp64": "fp64", "fp32": "fp32", "mixed": "bf16"}[prec]
+        self.sparse_precision, self.ladder_precision = PRECISION_MODES[prec]
         self.dense_dtype = torch.float64 if prec == "fp64" else torch.float32
         self.e_hf = getattr(mf, "e_hf", 0.0) if mf is not None else 0.0
 

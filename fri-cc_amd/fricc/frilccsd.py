@@ -22,7 +22,8 @@ import logging
 
 from . import _capi
 from .py_rccsd import SparseTensor4d
-from .friccsd import (FRICCSD, _check_m_keep, _contract_into, _mm_splitk, _prepare, _set_uniforms, _torch, _vp,
+from .dense import einsum as _de
+from .friccsd import (FRICCSD, _check_m_keep, _contract_into, _prepare, _set_uniforms, _torch, _vp,
                       default_fri_settings as _ccsd_defaults, _finish_step, _draw_uniforms, _O3V3, _PREC_ID)
 
 log = logging.getLogger("fricc")
@@ -76,9 +77,13 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
     t2 = t2.contiguous()
     lccd = bool(st["LCCD"])
     t1 = torch.zeros_like(t1_in) if lccd else t1_in
-    t1c = t1.to(cdt)
     t2c = t2.to(cdt) if cdt != f64 else t2
-    C = lambda a: a.to(f64) if cdt != f64 else a  # noqa: E731
+
+    def E(spec, A, B, out=None, alpha=1.0, beta=0.0):  # MFMA contraction in the dense precision
+        return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=cdt)
+
+    def E64(spec, A, B, out=None, alpha=1.0, beta=0.0):
+        return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=f64)
 
     # ---- compression: one tensor, or one per compressed term
     comp = st["compression"]
@@ -105,25 +110,24 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
         t1new = torch.zeros_like(t1)
     else:
         t1new = fov.clone()
-        t1new += t1 @ Fvv.T
-        t1new -= Foo.T @ t1
-        t1new += C(2 * (t1c.reshape(1, ov) @ P["ovvo"].reshape(ov, v * o)).reshape(v, o).T
-                   - torch.matmul(P["oovv"].reshape(o, ov, v), t1c.view(o, v, 1)).sum(0).reshape(o, v))
-        # 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] @ Z[(kcd),a]
-        t1new += C(_mm_splitk(t2c.reshape(o, -1), P["Z"].reshape(-1, v)))
-        t1new += C(torch.bmm(P["Aoo"].reshape(o * o, o, v), t2c.reshape(o * o, v, v).transpose(1, 2)).sum(0))
-        fovc = fov.to(cdt)
-        t1new += C(2 * torch.matmul(fovc.view(o, 1, 1, v), t2c).sum(0).reshape(o, v)
-                   - torch.matmul(fovc.reshape(1, ov), t2c.reshape(o, ov, v)).reshape(o, v))
+        E64("ic,ac->ia", t1, Fvv, out=t1new, beta=1.0)
+        E64("ki,ka->ia", Foo, t1, out=t1new, alpha=-1.0, beta=1.0)
+        # 2 sum_kc (kc|ai) t1[k,c] - sum_kc (ki|ac) t1[k,c]
+        E("kcai,kc->ia", P["ovvo"], t1, out=t1new, alpha=2.0, beta=1.0)
+        E("kiac,kc->ia", P["oovv"], t1, out=t1new, alpha=-1.0, beta=1.0)
+        # 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] Z[(kcd),a]
+        E("ikcd,kcda->ia", t2c, P["Z"], out=t1new, beta=1.0)
+        E("klic,klac->ia", P["Aoo"], t2c, out=t1new, beta=1.0)
+        E("kc,kica->ia", fov, t2c, out=t1new, alpha=2.0, beta=1.0)
+        E("kc,ikca->ia", fov, t2c, out=t1new, alpha=-1.0, beta=1.0)
 
     # ---- symmetrised T2 terms into S (fricc/frilccsd.py:168-224 "tmp + tmp^T")
     # Term 8: sum_c (ia|bc) t1[j,c];  Term 7: - sum_k (ia|jk) t1[k,b]
-    S = torch.matmul(t1c.unsqueeze(0), P["Qv"].reshape(o, v, v * v)).reshape(o, o, v, v)
-    S3 = S.reshape(o * o, v, v)
-    S3 -= (P["ovoo_ijak"].reshape(o * o * v, o) @ t1c).reshape(o * o, v, v)
+    S = E("jc,icab->ijab", t1, P["Qv"])
+    E("ijak,kb->ijab", P["ovoo_ijak"], t1, out=S, alpha=-1.0, beta=1.0)
     # Term 3: Fvv[a,c] t2[i,j,c,b];  Term 2: - Foo[k,i] t2[k,j,a,b]
-    S3.baddbmm_(Fvv.to(cdt).expand(o * o, v, v), t2c.reshape(o * o, v, v))
-    S.reshape(o, o * v * v).addmm_(Foo.to(cdt).T, t2c.reshape(o, -1), alpha=-1)
+    E("ac,ijcb->ijab", Fvv, t2c, out=S, beta=1.0)
+    E("ki,kjab->ijab", Foo, t2c, out=S, alpha=-1.0, beta=1.0)
     # Term 6: O^3V^3 with Wvoov[a,k,i,c] = (kc|ai), Wvovo[a,k,c,i] = (ki|ac)
     if "O^3V^3" in ccs:
         Av = eris.sparse_operand("ovvo_p", sp)
@@ -142,23 +146,27 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
     else:
         Wvoov = P["ovvo_p"].permute(3, 0, 2, 1)   # [a,k,i,c]
         Wvovo = P["oovv_p"].permute(3, 0, 1, 2)   # [a,k,c,i]
-        S += 2 * torch.einsum("akic,kjcb->ijab", Wvoov, t2c) - torch.einsum("akci,kjcb->ijab", Wvovo, t2c)
-        S -= torch.einsum("akic,kjbc->ijab", Wvoov, t2c) + torch.einsum("bkci,kjac->ijab", Wvovo, t2c)
+        E("akic,kjcb->ijab", Wvoov, t2c, out=S, alpha=2.0, beta=1.0)
+        E("akci,kjcb->ijab", Wvovo, t2c, out=S, alpha=-1.0, beta=1.0)
+        E("akic,kjbc->ijab", Wvoov, t2c, out=S, alpha=-1.0, beta=1.0)
+        E("bkci,kjac->ijab", Wvovo, t2c, out=S, alpha=-1.0, beta=1.0)
 
     # ---- direct T2 terms into acc (FP64): Term 4 (ladder) and Term 5 (0101)
     acc = torch.zeros((o, o, v, v), dtype=f64, device=t2.device)
     if "O^2V^4" in ccs:
-        _prepare(ts["2323"], "2323", (v, v, v, v), (o, o, v, v), sp)
-        _contract_into("abcd,ijcd->ijab", 1.0, eris.sparse_operand("ladder", sp), (v, v, v, v), ts["2323"], acc, sp)
+        spl = cc.ladder_precision
+        _prepare(ts["2323"], "2323", (v, v, v, v), (o, o, v, v), spl)
+        _contract_into("abcd,ijcd->ijab", 1.0, eris.sparse_operand("ladder", spl), (v, v, v, v), ts["2323"], acc,
+                       spl)
     else:
         if eris.ladder is None:
             raise RuntimeError("the uncompressed O^2V^4 term needs the FP64 ladder")
-        acc.reshape(o * o, v * v).addmm_(t2.reshape(o * o, v * v), P64["ladder"].reshape(v * v, v * v))
+        E64("ijcd,cdab->ijab", t2, P64["ladder"], out=acc, beta=1.0)
     if "O^4V^2" in ccs:
         _contract_into("klij,klab->ijab", 1.0, eris.sparse_operand("Woooo_b", sp), (o, o, o, o), ts["0101"], acc,
                        sp)
     else:
-        acc.reshape(o * o, v * v).addmm_(P64["Woooo_b"].reshape(o * o, o * o).T, t2.reshape(o * o, v * v))
+        E64("klij,klab->ijab", P64["Woooo_b"], t2, out=acc, beta=1.0)
 
     # ---- denominators, linearised energy and norm: FP64, fused
     eia = (mo_e_o[:, None] - mo_e_v).contiguous()

@@ -344,8 +344,8 @@ def contract_DTSpT(W, T, ouput, term, precision="fp64"):
         raise ValueError(f"contraction term {term!r} not implemented")
     if not isinstance(T, SparseTensor4d):
         raise TypeError("T must be a SparseTensor4d")
-    if precision not in ("fp64", "fp32", "bf16"):
-        raise ValueError("precision must be 'fp64', 'fp32' or 'bf16'")
+    if precision not in ("fp64", "fp32", "bf16", "bf16x1"):
+        raise ValueError("precision must be 'fp64', 'fp32', 'bf16' or 'bf16x1'")
     Wd, ws, _ = _dev4(W, "W")
     Od, os_, host = _dev4(ouput, "ouput", writeable=True)
     if T.nnz == 0:

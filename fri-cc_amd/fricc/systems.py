@@ -85,6 +85,20 @@ def bf16_planes(A, R, X, stream=None):
     return out
 
 
+def bf16_planes_hi(A, R, X, stream=None):
+    """BF16 hi-only operand of A viewed as (R, X) for single-pass products:
+    512-byte records [X/256][R][hi 256] (fricc_split_bf16_hi layout)."""
+    import torch
+
+    A = A.contiguous()
+    assert A.numel() == R * X and A.dtype in (torch.float64, torch.float32)
+    src = _capi.FP64 if A.dtype == torch.float64 else _capi.FP32
+    out = torch.empty(R * bf16_xblocks(X) * 256, dtype=torch.int16, device=A.device)
+    _capi.check(_capi.lib().fricc_split_bf16_hi(ctypes.c_void_p(A.data_ptr()), src, int(R), int(X),
+                                                ctypes.c_void_p(out.data_ptr()), _capi.stream_ptr(stream)))
+    return out
+
+
 def bf16_planes_lincomb(A, B, alpha, beta, R, X, stream=None):
     """bf16_planes of alpha*A + beta*B without forming the sum (fricc_split_bf16_lincomb)."""
     import torch
@@ -198,7 +212,8 @@ class DeviceERIs:
     def sparse_operand(self, name, precision):
         """Static A operand of a sparse contraction ("A1323", "A0112", "A0313",
         "Gx", "Gy", "ladder") in the kernel layout for ``precision``
-        ("fp64" | "fp32" | "bf16"), cached.  For FP32/BF16 the FP64 layout is
+        ("fp64" | "fp32" | "bf16" hi | lo records | "bf16x1" hi-only records),
+        cached.  For FP32/BF16 the FP64 layout is
         built only transiently."""
         import torch
 
@@ -215,6 +230,8 @@ class DeviceERIs:
                 raise RuntimeError("the FP64 ladder was released; its BF16/FP32 operand must be built before")
             if precision == "fp32":
                 out = A.to(torch.float32).contiguous()
+            elif precision == "bf16x1":
+                out = bf16_planes_hi(A, A.shape[0] * A.shape[1], A.shape[2] * A.shape[3])
             else:
                 out = bf16_planes(A, A.shape[0] * A.shape[1], A.shape[2] * A.shape[3])
             del A
@@ -290,8 +307,10 @@ class DeviceERIs:
                 if self.L is None:
                     raise KeyError("s_ab needs the ladder or the factors")
                 # sum_cd sum_Q L[Q,a,c] L[Q,b,d] = sum_Q M[Q,a] M[Q,b], M = sum_c L[Q,a,c]
+                from .dense import einsum
+
                 M = self.L[:, o:, o:].sum(2)
-                self._s_ab = (M.T @ M).contiguous()
+                self._s_ab = einsum("qa,qb->ab", M, M, compute=torch.float64)
             return self._s_ab.to(dtype)
         if name == "g_ka":
             return ovvv.sum(dim=(1, 3)).to(dtype)       # sum_cd (kd|ac) = sum_cd (kc|ad)
@@ -311,11 +330,12 @@ class DeviceERIs:
         Xb = bf16_xblocks(X)
         cc = min(v, max(1, int(chunk_bytes // (8 * v ** 3))))
         dev = self.L.device
+        rl = 512 if precision == "bf16" else 256  # record length: hi | lo, or hi only
         if precision == "fp32":
             out = torch.empty((v, v, v, v), dtype=torch.float32, device=dev)
         else:
-            out = torch.empty(X * Xb * 512, dtype=torch.int16, device=dev)
-            rec = out.view(Xb, X, 512)  # [column block][row (c,d)][hi | lo]
+            out = torch.empty(X * Xb * rl, dtype=torch.int16, device=dev)
+            rec = out.view(Xb, X, rl)  # [column block][row (c,d)][hi | lo] or [hi]
         Wc = torch.empty((cc, v, v, v), dtype=torch.float64, device=dev)
         for c0 in range(0, v, cc):
             c1 = min(v, c0 + cc)
@@ -328,8 +348,8 @@ class DeviceERIs:
             if precision == "fp32":
                 out.view(X, X)[c0 * v:c1 * v].copy_(rows)
             else:
-                part = bf16_planes(rows, n * v, X)
-                rec[:, c0 * v:c1 * v].copy_(part.view(Xb, n * v, 512))
+                part = (bf16_planes if precision == "bf16" else bf16_planes_hi)(rows, n * v, X)
+                rec[:, c0 * v:c1 * v].copy_(part.view(Xb, n * v, rl))
                 del part
         del Wc, W, rows
         return out

@@ -35,7 +35,10 @@ enum fricc_status {
 };
 
 enum fricc_compression { FRICC_LARGEST = 0, FRICC_FRI_SYSTEMATIC = 1 };
-enum fricc_precision { FRICC_FP64 = 0, FRICC_FP32 = 1, FRICC_BF16 = 2 };
+/* FRICC_BF16: BF16 hi and lo planes of each operand, three MFMA passes
+ * (hi*hi + hi*lo + lo*hi, about 16 significant bits per product);
+ * FRICC_BF16X1: the hi plane only, one pass (BF16 products, FP32 accumulate). */
+enum fricc_precision { FRICC_FP64 = 0, FRICC_FP32 = 1, FRICC_BF16 = 2, FRICC_BF16X1 = 3 };
 
 typedef struct fricc_sparse fricc_sparse; /* device-resident SparseTensor4d */
 
@@ -146,7 +149,8 @@ int fricc_sparse_device(const fricc_sparse* sp, const uint32_t** idx, const doub
  * term in {"0101","2323","1302","1202","1303","1203","1323","0112","0113",
  * "0313","0312"}; W and out are device FP64 4-D arrays with arbitrary strides
  * (in elements).  precision: FRICC_FP64, FRICC_FP32 (FP32 operands, FP32
- * accumulate) or FRICC_BF16 (BF16 operands on MFMA, FP32 accumulate). */
+ * accumulate), FRICC_BF16 (BF16 hi/lo operands, three MFMA passes, FP32
+ * accumulate) or FRICC_BF16X1 (BF16 operands, one MFMA pass, FP32 accumulate). */
 int fricc_contract(const char* term, const double* W, const int64_t W_shape[4], const int64_t W_strides[4],
                    fricc_sparse* T, double* out, const int64_t out_shape[4], const int64_t out_strides[4],
                    int precision, void* stream);
@@ -154,7 +158,7 @@ int fricc_contract(const char* term, const double* W, const int64_t W_shape[4], 
 /* Same contraction with the dense operand already laid out as the kernel
  * reads it: A[r][x] row-major, r = the contracted index pair in T order,
  * x = W's free index pair in output order, element type per precision
- * (double / float / bf16).  Used by the device-resident update, which builds
+ * (double / float / fricc_split_bf16 records / fricc_split_bf16_hi records).  Used by the device-resident update, which builds
  * its intermediates directly in this layout.  No counterpart in the reference. */
 int fricc_contract_prepared(const char* term, const void* A, const int64_t W_shape[4], fricc_sparse* T,
                             double* out, const int64_t out_shape[4], const int64_t out_strides[4], int precision,
@@ -171,6 +175,21 @@ int fricc_contract_prepared(const char* term, const void* A, const int64_t W_sha
 int fricc_contract_spec(const char* spec, double alpha, const void* W_or_A, int prepared, const int64_t W_shape[4],
                         const int64_t W_strides[4], fricc_sparse* T, void* out, int out_precision,
                         const int64_t out_shape[4], const int64_t out_strides[4], int precision, void* stream);
+
+/* C = alpha * einsum(spec, A, B) + beta * C for device tensors of any
+ * strides (in elements; stride 0 broadcasts): spec "abc,cd->abd" style, one
+ * letter per axis, at most 8 axes per tensor; letters in A, B and C are
+ * batch, in A and C (B and C) rows (columns), in A and B summed.  Element
+ * types *_precision FRICC_FP64 or FRICC_FP32 (converted to the compute type
+ * on load); compute_precision FRICC_FP64 (FP64 MFMA) or FRICC_FP32 (FP32
+ * MFMA).  Each group of letters must reduce to at most three strided levels
+ * (FRICC_EINVAL otherwise).  beta == 0: C is not read.  Replaces the dense
+ * NumPy/PySCF einsums of update_amps (fricc/friccsd.py:144-207, 309-376). */
+int fricc_dense_contract(const char* spec, double alpha, const void* A, int a_precision, int a_ndim,
+                         const int64_t* a_shape, const int64_t* a_strides, const void* B, int b_precision, int b_ndim,
+                         const int64_t* b_shape, const int64_t* b_strides, double beta, void* C, int c_precision,
+                         int c_ndim, const int64_t* c_shape, const int64_t* c_strides, int compute_precision,
+                         void* stream);
 
 /* (pq|rs) = sum_Q L[Q,p,q] L[Q,r,s] (sequential fma chain over Q) for
  * p in [rng[0],rng[1]), q in [rng[2],rng[3]), r in [rng[4],rng[5]),
@@ -195,6 +214,12 @@ int fricc_sparse_prepare(fricc_sparse* T, const char* term_or_spec, const int64_
  * past X are zero.  out holds R*ceil(X/256)*512 uint16.  No counterpart in
  * the reference (FP64 only). */
 int fricc_split_bf16(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream);
+
+/* Operand of FRICC_BF16X1 contractions: A[R][X] (FP64 or FP32) -> 512-byte
+ * records [ceil(X/256)][R][hi 256] bf16 (hi = bf16(a) at ((x/256)*R + r)*256
+ * + x%256, columns past X zero); out holds R*ceil(X/256)*256 uint16.  No
+ * counterpart in the reference. */
+int fricc_split_bf16_hi(const void* A, int src_precision, int64_t R, int64_t X, void* out, void* stream);
 
 /* fricc_split_bf16 of alpha*A + beta*B (A, B: same shape and precision; the
  * combination formed in that precision), so a combined operand needs no

@@ -41,6 +41,7 @@
 #include <math.h>
 #include <omp.h>
 #include <immintrin.h>
+#include <parallel/algorithm>
 
 #include <vector>
 
@@ -195,7 +196,10 @@ void orc_prefix_canonical(const double* p, int64_t n, double* P) {
 
 // ---------------------------------------------------------------------------
 // sigma: indices sorted by |x| descending, ties by ascending index.
-static void sigma_sort(const double* x, int64_t n, std::vector<int64_t>& sig, int64_t keep) {
+// full_sort: sort all n (the reference's argsort merge-sorts the whole vector
+// in parallel and keeps the first m, src/fri_utils.cpp:11-36; the FAITHFUL
+// mode does the same work, in libstdc++'s parallel sort), else a partial sort.
+static void sigma_sort(const double* x, int64_t n, std::vector<int64_t>& sig, int64_t keep, bool full_sort = false) {
   sig.resize(n);
   std::iota(sig.begin(), sig.end(), (int64_t)0);
   auto cmp = [x](int64_t a, int64_t b) {
@@ -203,7 +207,10 @@ static void sigma_sort(const double* x, int64_t n, std::vector<int64_t>& sig, in
     if (fa != fb) return fa > fb;
     return a < b;
   };
-  if (keep < n) {
+  if (full_sort) {
+    __gnu_parallel::sort(sig.begin(), sig.end(), cmp);
+    if (keep < n) sig.resize(keep);
+  } else if (keep < n) {
     std::partial_sort(sig.begin(), sig.begin() + keep, sig.end(), cmp);
     sig.resize(keep);
   } else {
@@ -239,13 +246,13 @@ int orc_compress(const double* x, int64_t n, int64_t m, int method, int mode, do
     if (!std::isfinite(x[i])) { info->status = 3; return 3; }
   std::vector<int64_t> sig;
   if (method == 0) {
-    sigma_sort(x, n, sig, m);
+    sigma_sort(x, n, sig, m, mode == 1);
     for (int64_t k = 0; k < m; k++) { out_idx[k] = sig[k]; out_val[k] = x[sig[k]]; }
     info->n_D = m;
     return 0;
   }
   // get_d_largest (src/fri_utils.cpp:400-437)
-  sigma_sort(x, n, sig, m);
+  sigma_sort(x, n, sig, m, mode == 1);
   int64_t K = m;
   double R = 0.0;
   if (mode == 0) {

@@ -13,6 +13,10 @@ Digests per iteration k (inputs of iteration 1 = the MP2 guess):
   idx_sha[k, s], val_sha[k, s]          SHA-256 of the kept flat indices
                                         (uint32 LE) and values (float64 LE) of
                                         sample s (0: t2_sparse, 1: t2_sparse_alt)
+  val_at[k, s]                          kept values at VAL_POSITIONS (after
+                                        iteration 1 the compressed T2 carries
+                                        the last-bit differences of the FP64
+                                        update, so values compare to 1e-10)
   nD[k, s], nsamp[k, s]                 |D| and n' of each sample
   t1[k]                                 t1 after the iteration (o x v)
   t2_at[k]                              t2 after the iteration at POSITIONS
@@ -34,10 +38,15 @@ NOCC, NVIR, NAUX, SEED, M = 20, 180, 600, 1, 1_000_000
 LEVEL_SHIFT = 2.0
 N_ITER = 3
 POS_SEED, N_POS, WEIGHT_SEED = 2024, 20000, 7
+VAL_SEED, N_VAL = 99, 4096
 
 
 def positions(n):
     return np.sort(np.random.default_rng(POS_SEED).choice(n, N_POS, replace=False))
+
+
+def val_positions(m):
+    return np.sort(np.random.default_rng(VAL_SEED).choice(m, N_VAL, replace=False))
 
 
 def weights(n):
@@ -57,8 +66,9 @@ def main(out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "c2.npz"))
     pos, w = positions(N), weights(N)
     us = [u for _, u in zip(range(2 * N_ITER), rccsd.GlibcUniforms(SEED))]
     s = rccsd.default_settings(m_keep=M, level_shift=LEVEL_SHIFT)
-    rec = {k: [] for k in ("E", "normt", "idx_sha", "val_sha", "nD", "nsamp", "t1", "t2_at", "t2_sum", "t2_sumsq",
-                           "t2_wsum")}
+    rec = {k: [] for k in ("E", "normt", "idx_sha", "val_sha", "val_at", "nD", "nsamp", "t1", "t2_at", "t2_sum",
+                           "t2_sumsq", "t2_wsum")}
+    vpos = val_positions(M)
     uit = iter(us)
     ecc = rccsd.energy(t1, t2, eris)
     for k in range(N_ITER):
@@ -70,6 +80,7 @@ def main(out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "c2.npz"))
         rec["normt"].append(normt)
         rec["idx_sha"].append([sha(x.idx.astype("<u4")) for x in (ts, tsa)])
         rec["val_sha"].append([sha(x.val.astype("<f8")) for x in (ts, tsa)])
+        rec["val_at"].append([x.val[vpos].copy() for x in (ts, tsa)])
         rec["nD"].append([x.info["n_D"] for x in (ts, tsa)])
         rec["nsamp"].append([x.info["n_sampled"] for x in (ts, tsa)])
         rec["t1"].append(t1.copy())
@@ -83,7 +94,7 @@ def main(out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "c2.npz"))
     np.savez_compressed(
         out, nocc=NOCC, nvir=NVIR, naux=NAUX, seed=SEED, m_keep=M, level_shift=LEVEL_SHIFT, uniforms=np.array(us),
         emp2=emp2, mp2_t2_sha=sha(rccsd.init_amps(eris)[2].astype("<f8")), pos_seed=POS_SEED, n_pos=N_POS,
-        weight_seed=WEIGHT_SEED, **{k: np.array(v) for k, v in rec.items()})
+        weight_seed=WEIGHT_SEED, val_seed=VAL_SEED, n_val=N_VAL, **{k: np.array(v) for k, v in rec.items()})
     print(f"wrote {out} ({time.time() - t0:.1f} s)")
 
 

@@ -1,0 +1,115 @@
+"""The dense MFMA contraction (fricc_dense_contract) against plain PyTorch
+einsum in FP64 on the CPU: permuted views, broadcasts (stride 0), ragged
+sizes, split-K shapes (tiny output, long sum), mixed FP32/FP64 operands,
+beta accumulation, every tile shape, and the products of the amplitude
+update at small sizes.  Tolerance per element: FP64 compute 1e-12 and FP32
+compute 2e-6 of sum_k |a||b| (the magnitude the sum's rounding scales with)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    from fricc import dense
+
+    assert torch.cuda.is_available()
+    return torch, dense
+
+
+def _ref(torch, spec, A, B):
+    a, b = A.detach().cpu().double(), B.detach().cpu().double()
+    return torch.einsum(spec, a, b), torch.einsum(spec, a.abs(), b.abs())
+
+
+def _check(torch, dense, spec, A, B, compute, beta=0.0, out=None):
+    ref, mag = _ref(torch, spec, A, B)
+    C0 = None
+    if out is not None:
+        C0 = out.detach().cpu().double().clone()
+    got = dense.einsum(spec, A, B, out=out, alpha=0.75, beta=beta, compute=compute)
+    want = 0.75 * ref + (beta * C0 if beta else 0.0)
+    tol = (1e-12 if compute == torch.float64 else 2e-6) * (0.75 * mag + (abs(beta) * C0.abs() if beta else 0.0)) + 1e-300
+    err = (got.detach().cpu().double() - want).abs()
+    assert torch.all(err <= tol), (spec, float((err / tol).max()))
+
+
+SPECS = [
+    # (spec, A shape, B shape)
+    ("mk,kn->mn", (130, 70), (70, 33)),        # ragged edges
+    ("mk,kn->mn", (5, 20000), (20000, 7)),     # split K
+    ("bmk,bkn->bmn", (9, 40, 17), (9, 17, 65)),  # batch
+    ("klcd,ilcd->ki", (6, 5, 11, 13), (4, 5, 11, 13)),
+    ("klad,kldc->ac", (5, 4, 19, 23), (5, 4, 23, 21)),
+    ("kd,kcda->ac", (7, 9), (7, 12, 9, 10)),   # GEMV over a 4-index operand
+    ("jc,icab->ijab", (6, 9), (5, 9, 10, 11)),
+    ("iakj,kb->ijab", (5, 7, 6, 4), (6, 9)),   # three levels in m
+    ("ka,kijb->ijab", (6, 8), (6, 5, 4, 9)),
+    ("ijka,kb->ijab", (4, 5, 6, 7), (6, 8)),
+    ("ac,ijcb->ijab", (9, 10), (4, 5, 10, 9)),
+    ("ki,kjab->ijab", (6, 5), (6, 4, 7, 8)),
+    ("lkic,jc->klij", (5, 6, 7, 9), (8, 9)),
+    ("kcli,la->kcia", (4, 6, 5, 7), (5, 9)),
+]
+
+
+@pytest.mark.parametrize("spec,sa,sb", SPECS)
+@pytest.mark.parametrize("compute", ["fp64", "fp32"])
+def test_dense_matches_torch(env, spec, sa, sb, compute):
+    torch, dense = env
+    g = torch.Generator().manual_seed(hash(spec) % 1000)
+    cdt = torch.float64 if compute == "fp64" else torch.float32
+    A = torch.randn(sa, generator=g, dtype=torch.float64).cuda()
+    B = torch.randn(sb, generator=g, dtype=torch.float64).cuda()
+    _check(torch, dense, spec, A, B, cdt)
+    if compute == "fp32":
+        _check(torch, dense, spec, A.float(), B, cdt)  # mixed element types
+
+
+@pytest.mark.parametrize("compute", ["fp64", "fp32"])
+def test_dense_strided_views_and_broadcast(env, compute):
+    torch, dense = env
+    cdt = torch.float64 if compute == "fp64" else torch.float32
+    g = torch.Generator().manual_seed(3)
+    base = torch.randn(7, 9, 6, 5, generator=g, dtype=torch.float64).cuda()
+    A = base.permute(2, 0, 3, 1)  # (6, 7, 5, 9), no unit stride in the last axis
+    B = torch.randn(9, 11, generator=g, dtype=torch.float64).cuda()
+    _check(torch, dense, "pqrs,st->pqrt", A, B, cdt)
+    v = torch.randn(11, generator=g, dtype=torch.float64).cuda()
+    Bb = v.expand(13, 11)  # stride 0 rows
+    M = torch.randn(8, 13, generator=g, dtype=torch.float64).cuda()
+    _check(torch, dense, "mk,kn->mn", M, Bb, cdt)
+    out = torch.randn(8, 11, generator=g, dtype=cdt).cuda()
+    _check(torch, dense, "mk,kn->mn", M, Bb, cdt, beta=-1.5, out=out)
+    # output into a permuted view, accumulated
+    big = torch.randn(11, 8, generator=g, dtype=torch.float64).cuda()
+    _check(torch, dense, "mk,kn->mn", M, Bb, cdt, beta=1.0, out=big.t())
+
+
+def test_dense_every_tile_shape_and_split(env):
+    torch, dense = env
+    g = torch.Generator().manual_seed(5)
+    for m in (3, 40, 100, 300):
+        for n in (2, 50, 120, 260):
+            for k in (1, 33, 5000):
+                A = torch.randn(m, k, generator=g, dtype=torch.float64).cuda()
+                B = torch.randn(k, n, generator=g, dtype=torch.float64).cuda()
+                _check(torch, dense, "mk,kn->mn", A, B, torch.float64)
+
+
+def test_dense_rejects_bad_specs(env):
+    torch, dense = env
+    from fricc import _capi
+
+    A = torch.zeros(3, 4, dtype=torch.float64, device="cuda")
+    B = torch.zeros(4, 5, dtype=torch.float64, device="cuda")
+    out = torch.zeros(3, 5, dtype=torch.float64, device="cuda")
+    with pytest.raises(_capi.FriccValueError):
+        dense.einsum("mk,kn->mq", A, B, out=out)
+    with pytest.raises(_capi.FriccValueError):
+        dense.einsum("mk,jn->mn", A, B, out=out)  # j only in B
+    with pytest.raises(_capi.FriccValueError):
+        dense.einsum("mk,kn->mn", A, torch.zeros(5, 5, dtype=torch.float64, device="cuda"), out=out)

@@ -139,6 +139,32 @@ def test_contraction_bf16_split(py_rccsd, term):
     npt.assert_allclose(out, ref, rtol=0, atol=2e-4 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("term", sorted(CASES))
+def test_contraction_bf16_single_pass(py_rccsd, term):
+    """Single-pass BF16 (hi-only operand records): each operand rounded to
+    BF16 (via FP32, round to nearest even) and the products accumulated in
+    FP32, so the result equals the FP64 einsum of the rounded operands to
+    FP32 accumulation error."""
+    import torch
+
+    no, nv = 7, 19
+    wshape, spec, alpha, oshape = CASES[term]
+    rng = np.random.default_rng(13)
+    w = rng.standard_normal(wshape(no, nv))
+    t2 = rng.standard_normal((no, no, nv, nv))
+    T = py_rccsd.SparseTensor4d(t2, t2.shape, t2.size // 5, "largest")
+    dense = np.zeros(t2.size)
+    dense[T.indices()] = T.values()
+    bf = lambda a: torch.from_numpy(a).float().to(torch.bfloat16).double().numpy()  # noqa: E731
+    ref = alpha * np.einsum(spec, bf(w), bf(dense).reshape(t2.shape))
+    out = np.zeros(oshape(no, nv))
+    py_rccsd.contract_DTSpT(w, T, out, term, precision="bf16x1")
+    npt.assert_allclose(out, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    # and it is a BF16 result: far from the FP64 one
+    exact = alpha * np.einsum(spec, w, dense.reshape(t2.shape))
+    assert np.abs(out - exact).max() > 1e-4 * np.abs(exact).max()
+
+
 def test_strided_views(py_rccsd):
     """W and ouput may be arbitrary strided numpy views (src/sparse_tensor.cpp:353-354)."""
     no, nv = 5, 9

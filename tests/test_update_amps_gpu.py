@@ -77,14 +77,16 @@ def test_update_amps_rccsd_t1t1_mode(env):
     assert np.abs(dt2.cpu().numpy() - rt2).max() / np.abs(rt2).max() < 1e-10
 
 
-@pytest.mark.parametrize("prec,tol,etol", [("fp32", 5e-6, 1e-6), ("mixed", 2e-5, 1e-5)])
+@pytest.mark.parametrize("prec,tol,etol", [("fp32", 5e-6, 1e-6), ("mixed", 2e-5, 1e-5), ("bf16-ladder", 5e-3, 1e-3)])
 def test_update_amps_precision_modes(env, prec, tol, etol):
     """Same T2, same uniforms: the precision modes stay within their stated
     relative error of the FP64 step (FP32: FP32 operands and accumulation;
     mixed: BF16 hi/lo MFMA for the sparse terms, about 16 significant bits
-    per product, FP32 dense GEMMs).  On this 6 x 24 system the few terms per
-    element do not average the product errors out, so the energy bound is
-    10 ppm of E_corr; at o=32, v=256 bench.py measures 2e-8 Ha."""
+    per product, FP32 dense GEMMs; bf16-ladder: the ladder's products in BF16,
+    8 significant bits, everything else FP32).  On this 6 x 24 system the few
+    terms per element do not average the product errors out, so the bounds
+    are loose; bench.py reports each mode's per-term and energy error at
+    o=32, v=256."""
     torch = env[0]
     o, v, naux = 6, 24, 40
     us = [0.42, 0.58]
@@ -96,6 +98,7 @@ def test_update_amps_precision_modes(env, prec, tol, etol):
     a1, a2 = cc64.update_amps(t1, t2, eris)
     b1, b2 = ccm.update_amps(t1, t2, erism)
     rel = (b2 - a2).abs().max().item() / a2.abs().max().item()
+    print(f"{prec}: t2 max rel error {rel:.3g}", flush=True)
     assert rel < tol, rel
     assert np.array_equal(cc64._slots[0].indices(), ccm._slots[0].indices())
     de = abs(cc64.energy(a1, a2, eris) - ccm.energy(b1, b2, erism))
