@@ -74,6 +74,7 @@ struct DArgs {
   void* partial;
   int a64, b64, c64;
   int a_kfast, b_kfast;  // the tile load walks k fastest (else m / n fastest)
+  int c_vec;             // C rows hold runs of 4 consecutive, 16-byte aligned columns
   Lvl Ab, Am, Ak, Bb, Bk, Bn, Cb, Cm, Cn;
   int64_t M, N, K, batch;
   int64_t ntm, ntn;
@@ -89,15 +90,48 @@ __device__ inline T ld_as(const void* p, int f64, int64_t off) {
   return f64 ? (T)((const double*)p)[off] : (T)((const float*)p)[off];
 }
 
-template <typename T, int BM, int BN>
+// V consecutive elements at off (16-byte aligned when V == 4: the host
+// checks that every stride but the unit one is a multiple of 4 and the base
+// is aligned), converted to T
+template <typename T, int V>
+__device__ inline void ld_vec(const void* p, int f64, int64_t off, T (&r)[V]) {
+  if constexpr (V == 4) {
+    if (f64) {
+      const double2 x = *(const double2*)((const double*)p + off);
+      const double2 y = *(const double2*)((const double*)p + off + 2);
+      r[0] = (T)x.x;
+      r[1] = (T)x.y;
+      r[2] = (T)y.x;
+      r[3] = (T)y.y;
+    } else {
+      const float4 x = *(const float4*)((const float*)p + off);
+      r[0] = (T)x.x;
+      r[1] = (T)x.y;
+      r[2] = (T)x.z;
+      r[3] = (T)x.w;
+    }
+  } else {
+    r[0] = ld_as<T>(p, f64, off);
+  }
+}
+
+// VA / VB: elements per load along the unit-stride index of A / B (4 or 1);
+// kfast flags pick which index that is (k, else m / n).
+template <typename T, int BM, int BN, int VA, int VB>
 __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
   constexpr int PA = BM + (sizeof(T) == 4 ? 16 : 0);  // LDS pitches: the two k rows one fragment read
   constexpr int PB = BN + (sizeof(T) == 4 ? 16 : 0);  // touches fall on different banks
-  constexpr int EA = BM * DBK / 256, EB = BN * DBK / 256;
+  constexpr int NGA = BM * DBK / VA, NGB = BN * DBK / VB;  // load groups of a tile
+  constexpr int GA = (NGA + 255) / 256, GB = (NGB + 255) / 256;
   constexpr int MB = BM / 32, NB = BN / 32;  // 16 x 16 blocks per wave
   typedef typename Acc<T>::type AT;
-  __shared__ T sA[2][DBK][PA];
-  __shared__ T sB[2][DBK][PB];
+  constexpr int WN = BN / 2, EP = WN + 4;  // epilogue: a wave's 16-row slab, padded
+  constexpr int SA_BYTES = 2 * DBK * PA * (int)sizeof(T), SB_BYTES = 2 * DBK * PB * (int)sizeof(T);
+  static_assert(4 * 16 * EP * (int)sizeof(T) <= SA_BYTES + SB_BYTES, "epilogue slabs must fit the staging LDS");
+  __shared__ __align__(16) unsigned char smem[SA_BYTES + SB_BYTES];
+  T(*sA)[DBK][PA] = (T(*)[DBK][PA])smem;
+  T(*sB)[DBK][PB] = (T(*)[DBK][PB])(smem + SA_BYTES);
+  __shared__ int64_t cmoff[BM], cnoff[BN];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int64_t tm = (int64_t)blockIdx.x % a.ntm, tn = (int64_t)blockIdx.x / a.ntm;
@@ -106,47 +140,90 @@ __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kbeg = (int64_t)sp * a.kchunk, kend = min(a.K, kbeg + a.kchunk);
   const int64_t abase = lvl_off(a.Ab, b), bbase = lvl_off(a.Bb, b);
+  for (int t = tid; t < BM; t += 256) cmoff[t] = (m0 + t < a.M) ? lvl_off(a.Cm, m0 + t) : 0;
+  for (int t = tid; t < BN; t += 256) cnoff[t] = (n0 + t < a.N) ? lvl_off(a.Cn, n0 + t) : 0;
 
-  // staging coordinates of this thread's elements
-  int amm[EA], akk[EA], bnn[EB], bkk[EB];
-  int64_t amoff[EA], bnoff[EB];
-  bool amok[EA], bnok[EB];
+  // staging coordinates of this thread's load groups: (row index mm / nn,
+  // first k) of V elements along the unit-stride index
+  int amm[GA], akk[GA], bnn[GB], bkk[GB];
+  int64_t amoff[GA], bnoff[GB];
+  bool amok[GA], bnok[GB];
 #pragma unroll
-  for (int j = 0; j < EA; j++) {
-    const int e = tid + 256 * j;
-    amm[j] = a.a_kfast ? e / DBK : e % BM;
-    akk[j] = a.a_kfast ? e % DBK : e / BM;
+  for (int j = 0; j < GA; j++) {
+    const int g = tid + 256 * j;
+    const bool in = g < NGA;
+    if (a.a_kfast) {
+      amm[j] = g / (DBK / VA);
+      akk[j] = (g % (DBK / VA)) * VA;
+    } else {
+      amm[j] = (g % (BM / VA)) * VA;
+      akk[j] = g / (BM / VA);
+    }
     const int64_t m = m0 + amm[j];
-    amok[j] = m < a.M;
+    amok[j] = in && m < a.M;
     amoff[j] = amok[j] ? abase + lvl_off(a.Am, m) : 0;
   }
 #pragma unroll
-  for (int j = 0; j < EB; j++) {
-    const int e = tid + 256 * j;
-    bnn[j] = a.b_kfast ? e / DBK : e % BN;
-    bkk[j] = a.b_kfast ? e % DBK : e / BN;
+  for (int j = 0; j < GB; j++) {
+    const int g = tid + 256 * j;
+    const bool in = g < NGB;
+    if (a.b_kfast) {
+      bnn[j] = g / (DBK / VB);
+      bkk[j] = (g % (DBK / VB)) * VB;
+    } else {
+      bnn[j] = (g % (BN / VB)) * VB;
+      bkk[j] = g / (BN / VB);
+    }
     const int64_t n = n0 + bnn[j];
-    bnok[j] = n < a.N;
+    bnok[j] = in && n < a.N;
     bnoff[j] = bnok[j] ? bbase + lvl_off(a.Bn, n) : 0;
   }
-  T ra[EA], rb[EB];
+  T ra[GA][VA], rb[GB][VB];
   auto gload = [&](int64_t k0) {
 #pragma unroll
-    for (int j = 0; j < EA; j++) {
+    for (int j = 0; j < GA; j++) {
       const int64_t k = k0 + akk[j];
-      ra[j] = (amok[j] && k < kend) ? ld_as<T>(a.A, a.a64, amoff[j] + lvl_off(a.Ak, k)) : (T)0;
+      if (amok[j] && k < kend) {
+        ld_vec<T, VA>(a.A, a.a64, amoff[j] + lvl_off(a.Ak, k), ra[j]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < VA; i++) ra[j][i] = (T)0;
+      }
     }
 #pragma unroll
-    for (int j = 0; j < EB; j++) {
+    for (int j = 0; j < GB; j++) {
       const int64_t k = k0 + bkk[j];
-      rb[j] = (bnok[j] && k < kend) ? ld_as<T>(a.B, a.b64, bnoff[j] + lvl_off(a.Bk, k)) : (T)0;
+      if (bnok[j] && k < kend) {
+        ld_vec<T, VB>(a.B, a.b64, bnoff[j] + lvl_off(a.Bk, k), rb[j]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < VB; i++) rb[j][i] = (T)0;
+      }
     }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int j = 0; j < EA; j++) sA[buf][akk[j]][amm[j]] = ra[j];
+    for (int j = 0; j < GA; j++) {
+      if (tid + 256 * j >= NGA) continue;
 #pragma unroll
-    for (int j = 0; j < EB; j++) sB[buf][bkk[j]][bnn[j]] = rb[j];
+      for (int i = 0; i < VA; i++) {
+        if (a.a_kfast)
+          sA[buf][akk[j] + i][amm[j]] = ra[j][i];
+        else
+          sA[buf][akk[j]][amm[j] + i] = ra[j][i];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < GB; j++) {
+      if (tid + 256 * j >= NGB) continue;
+#pragma unroll
+      for (int i = 0; i < VB; i++) {
+        if (a.b_kfast)
+          sB[buf][bkk[j] + i][bnn[j]] = rb[j][i];
+        else
+          sB[buf][bkk[j]][bnn[j] + i] = rb[j][i];
+      }
+    }
   };
 
   AT acc[MB][NB];
@@ -181,33 +258,88 @@ __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
       __syncthreads();
       cur ^= 1;
     }
+  } else {
+    __syncthreads();  // the offset tables
   }
   // epilogue: lane holds column lr of each 16 x 16 block, rows Acc<T>::row(lk, i)
   const int64_t cbase = lvl_off(a.Cb, b);
+  if (a.nsplit > 1) {
 #pragma unroll
-  for (int j = 0; j < NB; j++) {
-    const int64_t n = n0 + nw + 16 * j + lr;
-    if (n >= a.N) continue;
-    const int64_t cn = a.nsplit > 1 ? 0 : lvl_off(a.Cn, n);
+    for (int j = 0; j < NB; j++) {
+      const int64_t n = n0 + nw + 16 * j + lr;
+      if (n >= a.N) continue;
 #pragma unroll
-    for (int i = 0; i < MB; i++)
+      for (int i = 0; i < MB; i++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t m = m0 + mw + 16 * i + Acc<T>::row(lk, r);
-        if (m >= a.M) continue;
-        const T v = acc[i][j][r];
-        if (a.nsplit > 1) {
-          ((T*)a.partial)[(((int64_t)sp * a.batch + b) * a.M + m) * a.N + n] = v;
+        for (int r = 0; r < 4; r++) {
+          const int64_t m = m0 + mw + 16 * i + Acc<T>::row(lk, r);
+          if (m < a.M) ((T*)a.partial)[(((int64_t)sp * a.batch + b) * a.M + m) * a.N + n] = acc[i][j][r];
+        }
+    }
+    return;
+  }
+  // Each wave's 16-row slab of the tile goes through LDS so that lanes then
+  // read and write C in runs of 4 consecutive columns (16 consecutive lanes
+  // cover a slab row): coalesced, and 16-byte accesses where C allows.
+  T(*sE)[16][EP] = (T(*)[16][EP])smem;
+  constexpr int CH = 16 * WN / 4;  // 4-column chunks of a slab
+#pragma unroll
+  for (int i = 0; i < MB; i++) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) sE[w][Acc<T>::row(lk, r)][16 * j + lr] = acc[i][j][r];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < (CH + 63) / 64; q++) {
+      const int c = lane + 64 * q;
+      if (c >= CH) break;
+      const int row = c / (WN / 4), col = (c % (WN / 4)) * 4;
+      const int ml = mw + 16 * i + row;
+      if (m0 + ml >= a.M) continue;
+      const int nl = nw + col;
+      const int64_t rowoff = cbase + cmoff[ml];
+      double o[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) o[e] = a.alpha * (double)sE[w][row][col + e];
+      if (a.c_vec && n0 + nl < a.N) {
+        const int64_t off = rowoff + cnoff[nl];
+        if (a.c64) {
+          double2* p = (double2*)((double*)a.C + off);
+          if (a.beta != 0.0) {
+            const double2 x = p[0], y = p[1];
+            o[0] += a.beta * x.x;
+            o[1] += a.beta * x.y;
+            o[2] += a.beta * y.x;
+            o[3] += a.beta * y.y;
+          }
+          p[0] = make_double2(o[0], o[1]);
+          p[1] = make_double2(o[2], o[3]);
         } else {
-          const int64_t off = cbase + lvl_off(a.Cm, m) + cn;
-          double o = a.alpha * (double)v;
-          if (a.beta != 0.0) o += a.beta * (a.c64 ? ((const double*)a.C)[off] : (double)((const float*)a.C)[off]);
+          float4* p = (float4*)((float*)a.C + off);
+          if (a.beta != 0.0) {
+            const float4 x = *p;
+            o[0] += a.beta * x.x;
+            o[1] += a.beta * x.y;
+            o[2] += a.beta * x.z;
+            o[3] += a.beta * x.w;
+          }
+          *p = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          if (n0 + nl + e >= a.N) break;
+          const int64_t off = rowoff + cnoff[nl + e];
+          if (a.beta != 0.0) o[e] += a.beta * (a.c64 ? ((const double*)a.C)[off] : (double)((const float*)a.C)[off]);
           if (a.c64)
-            ((double*)a.C)[off] = o;
+            ((double*)a.C)[off] = o[e];
           else
-            ((float*)a.C)[off] = (float)o;
+            ((float*)a.C)[off] = (float)o[e];
         }
       }
+    }
   }
 }
 
@@ -305,16 +437,23 @@ static Lvl make_lvl(const std::vector<Letter>& lv, int op, int64_t* extent) {
 }
 
 template <typename T, int BM, int BN>
-static void launch_tile(const DArgs& a, hipStream_t s) {
+static void launch_tile(const DArgs& a, int va, int vb, hipStream_t s) {
   dim3 grid((unsigned)(a.ntm * a.ntn), (unsigned)(a.batch * a.nsplit));
-  k_dgemm<T, BM, BN><<<grid, 256, 0, s>>>(a);
+  if (va == 4 && vb == 4)
+    k_dgemm<T, BM, BN, 4, 4><<<grid, 256, 0, s>>>(a);
+  else if (va == 4)
+    k_dgemm<T, BM, BN, 4, 1><<<grid, 256, 0, s>>>(a);
+  else if (vb == 4)
+    k_dgemm<T, BM, BN, 1, 4><<<grid, 256, 0, s>>>(a);
+  else
+    k_dgemm<T, BM, BN, 1, 1><<<grid, 256, 0, s>>>(a);
 }
 
 template <typename T>
-static int launch_dgemm(DArgs a, int bm, int bn, hipStream_t s) {
+static int launch_dgemm(DArgs a, int bm, int bn, int va, int vb, hipStream_t s) {
 #define FRICC_TILE(X, Y)                \
   if (bm == X && bn == Y) {             \
-    launch_tile<T, X, Y>(a, s);         \
+    launch_tile<T, X, Y>(a, va, vb, s); \
     return OK;                          \
   }
   FRICC_TILE(128, 128)
@@ -433,11 +572,28 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
   const int bm = pick_tile(M), bnt = pick_tile(N);
   a.ntm = ceil_div(M, bm);
   a.ntn = ceil_div(N, bnt);
-  // split K when the output grid cannot fill the device
+  // 16-byte loads along the unit-stride index of an operand when every group
+  // of 4 elements there stays in one level and starts 16-byte aligned
+  auto mult4 = [](const Lvl& d) { return d.s1 % 4 == 0 && d.s2 % 4 == 0; };
+  auto vec_ok = [&](const void* p, const Lvl& fast, const Lvl& other, const Lvl& bat) {
+    return fast.s0 == 1 && fast.L0 % 4 == 0 && mult4(fast) && other.s0 % 4 == 0 && mult4(other) &&
+           bat.s0 % 4 == 0 && mult4(bat) && ((uintptr_t)p % 16) == 0;
+  };
+  const int va = a.a_kfast ? (vec_ok(A, a.Ak, a.Am, a.Ab) ? 4 : 1) : (vec_ok(A, a.Am, a.Ak, a.Ab) ? 4 : 1);
+  const int vb = a.b_kfast ? (vec_ok(B, a.Bk, a.Bn, a.Bb) ? 4 : 1) : (vec_ok(B, a.Bn, a.Bk, a.Bb) ? 4 : 1);
+  a.c_vec = (a.Cn.s0 == 1 && a.Cn.L0 % 4 == 0 && mult4(a.Cn) && a.Cm.s0 % 4 == 0 && mult4(a.Cm) &&
+             a.Cb.s0 % 4 == 0 && mult4(a.Cb) && ((uintptr_t)C % 16) == 0)
+                ? 1
+                : 0;
+  // split K when the output grid cannot fill the device (memory-bound
+  // products need several blocks per CU in flight)
   const int64_t tiles = a.ntm * a.ntn * nb;
   int nsplit = 1;
-  if (tiles < 512 && K >= 8 * DBK) {
-    nsplit = (int)std::min<int64_t>(std::min<int64_t>(ceil_div(1024, tiles), K / (4 * DBK)), 256);
+  if (tiles < 1024 && K >= 8 * DBK) {
+    // about 2048 blocks; tiny outputs (a few tiles, a very long sum) may take
+    // more splits, their partial sums being small
+    const int64_t cap = M * N <= 4096 ? 1024 : 256;
+    nsplit = (int)std::min<int64_t>(std::min<int64_t>(ceil_div(2048, tiles), K / (4 * DBK)), cap);
     nsplit = std::max(nsplit, 1);
   }
   a.kchunk = ceil_div(ceil_div(K, nsplit), DBK) * DBK;
@@ -452,9 +608,9 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
   }
   if (M == 0 || N == 0 || nb == 0) return OK;
   if (compute64)
-    FRICC_TRY(launch_dgemm<double>(a, bm, bnt, s));
+    FRICC_TRY(launch_dgemm<double>(a, bm, bnt, va, vb, s));
   else
-    FRICC_TRY(launch_dgemm<float>(a, bm, bnt, s));
+    FRICC_TRY(launch_dgemm<float>(a, bm, bnt, va, vb, s));
   if (a.nsplit > 1) {
     const int64_t total = nb * M * N;
     const unsigned g = (unsigned)std::min<int64_t>(ceil_div(total, 256), 4096);

@@ -384,11 +384,12 @@ def _update_amps_device(cc, t1, t2, eris):
     E("id,kcda->kcia", t1, P["P1"], out=A_voov, beta=1.0)
     A_vovo = P["oovv_p"].clone()
     E("id,kcda->kcia", t1, P["P2"], out=A_vovo, beta=1.0)
-    # - sum_l (kcli + sum_d ldkc t1[i,d]) t1[l,a];  - sum_l (lcki + sum_d lckd t1[i,d]) t1[l,a]
-    E("kcli,la->kcia", P["ovoo"], t1, out=A_voov, alpha=-1.0, beta=1.0)
-    E("kcli,la->kcia", U, t1, out=A_voov, alpha=-1.0, beta=1.0)
-    E("lcki,la->kcia", P["ovoo"], t1, out=A_vovo, alpha=-1.0, beta=1.0)
-    E("lcki,la->kcia", U, t1, out=A_vovo, alpha=-1.0, beta=1.0)
+    # - sum_l (kcli + sum_d ldkc t1[i,d]) t1[l,a];  - sum_l (lcki + sum_d lckd t1[i,d]) t1[l,a]:
+    # both read N1[p,c,q,i] = (pc|qi) + U[p,c,q,i], with (p,q) = (k,l) and (l,k)
+    N1 = P["ovoo"] + U
+    E("kcli,la->kcia", N1, t1, out=A_voov, alpha=-1.0, beta=1.0)
+    E("lcki,la->kcia", N1, t1, out=A_vovo, alpha=-1.0, beta=1.0)
+    del N1
     Wvoov = A_voov.permute(3, 0, 2, 1)  # [a,k,i,c] view
     Wvovo = A_vovo.permute(3, 0, 1, 2)  # [a,k,c,i] view
     if "O^3V^3" in ccs:
@@ -461,8 +462,9 @@ def _update_amps_device(cc, t1, t2, eris):
     Mt = E("kibc,jc->kijb", P["oovv"], t1)  # sum_c (ki|bc) t1[j,c]
     E("ka,kijb->ijab", t1, Mt, out=S, alpha=-1.0, beta=1.0)
     # - (sum_c (ia|kc) t1[j,c] + ovoo[i,a,j,k]) t1[k,b];  sum_c (ia|kc) t1[j,c] = U[i,a,k,j]
-    E("iakj,kb->ijab", U, t1, out=S, alpha=-1.0, beta=1.0)
-    E("iajk,kb->ijab", P["ovoo"], t1, out=S, alpha=-1.0, beta=1.0)
+    T2p = U + P["ovoo"].permute(0, 1, 3, 2)  # [i,a,k,j]
+    E("iakj,kb->ijab", T2p, t1, out=S, alpha=-1.0, beta=1.0)
+    del T2p
     # + Lvv[a,c] t2[i,j,c,b] - Loo[k,i] t2[k,j,a,b]
     E("ac,ijcb->ijab", Lvv, t2c, out=S, beta=1.0)
     E("ki,kjab->ijab", Loo, t2c, out=S, alpha=-1.0, beta=1.0)
