@@ -258,8 +258,13 @@ class Solve:
 
 
 # Whether solves' device work may run side by side: every kernel a step
-# launches must finish without waiting for other workgroups of its grid.
-OVERLAP_SAFE = False
+# launches must finish without waiting for other workgroups of its grid.  The
+# step's kernels are the library's own (compression, gathered and dense
+# contractions, finalize) and PyTorch's elementwise and reduction kernels;
+# none of them waits for another workgroup.  (Vendor stream-K GEMMs, whose
+# workgroups wait for partial tiles of their own grid, stalled side-by-side
+# replays when the dense remainder used them.)
+OVERLAP_SAFE = True
 
 
 class DeviceOrder:
@@ -306,6 +311,10 @@ class Server:
     (a run of several production-size solves side by side stopped making
     progress).
 
+    Side by side (``overlap``) every kernel of a step must make progress with
+    other solves' kernels resident on the device: all of them do (no kernel
+    waits for workgroups of its own grid; ``OVERLAP_SAFE``).
+
     A request is admitted while (a) the admitted solves' footprints plus its
     own stay within ``hbm_budget``, (b) the device still has room for its
     footprint and set-up transients and (c) it needs at most half the budget
@@ -322,10 +331,8 @@ class Server:
         if policy not in ("srpt", "rr"):
             raise ValueError(f"unknown policy {policy!r} (srpt, rr)")
         if overlap and not OVERLAP_SAFE:
-            raise ValueError("Server(overlap=True) is disabled: the dense remainder's vendor GEMMs (stream-K "
-                             "kernels whose workgroups wait for each other) stopped making progress when solves "
-                             "ran side by side; one solve's device work at a time (overlap=False) is the safe "
-                             "order")
+            raise ValueError("Server(overlap=True) is disabled: a step's kernels may wait for other workgroups "
+                             "of their grid, which side-by-side replays can starve")
         self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision

@@ -101,14 +101,33 @@ def test_srpt_second_slot_never_delays_the_shorter_solve():
     assert sv.Server(policy="rr")._may_follow(long_, [short])
 
 
-def test_overlap_refused_while_unsafe():
-    """Side-by-side replay stays off while a step's kernels may wait for
-    workgroups of their own grid (the vendor stream-K GEMMs)."""
+def test_overlap_refused_while_unsafe(monkeypatch):
+    """Side-by-side replay is refused whenever a step's kernels may wait for
+    workgroups of their own grid (OVERLAP_SAFE off), and allowed otherwise."""
     import pytest
 
-    if not sv.OVERLAP_SAFE:
-        with pytest.raises(ValueError, match="overlap"):
-            sv.Server(overlap=True)
+    monkeypatch.setattr(sv, "OVERLAP_SAFE", False)
+    with pytest.raises(ValueError, match="overlap"):
+        sv.Server(overlap=True)
+    monkeypatch.setattr(sv, "OVERLAP_SAFE", True)
+    assert sv.Server(overlap=True).overlap
+
+
+def test_step_uses_no_vendor_gemm():
+    """The amplitude updates call no PyTorch matrix product (which would run a
+    vendor BLAS kernel): every dense product goes through fricc.dense."""
+    import inspect
+    import re
+
+    from fricc import friccsd, frilccsd
+
+    for mod in (friccsd, frilccsd):
+        src = inspect.getsource(mod)
+        for pat in (r"torch\.(matmul|bmm|baddbmm|addmm|einsum|mm|mv)\(", r"\.(matmul|bmm|baddbmm_?|addmm_?|mv)\(",
+                    r"\w\)? @ \w"):
+            bad = [line for line in src.splitlines() if re.search(pat, line.split("#")[0])
+                   and not line.strip().startswith(('"', "'"))]
+            assert not bad, (mod.__name__, pat, bad[:3])
 
 
 def test_solve_failure_classification():

@@ -28,12 +28,7 @@ def _requests(sv):
     return reqs
 
 
-_OVERLAP_SKIP = pytest.mark.skip(reason="Server(overlap=True) is disabled while the dense remainder runs vendor "
-                                        "stream-K GEMMs: side-by-side replays stopped making progress twice")
-
-
-@pytest.mark.parametrize("overlap,policy", [(False, "srpt"), (False, "rr"),
-                                            pytest.param(True, "rr", marks=_OVERLAP_SKIP)])
+@pytest.mark.parametrize("overlap,policy", [(False, "srpt"), (False, "rr"), (True, "rr")])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_alone_equals_in_the_mix(serving, prec, overlap, policy):
     sv = serving
@@ -131,6 +126,27 @@ def test_one_solves_work_at_a_time(serving):
     ids = [rid for _, _, rid in iv]
     runs = 1 + sum(a != b for a, b in zip(ids, ids[1:]))
     assert runs > len(set(ids))
+
+
+def test_c5_sizes_side_by_side(serving):
+    """Three c5 sizes (10x90, 20x180, 32x256) replayed side by side on the
+    device (overlap), several solves at once: every solve completes and its
+    energies equal those it has alone."""
+    import torch
+
+    sv = serving
+    sizes = [(10, 90), (20, 180), (32, 256), (20, 180), (10, 90), (32, 256)]
+    reqs = [sv.Request(i, 0.0, o, v, m=int(0.03 * o * o * v * v), seed=700 + i, sys_seed=1, max_cycle=4, tol=0.0)
+            for i, (o, v) in enumerate(sizes)]
+    server = sv.Server(hbm_budget=240e9, precision="mixed", max_active=8, overlap=True)
+    res = server.run(reqs)
+    assert [r.iterations for r in res] == [4] * len(reqs) and not any(r.error for r in res)
+    assert server.max_concurrent >= 3
+    torch.cuda.empty_cache()
+    for r, q in zip(res, reqs):
+        assert r.energies == sv.solve_alone(q, precision="mixed"), (q.nocc, q.nvir)
+        torch.cuda.empty_cache()
+    print(f"c5 side by side: makespan {server.makespan:.2f} s, max concurrent {server.max_concurrent}")
 
 
 def test_c5_sizes_alone_equal_in_the_mix(serving):
