@@ -444,7 +444,7 @@ def cpu_model():
 def summarize_profile(path):
     """Markdown tables from a rocprofv3 --kernel-trace --stats output
     directory (its *kernel_stats.csv): per kernel the calls, total and average
-    duration, sorted by total time."""
+    duration, the 30 longest by total time; then per-iteration tables."""
     import csv
     import glob
 
@@ -454,14 +454,50 @@ def summarize_profile(path):
     rows = list(csv.DictReader(open(files[0])))
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
     out = ["| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
-    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:30]:
         name = r["Name"]
         name = name if len(name) <= 90 else name[:87] + "..."
         out.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                    f"{float(r['AverageNs']) / 1e3:.1f} | {100 * float(r['TotalDurationNs']) / tot:.1f} |")
     out.append(f"\nTotal kernel time {tot / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches "
                f"({os.path.basename(files[0])}).")
+    out += step_breakdown(path)
     return "\n".join(out)
+
+
+def step_breakdown(path, top=16):
+    """Per-iteration kernel tables from the same directory's *kernel_trace.csv:
+    an iteration is the dispatches after one k_t2_finalize (and its partial
+    sums) up to and including the next one's partial sums -- the two
+    compressions, the views, every contraction, the dense remainder and the
+    fused finalize."""
+    import collections
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(path, "**", "*kernel_trace.csv"), recursive=True))
+    if not files:
+        return []
+    rows = sorted(csv.DictReader(open(files[0])), key=lambda r: int(r["Start_Timestamp"]))
+    fin = [i for i, r in enumerate(rows) if "k_t2_finalize" in r["Kernel_Name"]]
+    out = []
+    for w in range(1, len(fin)):
+        seg = rows[fin[w - 1] + 2:fin[w] + 2]
+        wall = (int(seg[-1]["End_Timestamp"]) - int(seg[0]["Start_Timestamp"])) / 1e6
+        tot = collections.defaultdict(lambda: [0, 0])
+        for r in seg:
+            k = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            k = k if len(k) <= 70 else k[:67] + "..."
+            tot[k][0] += 1
+            tot[k][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        busy = sum(v[1] for v in tot.values()) / 1e6
+        out.append(f"\n### Iteration {w + 1} of the trace: {len(seg)} dispatches, kernel time {busy:.2f} ms, "
+                   f"wall {wall:.2f} ms\n")
+        out.append("| ms | calls | kernel |")
+        out.append("|---:|---:|---|")
+        for k, (cnt, d) in sorted(tot.items(), key=lambda x: -x[1][1])[:top]:
+            out.append(f"| {d / 1e6:.3f} | {cnt} | `{k}` |")
+    return out
 
 
 def main():
