@@ -195,6 +195,11 @@ int fricc_sample_systematic(const double* p, int64_t n, int64_t n_sample, double
   return sample_systematic(p, n, n_sample, u, out_idx, n_notfound, (hipStream_t)stream);
 }
 
+int fricc_sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* uniforms, double u,
+                         uint32_t* out_idx, void* stream) {
+  return sample_pivotal(p, n, n_sample, uniforms, u, out_idx, (hipStream_t)stream);
+}
+
 int fricc_sparse_create(const int64_t dims[4], fricc_sparse** out) {
   FRICC_CHECK(out != nullptr, "sparse_create: null output");
   int64_t n = 1;

@@ -23,6 +23,7 @@ enum Status {
   ENOMEM_ = 3,
   EPROB = 4,      // some p_i > 1 (the reference exits, src/fri_utils.cpp:383-395)
   ENONFINITE = 5, // non-finite entry in the vector to compress
+  ESAMPLE = 6,    // pivotal sampling ends with the wrong sample count (the reference exits)
 };
 
 #define FRICC_HIP(call)                                                          \
@@ -213,6 +214,37 @@ __device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* total, uint
   }
   __syncthreads();
   uint32_t ex = smem[w] + inc - v;
+  *total = smem[NT / WAVE];
+  __syncthreads();
+  return ex;
+}
+
+// Block-wide exclusive max-scan of u32 (0 before the first thread) for
+// blockDim.x == NT; writes the block maximum.
+template <int NT>
+__device__ inline uint32_t block_excl_max_u32(uint32_t v, uint32_t* total, uint32_t* smem /*NT/64+1*/) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    uint32_t t = __shfl_up(inc, d, WAVE);
+    if (lane >= d) inc = inc > t ? inc : t;
+  }
+  uint32_t ex = __shfl_up(inc, 1, WAVE);
+  if (lane == 0) ex = 0;
+  if (lane == WAVE - 1) smem[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < NT / WAVE; i++) {
+      uint32_t t = smem[i];
+      smem[i] = run;
+      run = run > t ? run : t;
+    }
+    smem[NT / WAVE] = run;
+  }
+  __syncthreads();
+  ex = ex > smem[w] ? ex : smem[w];
   *total = smem[NT / WAVE];
   __syncthreads();
   return ex;

@@ -14,6 +14,8 @@
 //   p_j     = (j in D) ? 0 : |x_j| / R_|D| * n'         n' = m - |D|
 //   P_j     = round(2^-96 * sum_{l<j} floor(p_l * 2^96))
 //   S_k     = j with P_j <= fl(k + u) < P_{j+1}, or 0 if no such j
+//             (systematic), or the pivotal chain's k-th emission (pivotal,
+//             src/fri_utils.cpp:63-127, on the exact prefix of FXP(p))
 //   output  = [sigma_0..sigma_{|D|-1}, S_0..S_{n'-1}],
 //             values [x_D, x_S / p_S]
 //
@@ -51,6 +53,9 @@ struct CompScalars {
   double u;        // the uniform of this call
   uint32_t ubad;   // u outside [0, 1]
   uint32_t dup;    // the output may repeat an index (two samples in one interval, or samples past the end)
+  uint64_t key;    // pivotal: SplitMix64 key of the step draws (the bits of u)
+  uint32_t pivbad; // pivotal: the chain ends with a sample count other than n'
+  uint32_t pad2;
 };
 
 // ---------------------------------------------------------------- pass A
@@ -460,7 +465,7 @@ __global__ void k_samp_tilesums(const double* __restrict__ x, const double* __re
   __shared__ u128 sm[NT / WAVE + 1];
   __shared__ double lds[LTILE];
   const CompScalars c = *sc;
-  if (c.n_sampled == 0) return;
+  if (c.n_sampled == 0 && pin == nullptr) return;
   const double nsd = (double)c.n_sampled;
   const int64_t tb = (int64_t)blockIdx.x * TILE;
   const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
@@ -473,7 +478,7 @@ __global__ void k_samp_tilesums(const double* __restrict__ x, const double* __re
     if (j < n) {
       const double v = tile_elem(lds, r);
       const double p = pin ? v : prob_of(v, (uint64_t)j, c, nsd);
-      bad |= !(p <= 1.0);  // also catches NaN (remaining norm 0, quirk A.6)
+      bad |= !(p >= 0.0 && p <= 1.0);  // also catches NaN (remaining norm 0, quirk A.6)
       s += fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
     }
   }
@@ -615,6 +620,274 @@ __global__ void k_samp_notfound(const double* __restrict__ x, const CompScalars*
   if (out_val) out_val[K + k] = x[0] / prob_of(x[0], 0, c, (double)c.n_sampled);
 }
 
+// ---------------------------------------------------------------- pivotal sampling
+// sample_pivotal (src/fri_utils.cpp:63-127).  The chain carries a candidate i
+// (initially 0) and a running mass a; step j (j >= 1, one uniform each)
+// either completes a unit of mass or not.  Not completing: i = j with
+// probability p_j / a.  Completing (a drops by 1): with probability
+// (1 - p_j) / (1 - a) emit i and set i = j, else emit j.  The chain stops
+// after the step whose emission makes n'; if it reaches the last step
+// without that, a mass within 1e-5 of 1 emits the candidate.
+//
+// Canonical mass: Q_{j+1} = sum_{l<=j} FXP(p_l) exactly (units 2^-96); the
+// units completed through step j are c_j = min(j, floor(Q_{j+1})) and
+// a_j = round(Q_{j+1} - c_j).  So the mass at every step is a function of the
+// prefix alone, and so is every acceptance probability: all steps decide in
+// parallel.  The candidate before step j is the last accepted step before j
+// (or 0) -- a max-scan over positions -- and a completing step j writes its
+// emission at output position c_j - 1.  The stop step is the first j >= 1
+// with c_j = n' (c is non-decreasing in steps of at most 1).
+//
+// Step j's uniform is U[j] when an array is given, else the j-th SplitMix64
+// output seeded with the call's key (Steele, Lea and Flood, OOPSLA 2014).
+
+__device__ inline double splitmix_u(uint64_t key, int64_t j) {
+  uint64_t z = key + (uint64_t)(j + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+// Units completed through step j (cn) and before it (cp), and the mass a_j,
+// from the exact prefix before step j (Qp) and FXP(p_j) (F).
+__device__ inline void piv_step(u128 Qp, u128 F, int64_t j, int64_t& cp, int64_t& cn, double& a) {
+  const u128 Qn = Qp + F;
+  const int64_t fn = (int64_t)(Qn >> 96), fp = (int64_t)(Qp >> 96);
+  cn = fn < j ? fn : j;
+  cp = j == 0 ? 0 : (fp < j - 1 ? fp : j - 1);
+  a = u128_to_double(Qn - ((u128)cn << 96), -96);
+}
+
+__device__ inline bool piv_accept(double p, double a, bool cross, double uj) {
+  return cross ? uj < (1.0 - p) / (1.0 - a) : uj < p / a;
+}
+
+// pst[0]: the stop step (~0 when none), pst[1]: the last accepted step overall.
+__global__ void k_piv_reset(uint64_t* pst) {
+  if (threadIdx.x == 0) {
+    pst[0] = ~0ull;
+    pst[1] = 0;
+  }
+}
+
+// Per thread and per tile: the last accepted step (thmax, tmax); the stop step.
+template <bool HASU>
+__global__ __launch_bounds__(NT) void k_piv_mark(const double* __restrict__ x, const double* __restrict__ pin,
+                                                 const double* __restrict__ U, int64_t n,
+                                                 const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
+                                                 const u128* __restrict__ thsum, uint32_t* __restrict__ tmax,
+                                                 uint32_t* __restrict__ thmax, uint64_t* __restrict__ pst) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  __shared__ uint32_t sm32[NT / WAVE + 1];
+  __shared__ double lds[LTILE];
+  __shared__ double ulds[HASU ? LTILE : 1];
+  const CompScalars c = *sc;
+  if (c.n_sampled == 0 && pin == nullptr) return;
+  const double nsd = (double)c.n_sampled;
+  const int64_t ns = (int64_t)c.n_sampled;
+  const int64_t tb = (int64_t)blockIdx.x * TILE;
+  const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
+  u128 tot;
+  u128 pre = block_excl_scan_u128<NT>(thsum[(int64_t)blockIdx.x * NT + threadIdx.x], &tot, sm) + tsum[blockIdx.x];
+  stage_tile(pin ? pin : x, tb, n, lds);
+  if (HASU) stage_tile(U, tb, n, ulds);
+  uint32_t last = 0;
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const int64_t j = base + r;
+    if (j < n) {
+      const double v = tile_elem(lds, r);
+      const double p = pin ? v : prob_of(v, (uint64_t)j, c, nsd);
+      const u128 F = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
+      if (j >= 1) {
+        int64_t cp, cn;
+        double a;
+        piv_step(pre, F, j, cp, cn, a);
+        const double uj = HASU ? tile_elem(ulds, r) : splitmix_u(c.key, j);
+        if (piv_accept(p, a, cn > cp, uj)) last = (uint32_t)j;
+        if (cn == ns && (j == 1 || cp < ns)) pst[0] = (uint64_t)j;  // the one stop step
+      }
+      pre += F;
+    }
+  }
+  thmax[(int64_t)blockIdx.x * NT + threadIdx.x] = last;
+  uint32_t bmax;
+  (void)block_excl_max_u32<NT>(last, &bmax, sm32);
+  if (threadIdx.x == 0) tmax[blockIdx.x] = bmax;
+}
+
+// Exclusive max-scan of the tile maxima in place (one block); pst[1] = overall.
+constexpr int PSN = 1024, PSI = 8;
+__global__ __launch_bounds__(PSN) void k_piv_scan(uint32_t* __restrict__ tmax, int64_t nt,
+                                                  uint64_t* __restrict__ pst) {
+  __shared__ uint32_t sm32[PSN / WAVE + 1];
+  uint32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nt; b0 += (int64_t)PSN * PSI) {
+    uint32_t v[PSI];
+    uint32_t tm = 0;
+#pragma unroll
+    for (int k = 0; k < PSI; k++) {
+      const int64_t i = b0 + (int64_t)threadIdx.x * PSI + k;
+      v[k] = i < nt ? tmax[i] : 0u;
+      tm = tm > v[k] ? tm : v[k];
+    }
+    uint32_t chunk;
+    uint32_t run = block_excl_max_u32<PSN>(tm, &chunk, sm32);
+    run = run > carry ? run : carry;
+#pragma unroll
+    for (int k = 0; k < PSI; k++) {
+      const int64_t i = b0 + (int64_t)threadIdx.x * PSI + k;
+      if (i < nt) tmax[i] = run;
+      run = run > v[k] ? run : v[k];
+    }
+    carry = carry > chunk ? carry : chunk;
+  }
+  if (threadIdx.x == 0) pst[1] = carry;
+}
+
+// Emission: threads whose steps complete a unit (up to the stop step) walk
+// them with the candidate carried in from the max-scan.
+template <bool HASU>
+__global__ __launch_bounds__(NT) void k_piv_emit(const double* __restrict__ x, const double* __restrict__ pin,
+                                                 const double* __restrict__ U, int64_t n,
+                                                 const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
+                                                 const u128* __restrict__ thsum, const uint32_t* __restrict__ tmax,
+                                                 const uint32_t* __restrict__ thmax, const uint64_t* __restrict__ pst,
+                                                 uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  __shared__ uint32_t sm32[NT / WAVE + 1];
+  __shared__ double lds[LTILE];
+  __shared__ double ulds[HASU ? LTILE : 1];
+  const CompScalars c = *sc;
+  if (c.n_sampled == 0 && pin == nullptr) return;
+  const double nsd = (double)c.n_sampled;
+  const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
+  const uint64_t jb = pst[0];
+  const int64_t jstop = jb < (uint64_t)n ? (int64_t)jb : n - 1;  // last step taken
+  const int64_t tb = (int64_t)blockIdx.x * TILE;
+  const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
+  const u128 st = thsum[(int64_t)blockIdx.x * NT + threadIdx.x];
+  u128 tot;
+  u128 pre = block_excl_scan_u128<NT>(st, &tot, sm) + tsum[blockIdx.x];
+  // does a step in [max(base,1), min(base+ITEMS-1, jstop)] complete a unit?
+  bool work = false;
+  {
+    const int64_t j0 = base > 1 ? base : 1;
+    const int64_t jl = base + ITEMS - 1 < n - 1 ? base + ITEMS - 1 : n - 1;
+    const int64_t j1 = jl < jstop ? jl : jstop;
+    if (j0 <= j1) {
+      if (j1 < jl) {
+        work = true;  // the stop step lies in this thread's range
+      } else {
+        const int64_t fb = (int64_t)(pre >> 96), fe = (int64_t)((pre + st) >> 96);
+        const int64_t cb = base == 0 ? 0 : (fb < base - 1 ? fb : base - 1);
+        const int64_t ce = fe < jl ? fe : jl;
+        work = ce > cb;
+      }
+    }
+  }
+  if (!__syncthreads_or(work)) return;
+  uint32_t bmax;
+  uint32_t cand = block_excl_max_u32<NT>(thmax[(int64_t)blockIdx.x * NT + threadIdx.x], &bmax, sm32);
+  cand = cand > tmax[blockIdx.x] ? cand : tmax[blockIdx.x];
+  stage_tile(pin ? pin : x, tb, n, lds);
+  if (HASU) stage_tile(U, tb, n, ulds);
+  if (!work) return;
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const int64_t j = base + r;
+    if (j < n && j <= jstop) {
+      const double v = tile_elem(lds, r);
+      const double p = pin ? v : prob_of(v, (uint64_t)j, c, nsd);
+      const u128 F = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
+      if (j >= 1) {
+        int64_t cp, cn;
+        double a;
+        piv_step(pre, F, j, cp, cn, a);
+        const double uj = HASU ? tile_elem(ulds, r) : splitmix_u(c.key, j);
+        const bool cross = cn > cp;
+        const bool acc = piv_accept(p, a, cross, uj);
+        if (cross && cn - 1 < ns) {
+          const uint32_t e = acc ? cand : (uint32_t)j;
+          out_idx[K + cn - 1] = e;
+          if (out_val) {
+            const double xe = acc ? x[e] : v;
+            out_val[K + cn - 1] = xe / (acc ? prob_of(xe, e, c, nsd) : p);
+          }
+        }
+        if (acc) cand = (uint32_t)j;
+      }
+      pre += F;
+    }
+  }
+}
+
+// The end of the chain (one thread): with no stop step, the last step's mass
+// within 1e-5 of 1 emits the candidate; any other count than n' is an exit of
+// the reference (src/fri_utils.cpp:100-123).
+__global__ void k_piv_final(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
+                            CompScalars* sc, const u128* __restrict__ total, const uint64_t* __restrict__ pst,
+                            uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
+  const CompScalars c = *sc;
+  if (c.n_sampled == 0 && pin == nullptr) return;
+  const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
+  if (pst[0] < (uint64_t)n) return;  // the chain stopped with n' emissions
+  if (n == 1) {
+    if (ns != 0) sc->pivbad = 1u;
+    return;
+  }
+  const u128 Q = *total;
+  const int64_t fq = (int64_t)(Q >> 96);
+  const int64_t cl = fq < n - 1 ? fq : n - 1;
+  const double a = u128_to_double(Q - ((u128)cl << 96), -96);
+  if (!(fabs(a - 1.0) < 1e-5)) {
+    sc->pivbad = 1u;
+    return;
+  }
+  if (cl < ns) {
+    const uint32_t e = (uint32_t)pst[1];
+    out_idx[K + cl] = e;
+    if (out_val) out_val[K + cl] = x[e] / prob_of(x[e], e, c, (double)ns);
+  }
+  if (cl + 1 != ns) sc->pivbad = 1u;
+}
+
+// Sample positions [K, cap) start as index 0, value 0: a chain that ends
+// short of n' (reported as an error) leaves no position unwritten.
+__global__ void k_piv_clear(const CompScalars* __restrict__ sc, int64_t cap, uint32_t* __restrict__ out_idx,
+                            double* __restrict__ out_val) {
+  const int64_t k = (int64_t)sc->K + (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (k >= cap) return;
+  out_idx[k] = 0u;
+  if (out_val) out_val[k] = 0.0;
+}
+
+// The pivotal passes after the probability prefix (k_samp_tilesums and its
+// tile scan, which left ts exclusive and tot = the total); cap = K + n' at
+// most (m, or n_sample for the free function).
+static int pivotal_passes(const double* x, const double* pin, const double* U, int64_t n, int64_t cap,
+                          CompScalars* sc, const u128* ts, const u128* ths, const u128* tot, uint32_t* tmax,
+                          uint32_t* thmax, uint64_t* pst, uint32_t* out_idx, double* out_val, hipStream_t s) {
+  const int64_t nt = ceil_div(n, TILE);
+  if (cap > 0) k_piv_clear<<<ceil_div(cap, NT), NT, 0, s>>>(sc, cap, out_idx, out_val);
+  k_piv_reset<<<1, 64, 0, s>>>(pst);
+  if (U) {
+    k_piv_mark<true><<<nt, NT, 0, s>>>(x, pin, U, n, sc, ts, ths, tmax, thmax, pst);
+  } else {
+    k_piv_mark<false><<<nt, NT, 0, s>>>(x, pin, U, n, sc, ts, ths, tmax, thmax, pst);
+  }
+  k_piv_scan<<<1, PSN, 0, s>>>(tmax, nt, pst);
+  if (U) {
+    k_piv_emit<true><<<nt, NT, 0, s>>>(x, pin, U, n, sc, ts, ths, tmax, thmax, pst, out_idx, out_val);
+  } else {
+    k_piv_emit<false><<<nt, NT, 0, s>>>(x, pin, U, n, sc, ts, ths, tmax, thmax, pst, out_idx, out_val);
+  }
+  k_piv_final<<<1, 1, 0, s>>>(x, pin, n, sc, tot, pst, out_idx, out_val);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
 // D part of the output (or all of it for "largest").
 __global__ void k_emit_sorted(const double* __restrict__ x, const uint32_t* __restrict__ vals,
                               const CompScalars* __restrict__ sc, int64_t cnt_if_largest,
@@ -639,6 +912,7 @@ __global__ void k_init_scalars(CompScalars* sc, int64_t m, int64_t n_sampled, do
     const double u = u_dev ? *u_dev : u_val;
     z.ubad = (u >= 0.0 && u <= 1.0) ? 0u : 1u;
     z.u = z.ubad ? 0.0 : u;  // an out-of-range u is reported, never used
+    z.key = dbits(z.u);
     *sc = z;
   }
   if (hist)
@@ -683,6 +957,11 @@ int comp_scalars_status(const void* host_sc, CompressInfo* info) {
     set_error("compress: a sampling probability exceeds 1 (reference exits, src/fri_utils.cpp:383-395)");
     return EPROB;
   }
+  if (h.pivbad) {
+    set_error("compress: pivotal sampling ended with a sample count other than n' (reference exits, "
+              "src/fri_utils.cpp:100-123)");
+    return ESAMPLE;
+  }
   return OK;
 }
 
@@ -700,18 +979,21 @@ struct CompScratch {
   uint32_t* bh;     // per-tile digit histograms [256][nt]
   u128* ts;         // nt + 1 tile sums
   u128* ths;        // nt * NT per-thread sums
+  uint32_t* tmax;   // pivotal: nt last accepted steps per tile (then their exclusive max-scan)
+  uint32_t* thmax;  // pivotal: nt * NT last accepted steps per thread
+  uint64_t* pst;    // pivotal: stop step, last accepted step overall
 };
 
 // Byte offsets of the parts of CompScratch inside one buffer (256-byte
 // aligned parts), and the total.
 struct CompLayout {
-  size_t off[12];
+  size_t off[15];
   size_t total;
 };
 
 static CompLayout comp_layout(int64_t n) {
   const int64_t nt = ceil_div(n, TILE);
-  const size_t sz[12] = {sizeof(CompScalars),
+  const size_t sz[15] = {sizeof(CompScalars),
                          (size_t)(4096 + 256) * 4,
                          (size_t)(nt + 1) * 4,
                          (size_t)scan_u32_sums_words(nt + 1) * 4,
@@ -722,10 +1004,13 @@ static CompLayout comp_layout(int64_t n) {
                          (size_t)n * 4,
                          (size_t)nt * 256 * 4,
                          (size_t)(nt + 1) * sizeof(u128),
-                         (size_t)nt * NT * sizeof(u128)};
+                         (size_t)nt * NT * sizeof(u128),
+                         (size_t)(nt + 1) * 4,
+                         (size_t)nt * NT * 4,
+                         16};
   CompLayout L;
   size_t o = 0;
-  for (int i = 0; i < 12; i++) {
+  for (int i = 0; i < 15; i++) {
     L.off[i] = o;
     o += (sz[i] + 255) / 256 * 256;
   }
@@ -747,13 +1032,30 @@ static CompScratch comp_scratch_at(uint8_t* base, const CompLayout& L) {
   w.bh = (uint32_t*)(base + L.off[9]);
   w.ts = (u128*)(base + L.off[10]);
   w.ths = (u128*)(base + L.off[11]);
+  w.tmax = (uint32_t*)(base + L.off[12]);
+  w.thmax = (uint32_t*)(base + L.off[13]);
+  w.pst = (uint64_t*)(base + L.off[14]);
   return w;
 }
 
 // Workspace slots of a compression (the resample reads the first sample's
 // scalars and sums back from the same slots).
 enum { WS_SC = 0, WS_HIST = 1, WS_TCNT = 2, WS_K0 = 3, WS_V0 = 4, WS_K1 = 5, WS_V1 = 6, WS_SH = 7,
-       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11, WS_SC2 = 12, WS_SSUM = 13 };
+       WS_BH = 8, WS_TS = 9, WS_TOT = 10, WS_THS = 11, WS_SC2 = 12, WS_SSUM = 13, WS_TMAX = 14,
+       WS_THMAX = 15, WS_PST = 16 };
+
+// The pivotal passes' scratch (slots of the calling thread's arena).
+static int piv_scratch_ws(int64_t n, uint32_t** tmax, uint32_t** thmax, uint64_t** pst) {
+  const int64_t nt = ceil_div(n, TILE);
+  void *a, *b, *c;
+  FRICC_TRY(ws_get(WS_TMAX, (size_t)(nt + 1) * 4, &a));
+  FRICC_TRY(ws_get(WS_THMAX, (size_t)nt * NT * 4, &b));
+  FRICC_TRY(ws_get(WS_PST, 16, &c));
+  *tmax = (uint32_t*)a;
+  *thmax = (uint32_t*)b;
+  *pst = (uint64_t*)c;
+  return OK;
+}
 
 static int comp_scratch_ws(int64_t n, CompScratch* w) {
   const int64_t nt = ceil_div(n, TILE);
@@ -782,13 +1084,14 @@ static int comp_scratch_ws(int64_t n, CompScratch* w) {
   w->bh = (uint32_t*)p_sh;
   w->ts = (u128*)p_ts;
   w->ths = (u128*)p_ths;
-  return OK;
+  return piv_scratch_ws(n, &w->tmax, &w->thmax, &w->pst);
 }
 
 static int check_compress_args(int64_t n, int64_t m, int method, double u_val, const double* u_dev) {
   FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "compress: vector length must be in [1, 2^31)");
   FRICC_CHECK(m > 0 && m <= n, "compress: need 0 < m <= n (fricc/friccsd.py:68-72)");
-  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC, "compress: unknown compression method");
+  FRICC_CHECK(method == COMP_LARGEST || method == COMP_FRI_SYSTEMATIC || method == COMP_FRI_PIVOTAL,
+              "compress: unknown compression method");
   FRICC_CHECK(u_dev || (u_val >= 0.0 && u_val <= 1.0), "compress: uniform must lie in [0, 1]");
   return OK;
 }
@@ -841,12 +1144,17 @@ static int compress_pipeline(const double* x, int64_t n, int64_t m, int method, 
     k_dtest<<<ntm, NT, 0, s>>>(kA, m, sc, ts);
     k_dfinal<<<1, NT, 0, s>>>(kA, vA, m, sc, ts);
     k_emit_sorted<<<ceil_div(m, NT), NT, 0, s>>>(x, vA, sc, -1, out_idx, out_val);
-    // --- systematic sampling of the remainder
+    // --- sampling of the remainder: exact prefix of FXP(p), then emission
     k_samp_tilesums<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, w.ths);
     k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, w.tot);
-    k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, w.ths, out_idx, out_val);
-    k_samp_final<<<1, 1, 0, s>>>(sc, w.tot);
-    k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
+    if (method == COMP_FRI_PIVOTAL) {
+      FRICC_TRY(pivotal_passes(x, nullptr, nullptr, n, m, sc, ts, w.ths, w.tot, w.tmax, w.thmax, w.pst, out_idx,
+                               out_val, s));
+    } else {
+      k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, w.ths, out_idx, out_val);
+      k_samp_final<<<1, 1, 0, s>>>(sc, w.tot);
+      k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
+    }
   }
   if (export_sc) k_export_scalars<<<1, 64, 0, s>>>(sc, (CompScalars*)export_sc);
   FRICC_HIP(hipGetLastError());
@@ -871,6 +1179,8 @@ __global__ void k_resample_scalars(const CompScalars* __restrict__ src, const do
   const double u = *u_dev;
   c.ubad = (u >= 0.0 && u <= 1.0) ? 0u : 1u;
   c.u = c.ubad ? 0.0 : u;
+  c.key = dbits(c.u);
+  c.pivbad = 0u;
   c.dup = 0u;
   c.Ptot = 0.0;
   c.k_notfound = 0;
@@ -910,7 +1220,13 @@ int compress_resample(const double* x, int64_t n, int64_t m, int method, const d
   // the kept part (all m entries for "largest", where K stays m: the second
   // sample is the first)
   k_copy_kept<<<(unsigned)std::min<int64_t>(ceil_div(m, NT), 2048), NT, 0, s>>>(idx, val, sc, m, out_idx, out_val);
-  if (method != COMP_LARGEST) {
+  if (method == COMP_FRI_PIVOTAL) {
+    uint32_t *tmax, *thmax;
+    uint64_t* pst;
+    FRICC_TRY(piv_scratch_ws(n, &tmax, &thmax, &pst));
+    FRICC_TRY(pivotal_passes(x, nullptr, nullptr, n, m, sc, (const u128*)p_ts, (const u128*)p_ths,
+                             (const u128*)p_tot, tmax, thmax, pst, out_idx, out_val, s));
+  } else if (method != COMP_LARGEST) {
     k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, (const u128*)p_ts, (const u128*)p_ths, out_idx, out_val);
     k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
     k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
@@ -1021,6 +1337,45 @@ int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, ui
   if (h.pflag) {
     set_error("sample_systematic: a probability exceeds 1");
     return EPROB;
+  }
+  return OK;
+}
+
+// Pivotal sampling of n_sample indices from given probabilities p[n] (each
+// in [0,1]); step j draws U[j] (device, n doubles; U[0] unused) or, with U
+// null, the SplitMix64 stream keyed by the bits of u.
+int sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* U, double u, uint32_t* out_idx,
+                   hipStream_t s) {
+  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "sample_pivotal: length must be in [1, 2^31)");
+  FRICC_CHECK(n_sample >= 0, "sample_pivotal: negative sample count");
+  FRICC_CHECK(U || (u >= 0.0 && u <= 1.0), "sample_pivotal: uniform must lie in [0, 1]");
+  void *p_sc, *p_ts, *p_tot, *p_ths;
+  const int64_t nt = ceil_div(n, TILE);
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
+  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  uint32_t *tmax, *thmax;
+  uint64_t* pst;
+  FRICC_TRY(piv_scratch_ws(n, &tmax, &thmax, &pst));
+  CompScalars* sc = (CompScalars*)p_sc;
+  u128* ts = (u128*)p_ts;
+  k_init_scalars<<<1, NT, 0, s>>>(sc, 0, n_sample, U ? 0.0 : u, nullptr, nullptr, 0);
+  k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (u128*)p_ths);
+  k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
+  FRICC_TRY(pivotal_passes(nullptr, p, U, n, n_sample, sc, ts, (const u128*)p_ths, (const u128*)p_tot, tmax, thmax,
+                           pst, out_idx, nullptr, s));
+  CompScalars h;
+  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  if (h.pflag) {
+    set_error("sample_pivotal: a probability lies outside [0, 1]");
+    return EPROB;
+  }
+  if (h.pivbad) {
+    set_error("sample_pivotal: the chain ended with a sample count other than n_sample (reference exits, "
+              "src/fri_utils.cpp:100-123)");
+    return ESAMPLE;
   }
   return OK;
 }

@@ -9,7 +9,7 @@
 
 namespace fricc {
 
-enum { COMP_LARGEST = 0, COMP_FRI_SYSTEMATIC = 1 };
+enum { COMP_LARGEST = 0, COMP_FRI_SYSTEMATIC = 1, COMP_FRI_PIVOTAL = 2 };
 // DT_BF16: hi | lo bf16 planes (three-pass products); DT_BF16X1: the hi plane
 // only (single-pass products)
 enum { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_BF16X1 = 3 };
@@ -43,6 +43,10 @@ const uint32_t* comp_scalars_dup_flag(const void* dev_sc);
 int comp_scalars_status(const void* host_sc, CompressInfo* info);
 int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
                       int64_t* n_notfound, hipStream_t s);
+// Pivotal sampling of n_sample indices from p[n]; step j draws U[j] (device,
+// n doubles) or, with U null, SplitMix64 keyed by the bits of u.
+int sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* U, double u, uint32_t* out_idx,
+                   hipStream_t s);
 
 // A gathered view of a sparse 4-index tensor as a matrix Tm[r, y]:
 // rows r = (i_rh, i_rl), columns y = (i_yh, i_yl) (positions into the tensor's

@@ -10,8 +10,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfricc_hip.so")
 
-FRICC_OK, FRICC_EINVAL, FRICC_EDEVICE, FRICC_ENOMEM, FRICC_EPROB, FRICC_ENONFINITE = range(6)
-LARGEST, FRI_SYSTEMATIC = 0, 1
+FRICC_OK, FRICC_EINVAL, FRICC_EDEVICE, FRICC_ENOMEM, FRICC_EPROB, FRICC_ENONFINITE, FRICC_ESAMPLE = range(7)
+LARGEST, FRI_SYSTEMATIC, FRI_PIVOTAL = 0, 1, 2
 FP64, FP32, BF16, BF16X1 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16x1": BF16X1}
 
@@ -25,7 +25,7 @@ EXPORTS = (
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
     "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
     "fricc_compress_noalloc", "fricc_compress_noalloc_max_n", "fricc_dense_contract",
-    "fricc_split_bf16_hi",
+    "fricc_split_bf16_hi", "fricc_sample_pivotal",
 )
 
 
@@ -81,6 +81,7 @@ def lib():
     L.fricc_compress_noalloc_max_n.restype = _i64
     L.fricc_compress_noalloc_max_n.argtypes = []
     L.fricc_sample_systematic.argtypes = [_vp, _i64, _i64, ctypes.c_double, _vp, ctypes.POINTER(_i64), _vp]
+    L.fricc_sample_pivotal.argtypes = [_vp, _i64, _i64, _vp, ctypes.c_double, _vp, _vp]
     L.fricc_sparse_create.argtypes = [I64x4, ctypes.POINTER(_vp)]
     L.fricc_sparse_destroy.argtypes = [_vp]
     L.fricc_sparse_compress.argtypes = [_vp, _vp, _i64, ctypes.c_int, ctypes.c_double, _vp]

@@ -279,6 +279,7 @@ def _update_amps_device(cc, t1, t2, eris):
     t0 = time.perf_counter()
     tm.start("compress")
     comp = st["compression"]
+    smeth = st["sampling_method"]
     ud = cc._udev()
     ts_alt = None
     if _needs_alt(cc) and (not st["unbiased"] or cc._iter < st["burn_in"]):
@@ -286,10 +287,10 @@ def _update_amps_device(cc, t1, t2, eris):
         # uniform for the second), which share everything but the sampling
         ualt = ud[1:2] if comp == "fri" else ud[0:1]
         ts, ts_alt = cc._slot(0, t2.shape).recompress_pair_async(cc._slot(1, t2.shape), t2, int(m_keep), comp,
-                                                                 ud[0:1], ualt)
+                                                                 ud[0:1], ualt, sampling_method=smeth)
         cc._used_slots = {0, 1}
     else:
-        ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1])
+        ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1], sampling_method=smeth)
         cc._used_slots = {0}
     if _needs_alt(cc) and ts_alt is None:
         # unbiased averaging after burn-in: t2_sparse_alt samples the running average
@@ -299,7 +300,8 @@ def _update_amps_device(cc, t1, t2, eris):
             cc._t2_avg = t2.clone()
         else:
             cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
-        ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt)
+        ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt,
+                                                        sampling_method=smeth)
         cc._used_slots = {0, 1}
     tm.stop("compress")
     timings["compress"] = time.perf_counter() - t0
@@ -731,8 +733,8 @@ class FRICCSD:
                 raise ValueError(f"Contraction ({c}) is not supported!")
         if settings["compression"] not in ("fri", "largest"):
             raise ValueError("compression must be 'fri' or 'largest'")
-        if settings["compression"] == "fri" and settings["sampling_method"] != "systematic":
-            raise NotImplementedError("only systematic sampling runs on the GPU path")
+        if settings["compression"] == "fri" and settings["sampling_method"] not in ("systematic", "pivotal"):
+            raise ValueError("sampling_method must be 'systematic' or 'pivotal'")
         if settings["precision"] not in PRECISION_MODES:
             raise ValueError(f"precision must be one of {sorted(PRECISION_MODES)}")
         self.frozen = frozen

@@ -92,11 +92,13 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
         # one sample of the same T2 per term, uniforms in term order: one
         # selection shared by all samples
         slots = [cc._slot(k, t2.shape) for k in range(len(TERM_ORDER))]
-        SparseTensor4d.compress_multi_async(slots, t2, m_keep, comp, ud[:len(TERM_ORDER)])
+        SparseTensor4d.compress_multi_async(slots, t2, m_keep, comp, ud[:len(TERM_ORDER)],
+                                            sampling_method=st["sampling_method"])
         ts = dict(zip(TERM_ORDER, slots))
         cc._used_slots = set(range(len(TERM_ORDER)))
     else:
-        one = cc._slot(0, t2.shape).recompress_async(t2, m_keep, comp, ud[0:1])
+        one = cc._slot(0, t2.shape).recompress_async(t2, m_keep, comp, ud[0:1],
+                                                     sampling_method=st["sampling_method"])
         ts = dict.fromkeys(TERM_ORDER, one)
         cc._used_slots = {0}
 

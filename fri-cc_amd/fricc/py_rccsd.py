@@ -12,10 +12,12 @@ Deviations, all deliberate:
 * The reference terminates the process (``exit(EXIT_FAILURE)``) on a bad
   compression string, an unknown term, p > 1 or an out-of-range index; here
   those raise ``ValueError`` / ``RuntimeError``.
-* ``sampling_method="pivotal"`` (the pybind default, src/py_rccsd.cpp:281) is
-  the serial pivotal sampler (src/fri_utils.cpp:63-127); it is not on the GPU
-  path and raises ``NotImplementedError`` for ``compression="fri"``.  FRICCSD
-  always passes ``"systematic"`` (fricc/friccsd.py:18,105).
+* ``sampling_method="pivotal"`` (the pybind default, src/py_rccsd.cpp:30,39)
+  runs the pivotal chain (src/fri_utils.cpp:63-127) on the GPU, all steps in
+  parallel on the exact prefix of p.  The reference draws rand()/RAND_MAX at
+  every step of the chain; here one uniform per call (drawn like the
+  systematic one, or given as ``u``) keys a SplitMix64 stream of step draws.
+  ``sample_pivotal`` also takes the step uniforms as an array.
 * An unknown sampling method raises ``ValueError`` (the reference silently
   emits index 0, src/fri_utils.cpp:443-444).
 """
@@ -66,6 +68,18 @@ def _draw(u):
     return draw_libc() if u is None else float(u)
 
 
+def _method(compression, sampling_method="systematic"):
+    if compression == "largest":
+        return _capi.LARGEST
+    if compression != "fri":
+        raise ValueError("compression must be 'largest' or 'fri'")
+    if sampling_method == "systematic":
+        return _capi.FRI_SYSTEMATIC
+    if sampling_method == "pivotal":
+        return _capi.FRI_PIVOTAL
+    raise ValueError("sampling_method must be 'pivotal' or 'systematic'")
+
+
 class SparseTensor4d:
     """Device-resident SparseTensor4d (include/sparse_tensor.hpp:13-152).
 
@@ -73,8 +87,9 @@ class SparseTensor4d:
     sampling_method="pivotal", verbose=False)`` compresses a dense (o,o,v,v)
     tensor; ``SparseTensor4d(dims, nnz)`` makes an empty one.  Entries are kept
     on the GPU in compression order: D (|x| descending) then the samples
-    (ascending flat index).  ``u`` (keyword only, not in the reference) supplies
-    the uniform instead of drawing ``rand()/RAND_MAX``.
+    (systematic: ascending flat index; pivotal: the chain's emission order).
+    ``u`` (keyword only, not in the reference) supplies the uniform instead of
+    drawing ``rand()/RAND_MAX``.
     """
 
     def __init__(self, *args, **kwargs):
@@ -128,23 +143,12 @@ class SparseTensor4d:
         if x.numel() != n:
             raise ValueError(f"tensor has {x.numel()} elements but dims {self.dims} give {n}")
         m = int(m)
-        if compression == "largest":
-            method = _capi.LARGEST
-            uu = 0.0
-        elif compression == "fri":
-            if sampling_method == "pivotal":
-                raise NotImplementedError(
-                    "pivotal sampling (src/fri_utils.cpp:63-127) is serial and not on the GPU path; "
-                    "use sampling_method='systematic'")
-            if sampling_method != "systematic":
-                raise ValueError("sampling_method must be 'pivotal' or 'systematic'")
-            method = _capi.FRI_SYSTEMATIC
-            uu = _draw(u)  # one rand() per systematic call, also when nothing is sampled
-        else:
-            raise ValueError("compression must be 'largest' or 'fri'")
+        method = _method(compression, sampling_method)
+        # one rand() per "fri" call, also when nothing is sampled
+        uu = 0.0 if method == _capi.LARGEST else _draw(u)
         if m < 0 or m > n:
             raise ValueError(f"m must lie in [0, {n}]")
-        self.u = uu if method == _capi.FRI_SYSTEMATIC else None
+        self.u = None if method == _capi.LARGEST else uu
         if m == 0:
             return
         t0 = time.perf_counter()
@@ -227,30 +231,26 @@ class SparseTensor4d:
                                                  val.ctypes.data_as(ctypes.c_void_p), idx.size, _capi.stream_ptr()))
         self._refresh_info()
 
-    def recompress(self, x, m, compression="fri", u=None, stream=None):
+    def recompress(self, x, m, compression="fri", u=None, stream=None, sampling_method="systematic"):
         """Compress a new device tensor into this object's storage (no reallocation
         when m does not grow).  Consumes one uniform for "fri"."""
         x = _as_device_vector(x)
-        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
-        if compression not in ("largest", "fri"):
-            raise ValueError("compression must be 'largest' or 'fri'")
-        uu = _draw(u) if method == _capi.FRI_SYSTEMATIC else 0.0
-        self.u = uu if method == _capi.FRI_SYSTEMATIC else None
+        method = _method(compression, sampling_method)
+        uu = _draw(u) if method != _capi.LARGEST else 0.0
+        self.u = uu if method != _capi.LARGEST else None
         _capi.check(_capi.lib().fricc_sparse_compress(self._h, ctypes.c_void_p(x.data_ptr()), int(m), method, uu,
                                                       _capi.stream_ptr(stream)))
         self._async = False
         self._refresh_info()
         return self
 
-    def recompress_async(self, x, m, compression, u_dev, stream=None):
+    def recompress_async(self, x, m, compression, u_dev, stream=None, sampling_method="systematic"):
         """``recompress`` with no host round trip (device-resident loop, graph
         capture): the uniform is read from ``u_dev`` (a one-element CUDA float64
         tensor, unused for "largest"); statistics and errors wait for
         ``sync_status``."""
         x = _as_device_vector(x)
-        if compression not in ("largest", "fri"):
-            raise ValueError("compression must be 'largest' or 'fri'")
-        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
+        method = _method(compression, sampling_method)
         _capi.check(_capi.lib().fricc_sparse_compress_async(self._h, ctypes.c_void_p(x.data_ptr()), int(m), method,
                                                             ctypes.c_void_p(u_dev.data_ptr()),
                                                             _capi.stream_ptr(stream)))
@@ -258,15 +258,14 @@ class SparseTensor4d:
         self._async = True
         return self
 
-    def recompress_pair_async(self, other, x, m, compression, u_dev, u_dev2, stream=None):
+    def recompress_pair_async(self, other, x, m, compression, u_dev, u_dev2, stream=None,
+                              sampling_method="systematic"):
         """``recompress_async`` of x into this tensor (uniform u_dev) and into
         ``other`` (uniform u_dev2), sharing everything but the sampling: each
         tensor ends bit for bit as its own ``recompress_async`` would leave it.
         Returns (self, other)."""
         x = _as_device_vector(x)
-        if compression not in ("largest", "fri"):
-            raise ValueError("compression must be 'largest' or 'fri'")
-        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
+        method = _method(compression, sampling_method)
         _capi.check(_capi.lib().fricc_sparse_compress_pair_async(
             self._h, other._h, ctypes.c_void_p(x.data_ptr()), int(m), method, ctypes.c_void_p(u_dev.data_ptr()),
             ctypes.c_void_p(u_dev2.data_ptr()), _capi.stream_ptr(stream)))
@@ -276,16 +275,14 @@ class SparseTensor4d:
         return self, other
 
     @staticmethod
-    def compress_multi_async(tensors, x, m, compression, u_dev, stream=None):
+    def compress_multi_async(tensors, x, m, compression, u_dev, stream=None, sampling_method="systematic"):
         """``recompress_async`` of x into every tensor of ``tensors``, tensor k
         with the uniform u_dev[k] (a CUDA float64 tensor of len(tensors)
         consecutive values), sharing everything but the sampling."""
         x = _as_device_vector(x)
-        if compression not in ("largest", "fri"):
-            raise ValueError("compression must be 'largest' or 'fri'")
+        method = _method(compression, sampling_method)
         if u_dev.numel() < len(tensors) or not u_dev.is_contiguous():
             raise ValueError("u_dev must hold one contiguous uniform per tensor")
-        method = _capi.LARGEST if compression == "largest" else _capi.FRI_SYSTEMATIC
         handles = (ctypes.c_void_p * len(tensors))(*[t._h.value for t in tensors])
         _capi.check(_capi.lib().fricc_sparse_compress_multi_async(
             handles, len(tensors), ctypes.c_void_p(x.data_ptr()), int(m), method, ctypes.c_void_p(u_dev.data_ptr()),
@@ -391,11 +388,7 @@ def argsort(array, m):
 
 def fri_compression(x, n_sample, sampling_method="pivotal", verbose=False, *, u=None):
     """(indices, values) of the FRI compression (src/fri_utils.cpp:439-522)."""
-    if sampling_method == "pivotal":
-        raise NotImplementedError("pivotal sampling is not on the GPU path; use 'systematic'")
-    if sampling_method != "systematic":
-        raise ValueError("sampling_method must be 'pivotal' or 'systematic'")
-    idx, val, info = _compress_raw(x, n_sample, _capi.FRI_SYSTEMATIC, _draw(u))
+    idx, val, info = _compress_raw(x, n_sample, _method("fri", sampling_method), _draw(u))
     if verbose:
         print(f"|D| = {info['n_D']}  n' = {info['n_sampled']}  remaining norm = {info['remaining_norm']}")
     return idx.tolist(), val.tolist()
@@ -440,9 +433,29 @@ def sample_systematic(n_sample, probs, *, u=None):
     return out[:n_sample].cpu().numpy().astype(np.int64).tolist()
 
 
-def sample_pivotal(n_sample, probs):
-    """Serial pivotal sampling (src/fri_utils.cpp:63-127): not on the GPU path."""
-    raise NotImplementedError("pivotal sampling is serial and not on the GPU path")
+def sample_pivotal(n_sample, probs, *, u=None, uniforms=None):
+    """Pivotal sampling (src/fri_utils.cpp:63-127) of n_sample indices, in the
+    order the chain emits them.  Step j draws ``uniforms[j]`` when given (n
+    values; uniforms[0] is not read -- e.g. the reference's rand()/RAND_MAX
+    stream shifted by one), else the SplitMix64 stream keyed by one uniform
+    (``u`` or one rand()/RAND_MAX draw).  Raises where the reference exits."""
+    torch = _torch()
+    pd = _as_device_vector(np.asarray(probs, dtype=np.float64) if not _is_torch(probs) else probs, "probs")
+    n_sample = int(n_sample)
+    Ud = None
+    uu = 0.0
+    if uniforms is not None:
+        Ud = _as_device_vector(np.asarray(uniforms, dtype=np.float64) if not _is_torch(uniforms) else uniforms,
+                               "uniforms")
+        if Ud.numel() < pd.numel():
+            raise ValueError("uniforms must hold one value per probability")
+    else:
+        uu = _draw(u)
+    out = torch.zeros(max(n_sample, 1), dtype=torch.int32, device="cuda")
+    _capi.check(_capi.lib().fricc_sample_pivotal(ctypes.c_void_p(pd.data_ptr()), pd.numel(), n_sample,
+                                                 ctypes.c_void_p(Ud.data_ptr() if Ud is not None else None), uu,
+                                                 ctypes.c_void_p(out.data_ptr()), _capi.stream_ptr()))
+    return out[:n_sample].cpu().numpy().astype(np.int64).tolist()
 
 
 def parallel_sample(n_sample, probs):

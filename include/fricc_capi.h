@@ -31,10 +31,18 @@ enum fricc_status {
   FRICC_EDEVICE = 2,    /* HIP runtime error                                     */
   FRICC_ENOMEM = 3,     /* device allocation failed                              */
   FRICC_EPROB = 4,      /* a sampling probability exceeds 1 (src/fri_utils.cpp:383-395) */
-  FRICC_ENONFINITE = 5  /* non-finite value in the vector to compress            */
+  FRICC_ENONFINITE = 5, /* non-finite value in the vector to compress           */
+  FRICC_ESAMPLE = 6     /* pivotal sampling ended with a sample count other than
+                           n' (reference exits, src/fri_utils.cpp:100-123)       */
 };
 
-enum fricc_compression { FRICC_LARGEST = 0, FRICC_FRI_SYSTEMATIC = 1 };
+/* FRICC_FRI_PIVOTAL: "fri" with sampling_method "pivotal" (the binding's
+ * default, src/py_rccsd.cpp:30,39).  Its chain draws one uniform per step
+ * (rand()/RAND_MAX in the reference, src/fri_utils.cpp:73); here step j draws
+ * the j-th output of SplitMix64 (Steele, Lea and Flood, OOPSLA 2014) seeded
+ * with the IEEE-754 bits of the call's uniform u, so one u per call still
+ * fixes the sample. */
+enum fricc_compression { FRICC_LARGEST = 0, FRICC_FRI_SYSTEMATIC = 1, FRICC_FRI_PIVOTAL = 2 };
 /* FRICC_BF16: BF16 hi and lo planes of each operand, three MFMA passes
  * (hi*hi + hi*lo + lo*hi, about 16 significant bits per product);
  * FRICC_BF16X1: the hi plane only, one pass (BF16 products, FP32 accumulate). */
@@ -58,13 +66,16 @@ int fricc_abi_version(void);
 const char* fricc_last_error(void);
 
 /* FRI compression of a device vector x[n] to m entries.
- * Replaces fri_compression(x, m, "systematic") (src/fri_utils.cpp:439-522,
- * include/fri_utils.hpp:30-32) and, for compression == FRICC_LARGEST, the
- * argsort-top-m path of the SparseTensor4d ctor (include/sparse_tensor.hpp:46-56).
- * `u` is the one uniform the reference draws per call as rand()/RAND_MAX
- * (src/fri_utils.cpp:47); it is ignored for FRICC_LARGEST.
+ * Replaces fri_compression(x, m, "systematic" | "pivotal")
+ * (src/fri_utils.cpp:439-522, include/fri_utils.hpp:30-32) and, for
+ * compression == FRICC_LARGEST, the argsort-top-m path of the SparseTensor4d
+ * ctor (include/sparse_tensor.hpp:46-56).
+ * `u` is the one uniform the reference draws per systematic call as
+ * rand()/RAND_MAX (src/fri_utils.cpp:47); for FRICC_FRI_PIVOTAL it keys the
+ * chain's step draws; it is ignored for FRICC_LARGEST.
  * out_idx/out_val: device arrays of m; flat indices, D first (|x| descending,
- * ties by index), then the samples in ascending order.  n < 2^31. */
+ * ties by index), then the samples (systematic: in ascending order; pivotal:
+ * in the order the chain emits them, all distinct).  n < 2^31. */
 int fricc_compress(const double* x, int64_t n, int64_t m, int compression, double u, uint32_t* out_idx,
                    double* out_val, fricc_compress_info* info, void* stream);
 
@@ -89,6 +100,18 @@ int64_t fricc_compress_noalloc_max_n(void);
  * n_sample.  *n_notfound (host) receives the number of such k. */
 int fricc_sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, uint32_t* out_idx,
                             int64_t* n_notfound, void* stream);
+
+/* sample_pivotal(n_sample, probs) (src/fri_utils.cpp:63-127) on device
+ * probabilities p[n] (each in [0,1], else FRICC_EPROB), with the chain's mass
+ * taken from the exact prefix of p (canonical: c_j = min(j, floor(Q_{j+1}))
+ * units completed through step j, Q the exact sum of floor(p * 2^96) * 2^-96,
+ * mass a_j = round(Q_{j+1} - c_j)).  Step j (1 <= j < n) draws uniforms[j]
+ * (device array of n doubles; uniforms[0] is not read) or, with uniforms
+ * NULL, the SplitMix64 stream keyed by the bits of u.  out_idx: device array
+ * of n_sample, in the order the chain emits.  FRICC_ESAMPLE where the
+ * reference exits (the chain ends with a count other than n_sample). */
+int fricc_sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* uniforms, double u,
+                         uint32_t* out_idx, void* stream);
 
 /* SparseTensor4d(dims, nnz) -- empty tensor (include/sparse_tensor.hpp:27-31). */
 int fricc_sparse_create(const int64_t dims[4], fricc_sparse** out);

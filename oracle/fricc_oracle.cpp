@@ -236,8 +236,98 @@ static int64_t systematic_scan(int64_t n_sample, const double* P, int64_t n, dou
   return notfound;
 }
 
+// ---------------------------------------------------------------------------
+// Pivotal sampling, sample_pivotal (src/fri_utils.cpp:63-127).
+//
+// Step j (j = 1 .. n-1) of the chain draws one uniform: U[j] when an array is
+// given (U[0] is never read), else the j-th output of SplitMix64 seeded with
+// `key` (Steele, Lea and Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014): z = key + (j+1) * 0x9e3779b97f4a7c15, the mix13
+// finalizer, the top 53 bits times 2^-53.  The reference draws rand()/RAND_MAX
+// at every step (:73); passing that stream as U reproduces its draws.
+static inline uint64_t splitmix_at(uint64_t key, int64_t j) {
+  uint64_t z = key + (uint64_t)(j + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+static inline double piv_uniform(const double* U, uint64_t key, int64_t j) {
+  return U ? U[j] : (double)(splitmix_at(key, j) >> 11) * 0x1.0p-53;
+}
+
+void orc_splitmix_uniforms(uint64_t key, int64_t n, double* out) {
+  for (int64_t j = 0; j < n; j++) out[j] = piv_uniform(nullptr, key, j);
+}
+
+// The chain: a candidate i (initially 0) and the running mass a.  At a step
+// that does not complete a unit of mass (a + p_j < 1) the candidate becomes j
+// with probability p_j / a; at one that does, a drops by 1 and either the
+// candidate is emitted and replaced by j (probability (1 - p_j) / (1 - a)) or
+// j is emitted.  The loop stops once n_sample indices are out; at the last
+// step a mass within 1e-5 of 1 emits the candidate.  Every other ending, and
+// an acceptance probability outside [0, 1], is an exit of the reference
+// (:85-98, :103-123): status 5.
+//   FAITHFUL  (mode 1): a is the reference's FP64 running sum (:72-84).
+//   CANONICAL (mode 0): the mass comes from the exact prefix
+//     Q_{j+1} = sum_{l<=j} FXP(p_l) (units of 2^-96): the units completed by
+//     step j are c_j = min(j, floor(Q_{j+1})) and a_j = round(Q_{j+1} - c_j);
+//     the reference's sequence in exact arithmetic (the min covers p_0 = 1,
+//     which the reference carries as a = 1 into step 1).
+// p outside [0, 1] (or NaN): status 1.  S receives min(count, n_sample)
+// indices, *cnt the number the chain emitted.
+int orc_pivotal(const double* p, int64_t n, int64_t ns, const double* U, uint64_t key, int mode, int64_t* S,
+                int64_t* cnt) {
+  *cnt = 0;
+  if (n <= 0 || ns < 0) return 2;
+  for (int64_t j = 0; j < n; j++)
+    if (!(p[j] >= 0.0 && p[j] <= 1.0)) return 1;
+  int64_t i = 0, k = 0, c = 0;
+  double a = p[0];
+  u128 Q = fx_of(dbits(p[0]), 1023u);
+  auto push = [&](int64_t v) {
+    if (k < ns) S[k] = v;
+    k++;
+  };
+  for (int64_t j = 1; j < n; j++) {
+    const double rn = piv_uniform(U, key, j);
+    bool cross;
+    if (mode == 1) {
+      a += p[j];
+      cross = !(a < 1.0);
+      if (cross) a -= 1.0;
+    } else {
+      Q += fx_of(dbits(p[j]), 1023u);
+      const int64_t cn = std::min<int64_t>(j, (int64_t)(Q >> 96));
+      cross = cn > c;
+      c = cn;
+      a = u128_to_double(Q - ((u128)c << 96), -96);
+    }
+    if (!cross) {
+      if (rn < p[j] / a) i = j;
+    } else {
+      const double pa = (1.0 - p[j]) / (1.0 - a);
+      if (pa > 1.0 || pa < 0.0) { *cnt = k; return 5; }
+      if (rn < pa) {
+        push(i);
+        i = j;
+      } else {
+        push(j);
+      }
+    }
+    if (k == ns) break;
+    if (j == n - 1) {
+      if (fabs(a - 1.0) < 1e-5) push(i);
+      else { *cnt = k; return 5; }
+    }
+  }
+  *cnt = k;
+  return k == ns ? 0 : 5;
+}
+
 // FRI / largest compression.  method 0 = "largest" (include/sparse_tensor.hpp:46-56),
-// method 1 = "fri" + "systematic" (include/sparse_tensor.hpp:58-67).
+// method 1 = "fri" + "systematic" (include/sparse_tensor.hpp:58-67), method 2 =
+// "fri" + "pivotal" (src/fri_utils.cpp:486-489), its step draws SplitMix64
+// keyed by the bits of u (piv_uniform).
 int orc_compress(const double* x, int64_t n, int64_t m, int method, int mode, double u,
                  int64_t* out_idx, double* out_val, orc_info* info) {
   memset(info, 0, sizeof(*info));
@@ -308,7 +398,16 @@ int orc_compress(const double* x, int64_t n, int64_t m, int method, int mode, do
       std::partial_sum(p.begin(), p.end(), P.begin() + 1);
     }
     info->p_total = P[n];
-    info->n_notfound = systematic_scan(ns, P.data(), n, u, S);
+    if (method == 2) {
+      S.assign(ns, 0);
+      int64_t cnt = 0;
+      if (orc_pivotal(p.data(), n, ns, nullptr, dbits(u), mode, S.data(), &cnt) != 0) {
+        info->status = 5;
+        return 5;
+      }
+    } else {
+      info->n_notfound = systematic_scan(ns, P.data(), n, u, S);
+    }
   } else {
     info->p_total = 0.0;
     info->n_notfound = 0;

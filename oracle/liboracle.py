@@ -15,6 +15,7 @@ CANONICAL = 0
 FAITHFUL = 1
 LARGEST = 0
 FRI_SYSTEMATIC = 1
+FRI_PIVOTAL = 2
 
 
 class OrcInfo(ctypes.Structure):
@@ -57,6 +58,10 @@ def lib():
         L.orc_eri_block_plain.argtypes = L.orc_eri_block.argtypes
         L.orc_eri_block_plain.restype = None
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_pivotal.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, dp, ctypes.c_uint64, ctypes.c_int, ip, ip]
+        L.orc_pivotal.restype = ctypes.c_int
+        L.orc_splitmix_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_int64, dp]
+        L.orc_splitmix_uniforms.restype = None
         _LIB = L
     return _LIB
 
@@ -82,6 +87,37 @@ def compress(x, m, method=FRI_SYSTEMATIC, u=0.0, mode=CANONICAL):
     d = info.as_dict()
     d["status"] = st
     return idx[:m], val[:m], d
+
+
+def key_of(u):
+    """The 64-bit SplitMix64 key of a call's uniform u: its IEEE-754 bits."""
+    return int(np.array([u], dtype="<f8").view("<u8")[0])
+
+
+def splitmix_uniforms(key, n):
+    """The step uniforms of pivotal sampling when no array is given: element j
+    is the j-th SplitMix64 output seeded with key, top 53 bits times 2^-53."""
+    out = np.zeros(max(int(n), 1))
+    lib().orc_splitmix_uniforms(ctypes.c_uint64(int(key)), int(n), _dp(out))
+    return out[:int(n)]
+
+
+def pivotal(p, n_sample, U=None, key=0, mode=CANONICAL):
+    """sample_pivotal (src/fri_utils.cpp:63-127).  Step j draws U[j] (U[0]
+    unused) or, with U None, the SplitMix64 stream of key.  Returns
+    (S int64[n_sample], count emitted, status): status 0, 1 (p outside [0, 1])
+    or 5 (a reference exit: wrong number of samples)."""
+    p = np.ascontiguousarray(p, dtype=np.float64).ravel()
+    ns = int(n_sample)
+    S = np.zeros(max(ns, 1), dtype=np.int64)
+    cnt = ctypes.c_int64(0)
+    Up = None
+    if U is not None:
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        assert U.size >= p.size
+        Up = _dp(U)
+    st = lib().orc_pivotal(_dp(p), p.size, ns, Up, ctypes.c_uint64(int(key)), int(mode), _ip(S), ctypes.byref(cnt))
+    return S[:ns], cnt.value, st
 
 
 def norm_canonical(x):

@@ -238,9 +238,12 @@ def amplitudes_to_vector(t1, t2):
 class SparseT:
     """Decompressed view of a compressed tensor (SparseTensor4d semantics)."""
 
-    def __init__(self, t2, m, compression, u, mode):
+    def __init__(self, t2, m, compression, u, mode, sampling="systematic"):
         self.dims = t2.shape
-        method = liboracle.FRI_SYSTEMATIC if compression == "fri" else liboracle.LARGEST
+        if compression != "fri":
+            method = liboracle.LARGEST
+        else:
+            method = liboracle.FRI_PIVOTAL if sampling == "pivotal" else liboracle.FRI_SYSTEMATIC
         self.idx, self.val, self.info = liboracle.compress(t2.ravel(), m, method, u, mode)
         if self.info["status"] != 0:
             raise RuntimeError(f"oracle compression failed: {self.info}")
@@ -344,16 +347,17 @@ def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False, state=Non
     t2 = np.ascontiguousarray(t2)
     comp = settings["compression"]
     mode = settings["mode"]
+    smeth = settings.get("sampling_method", "systematic")
 
     def draw():
         return next(uniforms) if comp == "fri" else 0.0
 
-    ts = SparseT(t2, m, comp, draw(), mode)
+    ts = SparseT(t2, m, comp, draw(), mode, smeth)
     ts_alt = None
     if "O^3V^3" in cc or "O^4V^2" in cc:
         it_now = 0 if state is None else state.get("iter", 0)
         if not settings["unbiased"] or it_now < settings["burn_in"]:
-            ts_alt = SparseT(t2, m, comp, draw(), mode)
+            ts_alt = SparseT(t2, m, comp, draw(), mode, smeth)
         else:
             # fricc/friccsd.py:117-136: after the burn-in the second sample is
             # taken of the running average Y(t+1) = (1 - 1/t) Y(t) + X(t+1)/t,
@@ -364,7 +368,7 @@ def update_amps(t1, t2, eris, settings, uniforms, return_sparse=False, state=Non
                 state["t2_avg"] = t2.copy()
             else:
                 state["t2_avg"] = t2 / it + (it - 1) / it * state["t2_avg"]
-            ts_alt = SparseT(np.ascontiguousarray(state["t2_avg"]), m, comp, draw(), mode)
+            ts_alt = SparseT(np.ascontiguousarray(state["t2_avg"]), m, comp, draw(), mode, smeth)
 
     Foo = cc_Foo(t1, t2, eris)
     Fvv = cc_Fvv(t1, t2, eris)

@@ -135,8 +135,11 @@ def test_free_functions(mods):
     assert np.array_equal(np.array(p), p_ref)
     S = py_rccsd.sample_systematic(900 - len(D), np.array(p), u=0.2)
     assert S == oi[info["n_D"]:].tolist()
-    with pytest.raises(NotImplementedError):
-        py_rccsd.sample_pivotal(3, [0.5, 0.5])
+    idx, val = py_rccsd.fri_compression(x, 900, u=0.2)  # the binding's default: pivotal
+    oi, ov, info = lo.compress(x, 900, lo.FRI_PIVOTAL, 0.2)
+    assert idx == oi.tolist() and val == ov.tolist()
+    S = py_rccsd.sample_pivotal(900 - len(D), np.array(p), u=0.2)
+    assert S == oi[info["n_D"]:].tolist()
 
 
 def test_uniform_drawn_from_libc_rand(mods):

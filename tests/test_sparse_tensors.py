@@ -196,8 +196,6 @@ def test_errors_raise_instead_of_exit(py_rccsd):
         py_rccsd.SparseTensor4d(t2, t2.shape, 10, "bogus")
     with pytest.raises(ValueError):
         py_rccsd.SparseTensor4d(t2, t2.shape, 10, "fri", "bogus")
-    with pytest.raises(NotImplementedError):
-        py_rccsd.SparseTensor4d(t2, t2.shape, 10, "fri", "pivotal")
     with pytest.raises(TypeError):
         py_rccsd.SparseTensor4d(t2.astype(np.float32), t2.shape, 10, "largest")
     with pytest.raises(TypeError):

@@ -31,7 +31,8 @@ def _pair(env, o, v, naux, seed, settings, us):
     oeris, _ = rccsd.synthetic_eris(o, v, naux, seed)
     s = rccsd.default_settings(**{k: settings[k] for k in settings if k in ("m_keep", "compression",
                                                                             "compressed_contractions",
-                                                                            "vvvv_t1t1", "level_shift")})
+                                                                            "vvvv_t1t1", "level_shift",
+                                                                            "sampling_method")})
     return cc, eris, oeris, s
 
 
@@ -61,6 +62,28 @@ def test_update_amps_fp64_matches_oracle(env, ccs, comp):
         assert np.array_equal(cc._slots[0].indices(), ts.idx)
     if ts_alt is not None:
         assert np.array_equal(cc._slots[1].indices(), ts_alt.idx)
+    e2 = np.abs(dt2.cpu().numpy() - rt2).max() / np.abs(rt2).max()
+    e1 = np.abs(dt1.cpu().numpy() - rt1).max() / np.abs(rt1).max()
+    assert e2 < 1e-10 and e1 < 1e-10, (e1, e2)
+
+
+def test_update_amps_pivotal_matches_oracle(env):
+    """sampling_method="pivotal" (the binding's default, src/py_rccsd.cpp:30):
+    both samples' index sets and values bit for bit against the oracle's
+    canonical pivotal chain, the update within 1e-10."""
+    torch = env[0]
+    settings = {"m_keep": 700, "compression": "fri", "sampling_method": "pivotal",
+                "compressed_contractions": ["O^2V^4", "O^3V^3", "O^4V^2"]}
+    us = [0.42, 0.58]
+    cc, eris, oeris, s = _pair(env, 4, 13, 30, 7, settings, us)
+    _, t1, t2 = env[3].init_amps(oeris)
+    t1, t2 = _perturbed(t1, t2, 2)
+    rt1, rt2, (ts, ts_alt) = env[3].update_amps(t1, t2, oeris, s, iter(us), return_sparse=True)
+    dt1, dt2 = cc.update_amps(torch.tensor(t1, device="cuda"), torch.tensor(t2, device="cuda"), eris)
+    for slot, ref in ((cc._slots[0], ts), (cc._slots[1], ts_alt)):
+        assert np.array_equal(slot.indices(), ref.idx)
+        assert np.array_equal(slot.values().view(np.int64), ref.val.view(np.int64))
+    assert len(set(cc._slots[0].indices().tolist())) == 700
     e2 = np.abs(dt2.cpu().numpy() - rt2).max() / np.abs(rt2).max()
     e1 = np.abs(dt1.cpu().numpy() - rt1).max() / np.abs(rt1).max()
     assert e2 < 1e-10 and e1 < 1e-10, (e1, e2)

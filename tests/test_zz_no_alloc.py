@@ -1,7 +1,8 @@
 """fricc_compress_noalloc (scratch in a fixed device array, no allocation)
 against fricc_compress (scratch from the library's workspace) through the C
 ABI: indices, values and statistics bit for bit, on two calls in a row with
-the second on a smaller input, for FRI systematic sampling and "largest"."""
+the second on a smaller input, for FRI systematic and pivotal sampling and
+"largest"."""
 import ctypes
 
 import numpy as np
@@ -33,7 +34,7 @@ def _run(fn, xd, m, method, u):
     return idx.cpu().numpy().view(np.uint32).copy(), val.cpu().numpy().copy(), info.as_dict()
 
 
-@pytest.mark.parametrize("method", [1, 0])  # FRICC_FRI_SYSTEMATIC, FRICC_LARGEST
+@pytest.mark.parametrize("method", [1, 0, 2])  # FRICC_FRI_SYSTEMATIC, FRICC_LARGEST, FRICC_FRI_PIVOTAL
 def test_noalloc_equals_compress_twice(method):
     import torch
 
