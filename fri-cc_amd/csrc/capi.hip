@@ -200,6 +200,11 @@ int fricc_sample_pivotal(const double* p, int64_t n, int64_t n_sample, const dou
   return sample_pivotal(p, n, n_sample, uniforms, u, out_idx, (hipStream_t)stream);
 }
 
+int fricc_parallel_sample(const double* p, int64_t n, int64_t n_sample, int64_t n_chunks, const double* uniforms,
+                          double u, uint32_t* out_idx, void* stream) {
+  return parallel_sample(p, n, n_sample, n_chunks, uniforms, u, out_idx, (hipStream_t)stream);
+}
+
 int fricc_sparse_create(const int64_t dims[4], fricc_sparse** out) {
   FRICC_CHECK(out != nullptr, "sparse_create: null output");
   int64_t n = 1;

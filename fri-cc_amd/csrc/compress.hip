@@ -888,6 +888,342 @@ static int pivotal_passes(const double* x, const double* pin, const double* U, i
   return OK;
 }
 
+// ---------------------------------------------------------------- parallel_sample
+// Thread-partitioned pivotal sampling (src/fri_utils.cpp:176-362) with T
+// chunks for the reference's T OpenMP threads; chunk j = [j*step, (j+1)*step),
+// the last one taking the remainder, step = n / T.  Canonical sums as in the
+// pivotal chain: |q| and |q_j| are exact sums of FXP(q) rounded once; the move
+// of a chunk toward its sample count keeps an exact running sum.  Draws by
+// position: the chunk chain's step j takes position j*step, chunk j's chain
+// its step i position j*step + i.
+
+struct PsPlan {
+  int64_t g, lb;  // samples of the chunk, its first output position
+  double a, t;    // expected count n'|q_j|/|q| and its fractional part
+  int32_t dir;    // move: +1 up (g > a), -1 down (g < a), 0 none
+  int32_t pad;
+};
+
+// Exact sum of FXP(q) per chunk (one block each); q outside [0, 1] flagged.
+__global__ __launch_bounds__(NT) void k_ps_chunksum(const double* __restrict__ q, int64_t n, int64_t T, int64_t step,
+                                                    u128* __restrict__ QJ, CompScalars* sc) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  const int64_t j = blockIdx.x;
+  const int64_t lo = j * step, hi = j == T - 1 ? n : lo + step;
+  u128 s = 0;
+  uint32_t bad = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += NT) {
+    const double v = q[i];
+    bad |= !(v >= 0.0 && v <= 1.0);
+    s += fx_of(dbits(v >= 0.0 && v <= 1.0 ? v : 0.0), 1023u);
+  }
+  if (bad) atomicOr(&sc->pflag, 1u);
+  u128 tot;
+  (void)block_excl_scan_u128<NT>(s, &tot, sm);
+  if (threadIdx.x == 0) QJ[j] = tot;
+}
+
+// The plan (one thread): expected counts, the chunk chain over the
+// fractional parts (canonical, serial: T is small), output offsets.
+__global__ void k_ps_plan(const u128* __restrict__ QJ, int64_t T, int64_t step, const double* __restrict__ U,
+                          PsPlan* __restrict__ plan, CompScalars* sc) {
+  if (threadIdx.x != 0) return;
+  const int64_t ns = (int64_t)sc->n_sampled;
+  const uint64_t key = sc->key;
+  u128 Qt = 0;
+  for (int64_t j = 0; j < T; j++) Qt += QJ[j];
+  const double qn = u128_to_double(Qt, -96);
+  if (qn == 0.0) {
+    for (int64_t j = 0; j < T; j++) plan[j] = PsPlan{0, 0, 0.0, 0.0, 0, 0};
+    if (ns != 0) sc->pivbad = 1u;
+    return;
+  }
+  int64_t c = ns;
+  for (int64_t j = 0; j < T; j++) {
+    const double a = (double)ns * u128_to_double(QJ[j], -96) / qn;
+    const double fa = floor(a);
+    plan[j] = PsPlan{(int64_t)fa, 0, a, a - fa, 0, 0};
+    c -= (int64_t)fa;
+  }
+  if (c < 0) {
+    sc->pivbad = 1u;
+    return;
+  }
+  // the chunk chain over t (the canonical chain of sample_pivotal, serially)
+  {
+    int64_t cand = 0, k = 0, cprev = 0;
+    u128 Q = fx_of(dbits(plan[0].t), 1023u);
+    bool done = false;
+    for (int64_t j = 1; j < T && !done; j++) {
+      const double tj = plan[j].t;
+      const double uj = U ? U[j * step] : splitmix_u(key, j * step);
+      Q += fx_of(dbits(tj), 1023u);
+      const int64_t fq = (int64_t)(Q >> 96);
+      const int64_t cn = fq < j ? fq : j;
+      const bool cross = cn > cprev;
+      cprev = cn;
+      const double a = u128_to_double(Q - ((u128)cn << 96), -96);
+      if (!cross) {
+        if (uj < tj / a) cand = j;
+      } else {
+        const double pa = (1.0 - tj) / (1.0 - a);
+        if (pa > 1.0 || pa < 0.0) {
+          sc->pivbad = 1u;
+          return;
+        }
+        if (uj < pa) {
+          if (k < c) plan[cand].g += 1;
+          k++;
+          cand = j;
+        } else {
+          if (k < c) plan[j].g += 1;
+          k++;
+        }
+      }
+      if (k == c) {
+        done = true;
+      } else if (j == T - 1) {
+        if (fabs(a - 1.0) < 1e-5) {
+          if (k < c) plan[cand].g += 1;
+          k++;
+        } else {
+          sc->pivbad = 1u;
+          return;
+        }
+      }
+    }
+    if (k != c) {
+      sc->pivbad = 1u;
+      return;
+    }
+  }
+  int64_t lb = 0;
+  for (int64_t j = 0; j < T; j++) {
+    plan[j].lb = lb;
+    lb += plan[j].g;
+    const double gd = (double)plan[j].g;
+    plan[j].dir = gd > plan[j].a ? 1 : (gd < plan[j].a ? -1 : 0);
+  }
+}
+
+// Chunk range [lo, hi) staged tile by tile: the TILE elements from lo + tb.
+__device__ inline void stage_range(const double* __restrict__ src, int64_t base, int64_t end, double* lds) {
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const int e = r * NT + threadIdx.x;
+    const int64_t i = base + e;
+    lds[e + e / ITEMS] = i < end ? src[i] : 0.0;
+  }
+  __syncthreads();
+}
+
+// The move of each chunk toward g (one block per chunk), in index order:
+// up y = min(1, q/t), down y = max(0, (q - t)/(1 - t)); the running sum
+// S_i = |q_j| + sum_{l<=i} (FXP(y_l) - FXP(q_l)) (exact, monotone) first
+// reaching g (up: S >= g, down: S <= g) at i, which takes FXP(y_i) + g - S_i;
+// later elements keep q.  Writes the chunk's values to qa.
+__global__ __launch_bounds__(NT) void k_ps_move(const double* __restrict__ q, int64_t n, int64_t T, int64_t step,
+                                                const u128* __restrict__ QJ, const PsPlan* __restrict__ plan,
+                                                double* __restrict__ qa, CompScalars* sc) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  __shared__ double lds[LTILE];
+  __shared__ int cross_seen;
+  const int64_t j = blockIdx.x;
+  const PsPlan P = plan[j];
+  if (P.dir == 0) return;
+  const bool up = P.dir > 0;
+  const int64_t lo = j * step, hi = j == T - 1 ? n : lo + step;
+  const double tj = P.t;
+  const u128 G = (u128)P.g << 96, Q0 = QJ[j];
+  u128 D = 0;  // |sum of the changes so far|
+  if (threadIdx.x == 0) cross_seen = 0;
+  __syncthreads();
+  for (int64_t tb = lo; tb < hi; tb += TILE) {
+    stage_range(q, tb, hi, lds);
+    const bool before = cross_seen == 0;
+    const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
+    u128 ds = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+      const int64_t i = base + r;
+      if (i < hi) {
+        const double v = tile_elem(lds, r);
+        double y;
+        if (up) {
+          const double rr = v / tj;
+          y = rr < 1.0 ? rr : 1.0;
+        } else {
+          const double rr = (v - tj) / (1.0 - tj);
+          y = 0.0 < rr ? rr : 0.0;
+        }
+        const u128 Fy = fx_of(dbits(y), 1023u), Fq = fx_of(dbits(v), 1023u);
+        ds += up ? Fy - Fq : Fq - Fy;
+      }
+    }
+    u128 tot;
+    u128 pre = block_excl_scan_u128<NT>(ds, &tot, sm) + D;
+    bool found = false;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+      const int64_t i = base + r;
+      if (i < hi) {
+        const double v = tile_elem(lds, r);
+        double out = v;
+        if (before) {
+          double y;
+          if (up) {
+            const double rr = v / tj;
+            y = rr < 1.0 ? rr : 1.0;
+          } else {
+            const double rr = (v - tj) / (1.0 - tj);
+            y = 0.0 < rr ? rr : 0.0;
+          }
+          const u128 Fy = fx_of(dbits(y), 1023u), Fq = fx_of(dbits(v), 1023u);
+          const u128 Sp = up ? Q0 + pre : Q0 - pre;
+          pre += up ? Fy - Fq : Fq - Fy;
+          const u128 S = up ? Q0 + pre : Q0 - pre;
+          const bool hit = up ? S >= G : S <= G;
+          const bool hitp = i == lo ? false : (up ? Sp >= G : Sp <= G);
+          if (!hitp) {
+            out = y;
+            if (hit) {
+              // negative (or above 1) only when the chunk held more than g
+              // already (q not summing to n'): reported like any p outside [0, 1]
+              out = Fy + G >= S ? u128_to_double(Fy + G - S, -96) : -u128_to_double(S - Fy - G, -96);
+              if (!(out >= 0.0 && out <= 1.0)) atomicOr(&sc->pflag, 1u);
+              found = true;
+            }
+          }
+        }
+        qa[i] = out;
+      }
+    }
+    D += tot;
+    if (__syncthreads_or(found) && threadIdx.x == 0) cross_seen = 1;
+    __syncthreads();
+  }
+}
+
+// Each chunk's own canonical pivotal chain (one block per chunk, its tiles in
+// order): g samples written at plan.lb, indices shifted by the chunk start.
+template <bool HASU>
+__global__ __launch_bounds__(NT) void k_ps_chain(const double* __restrict__ q, const double* __restrict__ qa,
+                                                 const double* __restrict__ U, int64_t n, int64_t T, int64_t step,
+                                                 int64_t ns_total, const PsPlan* __restrict__ plan, CompScalars* sc,
+                                                 uint32_t* __restrict__ out_idx) {
+  __shared__ u128 sm[NT / WAVE + 1];
+  __shared__ uint32_t sm32[NT / WAVE + 1];
+  __shared__ double lds[LTILE];
+  __shared__ double ulds[HASU ? LTILE : 1];
+  __shared__ int stop_seen;
+  const int64_t j = blockIdx.x;
+  const PsPlan P = plan[j];
+  const uint64_t key = sc->key;
+  const int64_t lo = j * step, hi = j == T - 1 ? n : lo + step, len = hi - lo;
+  const double* src = P.dir != 0 ? qa : q;
+  const int64_t g = P.g;
+  u128 Q = 0;          // exact prefix before the tile
+  uint32_t cenc = 0;   // candidate so far, local index + 1 (0: the initial candidate, local 0)
+  uint32_t bad = 0;
+  if (threadIdx.x == 0) stop_seen = 0;
+  __syncthreads();
+  for (int64_t tb = 0; tb < len; tb += TILE) {
+    stage_range(src, lo + tb, hi, lds);
+    if (HASU) stage_range(U, lo + tb, hi, ulds);
+    const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;  // local index of the thread's first element
+    u128 st = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+      const double v = tile_elem(lds, r);
+      if (base + r < len) {
+        bad |= !(v >= 0.0 && v <= 1.0);
+        st += fx_of(dbits(v >= 0.0 && v <= 1.0 ? v : 0.0), 1023u);
+      }
+    }
+    u128 tot;
+    const u128 pre = block_excl_scan_u128<NT>(st, &tot, sm) + Q;
+    // decisions: the last accepted step of this thread, and the stop step
+    uint32_t last = 0;
+    bool stop = false;
+    {
+      u128 Qp = pre;
+#pragma unroll
+      for (int r = 0; r < ITEMS; r++) {
+        const int64_t li = base + r;
+        if (li < len) {
+          const double v = tile_elem(lds, r);
+          const double p = v >= 0.0 && v <= 1.0 ? v : 0.0;
+          const u128 F = fx_of(dbits(p), 1023u);
+          if (li >= 1) {
+            int64_t cp, cn;
+            double a;
+            piv_step(Qp, F, li, cp, cn, a);
+            const double uj = HASU ? tile_elem(ulds, r) : splitmix_u(key, lo + li);
+            if (piv_accept(v, a, cn > cp, uj)) last = (uint32_t)(li + 1);
+            if (cn == g && (li == 1 || cp < g)) stop = true;
+          }
+          Qp += F;
+        }
+      }
+    }
+    uint32_t bmax;
+    uint32_t cin = block_excl_max_u32<NT>(last, &bmax, sm32);
+    cin = cin > cenc ? cin : cenc;
+    // emissions at completing steps
+    {
+      u128 Qp = pre;
+      uint32_t cand = cin;
+#pragma unroll
+      for (int r = 0; r < ITEMS; r++) {
+        const int64_t li = base + r;
+        if (li < len) {
+          const double v = tile_elem(lds, r);
+          const double p = v >= 0.0 && v <= 1.0 ? v : 0.0;
+          const u128 F = fx_of(dbits(p), 1023u);
+          if (li >= 1) {
+            int64_t cp, cn;
+            double a;
+            piv_step(Qp, F, li, cp, cn, a);
+            const double uj = HASU ? tile_elem(ulds, r) : splitmix_u(key, lo + li);
+            const bool cross = cn > cp;
+            const bool acc = piv_accept(v, a, cross, uj);
+            if (cross && cn - 1 < g && P.lb + cn - 1 < ns_total) {
+              const int64_t e = acc ? (cand == 0 ? 0 : (int64_t)cand - 1) : li;
+              out_idx[P.lb + cn - 1] = (uint32_t)(lo + e);
+            }
+            if (acc) cand = (uint32_t)(li + 1);
+          }
+          Qp += F;
+        }
+      }
+    }
+    Q += tot;
+    cenc = cenc > bmax ? cenc : bmax;
+    if (__syncthreads_or(stop) && threadIdx.x == 0) stop_seen = 1;
+    __syncthreads();
+    if (stop_seen) break;
+  }
+  if (__syncthreads_or(bad)) {
+    if (threadIdx.x == 0) atomicOr(&sc->pflag, 1u);
+    return;
+  }
+  if (threadIdx.x != 0 || stop_seen) return;
+  // no stop step: the last step's mass within 1e-5 of 1 emits the candidate
+  if (len == 1) {
+    if (g != 0) sc->pivbad = 1u;
+    return;
+  }
+  const int64_t fq = (int64_t)(Q >> 96);
+  const int64_t cl = fq < len - 1 ? fq : len - 1;
+  const double a = u128_to_double(Q - ((u128)cl << 96), -96);
+  if (!(fabs(a - 1.0) < 1e-5)) {
+    sc->pivbad = 1u;
+    return;
+  }
+  if (cl < g && P.lb + cl < ns_total) out_idx[P.lb + cl] = (uint32_t)(lo + (cenc == 0 ? 0 : (int64_t)cenc - 1));
+  if (cl + 1 != g) sc->pivbad = 1u;
+}
+
 // D part of the output (or all of it for "largest").
 __global__ void k_emit_sorted(const double* __restrict__ x, const uint32_t* __restrict__ vals,
                               const CompScalars* __restrict__ sc, int64_t cnt_if_largest,
@@ -1374,6 +1710,48 @@ int sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* U
   }
   if (h.pivbad) {
     set_error("sample_pivotal: the chain ended with a sample count other than n_sample (reference exits, "
+              "src/fri_utils.cpp:100-123)");
+    return ESAMPLE;
+  }
+  return OK;
+}
+
+// parallel_sample of n_sample indices from q[n] with T chunks.
+int parallel_sample(const double* q, int64_t n, int64_t n_sample, int64_t T, const double* U, double u,
+                    uint32_t* out_idx, hipStream_t s) {
+  FRICC_CHECK(n > 0 && n < (int64_t)1 << 31, "parallel_sample: length must be in [1, 2^31)");
+  FRICC_CHECK(n_sample >= 0, "parallel_sample: negative sample count");
+  FRICC_CHECK(T >= 1 && T <= n && T <= 65536, "parallel_sample: chunk count must lie in [1, min(n, 65536)]");
+  FRICC_CHECK(U || (u >= 0.0 && u <= 1.0), "parallel_sample: uniform must lie in [0, 1]");
+  const int64_t step = n / T;
+  void *p_sc, *p_qj, *p_plan, *p_qa;
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_TS, (size_t)T * sizeof(u128), &p_qj));
+  FRICC_TRY(ws_get(WS_PST, (size_t)T * sizeof(PsPlan), &p_plan));
+  FRICC_TRY(ws_get(WS_K0, (size_t)n * 8, &p_qa));
+  CompScalars* sc = (CompScalars*)p_sc;
+  const u128* QJ = (const u128*)p_qj;
+  PsPlan* plan = (PsPlan*)p_plan;
+  k_init_scalars<<<1, NT, 0, s>>>(sc, 0, n_sample, U ? 0.0 : u, nullptr, nullptr, 0);
+  k_ps_chunksum<<<T, NT, 0, s>>>(q, n, T, step, (u128*)p_qj, sc);
+  k_ps_plan<<<1, 64, 0, s>>>(QJ, T, step, U, plan, sc);
+  k_ps_move<<<T, NT, 0, s>>>(q, n, T, step, QJ, plan, (double*)p_qa, sc);
+  if (n_sample > 0) k_piv_clear<<<ceil_div(n_sample, NT), NT, 0, s>>>(sc, n_sample, out_idx, nullptr);
+  if (U) {
+    k_ps_chain<true><<<T, NT, 0, s>>>(q, (const double*)p_qa, U, n, T, step, n_sample, plan, sc, out_idx);
+  } else {
+    k_ps_chain<false><<<T, NT, 0, s>>>(q, (const double*)p_qa, U, n, T, step, n_sample, plan, sc, out_idx);
+  }
+  FRICC_HIP(hipGetLastError());
+  CompScalars h;
+  FRICC_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FRICC_HIP(hipStreamSynchronize(s));
+  if (h.pflag) {
+    set_error("parallel_sample: a probability lies outside [0, 1]");
+    return EPROB;
+  }
+  if (h.pivbad) {
+    set_error("parallel_sample: a chain ended with a sample count other than asked (reference exits, "
               "src/fri_utils.cpp:100-123)");
     return ESAMPLE;
   }

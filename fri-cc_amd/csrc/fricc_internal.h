@@ -47,6 +47,9 @@ int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, ui
 // n doubles) or, with U null, SplitMix64 keyed by the bits of u.
 int sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* U, double u, uint32_t* out_idx,
                    hipStream_t s);
+// parallel_sample with T chunks (draws by position as in sample_pivotal).
+int parallel_sample(const double* q, int64_t n, int64_t n_sample, int64_t T, const double* U, double u,
+                    uint32_t* out_idx, hipStream_t s);
 
 // A gathered view of a sparse 4-index tensor as a matrix Tm[r, y]:
 // rows r = (i_rh, i_rl), columns y = (i_yh, i_yl) (positions into the tensor's

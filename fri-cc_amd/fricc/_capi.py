@@ -25,7 +25,7 @@ EXPORTS = (
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
     "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
     "fricc_compress_noalloc", "fricc_compress_noalloc_max_n", "fricc_dense_contract",
-    "fricc_split_bf16_hi", "fricc_sample_pivotal",
+    "fricc_split_bf16_hi", "fricc_sample_pivotal", "fricc_parallel_sample",
 )
 
 
@@ -82,6 +82,7 @@ def lib():
     L.fricc_compress_noalloc_max_n.argtypes = []
     L.fricc_sample_systematic.argtypes = [_vp, _i64, _i64, ctypes.c_double, _vp, ctypes.POINTER(_i64), _vp]
     L.fricc_sample_pivotal.argtypes = [_vp, _i64, _i64, _vp, ctypes.c_double, _vp, _vp]
+    L.fricc_parallel_sample.argtypes = [_vp, _i64, _i64, _i64, _vp, ctypes.c_double, _vp, _vp]
     L.fricc_sparse_create.argtypes = [I64x4, ctypes.POINTER(_vp)]
     L.fricc_sparse_destroy.argtypes = [_vp]
     L.fricc_sparse_compress.argtypes = [_vp, _vp, _i64, ctypes.c_int, ctypes.c_double, _vp]

@@ -458,6 +458,29 @@ def sample_pivotal(n_sample, probs, *, u=None, uniforms=None):
     return out[:n_sample].cpu().numpy().astype(np.int64).tolist()
 
 
-def parallel_sample(n_sample, probs):
-    """Thread-partitioned pivotal sampling (src/fri_utils.cpp:176-362): not on the GPU path."""
-    raise NotImplementedError("parallel pivotal sampling is not on the GPU path")
+PARALLEL_CHUNKS = 256
+
+
+def parallel_sample(n_sample, probs, *, n_chunks=None, u=None, uniforms=None):
+    """Thread-partitioned pivotal sampling (src/fri_utils.cpp:176-362) with
+    ``n_chunks`` chunks (default min(256, len(probs))) in place of the
+    reference's OpenMP threads, whose count its result depends on.  Draws as
+    ``sample_pivotal``, by position.  Raises where the reference exits."""
+    torch = _torch()
+    pd = _as_device_vector(np.asarray(probs, dtype=np.float64) if not _is_torch(probs) else probs, "probs")
+    n_sample = int(n_sample)
+    T = int(min(PARALLEL_CHUNKS, pd.numel()) if n_chunks is None else n_chunks)
+    Ud = None
+    uu = 0.0
+    if uniforms is not None:
+        Ud = _as_device_vector(np.asarray(uniforms, dtype=np.float64) if not _is_torch(uniforms) else uniforms,
+                               "uniforms")
+        if Ud.numel() < pd.numel():
+            raise ValueError("uniforms must hold one value per probability")
+    else:
+        uu = _draw(u)
+    out = torch.zeros(max(n_sample, 1), dtype=torch.int32, device="cuda")
+    _capi.check(_capi.lib().fricc_parallel_sample(ctypes.c_void_p(pd.data_ptr()), pd.numel(), n_sample, T,
+                                                  ctypes.c_void_p(Ud.data_ptr() if Ud is not None else None), uu,
+                                                  ctypes.c_void_p(out.data_ptr()), _capi.stream_ptr()))
+    return out[:n_sample].cpu().numpy().astype(np.int64).tolist()

@@ -113,6 +113,22 @@ int fricc_sample_systematic(const double* p, int64_t n, int64_t n_sample, double
 int fricc_sample_pivotal(const double* p, int64_t n, int64_t n_sample, const double* uniforms, double u,
                          uint32_t* out_idx, void* stream);
 
+/* parallel_sample(n_sample, probs) (src/fri_utils.cpp:176-362) with n_chunks
+ * chunks in place of the reference's OpenMP threads (its result depends on
+ * that count): chunk j = [j*step, (j+1)*step), the last one taking the
+ * remainder, step = n / n_chunks.  Expected counts a_j = n_sample |p_j| / |p|
+ * (canonical exact sums, rounded once), a pivotal chain over their fractional
+ * parts picks the chunks that get one more sample, each chunk's
+ * probabilities move in index order toward that count (up: min(1, p/t), down:
+ * max(0, (p - t)/(1 - t)), exact running sum, the crossing element takes the
+ * remainder), then each chunk runs the canonical pivotal chain; chunk j's
+ * samples follow those of chunks 0..j-1.  Draws by position: the chunk
+ * chain's step j takes uniforms[j*step], chunk j's step i uniforms[j*step+i]
+ * (or the SplitMix64 stream keyed by u at those positions).  n_chunks in
+ * [1, min(n, 65536)].  FRICC_EPROB / FRICC_ESAMPLE as fricc_sample_pivotal. */
+int fricc_parallel_sample(const double* p, int64_t n, int64_t n_sample, int64_t n_chunks, const double* uniforms,
+                          double u, uint32_t* out_idx, void* stream);
+
 /* SparseTensor4d(dims, nnz) -- empty tensor (include/sparse_tensor.hpp:27-31). */
 int fricc_sparse_create(const int64_t dims[4], fricc_sparse** out);
 int fricc_sparse_destroy(fricc_sparse* sp);

@@ -324,6 +324,121 @@ int orc_pivotal(const double* p, int64_t n, int64_t ns, const double* U, uint64_
   return k == ns ? 0 : 5;
 }
 
+// ---------------------------------------------------------------------------
+// Thread-partitioned pivotal sampling, parallel_sample (src/fri_utils.cpp:176-362),
+// with T chunks in place of the reference's T OpenMP threads: chunk j is
+// [j*step, (j+1)*step) (the last one takes the remainder), step = n / T.
+//   a_j = n' * |q_j| / |q| (FP64, that order), t_j = a_j - floor(a_j),
+//   g_j = floor(a_j); c = n' - sum g_j more samples go to the chunks a pivotal
+//   chain over t picks (g_j += 1); each chunk's probabilities are then moved
+//   in index order toward a sum of g_j -- up: y = min(1, q/t_j), down:
+//   y = max(0, (q - t_j)/(1 - t_j)) -- until the running sum crosses g_j,
+//   where the crossing element takes what makes the sum g_j exactly; the
+//   chunk then runs its own pivotal chain for g_j samples, written at the
+//   chunk's offset (prefix of g) with indices shifted by j*step.
+// Draws: the chunk chain's step j takes the uniform of position j*step, the
+// chain of chunk j its step i the uniform of position j*step + i (positions
+// never shared; piv_uniform).
+// CANONICAL (mode 0): |q| and |q_j| are the exact sums of FXP(q) rounded
+// once, the running sum of the move is exact (FXP units), the crossing
+// element takes round(FXP(y) + g - S), and every chain is the canonical one.
+// FAITHFUL (mode 1): the reference's FP64 sums (sequential) and chains.
+// Returns 0, 1 (q outside [0, 1]), 2 (T outside [1, n]) or 5 (a reference
+// exit in one of the chains, or fewer than 0 extra samples).
+int orc_parallel_sample(const double* q, int64_t n, int64_t ns, int64_t T, const double* U, uint64_t key, int mode,
+                        int64_t* S) {
+  if (n <= 0 || ns < 0 || T < 1 || T > n) return 2;
+  for (int64_t i = 0; i < n; i++)
+    if (!(q[i] >= 0.0 && q[i] <= 1.0)) return 1;
+  const int64_t step = n / T, rem = n % T;
+  std::vector<int64_t> lo(T), hi(T), g(T);
+  std::vector<u128> QJ(T, 0);
+  std::vector<double> qjn(T), a(T), t(T);
+  u128 Qt = 0;
+  double qn = 0.0;
+  for (int64_t j = 0; j < T; j++) {
+    lo[j] = j * step;
+    hi[j] = (j + 1) * step + (j == T - 1 ? rem : 0);
+    double sj = 0.0;
+    for (int64_t i = lo[j]; i < hi[j]; i++) {
+      QJ[j] += fx_of(dbits(q[i]), 1023u);
+      sj += fabs(q[i]);
+    }
+    Qt += QJ[j];
+    qjn[j] = mode == 1 ? sj : u128_to_double(QJ[j], -96);
+  }
+  if (mode == 1) {
+    for (int64_t i = 0; i < n; i++) qn += fabs(q[i]);
+  } else {
+    qn = u128_to_double(Qt, -96);
+  }
+  if (qn == 0.0) return ns == 0 ? 0 : 5;
+  int64_t c = ns;
+  for (int64_t j = 0; j < T; j++) {
+    a[j] = (double)ns * qjn[j] / qn;
+    t[j] = a[j] - floor(a[j]);
+    g[j] = (int64_t)floor(a[j]);
+    c -= g[j];
+  }
+  if (c < 0) return 5;
+  {
+    std::vector<double> Uc(T);
+    for (int64_t j = 0; j < T; j++) Uc[j] = piv_uniform(U, key, j * step);
+    std::vector<int64_t> sp(std::max<int64_t>(c, 1));
+    int64_t cnt = 0;
+    const int st = orc_pivotal(t.data(), T, c, Uc.data(), 0, mode, sp.data(), &cnt);
+    if (st != 0) return st;
+    for (int64_t k = 0; k < c; k++) g[sp[k]] += 1;
+  }
+  int64_t lb = 0;
+  for (int64_t j = 0; j < T; j++) {
+    const int64_t len = hi[j] - lo[j];
+    std::vector<double> qa(q + lo[j], q + hi[j]);
+    const bool up = (double)g[j] > a[j], down = (double)g[j] < a[j];
+    if (up || down) {
+      const double tj = t[j];
+      u128 Sx = QJ[j];
+      const u128 G = (u128)g[j] << 96;
+      double sj = qjn[j];
+      for (int64_t i = 0; i < len; i++) {
+        double y;
+        if (up) {
+          const double r = qa[i] / tj;
+          y = r < 1.0 ? r : 1.0;
+        } else {
+          const double r = (qa[i] - tj) / (1.0 - tj);
+          y = 0.0 < r ? r : 0.0;
+        }
+        bool cross;
+        if (mode == 1) {
+          sj += y - qa[i];
+          cross = up ? sj >= (double)g[j] : sj <= (double)g[j];
+          qa[i] = y;
+          if (cross) qa[i] = y + (double)g[j] - sj;
+        } else {
+          const u128 Fy = fx_of(dbits(y), 1023u);
+          Sx = Sx - fx_of(dbits(qa[i]), 1023u) + Fy;
+          cross = up ? Sx >= G : Sx <= G;
+          qa[i] = y;
+          // Fy + G - S, negative when a chunk already held more than g (q
+          // not summing to n'): the chain then rejects it as outside [0, 1]
+          if (cross) qa[i] = Fy + G >= Sx ? u128_to_double(Fy + G - Sx, -96) : -u128_to_double(Sx - Fy - G, -96);
+        }
+        if (cross) break;
+      }
+    }
+    std::vector<double> Uj(len);
+    for (int64_t i = 0; i < len; i++) Uj[i] = piv_uniform(U, key, lo[j] + i);
+    std::vector<int64_t> Sj(std::max<int64_t>(g[j], 1));
+    int64_t cnt = 0;
+    const int st = orc_pivotal(qa.data(), len, g[j], Uj.data(), 0, mode, Sj.data(), &cnt);
+    if (st != 0) return st;
+    for (int64_t k = 0; k < g[j]; k++) S[lb + k] = Sj[k] + j * step;
+    lb += g[j];
+  }
+  return 0;
+}
+
 // FRI / largest compression.  method 0 = "largest" (include/sparse_tensor.hpp:46-56),
 // method 1 = "fri" + "systematic" (include/sparse_tensor.hpp:58-67), method 2 =
 // "fri" + "pivotal" (src/fri_utils.cpp:486-489), its step draws SplitMix64

@@ -60,6 +60,9 @@ def lib():
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_pivotal.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, dp, ctypes.c_uint64, ctypes.c_int, ip, ip]
         L.orc_pivotal.restype = ctypes.c_int
+        L.orc_parallel_sample.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, dp, ctypes.c_uint64,
+                                          ctypes.c_int, ip]
+        L.orc_parallel_sample.restype = ctypes.c_int
         L.orc_splitmix_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_int64, dp]
         L.orc_splitmix_uniforms.restype = None
         _LIB = L
@@ -118,6 +121,24 @@ def pivotal(p, n_sample, U=None, key=0, mode=CANONICAL):
         Up = _dp(U)
     st = lib().orc_pivotal(_dp(p), p.size, ns, Up, ctypes.c_uint64(int(key)), int(mode), _ip(S), ctypes.byref(cnt))
     return S[:ns], cnt.value, st
+
+
+def parallel_sample(q, n_sample, n_chunks, U=None, key=0, mode=CANONICAL):
+    """parallel_sample (src/fri_utils.cpp:176-362) with n_chunks chunks in place
+    of the reference's OpenMP threads.  Returns (S int64[n_sample], status):
+    0, 1 (q outside [0, 1]), 2 (n_chunks outside [1, n]) or 5 (a reference
+    exit)."""
+    q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+    ns = int(n_sample)
+    S = np.zeros(max(ns, 1), dtype=np.int64)
+    Up = None
+    if U is not None:
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        assert U.size >= q.size
+        Up = _dp(U)
+    st = lib().orc_parallel_sample(_dp(q), q.size, ns, int(n_chunks), Up, ctypes.c_uint64(int(key)), int(mode),
+                                   _ip(S))
+    return S[:ns], st
 
 
 def norm_canonical(x):

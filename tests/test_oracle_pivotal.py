@@ -205,3 +205,139 @@ def test_fri_pivotal_unbiased():
     err = np.abs(acc / runs - x).sum() / np.abs(x).sum()
     assert err < 0.03, err
     assert math.isfinite(err)
+
+
+# ---------------------------------------------------------------- parallel_sample
+# parallel_sample (src/fri_utils.cpp:176-362) with T chunks for the reference's
+# T OpenMP threads.  Pinned by: T = 1 is sample_pivotal itself; chunks with
+# integer sums need no extra samples and no move, so the result is each
+# chunk's own pivotal chain at its offset; and the inclusion probabilities,
+# predicted from the reference's move (:290-316) restated in exact rationals
+# (P(chunk j gets g_j + 1) = t_j by the pivotal design), match the oracle's
+# frequencies over many draws.
+
+def _chunks(n, T):
+    step = n // T
+    return [(j * step, (j + 1) * step + (n % T if j == T - 1 else 0)) for j in range(T)]
+
+
+@pytest.mark.parametrize("mode", [lo.CANONICAL, lo.FAITHFUL])
+def test_parallel_one_chunk_is_pivotal(mode):
+    """One chunk: a_0 = n' |q| / |q|, which is n' whenever that FP64 quotient
+    is exact (always on dyadic q); no extra sample, no move, and the chunk's
+    chain is sample_pivotal on the same draws."""
+    rng = np.random.default_rng(3)
+    checked = 0
+    for n, ns, dyadic in ((50, 7, True), (3000, 400, True), (10_000, 900, False), (777, 50, False)):
+        q = _dyadic_p(n, n, denom=64) if dyadic else rng.random(n)
+        if dyadic:
+            ns = int(round(q.sum()))
+        else:
+            q = q / q.sum() * ns
+        qn = math.fsum(q) if mode == lo.CANONICAL else float(np.cumsum(q)[-1])
+        if ns * qn / qn != ns:
+            continue
+        U = rng.random(n)
+        S1, st1 = lo.parallel_sample(q, ns, 1, U=U, mode=mode)
+        S2, cnt, st2 = lo.pivotal(q, ns, U=U, mode=mode)
+        assert st1 == 0 and st2 == 0
+        assert np.array_equal(S1, S2)
+        checked += 1
+    assert checked >= 2
+
+
+@pytest.mark.parametrize("T", [2, 5, 16])
+def test_parallel_integer_chunks_are_independent_chains(T):
+    rng = np.random.default_rng(T)
+    n = 4000
+    q = np.zeros(n)
+    for lo_, hi_ in _chunks(n, T):  # dyadic values, integer sum per chunk
+        k = rng.integers(0, 65, hi_ - lo_)
+        r, j = int(k.sum()) % 64, 0
+        while r:
+            d = min(r, int(k[j]))
+            k[j] -= d
+            r -= d
+            j += 1
+        q[lo_:hi_] = k / 64.0
+    ns = int(round(q.sum()))
+    U = rng.random(n)
+    for mode in (lo.CANONICAL, lo.FAITHFUL):
+        S, st = lo.parallel_sample(q, ns, T, U=U, mode=mode)
+        assert st == 0
+        off = 0
+        for lo_, hi_ in _chunks(n, T):
+            g = int(round(q[lo_:hi_].sum()))
+            Sj, cnt, stj = lo.pivotal(q[lo_:hi_], g, U=U[lo_:hi_], mode=mode)
+            assert stj == 0
+            assert np.array_equal(S[off:off + g], Sj + lo_)
+            off += g
+
+
+def _moved(q, tj, g, up):
+    """The reference's move of one chunk toward a sum of g, in exact rationals."""
+    qa = [Fraction(float(v)) for v in q]
+    s = sum(qa)
+    for i in range(len(qa)):
+        y = min(Fraction(1), qa[i] / tj) if up else max(Fraction(0), (qa[i] - tj) / (1 - tj))
+        s += y - qa[i]
+        qa[i] = y
+        if (s >= g) if up else (s <= g):
+            qa[i] = y + g - s
+            break
+    return qa
+
+
+def test_parallel_inclusion_probabilities():
+    rng = np.random.default_rng(11)
+    n, T, ns = 30, 3, 7
+    q = rng.random(n) ** 2
+    q = q / q.sum() * ns
+    assert q.max() < 1
+    pi = np.zeros(n)
+    Fq = [Fraction(float(v)) for v in q]
+    tot = sum(Fq)
+    for lo_, hi_ in _chunks(n, T):
+        a = ns * sum(Fq[lo_:hi_]) / tot
+        g0 = math.floor(a)
+        t = a - g0
+        up = _moved(q[lo_:hi_], t, g0 + 1, True) if t else Fq[lo_:hi_]
+        dn = _moved(q[lo_:hi_], t, g0, False) if t else Fq[lo_:hi_]
+        pi[lo_:hi_] = [float(t * u + (1 - t) * d) for u, d in zip(up, dn)]
+    runs = 40000
+    hits = np.zeros(n)
+    for r in range(runs):
+        S, st = lo.parallel_sample(q, ns, T, key=77 + r)
+        assert st == 0 and len(set(S.tolist())) == ns
+        hits[S] += 1
+    se = np.sqrt(pi * (1 - pi) / runs) + 1e-9
+    assert np.all(np.abs(hits / runs - pi) <= 5 * se), np.abs(hits / runs - pi) / se
+    assert abs(pi.sum() - ns) < 1e-9
+
+
+@pytest.mark.parametrize("T", [2, 7, 64])
+def test_parallel_invariants(T):
+    rng = np.random.default_rng(100 + T)
+    n, ns = 20_000, 3_000
+    q = rng.random(n) * np.exp(-rng.uniform(0, 2, n))
+    q = q / q.sum() * ns
+    assert q.max() <= 1.0
+    for mode in (lo.CANONICAL, lo.FAITHFUL):
+        S, st = lo.parallel_sample(q, ns, T, key=5, mode=mode)
+        assert st == 0 and len(set(S.tolist())) == ns
+        off = 0
+        for lo_, hi_ in _chunks(n, T):
+            a = ns * q[lo_:hi_].sum() / q.sum()
+            cnt = int(((S >= lo_) & (S < hi_)).sum())
+            assert cnt in (math.floor(a), math.floor(a) + 1) or abs(a - round(a)) < 1e-9
+            assert np.all((S[off:off + cnt] >= lo_) & (S[off:off + cnt] < hi_))  # chunk order
+            off += cnt
+
+
+def test_parallel_bad_arguments():
+    q = np.full(10, 0.5)
+    assert lo.parallel_sample(q, 5, 0)[1] == 2
+    assert lo.parallel_sample(q, 5, 11)[1] == 2
+    assert lo.parallel_sample(np.array([0.5, 1.5]), 2, 1)[1] == 1
+    assert lo.parallel_sample(np.zeros(10), 0, 3)[1] == 0
+    assert lo.parallel_sample(np.zeros(10), 2, 3)[1] == 5

@@ -187,3 +187,61 @@ def test_noalloc_pivotal_matches_oracle(mods):
     oi, ov, oinfo = lo.compress(x, m, lo.FRI_PIVOTAL, 0.45)
     assert np.array_equal(idx.cpu().numpy().astype(np.int64), oi)
     assert np.array_equal(val.cpu().numpy().view(np.int64), ov.view(np.int64))
+
+
+# ---------------------------------------------------------------- parallel_sample
+def _ps_gpu(mods, q, ns, T, U=None, u=None):
+    py_rccsd = mods[0]
+    return np.array(py_rccsd.parallel_sample(ns, q, n_chunks=T, u=u, uniforms=U), dtype=np.int64)
+
+
+def _ps_check(mods, q, ns, T, U=None, u=None):
+    _, lo, _, _capi = mods
+    key = lo.key_of(u) if U is None else 0
+    S, st = lo.parallel_sample(q, ns, T, U=U, key=key)
+    if st == 0:
+        G = _ps_gpu(mods, q, ns, T, U=U, u=u)
+        assert np.array_equal(G, S), (q.size, ns, T, np.flatnonzero(G != S)[:10])
+    else:
+        want = {1: _capi.FRICC_EPROB, 5: _capi.FRICC_ESAMPLE}[st]
+        with pytest.raises(_capi.FriccError) as e:
+            _ps_gpu(mods, q, ns, T, U=U, u=u)
+        assert e.value.status == want
+    return st
+
+
+@pytest.mark.parametrize("n,ns,T", [(100, 20, 3), (5000, 700, 7), (4096 * 3 + 5, 3000, 2), (70_000, 9_000, 64),
+                                    (1_000_003, 120_000, 256), (3_000_000, 400_000, 7), (200_000, 30_000, 4096)])
+def test_parallel_sample_bitexact(mods, n, ns, T):
+    p = _probs(n, ns, 3 * n + T)
+    print(f"parallel_sample n={n} ns={ns} T={T}", flush=True)
+    assert _ps_check(mods, p, ns, T, U=np.random.default_rng(n).random(n)) == 0
+    assert _ps_check(mods, p, ns, T, u=0.77) == 0
+
+
+def test_parallel_sample_edges(mods):
+    py_rccsd, lo, _, _capi = mods
+    rng = np.random.default_rng(4)
+    p = _probs(3000, 500, 9)
+    U = rng.random(p.size)
+    # one chunk (its expected count exactly n' here): the plain chain
+    assert lo.parallel_sample(p, 500, 1, U=U)[1] == 0
+    assert _ps_gpu(mods, p, 500, 1, U=U).tolist() == py_rccsd.sample_pivotal(500, p, uniforms=U)
+    # integer chunk sums: no extra samples, no move
+    q = np.zeros(8192)
+    q[:4096] = 0.25
+    q[4096:] = 0.125
+    assert _ps_check(mods, q, 1536, 2, U=U[:1].repeat(8192)) == 0
+    # one element per chunk (a chunk asked for one sample exits, as in the reference)
+    _ps_check(mods, np.full(64, 0.5), 32, 64, U=np.zeros(64))
+    # no mass
+    assert py_rccsd.parallel_sample(0, np.zeros(100), n_chunks=4, u=0.5) == []
+    with pytest.raises(_capi.FriccError) as e:
+        py_rccsd.parallel_sample(3, np.zeros(100), n_chunks=4, u=0.5)
+    assert e.value.status == _capi.FRICC_ESAMPLE
+    with pytest.raises(_capi.FriccError) as e:
+        py_rccsd.parallel_sample(1, np.array([0.5, 1.5]), n_chunks=1, u=0.5)
+    assert e.value.status == _capi.FRICC_EPROB
+    for T in (0, 101):
+        with pytest.raises(ValueError):
+            py_rccsd.parallel_sample(3, np.full(100, 0.03), n_chunks=T, u=0.5)
