@@ -188,36 +188,8 @@ __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t 
 }
 
 int g_order = 1;
+int g_fp_lds = 1;  // FP64/FP32 through the LDS pipeline where the geometry allows
 
-int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const OutMap& m, double alpha,
-                  hipStream_t s) {
-  FRICC_CHECK(v->TY == GT_Y, "gathered_gemm: view tile width must be 64");
-  FRICC_CHECK(v->dtype == adtype, "gathered_gemm: operand dtypes differ");
-  FRICC_CHECK(X > 0 && X < ((int64_t)1 << 31), "gathered_gemm: bad X");
-  if (v->ntiles == 0) return OK;
-  OutDev om;
-  om.out = m.out;
-  om.of32 = m.of32;
-  om.Xl = (uint32_t)m.Xl;
-  om.Yl = (uint32_t)v->Yl;
-  om.oxh = m.oxh;
-  om.oxl = m.oxl;
-  om.oyh = m.oyh;
-  om.oyl = m.oyl;
-  const int64_t nbx = ceil_div(X, GT_X);
-  const int64_t nb = nbx * v->ntiles;
-  if (adtype == DT_F64) {
-    k_ggemm<double><<<nb, 256, 0, s>>>((const double*)A, X, v->tile_off, v->rowlist, (const double*)v->B, nbx,
-                                       v->ntiles, v->Y, om, alpha, g_order);
-  } else if (adtype == DT_F32) {
-    k_ggemm<float><<<nb, 256, 0, s>>>((const float*)A, X, v->tile_off, v->rowlist, (const float*)v->B, nbx,
-                                      v->ntiles, v->Y, om, alpha, g_order);
-  } else {
-    FRICC_CHECK(false, "gathered_gemm: unsupported dtype");
-  }
-  FRICC_HIP(hipGetLastError());
-  return OK;
-}
 
 // ---------------------------------------------------------------------------
 // BF16 gathered contraction on v_mfma_f32_16x16x32_bf16 (FP32 accumulate).
@@ -568,6 +540,222 @@ __global__ void k_splitk_reduce(const float* __restrict__ partial, int nsplit, i
           (int64_t)(xx / om.Xl) * om.oxh + (int64_t)(xx % om.Xl) * om.oxl + (int64_t)(yy / om.Yl) * om.oyh +
               (int64_t)(yy % om.Yl) * om.oyl,
           alpha, (double)acc);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 gathered contraction through LDS (v_mfma_f64_16x16x4f64,
+// v_mfma_f32_16x16x4f32).  A workgroup owns BXF x 64 outputs: x in
+// [bx*BXF, +BXF) with BXF = 1 KB of A per row (128 doubles or 256 floats) and
+// one 64-column view tile.  Per stage it gathers 16 rows: each row's 1 KB
+// segment of A is one global->LDS DMA (rows at a padded pitch so the four
+// rows one MFMA K step reads sit on different banks), the tile's 16 B rows
+// are contiguous bytes of the view (4 or 8 KB).  Three stages in flight; the
+// waves (2 x 2) own BXF/2 x 32 outputs each.  Each A element of a gathered
+// row is fetched once per workgroup (the register-loading kernel above read
+// it once per lane group).  Needs X a multiple of 16 bytes' worth of
+// elements and a 16-byte aligned A, so every 16-byte chunk lies inside one
+// row.
+template <typename T>
+struct FpGeom {
+  static constexpr int BXF = 1024 / (int)sizeof(T);  // x per workgroup: one 1 KB segment per row
+  static constexpr int KR = 16;                       // gathered rows per stage (4 MFMA K steps)
+  static constexpr int APITCH = 1024 + (sizeof(T) == 8 ? 128 : 64);
+  static constexpr int BROW = GT_Y * (int)sizeof(T);  // bytes of one view row
+  static constexpr int ABYTES = KR * APITCH;
+  static constexpr int BBYTES = KR * BROW;
+  static constexpr int PERB = BBYTES / 1024 / 4;      // view DMAs per wave per stage
+  static constexpr int WX = BXF / 2, WY = GT_Y / 2;   // outputs per wave
+  static constexpr int MS = WX / 16, NQ = WY / 16;    // MFMA blocks per wave
+};
+
+template <typename T>
+struct FpStage {
+  uint8_t A[FpGeom<T>::ABYTES];
+  uint8_t B[FpGeom<T>::BBYTES];
+};
+
+struct FArgs {
+  const void* A;  // [R][X]
+  int64_t X, Y;
+  const uint32_t* tile_off;
+  const uint32_t* tile_order;  // tiles by decreasing row count
+  const uint32_t* rowlist;
+  const void* B;  // view [rows][64]
+  int64_t nbx, ntiles;
+  OutDev om;
+  double alpha;
+  int order;
+};
+
+template <typename T>
+__device__ inline T lds_ld(const uint8_t* base, uint32_t off) {
+  return *(const T*)(base + off);
+}
+
+template <typename T>
+__device__ inline void fp_issue(const FArgs& a, cu32* rl, int64_t aoff, uint32_t k, FpStage<T>& sm, int w,
+                                int lane) {
+  typedef FpGeom<T> G;
+  const uint32_t la = lds_addr(&sm.A[0]), lb = lds_addr(&sm.B[0]);
+  const T* Ab = (const T*)a.A;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint32_t r = rl[k + 4 * w + i];
+    dma16(Ab + (int64_t)r * a.X + aoff, la + (uint32_t)((4 * w + i) * G::APITCH));
+  }
+  const uint8_t* Bb = (const uint8_t*)a.B + (int64_t)k * G::BROW;
+#pragma unroll
+  for (int j = 0; j < G::PERB; j++) {
+    const int c = w * G::PERB + j;
+    dma16(Bb + c * 1024 + 16 * lane, lb + (uint32_t)(c * 1024));
+  }
+}
+
+template <typename T>
+__device__ inline void fp_compute(const FpStage<T>& sm, typename Acc<T>::type (&acc)[FpGeom<T>::MS][FpGeom<T>::NQ],
+                                  int wx, int wy, int lr, int lk) {
+  typedef FpGeom<T> G;
+#pragma unroll
+  for (int kk = 0; kk < G::KR / 4; kk++) {
+    T av[G::MS], bv[G::NQ];
+    const uint32_t ar = (uint32_t)((4 * kk + lk) * G::APITCH), br = (uint32_t)((4 * kk + lk) * G::BROW);
+#pragma unroll
+    for (int s = 0; s < G::MS; s++) av[s] = lds_ld<T>(sm.A, ar + (uint32_t)((wx * G::WX + 16 * s + lr) * sizeof(T)));
+#pragma unroll
+    for (int q = 0; q < G::NQ; q++) bv[q] = lds_ld<T>(sm.B, br + (uint32_t)((wy * G::WY + 16 * q + lr) * sizeof(T)));
+#pragma unroll
+    for (int s = 0; s < G::MS; s++)
+#pragma unroll
+      for (int q = 0; q < G::NQ; q++) acc[s][q] = Acc<T>::mfma(av[s], bv[q], acc[s][q]);
+  }
+}
+
+template <typename T, int S>
+__global__ __launch_bounds__(256, 2) void k_ggemm_fp(FArgs a) {
+  typedef FpGeom<T> G;
+  typedef typename Acc<T>::type AT;
+  constexpr int LW = 4 + G::PERB;  // DMA instructions per wave per stage
+  __shared__ FpStage<T> st[S];
+  int64_t bx, ti;
+  xcd_remap(a.nbx, a.ntiles, a.order, bx, ti);
+  const int64_t t = a.tile_order[ti];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wx = w & 1, wy = w >> 1;
+  const uint32_t k_beg = a.tile_off[t], k_end = a.tile_off[t + 1];
+  const int64_t x0 = bx * G::BXF;
+  // this lane's 16-byte chunk of each row segment; chunks past X read the
+  // row's start instead (their columns are never stored)
+  const int64_t xl = x0 + lane * (16 / (int64_t)sizeof(T));
+  const int64_t aoff = xl < a.X ? xl : 0;
+  cu32* rl = (cu32*)a.rowlist;
+  AT acc[G::MS][G::NQ];
+#pragma unroll
+  for (int s = 0; s < G::MS; s++)
+#pragma unroll
+    for (int q = 0; q < G::NQ; q++) acc[s][q] = AT{0, 0, 0, 0};
+  uint32_t k = k_beg;
+  if (k < k_end) {
+#pragma unroll
+    for (int s = 0; s < S - 1; s++)
+      if (k + s * G::KR < k_end) fp_issue<T>(a, rl, aoff, k + s * G::KR, st[s], w, lane);
+    int cur = 0;
+    for (; k < k_end; k += G::KR) {
+      if (k + G::KR < k_end)
+        wait_vm<LW * (S - 2)>();
+      else
+        wait_vm<0>();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      const int nxt = cur == 0 ? S - 1 : cur - 1;
+      if (k + (S - 1) * G::KR < k_end) fp_issue<T>(a, rl, aoff, k + (S - 1) * G::KR, st[nxt], w, lane);
+      fp_compute<T>(st[cur], acc, wx, wy, lr, lk);
+      cur = cur == S - 1 ? 0 : cur + 1;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < G::NQ; q++) {
+    const int64_t y = t * GT_Y + wy * G::WY + 16 * q + lr;
+    if (y >= a.Y) continue;
+    const uint32_t yy = (uint32_t)y;
+    const int64_t oy = (int64_t)(yy / a.om.Yl) * a.om.oyh + (int64_t)(yy % a.om.Yl) * a.om.oyl;
+#pragma unroll
+    for (int h = 0; h < G::MS / 4; h++) {
+      int64_t off[16];
+      bool ok[16];
+      double vv[16];
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int64_t x = x0 + wx * G::WX + 16 * (4 * h + s) + Acc<T>::row(lk, i);
+          ok[4 * s + i] = x < a.X;
+          off[4 * s + i] = out_xoff(a.om, x < a.X ? x : 0) + oy;
+          vv[4 * s + i] = (double)acc[4 * h + s][q][i];
+        }
+      if (a.om.of32)
+        out_add_batch<float, 16>((float*)a.om.out, off, ok, a.alpha, vv);
+      else
+        out_add_batch<double, 16>((double*)a.om.out, off, ok, a.alpha, vv);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const OutMap& m, double alpha,
+                  hipStream_t s) {
+  FRICC_CHECK(v->TY == GT_Y, "gathered_gemm: view tile width must be 64");
+  FRICC_CHECK(v->dtype == adtype, "gathered_gemm: operand dtypes differ");
+  FRICC_CHECK(X > 0 && X < ((int64_t)1 << 31), "gathered_gemm: bad X");
+  if (v->ntiles == 0) return OK;
+  OutDev om;
+  om.out = m.out;
+  om.of32 = m.of32;
+  om.Xl = (uint32_t)m.Xl;
+  om.Yl = (uint32_t)v->Yl;
+  om.oxh = m.oxh;
+  om.oxl = m.oxl;
+  om.oyh = m.oyh;
+  om.oyl = m.oyl;
+  const size_t esz = adtype == DT_F64 ? 8 : 4;
+  if (g_fp_lds && v->TY == GT_Y && X % (int64_t)(16 / esz) == 0 && ((uintptr_t)A & 15) == 0 && v->tile_order) {
+    FArgs a;
+    a.A = A;
+    a.X = X;
+    a.Y = v->Y;
+    a.tile_off = v->tile_off;
+    a.tile_order = v->tile_order;
+    a.rowlist = v->rowlist;
+    a.B = v->B;
+    a.ntiles = v->ntiles;
+    a.om = om;
+    a.alpha = alpha;
+    a.order = g_order;
+    if (adtype == DT_F64) {
+      a.nbx = ceil_div(X, FpGeom<double>::BXF);
+      k_ggemm_fp<double, 3><<<a.nbx * a.ntiles, 256, 0, s>>>(a);
+    } else {
+      a.nbx = ceil_div(X, FpGeom<float>::BXF);
+      k_ggemm_fp<float, 3><<<a.nbx * a.ntiles, 256, 0, s>>>(a);
+    }
+    FRICC_HIP(hipGetLastError());
+    return OK;
+  }
+  const int64_t nbx = ceil_div(X, GT_X);
+  const int64_t nb = nbx * v->ntiles;
+  if (adtype == DT_F64) {
+    k_ggemm<double><<<nb, 256, 0, s>>>((const double*)A, X, v->tile_off, v->rowlist, (const double*)v->B, nbx,
+                                       v->ntiles, v->Y, om, alpha, g_order);
+  } else if (adtype == DT_F32) {
+    k_ggemm<float><<<nb, 256, 0, s>>>((const float*)A, X, v->tile_off, v->rowlist, (const float*)v->B, nbx,
+                                      v->ntiles, v->Y, om, alpha, g_order);
+  } else {
+    FRICC_CHECK(false, "gathered_gemm: unsupported dtype");
+  }
+  FRICC_HIP(hipGetLastError());
+  return OK;
 }
 
 int g_bf16_width = 0;  // 0: by operand size

@@ -104,6 +104,28 @@ def test_contraction_dense_random_fp64_tight(py_rccsd, term):
     npt.assert_allclose(out - out0, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("term", sorted(CASES))
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_contraction_lds_pipeline_shapes(py_rccsd, term, precision, tol):
+    """o=8, v=40: every A width (64, 320, 1600 columns) is a whole number of
+    16-byte chunks, so the FP64/FP32 products run the LDS-staged kernel, with
+    partial last column blocks (1600 = 12.5 x 128 = 6.25 x 256) and many
+    64-column tiles; checked against the einsum of the kept entries."""
+    no, nv = 8, 40
+    wshape, spec, alpha, oshape = CASES[term]
+    rng = np.random.default_rng(hash(term) % 997 + 1)
+    w = rng.standard_normal(wshape(no, nv))
+    t2 = rng.standard_normal((no, no, nv, nv))
+    T = py_rccsd.SparseTensor4d(t2, t2.shape, t2.size // 3, "largest")
+    dense = np.zeros(t2.size)
+    dense[T.indices()] = T.values()
+    ref = alpha * np.einsum(spec, w, dense.reshape(t2.shape))
+    out = rng.standard_normal(oshape(no, nv))
+    out0 = out.copy()
+    py_rccsd.contract_DTSpT(w, T, out, term, precision=precision)
+    npt.assert_allclose(out - out0, ref, rtol=0, atol=tol * np.abs(ref).max())
+
+
 @pytest.mark.parametrize("term", ["2323", "1302", "0312"])
 def test_contraction_fp32(py_rccsd, term):
     no, nv = 6, 13
