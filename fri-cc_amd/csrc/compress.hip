@@ -345,7 +345,7 @@ __global__ void k_dfx_tilesums(const uint64_t* __restrict__ keys, int64_t m, con
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
 }
 
-// Single block, exclusive scan of u128 tile sums in place.
+// Single block, exclusive scan of u128 tile sums in place; sums[nt] = total.
 __global__ void k_scan_u128_sums(u128* sums, int64_t nt, u128* total) {
   __shared__ u128 sm[1024 / WAVE + 1];
   const int64_t chunk = (nt + 1023) / 1024;
@@ -359,7 +359,10 @@ __global__ void k_scan_u128_sums(u128* sums, int64_t nt, u128* total) {
     sums[i] = ex;
     ex += t;
   }
-  if (threadIdx.x == 0 && total) *total = tot;
+  if (threadIdx.x == 0) {
+    sums[nt] = tot;  // (nt + 1 entries) the end of the last tile
+    if (total) *total = tot;
+  }
 }
 
 __global__ void k_dtest(const uint64_t* __restrict__ keys, int64_t m, CompScalars* sc,
@@ -458,10 +461,11 @@ __device__ inline double tile_elem(const double* lds, int r) {
 
 // pin != nullptr: sample from the given probabilities (sample_systematic,
 // src/fri_utils.cpp:129-159) instead of deriving them from x and D.
-// Per tile: the exact sum of FXP(p) (tsum) and per thread (thsum, in thread
-// order), the latter so that emission recomputes only where it emits.
+// Per tile: the exact sum of FXP(p) (tsum), and per thread the exact sum of
+// FXP(p) over the tile's earlier threads (thpre), so that later passes find
+// any thread's prefix without a block-wide scan.
 __global__ void k_samp_tilesums(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
-                                CompScalars* sc, u128* __restrict__ tsum, u128* __restrict__ thsum) {
+                                CompScalars* sc, u128* __restrict__ tsum, u128* __restrict__ thpre) {
   __shared__ u128 sm[NT / WAVE + 1];
   __shared__ double lds[LTILE];
   const CompScalars c = *sc;
@@ -483,10 +487,20 @@ __global__ void k_samp_tilesums(const double* __restrict__ x, const double* __re
     }
   }
   if (bad) atomicOr(&sc->pflag, 1u);
-  thsum[(int64_t)blockIdx.x * NT + threadIdx.x] = s;
   u128 tot;
-  block_excl_scan_u128<NT>(s, &tot, sm);
+  thpre[(int64_t)blockIdx.x * NT + threadIdx.x] = block_excl_scan_u128<NT>(s, &tot, sm);
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+// Exact prefix of FXP(p) before the first element of thread t of tile b and
+// after its last one (tsum scanned: tile b's exclusive prefix at b, the total
+// at nt).
+__device__ inline void thread_range(const u128* __restrict__ tsum, const u128* __restrict__ thpre, int64_t b, int t,
+                                    u128& lo, u128& hi) {
+  const u128 t0 = tsum[b];
+  const int64_t g = b * NT + t;
+  lo = thpre[g] + t0;
+  hi = t < NT - 1 ? thpre[g + 1] + t0 : tsum[b + 1];
 }
 
 __device__ inline int64_t first_k_at_least(double P, double u) {
@@ -529,51 +543,48 @@ __device__ inline void above_threshold(double a, u128& T, bool& tie) {
   }
 }
 
-// Emission.  A thread's elements hold lattice points k+u exactly when its
-// whole range [P(first), P(last+1)) does (first_k_at_least of the rounded
-// prefix is monotone), so only those threads recompute their elements.
-__global__ void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin, int64_t n,
-                            CompScalars* sc, const u128* __restrict__ tsum,
-                            const u128* __restrict__ thsum, uint32_t* __restrict__ out_idx,
-                            double* __restrict__ out_val) {
-  __shared__ u128 sm[NT / WAVE + 1];
-  __shared__ double lds[LTILE];
+// Emission, one thread per ITEMS consecutive elements and no cooperation
+// between threads: a thread's elements hold lattice points k+u exactly when
+// its whole range [P(first), P(last+1)) does (first_k_at_least of the rounded
+// prefix is monotone), so only those threads read their elements (one 128-byte
+// line each) and walk them.
+__global__ __launch_bounds__(NT) void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin,
+                                                  int64_t n, CompScalars* sc, const u128* __restrict__ tsum,
+                                                  const u128* __restrict__ thpre, uint32_t* __restrict__ out_idx,
+                                                  double* __restrict__ out_val) {
   const CompScalars c = *sc;
   if (c.n_sampled == 0) return;
   const double u = c.u;
   const double nsd = (double)c.n_sampled;
   const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
-  const int64_t tb = (int64_t)blockIdx.x * TILE;
-  const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
-  const u128 st = thsum[(int64_t)blockIdx.x * NT + threadIdx.x];
-  u128 tot;
-  u128 pre = block_excl_scan_u128<NT>(st, &tot, sm) + tsum[blockIdx.x];
-  bool work = false;
+  u128 pre, end;
+  thread_range(tsum, thpre, blockIdx.x, threadIdx.x, pre, end);
+  const double Plo = u128_to_double(pre, -96), Phi = u128_to_double(end, -96);
+  if (!(Phi > Plo)) return;
+  int64_t kc = first_k_at_least(Plo, u);
   {
-    const double Plo = u128_to_double(pre, -96), Phi = u128_to_double(pre + st, -96);
-    if (Phi > Plo) {
-      int64_t klo = first_k_at_least(Plo, u), khi = first_k_at_least(Phi, u);
-      if (khi > ns) khi = ns;
-      work = klo < khi;
-    }
+    int64_t khi = first_k_at_least(Phi, u);
+    if (khi > ns) khi = ns;
+    if (kc >= khi) return;
   }
-  if (!__syncthreads_or(work)) return;
-  stage_tile(pin ? pin : x, tb, n, lds);
-  if (!work) return;
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  const double* src = pin ? pin : x;
+  double vals[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) vals[r] = base + r < n ? src[base + r] : 0.0;
   // Walk the lattice alongside the elements: kc = first k with fl(k+u) >= P
   // of the running prefix.  Element j owns the k with P_j <= fl(k+u) <
   // P_{j+1}, i.e. those kc passes while fl(kc+u) < P_{j+1} (the same set as
   // first_k_at_least(P_j) .. first_k_at_least(P_{j+1}), by monotonicity).
   // fl(kc+u) < P_{j+1} is tested on the exact prefix against the threshold of
   // fl(kc+u) (above_threshold), so the prefix is rounded to a double only here
-  int64_t kc = first_k_at_least(u128_to_double(pre, -96), u);
   u128 T;
   bool tie;
   above_threshold((double)kc + u, T, tie);
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
     const int64_t j = base + r;
-    const double v = tile_elem(lds, r);
+    const double v = vals[r];
     const double p = j < n ? (pin ? v : prob_of(v, (uint64_t)j, c, nsd)) : 0.0;
     const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
     pre += f;
@@ -676,9 +687,8 @@ template <bool HASU>
 __global__ __launch_bounds__(NT) void k_piv_mark(const double* __restrict__ x, const double* __restrict__ pin,
                                                  const double* __restrict__ U, int64_t n,
                                                  const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
-                                                 const u128* __restrict__ thsum, uint32_t* __restrict__ tmax,
+                                                 const u128* __restrict__ thpre, uint32_t* __restrict__ tmax,
                                                  uint32_t* __restrict__ thmax, uint64_t* __restrict__ pst) {
-  __shared__ u128 sm[NT / WAVE + 1];
   __shared__ uint32_t sm32[NT / WAVE + 1];
   __shared__ double lds[LTILE];
   __shared__ double ulds[HASU ? LTILE : 1];
@@ -688,8 +698,8 @@ __global__ __launch_bounds__(NT) void k_piv_mark(const double* __restrict__ x, c
   const int64_t ns = (int64_t)c.n_sampled;
   const int64_t tb = (int64_t)blockIdx.x * TILE;
   const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
-  u128 tot;
-  u128 pre = block_excl_scan_u128<NT>(thsum[(int64_t)blockIdx.x * NT + threadIdx.x], &tot, sm) + tsum[blockIdx.x];
+  u128 pre, pend;
+  thread_range(tsum, thpre, blockIdx.x, threadIdx.x, pre, pend);
   stage_tile(pin ? pin : x, tb, n, lds);
   if (HASU) stage_tile(U, tb, n, ulds);
   uint32_t last = 0;
@@ -752,10 +762,9 @@ template <bool HASU>
 __global__ __launch_bounds__(NT) void k_piv_emit(const double* __restrict__ x, const double* __restrict__ pin,
                                                  const double* __restrict__ U, int64_t n,
                                                  const CompScalars* __restrict__ sc, const u128* __restrict__ tsum,
-                                                 const u128* __restrict__ thsum, const uint32_t* __restrict__ tmax,
+                                                 const u128* __restrict__ thpre, const uint32_t* __restrict__ tmax,
                                                  const uint32_t* __restrict__ thmax, const uint64_t* __restrict__ pst,
                                                  uint32_t* __restrict__ out_idx, double* __restrict__ out_val) {
-  __shared__ u128 sm[NT / WAVE + 1];
   __shared__ uint32_t sm32[NT / WAVE + 1];
   __shared__ double lds[LTILE];
   __shared__ double ulds[HASU ? LTILE : 1];
@@ -767,9 +776,9 @@ __global__ __launch_bounds__(NT) void k_piv_emit(const double* __restrict__ x, c
   const int64_t jstop = jb < (uint64_t)n ? (int64_t)jb : n - 1;  // last step taken
   const int64_t tb = (int64_t)blockIdx.x * TILE;
   const int64_t base = tb + (int64_t)threadIdx.x * ITEMS;
-  const u128 st = thsum[(int64_t)blockIdx.x * NT + threadIdx.x];
-  u128 tot;
-  u128 pre = block_excl_scan_u128<NT>(st, &tot, sm) + tsum[blockIdx.x];
+  u128 pre, pend;
+  thread_range(tsum, thpre, blockIdx.x, threadIdx.x, pre, pend);
+  const u128 st = pend - pre;
   // does a step in [max(base,1), min(base+ITEMS-1, jstop)] complete a unit?
   bool work = false;
   {
