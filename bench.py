@@ -319,8 +319,8 @@ def replica_bench(args):
 
 def traffic_child(args):
     """Run under rocprofv3 --pmc by measure_traffic: one calibration copy of
-    COPY_BYTES, then warm iterations of the headline mode, whose ladder
-    launch is read."""
+    COPY_BYTES, then the headline mode's warm-up and two iterations of the
+    timed window, whose longest ladder launch is read."""
     import ctypes
 
     import torch
@@ -338,15 +338,17 @@ def traffic_child(args):
     cfg = CONFIGS[args.config]
     sysm = SyntheticSystem.from_config(args.config)
     eris = DeviceERIs.from_factors(sysm.L, sysm.mo_energy, sysm.nocc)
-    run_mode(eris, cfg, args.headline, 1, 2, args.seed)
+    run_mode(eris, cfg, args.headline, 2, args.warmup, args.seed)
     torch.cuda.synchronize()
 
 
 def measure_traffic(args, head):
-    """HBM bytes of one ladder launch from two rocprofv3 --pmc runs of
-    traffic_child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
-    by what the same counter reads for the COPY_BYTES copy (which moves
-    exactly COPY_BYTES each way).  Returns (dict or None, note)."""
+    """HBM bytes and MFMA use of one ladder launch from three rocprofv3 --pmc
+    runs of traffic_child: FETCH_SIZE, then WRITE_SIZE (KiB per dispatch),
+    each scaled by what the same counter reads for the COPY_BYTES copy (which
+    moves exactly COPY_BYTES each way); then the derived MfmaUtil (MFMA-busy
+    cycles over active cycles x SIMDs, %) and MfmaFlopsBF16 (BF16 MFMA flops
+    issued).  Returns (dict or None, note)."""
     import csv
     import glob
     import shutil
@@ -359,33 +361,40 @@ def measure_traffic(args, head):
     base = tempfile.mkdtemp(prefix="fricc_pmc_", dir="/tmp")
     env = dict(os.environ, TMPDIR="/tmp")
     got = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = os.path.join(base, counter)
-        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
-               os.path.abspath(__file__), "--traffic-child", "--config", args.config, "--seed", str(args.seed),
-               "--headline", head]
+    for counters in (("FETCH_SIZE",), ("WRITE_SIZE",), ("MfmaUtil", "MfmaFlopsBF16")):
+        d = os.path.join(base, counters[0])
+        cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               "-u", os.path.abspath(__file__), "--traffic-child", "--config", args.config, "--seed",
+               str(args.seed), "--headline", head, "--warmup", str(args.warmup)]
         try:
             subprocess.run(cmd, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env,
                            cwd="/tmp", check=False)
         except subprocess.TimeoutExpired:
-            return None, f"{counter} pass timed out"
+            return None, f"{counters[0]} pass timed out"
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if not files:
-            return None, f"{counter} pass wrote no counters"
+            return None, f"{counters[0]} pass wrote no counters"
         rows = list(csv.DictReader(open(files[0])))
-        copy = [float(r["Counter_Value"]) for r in rows if "k_stream_copy" in r["Kernel_Name"]]
-        gem = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]), float(r["Counter_Value"]))
-               for r in rows if "k_ggemm_bf16" in r["Kernel_Name"]]
-        if not copy or not gem:
-            return None, f"{counter} pass: copy or ladder dispatch missing"
-        got[counter] = (copy[0], max(gem)[1])  # the ladder is the longest gathered launch
+        for c in counters:
+            crow = [r for r in rows if r["Counter_Name"] == c]
+            copy = [float(r["Counter_Value"]) for r in crow if "k_stream_copy" in r["Kernel_Name"]]
+            gem = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]), float(r["Counter_Value"]))
+                   for r in crow if "k_ggemm_bf16" in r["Kernel_Name"]]
+            if not gem:
+                return None, f"{c} pass: ladder dispatch missing"
+            # the ladder is the longest gathered launch of the window
+            got[c] = (copy[0] if copy else None, max(gem)[1], max(gem)[0])
     shutil.rmtree(base, ignore_errors=True)
+    if got["FETCH_SIZE"][0] is None or got["WRITE_SIZE"][0] is None:
+        return None, "copy dispatch missing"
     f_fetch = COPY_BYTES / (got["FETCH_SIZE"][0] * 1024.0)
     f_write = COPY_BYTES / (got["WRITE_SIZE"][0] * 1024.0)
     fetch = got["FETCH_SIZE"][1] * 1024.0 * f_fetch
     write = got["WRITE_SIZE"][1] * 1024.0 * f_write
     return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "fetch_scale": f_fetch,
-            "write_scale": f_write}, "ok"
+            "write_scale": f_write, "mfma_util_pct": got["MfmaUtil"][1],
+            "mfma_flops_issued": got["MfmaFlopsBF16"][1],
+            "launch_ms_under_profiler": got["FETCH_SIZE"][2] / 1e6}, "ok"
 
 
 def serve_bench(args):
@@ -441,7 +450,7 @@ def cpu_model():
     return {"model": name, "logical_cpus": os.cpu_count()}
 
 
-def summarize_profile(path):
+def summarize_profile(path, only=None):
     """Markdown tables from a rocprofv3 --kernel-trace --stats output
     directory (its *kernel_stats.csv): per kernel the calls, total and average
     duration, the 30 longest by total time; then per-iteration tables."""
@@ -461,11 +470,11 @@ def summarize_profile(path):
                    f"{float(r['AverageNs']) / 1e3:.1f} | {100 * float(r['TotalDurationNs']) / tot:.1f} |")
     out.append(f"\nTotal kernel time {tot / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches "
                f"({os.path.basename(files[0])}).")
-    out += step_breakdown(path)
+    out += step_breakdown(path, only=only)
     return "\n".join(out)
 
 
-def step_breakdown(path, top=16):
+def step_breakdown(path, top=16, only=None):
     """Per-iteration kernel tables from the same directory's *kernel_trace.csv:
     an iteration is the dispatches after one k_t2_finalize (and its partial
     sums) up to and including the next one's partial sums -- the two
@@ -481,8 +490,12 @@ def step_breakdown(path, top=16):
     rows = sorted(csv.DictReader(open(files[0])), key=lambda r: int(r["Start_Timestamp"]))
     fin = [i for i, r in enumerate(rows) if "k_t2_finalize" in r["Kernel_Name"]]
     out = []
+    longest = {}
     for w in range(1, len(fin)):
         seg = rows[fin[w - 1] + 2:fin[w] + 2]
+        lad = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in seg if "k_ggemm_bf16" in r["Kernel_Name"]]
+        if lad:
+            longest[w + 1] = max(lad) / 1e6
         wall = (int(seg[-1]["End_Timestamp"]) - int(seg[0]["Start_Timestamp"])) / 1e6
         tot = collections.defaultdict(lambda: [0, 0])
         for r in seg:
@@ -491,12 +504,21 @@ def step_breakdown(path, top=16):
             tot[k][0] += 1
             tot[k][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
         busy = sum(v[1] for v in tot.values()) / 1e6
+        if only and (w + 1) not in only:
+            continue
         out.append(f"\n### Iteration {w + 1} of the trace: {len(seg)} dispatches, kernel time {busy:.2f} ms, "
                    f"wall {wall:.2f} ms\n")
         out.append("| ms | calls | kernel |")
         out.append("|---:|---:|---|")
         for k, (cnt, d) in sorted(tot.items(), key=lambda x: -x[1][1])[:top]:
             out.append(f"| {d / 1e6:.3f} | {cnt} | `{k}` |")
+    if longest:
+        out.append("\n### Ladder launch (the longest `k_ggemm_bf16` of each iteration), ms\n")
+        out.append(" ".join(f"{w}:{t:.3f}" for w, t in sorted(longest.items())))
+        win = [t for w, t in longest.items() if 6 <= w <= 25]
+        if win:
+            out.append(f"\nMean over iterations 6..25 (the bench's timed window): {sum(win) / len(win):.3f} ms "
+                       f"over {len(win)} launches.")
     return out
 
 
@@ -527,9 +549,11 @@ def main():
     ap.add_argument("--lccsd-steps", type=int, default=3)
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-profile", default=None, help="markdown summary of a rocprofv3 output dir; no GPU")
+    ap.add_argument("--profile-iterations", default=None, help="iterations to tabulate in the summary, e.g. 6,15,25")
     args = ap.parse_args()
     if args.summarize_profile:
-        print(summarize_profile(args.summarize_profile))
+        only = {int(x) for x in args.profile_iterations.split(",")} if args.profile_iterations else None
+        print(summarize_profile(args.summarize_profile, only))
         return
     if args.traffic_child:
         traffic_child(args)
