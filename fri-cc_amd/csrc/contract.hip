@@ -765,12 +765,14 @@ int bf16_view_width(int64_t R, int64_t X) {
   // operands too large to stay in the caches are streamed once per tile:
   // wider tiles cut that traffic (A is R*X*4 bytes) but hold more active rows
   // each (lower fill, more MFMA work per entry).  Measured at o=32, v=256,
-  // m=4e6: the O^3V^3 operands (0.27 GB) are fastest at 128, Gx/Gy (2.1 GB)
-  // at 256; the ladder (17 GB) at 256 over the first iterations but at 128
-  // (5.9 against 6.4 ms) from the tenth on, where the samples spread over
-  // more rows.
-  const double bytes = (double)R * (double)X * 4.0;
-  return bytes > 1.0e9 && bytes <= 8.0e9 ? 256 : 128;
+  // m=4e6 after ten iterations: 128 columns for every operand -- the O^3V^3
+  // ones (0.27 GB), Gx/Gy (2.1 GB; 0.84 against 0.91 ms at 256) and the
+  // ladder (17 GB; 5.9 against 6.4 ms; 256 was ahead only over the first
+  // iterations, before the samples spread over more rows).  One width also
+  // lets the ladder, Gx and Gy share one view of T.
+  (void)R;
+  (void)X;
+  return 128;
 }
 
 int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, const OutMap& m, double alpha,

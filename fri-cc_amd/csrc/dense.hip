@@ -343,6 +343,21 @@ __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
   }
 }
 
+// First level of a long split-K sum over small outputs: group g of PSG
+// consecutive splits summed in split order, for every element (grid: element
+// blocks x groups), into out[g][e].
+constexpr int PSG = 32;
+template <typename T>
+__global__ void k_dgemm_reduce_groups(const T* __restrict__ partial, int nsplit, int64_t total,
+                                      T* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int q0 = blockIdx.y * PSG, q1 = min(nsplit, q0 + PSG);
+  T s = 0;
+  for (int q = q0; q < q1; q++) s += partial[(int64_t)q * total + e];
+  out[(int64_t)blockIdx.y * total + e] = s;
+}
+
 // C = alpha * sum over splits (in split order) + beta * C
 template <typename T>
 __global__ void k_dgemm_reduce(DArgs a) {
@@ -613,6 +628,21 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
     FRICC_TRY(launch_dgemm<float>(a, bm, bnt, va, vb, s));
   if (a.nsplit > 1) {
     const int64_t total = nb * M * N;
+    if (a.nsplit > 2 * PSG && ceil_div(total, 256) < 512) {
+      // few outputs and many splits: one thread per element would sum
+      // hundreds of partials on a handful of CUs; sum groups of PSG splits
+      // in parallel first (a fixed order: deterministic)
+      const int ng = (int)ceil_div(a.nsplit, PSG);
+      void* p2 = nullptr;
+      FRICC_TRY(ws_get(51, (size_t)ng * total * (compute64 ? 8 : 4), &p2));
+      const dim3 gg((unsigned)ceil_div(total, 256), (unsigned)ng);
+      if (compute64)
+        k_dgemm_reduce_groups<double><<<gg, 256, 0, s>>>((const double*)a.partial, a.nsplit, total, (double*)p2);
+      else
+        k_dgemm_reduce_groups<float><<<gg, 256, 0, s>>>((const float*)a.partial, a.nsplit, total, (float*)p2);
+      a.partial = p2;
+      a.nsplit = ng;
+    }
     const unsigned g = (unsigned)std::min<int64_t>(ceil_div(total, 256), 4096);
     if (compute64)
       k_dgemm_reduce<double><<<g, 256, 0, s>>>(a);

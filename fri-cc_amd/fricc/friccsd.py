@@ -276,35 +276,51 @@ def _update_amps_device(cc, t1, t2, eris):
     C = lambda a: a.to(f64) if cdt != f64 else a  # noqa: E731
 
     # ---- compression (fricc/friccsd.py:99-138)
+    # The two samples of T2 run on the arena's side stream while the dense
+    # products that do not read the compressed T (Fock intermediates, the
+    # t1 equation, the W builds) run on this one; the stream joins before
+    # the first compressed contraction.  The unbiased branch (iteration
+    # dependent, never recorded) stays on one stream.
     t0 = time.perf_counter()
-    tm.start("compress")
     comp = st["compression"]
     smeth = st["sampling_method"]
     ud = cc._udev()
     ts_alt = None
-    if _needs_alt(cc) and (not st["unbiased"] or cc._iter < st["burn_in"]):
-        # t2_sparse and t2_sparse_alt: two samples of the same T2 (the second
-        # uniform for the second), which share everything but the sampling
-        ualt = ud[1:2] if comp == "fri" else ud[0:1]
-        ts, ts_alt = cc._slot(0, t2.shape).recompress_pair_async(cc._slot(1, t2.shape), t2, int(m_keep), comp,
-                                                                 ud[0:1], ualt, sampling_method=smeth)
-        cc._used_slots = {0, 1}
-    else:
-        ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1], sampling_method=smeth)
-        cc._used_slots = {0}
-    if _needs_alt(cc) and ts_alt is None:
-        # unbiased averaging after burn-in: t2_sparse_alt samples the running average
-        ualt = ud[1:2] if comp == "fri" else ud[0:1]
-        it = cc._iter - st["burn_in"]
-        if it == 0:
-            cc._t2_avg = t2.clone()
+    main = torch.cuda.current_stream()
+    averaging = _needs_alt(cc) and st["unbiased"] and cc._iter >= st["burn_in"]
+    side = None if averaging else arena_side_stream(cc.arena)
+    if side is not None:
+        side.wait_stream(main)
+    with torch.cuda.stream(side if side is not None else main):
+        tm.start("compress")
+        if _needs_alt(cc) and not averaging:
+            # t2_sparse and t2_sparse_alt: two samples of the same T2 (the
+            # second uniform for the second), which share everything but the
+            # sampling
+            ualt = ud[1:2] if comp == "fri" else ud[0:1]
+            ts, ts_alt = cc._slot(0, t2.shape).recompress_pair_async(cc._slot(1, t2.shape), t2, int(m_keep), comp,
+                                                                     ud[0:1], ualt, sampling_method=smeth)
+            cc._used_slots = {0, 1}
         else:
-            cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
-        ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt,
-                                                        sampling_method=smeth)
-        cc._used_slots = {0, 1}
-    tm.stop("compress")
+            ts = cc._slot(0, t2.shape).recompress_async(t2, int(m_keep), comp, ud[0:1], sampling_method=smeth)
+            cc._used_slots = {0}
+        if averaging:
+            # unbiased averaging after burn-in: t2_sparse_alt samples the running average
+            ualt = ud[1:2] if comp == "fri" else ud[0:1]
+            it = cc._iter - st["burn_in"]
+            if it == 0:
+                cc._t2_avg = t2.clone()
+            else:
+                cc._t2_avg = t2 / it + (it - 1) / it * cc._t2_avg
+            ts_alt = cc._slot(1, t2.shape).recompress_async(cc._t2_avg, int(m_keep), comp, ualt,
+                                                            sampling_method=smeth)
+            cc._used_slots = {0, 1}
+        tm.stop("compress")
     timings["compress"] = time.perf_counter() - t0
+
+    def join():
+        if side is not None:
+            main.wait_stream(side)
 
     # every dense product below is one MFMA contraction (fricc.dense): E in
     # the dense precision (FP32 in the reduced modes), E64 in FP64; operands
@@ -375,9 +391,7 @@ def _update_amps_device(cc, t1, t2, eris):
     # sum_c Y[l,k,i,c] t1[j,c], the second sum_c Y[k,l,j,c] t1[i,c]
     Woooo = E64("lkic,jc->klij", P64["ovoo_Y"], t1)
     E64("kljc,ic->klij", P64["ovoo_Y"], t1, out=Woooo, beta=1.0)
-    if "O^4V^2" in ccs:
-        _contract_into("kcld,ijcd->klij", 1.0, eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
-    else:
+    if "O^4V^2" not in ccs:
         E64("kcld,ijcd->klij", eris.ovov, t2, out=Woooo, beta=1.0)
     E("ic,kclj->klij", t1, U, out=Woooo, beta=1.0)  # sum_cd (kc|ld) t1[i,c] t1[j,d]
     Woooo += eris.oooo.permute(0, 2, 1, 3)
@@ -394,6 +408,9 @@ def _update_amps_device(cc, t1, t2, eris):
     del N1
     Wvoov = A_voov.permute(3, 0, 2, 1)  # [a,k,i,c] view
     Wvovo = A_vovo.permute(3, 0, 1, 2)  # [a,k,c,i] view
+    join()  # the compressed contractions start here
+    if "O^4V^2" in ccs:
+        _contract_into("kcld,ijcd->klij", 1.0, eris.sparse_operand("A1323", sp), (o, v, o, v), ts_alt, Woooo, sp)
     if "O^3V^3" in ccs:
         A0112 = eris.sparse_operand("A0112", sp)
         A0313 = eris.sparse_operand("A0313", sp)
@@ -545,6 +562,17 @@ def amplitudes_to_vector_norm(t1new, t2new, t1, t2):
 
 
 _ARENA_STREAMS = {}
+
+
+def arena_side_stream(arena):
+    """A second stream per arena and device (reused), on which a step runs
+    its compressions while the dense products run on the step's own stream."""
+    torch = _torch()
+    key = (torch.cuda.current_device(), int(arena), "side")
+    st = _ARENA_STREAMS.get(key)
+    if st is None:
+        st = _ARENA_STREAMS[key] = torch.cuda.Stream()
+    return st
 
 
 def arena_stream(arena):

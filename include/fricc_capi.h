@@ -286,7 +286,7 @@ int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void
                       const double* lovov, const double* t1new, double* sums, void* stream);
 
 /* Column-tile width of the gathered views of FRICC_BF16 contractions: 0 =
- * automatic (256 for operands of 1 to 8 GB as FP32, else 128), or a
+ * automatic (128), or a
  * multiple of 128.  Tuning knob; no counterpart in the reference. */
 int fricc_set_bf16_view_width(int width);
 
