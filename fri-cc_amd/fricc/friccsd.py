@@ -135,6 +135,26 @@ def _prepare(T, spec, wshape, oshape, precision):
                                                  _capi.stream_ptr()))
 
 
+def _prepare_views(cc, ts, ts_alt, o, v, sp, ccs):
+    """Build the gathered views of T that the step's compressed contractions
+    read (each contraction finds its view built and reuses it)."""
+    if ts_alt is not None and "O^4V^2" in ccs:
+        _prepare(ts_alt, "kcld,ijcd->klij", (o, v, o, v), (o, o, o, o), sp)
+    if ts_alt is not None and "O^3V^3" in ccs:
+        _prepare(ts_alt, "ldkc,ilda->akic", (o, v, o, v), (v, o, o, v), sp)
+        _prepare(ts_alt, "ldkc,ilad->akic", (o, v, o, v), (v, o, o, v), sp)
+        _prepare(ts_alt, "lckd,ilda->akci", (o, v, o, v), (v, o, v, o), sp)
+    if "O^2V^4" in ccs:
+        _prepare(ts, "kdac,ijcd->ijka", (o, v, v, v), (o, o, o, v), sp)
+        _prepare(ts, "abcd,ijcd->ijab", (v, v, v, v), (o, o, v, v), cc.ladder_precision)
+    if "O^4V^2" in ccs:
+        _prepare(ts, "klij,klab->ijab", (o, o, o, o), (o, o, v, v), sp)
+    if "O^3V^3" in ccs:
+        _prepare(ts, "akic,kjcb->ijab", (v, o, o, v), (o, o, v, v), sp)
+        _prepare(ts, "akic,kjbc->ijab", (v, o, o, v), (o, o, v, v), sp)
+        _prepare(ts, "bkci,kjac->ijab", (v, o, v, o), (o, o, v, v), sp)
+
+
 def _dyn_operand(A, R, X, precision):
     """Per-iteration A operand (Woooo, Wvoov, Wvovo) in the kernel precision."""
     torch = _torch()
@@ -292,6 +312,8 @@ def _update_amps_device(cc, t1, t2, eris):
     if side is not None:
         side.wait_stream(main)
     with torch.cuda.stream(side if side is not None else main):
+        if side is not None:
+            _capi.check(_capi.lib().fricc_workspace_select(side_arena(cc.arena)))
         tm.start("compress")
         if _needs_alt(cc) and not averaging:
             # t2_sparse and t2_sparse_alt: two samples of the same T2 (the
@@ -316,6 +338,10 @@ def _update_amps_device(cc, t1, t2, eris):
                                                             sampling_method=smeth)
             cc._used_slots = {0, 1}
         tm.stop("compress")
+        if side is not None:
+            # the views every compressed contraction reads, built here too
+            _prepare_views(cc, ts, ts_alt, o, v, sp, ccs)
+        _capi.check(_capi.lib().fricc_workspace_select(int(cc.arena)))
     timings["compress"] = time.perf_counter() - t0
 
     def join():
@@ -564,6 +590,15 @@ def amplitudes_to_vector_norm(t1new, t2new, t1, t2):
 _ARENA_STREAMS = {}
 
 
+def side_arena(arena):
+    """The scratch arena of a step's side-stream work (its compressions),
+    so that nothing the side stream writes shares a slot with the kernels the
+    main stream runs at the same time."""
+    if not 0 <= int(arena) < 32:
+        raise ValueError("solves use arenas 0..31 (their side arenas are 32..63)")
+    return int(arena) + 32
+
+
 def arena_side_stream(arena):
     """A second stream per arena and device (reused), on which a step runs
     its compressions while the dense products run on the step's own stream."""
@@ -643,10 +678,11 @@ class GraphStep:
         with torch.cuda.graph(self.graph, stream=st):
             self.out = cc._device_step(self.t1, self.t2, eris)
         torch.cuda.synchronize()
-        # the graph holds the addresses of the arena's slots: while it lives,
+        # the graph holds the addresses of the arenas' slots: while it lives,
         # growing one of them (an eager call on a larger shape on this arena)
         # is an error instead of freeing memory the graph uses
         _pin_arena(cc.arena, self)
+        _pin_arena(side_arena(cc.arena), self)
 
     def issue(self):
         """Queue one iteration on the current stream (uniforms, replay); no wait."""

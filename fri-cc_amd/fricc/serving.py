@@ -148,11 +148,14 @@ def _solve_failure(err):
 
 
 def _release_arenas(n):
-    """Free the library scratch of arenas 1..n (waits for the device)."""
+    """Free the library scratch of arenas 1..n and of their side arenas
+    (waits for the device)."""
     from . import _capi
+    from .friccsd import side_arena
 
     for a in range(1, n + 1):
         _capi.check(_capi.lib().fricc_workspace_release(a))
+        _capi.check(_capi.lib().fricc_workspace_release(side_arena(a)))
 
 
 def system_key(req):
@@ -336,7 +339,7 @@ class Server:
         self.progress = progress  # called with each admission / retirement / deferral record
         self.budget = float(hbm_budget)
         self.precision = precision
-        self.max_active = min(int(max_active), 63)
+        self.max_active = min(int(max_active), 31)  # arenas 1..31; 33..63 are their side arenas
         self.overlap = bool(overlap)
         self.policy = policy
         # (request id, start event, end event) of every queued iteration when
@@ -359,11 +362,13 @@ class Server:
         """Free the library scratch of idle arenas (grow-only buffers a retired
         solve left behind).  Waits for the device once."""
         from . import _capi
+        from .friccsd import side_arena
 
         torch = _torch()
         rel = sorted(self._dirty & set(idle))
         for a in rel:
             _capi.check(_capi.lib().fricc_workspace_release(a))
+            _capi.check(_capi.lib().fricc_workspace_release(side_arena(a)))
             self._dirty.discard(a)
         torch.cuda.empty_cache()
         return bool(rel)
