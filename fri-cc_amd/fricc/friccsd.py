@@ -300,7 +300,9 @@ def _update_amps_device(cc, t1, t2, eris):
     # products that do not read the compressed T (Fock intermediates, the
     # t1 equation, the W builds) run on this one; the stream joins before
     # the first compressed contraction.  The unbiased branch (iteration
-    # dependent, never recorded) stays on one stream.
+    # dependent) and recorded steps (cc._single_stream: a captured graph that
+    # forked to a second stream kept its intermediate buffers allocated after
+    # it was destroyed, on this HIP/PyTorch stack) stay on one stream.
     t0 = time.perf_counter()
     comp = st["compression"]
     smeth = st["sampling_method"]
@@ -308,7 +310,8 @@ def _update_amps_device(cc, t1, t2, eris):
     ts_alt = None
     main = torch.cuda.current_stream()
     averaging = _needs_alt(cc) and st["unbiased"] and cc._iter >= st["burn_in"]
-    side = None if averaging else arena_side_stream(cc.arena)
+    single = averaging or getattr(cc, "_single_stream", False)
+    side = None if single else arena_side_stream(cc.arena)
     if side is not None:
         side.wait_stream(main)
     with torch.cuda.stream(side if side is not None else main):
@@ -662,6 +665,7 @@ class GraphStep:
             raise ValueError("graph mode cannot run with a section timer")
         _check_m_keep(cc, t2)
         self.cc, self.eris = cc, eris
+        cc._single_stream = True  # recorded on one stream (see _update_amps_device)
         self.t1 = t1.detach().clone().contiguous()
         self.t2 = t2.detach().clone().contiguous()
         # warm-up step on the arena's stream: allocates every workspace, view

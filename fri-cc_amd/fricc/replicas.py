@@ -77,6 +77,8 @@ class ReplicaBatch:
 
     def _record(self):
         torch = _torch()
+        for cc in self.reps:
+            cc._single_stream = True  # recorded on one stream (see friccsd._update_amps_device)
         st = arena_stream(self.reps[0].arena)  # warm-up and capture on the arena's stream
         st.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(st):  # warm-up: every workspace, view and operand allocated
