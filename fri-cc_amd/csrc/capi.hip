@@ -7,6 +7,7 @@
 namespace fricc {
 const char* last_error();
 extern int g_order;
+extern int g_split;
 extern int g_bf16_width;
 
 // The 11 contraction terms as index strings (src/sparse_tensor.cpp:11-346;
@@ -481,6 +482,12 @@ int fricc_set_bf16_view_width(int width) {
 int fricc_set_gemm_order(int order) {
   FRICC_CHECK(order == 0 || order == 1, "set_gemm_order: order must be 0 or 1");
   g_order = order;
+  return OK;
+}
+
+int fricc_set_gemm_split(int nsplit) {
+  FRICC_CHECK(nsplit >= 0 && nsplit <= 16, "set_gemm_split: 0 (automatic) to 16");
+  g_split = nsplit;
   return OK;
 }
 

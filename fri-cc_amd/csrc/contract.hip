@@ -188,6 +188,7 @@ __global__ __launch_bounds__(256) void k_ggemm(const T* __restrict__ A, int64_t 
 }
 
 int g_order = 1;
+int g_split = 0;   // forced K split of the BF16 gathered GEMM (0: automatic)
 int g_fp_lds = 1;  // FP64/FP32 through the LDS pipeline where the geometry allows
 
 
@@ -319,16 +320,54 @@ struct StageSrc {
 // Wave w stages A rows 8w..8w+7 (one 1 KB record per DMA: lanes 0-31 the hi
 // half, 32-63 the lo half) and a quarter of the B chunks (64 columns of one
 // 8-row group and plane per DMA).
+// The 8 gathered row indices wave w stages at K step k.  Inside the
+// pipeline they come from LDS: one iteration before a stage's DMAs are
+// issued, lanes 0-1 of each wave DMA that stage's 8 indices (32 bytes) from
+// the row list into the wave's slot of a two-slot LDS table (row_dma), in
+// issue order with the A/B DMAs, so the wait at the top of the next
+// iteration also covers them; the wave then reads them with one LDS read
+// before the barrier.  A scalar load here (row_scalar, used by the prologue
+// only) would put its round trip between the barrier and the MFMAs, and a
+// vector load into registers would make the compiler wait for the stage's
+// DMAs before the MFMAs (it cannot see that the DMAs were issued later).
+struct RowIdx {
+  uint4 lo, hi;
+};
+
+__device__ inline RowIdx row_scalar(const StageSrc& p, uint32_t k, int w) {
+  cu32* rl = p.glist + k + 8 * w;
+  RowIdx r;
+  r.lo = uint4{rl[0], rl[1], rl[2], rl[3]};
+  r.hi = uint4{rl[4], rl[5], rl[6], rl[7]};
+  return r;
+}
+
+// LDS table of prefetched row indices: [slot][wave] 8 indices
+struct RowTab {
+  uint4 r[2][4][2];
+};
+
+__device__ inline void row_dma(const StageSrc& p, uint32_t k, int w, int lane, RowTab& tab, int slot) {
+  // k + 8w is a multiple of 8 and the row list is 16-byte aligned: lane l
+  // (0 or 1) moves indices 4l..4l+3, 16 bytes, to tab.r[slot][w][l]
+  if (lane < 2) dma16((const uint32_t*)p.glist + k + 8 * w + 4 * lane, lds_addr(&tab.r[slot][w][0]));
+}
+
+__device__ inline RowIdx row_lds(const RowTab& tab, int slot, int w) {
+  RowIdx r;
+  r.lo = tab.r[slot][w][0];
+  r.hi = tab.r[slot][w][1];
+  return r;
+}
+
 template <int NPASS, int BYB, bool HILO>
 __device__ inline void stage_issue(const StageSrc& p, uint32_t k, int64_t bcol, StageMem<BYB, HILO>& sm, int w,
-                                   int lane) {
+                                   int lane, const RowIdx& ri) {
   static_assert(BX == 256 && (BYB == 128 || BYB == 256) && KG == 4 && KR == 32,
                 "wave assignment assumes 4 waves, BX 256, BYB 128 or 256");
   typedef AGeom<HILO> G;
   const uint32_t la = lds_addr(&sm.A[0]), lb = lds_addr(&sm.B[0][0][0]);
-  cu32* rl = p.glist + k + 8 * w;
-  const uint32_t r0 = rl[0], r1 = rl[1], r2 = rl[2], r3 = rl[3], r4 = rl[4], r5 = rl[5], r6 = rl[6], r7 = rl[7];
-  const uint32_t rr[8] = {r0, r1, r2, r3, r4, r5, r6, r7};
+  const uint32_t rr[8] = {ri.lo.x, ri.lo.y, ri.lo.z, ri.lo.w, ri.hi.x, ri.hi.y, ri.hi.z, ri.hi.w};
   // one record per DMA: 64 lanes for a hi | lo record, lanes 0-31 for a hi one
 #pragma unroll
   for (int i = 0; i < 8; i++)
@@ -451,6 +490,7 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   constexpr int LW = 8 + 2 * PER;                   // DMA instructions per wave per stage
   constexpr int LWP = NPASS > 1 ? LW : LW - PER;    // the lo planes of B are skipped with one pass
   __shared__ StageMem<BYB, HILO> st[S];
+  __shared__ RowTab rtab;
   static_assert(S == 2 || S == 3, "2 or 3 stages");
   int64_t bx, yb;
   int sp;
@@ -504,28 +544,256 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16(B16Args a) {
   // ahead, then runs the MFMAs of the current stage.  The fences around the
   // barrier order LDS accesses only, so the explicit vmcnt wait is the only
   // wait on the DMA.
+  // Vector-memory issue order per iteration: the row-index DMA of the stage
+  // S ahead (clamped to a stage of this workgroup, so always issued), then
+  // the DMAs of the stage S-1 ahead.  The wait at the top of an iteration
+  // (all but the last LWP operations, the next stage's DMAs) so also
+  // completes the index DMA issued just before them, one iteration earlier;
+  // each wave reads its own indices, so no barrier is needed before the read.
   uint32_t k = g_beg;
   if (k < g_end) {
 #pragma unroll
-    for (int s = 0; s < S - 1; s++)
-      if (k + s * KR < g_end) stage_issue<NPASS, BYB, HILO>(ps, k + s * KR, bcol, st[s], w, lane);
-    int cur = 0;
+    for (int s = 0; s < S - 1; s++) {
+      if (s == S - 2) row_dma(ps, k + (S - 1) * KR < g_end ? k + (S - 1) * KR : k, w, lane, rtab, (S - 1) & 1);
+      if (k + s * KR < g_end)
+        stage_issue<NPASS, BYB, HILO>(ps, k + s * KR, bcol, st[s], w, lane, row_scalar(ps, k + s * KR, w));
+    }
+    int cur = 0, slot = (S - 1) & 1;  // slot of the indices of stage (k - g_beg)/KR + S - 1
     for (; k < g_end; k += KR) {
       if (S > 2 && k + KR < g_end)
         wait_vm<(S > 2 ? LWP : 0)>();
       else
         wait_vm<0>();
+      // the DMA'd indices are read only after the wait above
+      asm volatile("" ::: "memory");
+      const RowIdx ri = row_lds(rtab, slot, w);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
       __builtin_amdgcn_s_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
       const int nxt = cur == 0 ? S - 1 : cur - 1;  // (cur + S - 1) % S
-      if (k + (S - 1) * KR < g_end) stage_issue<NPASS, BYB, HILO>(ps, k + (S - 1) * KR, bcol, st[nxt], w, lane);
+      if (k + (S - 1) * KR < g_end) {
+        row_dma(ps, k + S * KR < g_end ? k + S * KR : k, w, lane, rtab, slot ^ 1);
+        stage_issue<NPASS, BYB, HILO>(ps, k + (S - 1) * KR, bcol, st[nxt], w, lane, ri);
+      }
       stage_compute<NPASS, BYB, HILO>(st[cur], acc, wx, wy, lr, lk);
       cur = cur == S - 1 ? 0 : cur + 1;
+      slot ^= 1;
     }
   }
   // D layout: lane holds column lane%16 of each 16x16 block, rows 4*(lane/16)+i
   bf16_epilogue<NQ>(std::make_index_sequence<NQ>{}, a, acc, x0, y0, wx, wy, lr, lk, sp);
+}
+
+// ---------------------------------------------------------------------------
+// The 256 x 128 block kernel with its LDS reads software-pipelined.
+//
+// Same operands, staging, block mapping and epilogue as k_ggemm_bf16 with
+// BYB = 128 and 3 LDS stages; what changes is when a wave reads LDS.  A wave
+// holds the MFMA fragments of a whole K step in registers (A: 8 blocks of 16
+// x, B: 4 blocks of 16 y, hi and lo) and reads the next step's fragments
+// while its MFMAs run on the current ones, so no MFMA waits on an LDS read
+// issued after the barrier: A block s of the next step is read into the
+// registers of A block s as soon as this step's MFMAs of block s are issued,
+// and B fragment q (read by every block) into its registers once the last
+// block's MFMAs with q are issued.  One register set, read in place: a second
+// set swapped every step would need the loop unrolled by two, and the
+// compiler then splits the accumulators between register files.  The
+// global->LDS DMAs of the stage S steps ahead are issued in pieces between
+// the MFMA blocks (sched_barrier keeps the pieces where they are); one
+// barrier per K step remains.
+template <int NPASS>
+struct PA {
+  bf8 ah[8];
+  bf8 al[NPASS > 1 ? 8 : 1];
+};
+template <int NPASS>
+struct PB {
+  uint4 bh[4];
+  uint4 bl[NPASS > 1 ? 4 : 1];
+};
+
+// A fragment s (x block wx*128 + 16 s) of one stage
+template <int NPASS, bool HILO>
+__device__ inline void frag_read_a(const StageMem<128, HILO>& sm, PA<NPASS>& f, int s, uint32_t r0, uint32_t r1,
+                                   int wx) {
+  const uint32_t c = (uint32_t)((wx * 128 + 16 * s) * 2);
+  f.ah[s] = cat_bf8(tr_read(sm, r0 + c), tr_read(sm, r1 + c));
+  if (NPASS > 1) f.al[s] = cat_bf8(tr_read(sm, r0 + BX * 2 + c), tr_read(sm, r1 + BX * 2 + c));
+}
+
+// B fragment q (y block wy*64 + 16 q) of one stage
+template <int NPASS, bool HILO>
+__device__ inline void frag_read_b(const StageMem<128, HILO>& sm, PB<NPASS>& f, int q, int wy, int lr, int lk) {
+  f.bh[q] = sm.B[0][lk][wy * 64 + 16 * q + lr];
+  if (NPASS > 1) f.bl[q] = sm.B[1][lk][wy * 64 + 16 * q + lr];
+}
+
+template <int NPASS>
+__device__ inline void frag_mfma(const PA<NPASS>& fa, const PB<NPASS>& fb, f4 (&acc)[8][4], int s, int q) {
+  acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa.ah[s], as_bf8(fb.bh[q]), acc[s][q], 0, 0, 0);
+  if (NPASS > 1) {
+    acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa.ah[s], as_bf8(fb.bl[q]), acc[s][q], 0, 0, 0);
+    acc[s][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa.al[s], as_bf8(fb.bh[q]), acc[s][q], 0, 0, 0);
+  }
+}
+
+// DMA piece s of the stage at K step k (wave w): A row 8w + s, and for s < 2
+// the B chunk s (hi, and lo with three passes) -- stage_issue in pieces
+template <int NPASS, bool HILO>
+__device__ inline void dma_piece(const StageSrc& p, uint32_t k, int64_t bcol, StageMem<128, HILO>& sm, int w,
+                                 int lane, uint32_t row, int s) {
+  typedef AGeom<HILO> G;
+  if (HILO || lane < 32) dma16(p.A + (int64_t)row * G::REC, lds_addr(&sm.A[0]) + G::off(8 * w + s));
+  if (s < 2) {
+    const int c = w * 2 + s, gl = c / 2, yc = c % 2;
+    const int64_t off = (int64_t)((k >> 3) + gl) * p.TY + bcol + yc * 64 + lane;
+    const uint32_t l = lds_addr(&sm.B[0][0][0]) + (uint32_t)((gl * 128 + yc * 64) * 16);
+    dma16(p.Bhi + off, l);
+    if (NPASS > 1) dma16(p.Blo + off, l + (uint32_t)(KG * 128 * 16));
+  }
+}
+
+__device__ inline uint32_t row_of(const RowIdx& r, int s) {
+  switch (s) {
+    case 0: return r.lo.x;
+    case 1: return r.lo.y;
+    case 2: return r.lo.z;
+    case 3: return r.lo.w;
+    case 4: return r.hi.x;
+    case 5: return r.hi.y;
+    case 6: return r.hi.z;
+    default: return r.hi.w;
+  }
+}
+
+template <int NPASS, bool HILO>
+__global__ __launch_bounds__(256, 1) void k_ggemm_bf16_pl(B16Args a) {
+  constexpr int S = 3;
+  constexpr int LW = 8 + 2 * 2;                  // DMA instructions per wave per stage
+  constexpr int LWP = NPASS > 1 ? LW : LW - 2;   // the lo planes of B are skipped with one pass
+  __shared__ StageMem<128, HILO> st[S];
+  __shared__ RowTab rtab;
+  int64_t bx, yb;
+  int sp;
+  {
+    const int64_t nb = a.nbx * a.nyb * a.nsplit;
+    int64_t id = blockIdx.x;
+    const int64_t per = nb / 8;
+    if (per * 8 == nb) id = (id % 8) * per + id / 8;
+    sp = (int)(id % a.nsplit);
+    id /= a.nsplit;
+    if (a.order == 0) {
+      yb = id / a.nbx;
+      bx = id % a.nbx;
+    } else {
+      bx = id / a.nyb;
+      yb = id % a.nyb;
+    }
+  }
+  const int64_t sub = a.TY / 128;
+  const int64_t t = a.tile_order[yb / sub];
+  const int64_t bcol = (yb % sub) * 128;  // first column of this block within the tile
+  const int64_t y0 = t * a.TY + bcol;
+  uint32_t g_beg = a.tile_off[t], g_end = a.tile_off[t + 1];
+  if (a.nsplit > 1) {
+    const uint32_t nst = (g_end - g_beg) / KR;
+    const uint32_t per = (nst + a.nsplit - 1) / a.nsplit;
+    const uint32_t s0 = min(nst, per * (uint32_t)sp), s1 = min(nst, per * (uint32_t)(sp + 1));
+    g_end = g_beg + s1 * KR;
+    g_beg = g_beg + s0 * KR;
+  }
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wx = w & 1, wy = w >> 1;
+  const int64_t x0 = bx * BX;
+  StageSrc ps;
+  ps.A = a.A + (bx * a.R) * AGeom<HILO>::REC + 8 * lane;
+  ps.Bhi = a.Bhi;
+  ps.Blo = a.Blo;
+  ps.glist = (cu32*)a.glist;
+  ps.TY = a.TY;
+  typedef AGeom<HILO> G;
+  // this lane's transposed-read addresses (rows 8*lk + lr/4 and +4)
+  const uint32_t r0 = G::off(8 * lk + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
+  const uint32_t r1 = G::off(8 * lk + 4 + (lr >> 2)) + (uint32_t)((lr & 3) * 8);
+  f4 acc[8][4];
+#pragma unroll
+  for (int s = 0; s < 8; s++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[s][q] = f4{0, 0, 0, 0};
+  if (g_beg < g_end) {
+    // Prologue: stages 0..S-1 (scalar row indices), the row-index DMA of
+    // stage S just before the DMAs of stage S-1; then stage 0's fragments.
+    const uint32_t k0 = g_beg;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t ks = k0 + s * KR;
+      if (s == S - 1 && ks < g_end) row_dma(ps, k0 + S * KR < g_end ? k0 + S * KR : k0, w, lane, rtab, S & 1);
+      if (ks < g_end) {
+        const RowIdx ri = row_scalar(ps, ks, w);
+#pragma unroll
+        for (int i = 0; i < 8; i++) dma_piece<NPASS, HILO>(ps, ks, bcol, st[s], w, lane, row_of(ri, i), i);
+      }
+    }
+    if (k0 + 2 * KR < g_end)
+      wait_vm<2 * LWP + 1>();
+    else if (k0 + KR < g_end)
+      wait_vm<LWP>();
+    else
+      wait_vm<0>();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    PA<NPASS> fa;
+    PB<NPASS> fb;
+#pragma unroll
+    for (int s = 0; s < 8; s++) frag_read_a<NPASS, HILO>(st[0], fa, s, r0, r1, wx);
+#pragma unroll
+    for (int q = 0; q < 4; q++) frag_read_b<NPASS, HILO>(st[0], fb, q, wy, lr, lk);
+
+    // Step j (K rows [k, k+32)): its fragments are in fa and fb.  Vector-memory
+    // order per step: the row-index DMA of step j+S+1, then the LWP DMAs of
+    // step j+S.  The wait at the top (all but the last LWP operations) so
+    // completes step j+1's DMAs, issued two steps earlier, and the indices
+    // of step j+S, issued first in the previous step.
+    uint32_t k = k0;
+    int cur = 0, slot = S & 1;
+    while (k < g_end) {
+      const bool more2 = k + 2 * KR < g_end, issue = k + S * KR < g_end;
+      if (more2)
+        wait_vm<LWP>();
+      else
+        wait_vm<0>();
+      asm volatile("" ::: "memory");  // the DMA'd indices are read after the wait
+      const RowIdx ri = row_lds(rtab, slot, w);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      const int nb = cur == S - 1 ? 0 : cur + 1;  // buffer of step j+1
+      const uint32_t kd = k + S * KR;             // step whose DMAs go into buffer cur
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        if (issue) {
+          if (s == 0) row_dma(ps, kd + KR < g_end ? kd + KR : k, w, lane, rtab, slot ^ 1);
+          dma_piece<NPASS, HILO>(ps, kd, bcol, st[cur], w, lane, row_of(ri, s), s);
+        }
+        // next step's fragments, read unconditionally: past the last step
+        // buffer nb holds stale data whose fragments are never used
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          frag_mfma<NPASS>(fa, fb, acc, s, q);
+          if (s == 7) frag_read_b<NPASS, HILO>(st[nb], fb, q, wy, lr, lk);
+        }
+        frag_read_a<NPASS, HILO>(st[nb], fa, s, r0, r1, wx);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      cur = nb;
+      slot ^= 1;
+      k += KR;
+    }
+  }
+  bf16_epilogue<4>(std::make_index_sequence<4>{}, a, acc, x0, y0, wx, wy, lr, lk, sp);
 }
 
 // out[x, y] += alpha * sum_s partial[s][x][y], splits added in order
@@ -812,10 +1080,15 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   a.order = g_order;
   a.alpha = alpha;
   a.partial = nullptr;
-  // small grids: split each tile's K range, partial sums added in order
+  // grids of fewer workgroups than CUs: split each tile's K range, partial
+  // sums added in order (at 256 workgroups, the t1-dressing contractions at
+  // o=32 v=256, splitting in two was slower: 0.98 against 0.91 ms, plus the
+  // reduce)
   const int64_t nb0 = a.nbx * a.nyb;
   int nsplit = 1;
-  if (nb0 < 512) {
+  if (g_split > 0) {
+    nsplit = g_split;
+  } else if (nb0 < 256) {
     nsplit = (int)std::min<int64_t>(16, ceil_div(512, nb0));
     while (nsplit > 1 && (double)nsplit * X * v->Y * 4.0 > 1.0e9) nsplit--;
   }
@@ -835,11 +1108,11 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
       k_ggemm_bf16<1, 256, 2, true><<<nb, 256, 0, s>>>(a);
   } else {
     if (!hilo)
-      k_ggemm_bf16<1, 128, 3, false><<<nb, 256, 0, s>>>(a);
+      k_ggemm_bf16_pl<1, false><<<nb, 256, 0, s>>>(a);
     else if (npass == 3)
-      k_ggemm_bf16<3, 128, 3, true><<<nb, 256, 0, s>>>(a);
+      k_ggemm_bf16_pl<3, true><<<nb, 256, 0, s>>>(a);
     else
-      k_ggemm_bf16<1, 128, 3, true><<<nb, 256, 0, s>>>(a);
+      k_ggemm_bf16_pl<1, true><<<nb, 256, 0, s>>>(a);
   }
   FRICC_HIP(hipGetLastError());
   if (nsplit > 1) k_splitk_reduce<<<ceil_div(X * v->Y, 256), 256, 0, s>>>(a.partial, nsplit, X, v->Y, a.om, alpha);

@@ -317,6 +317,12 @@ int fricc_stream_copy(const void* src, void* dst, int64_t bytes, void* stream);
  * reference. */
 int fricc_set_gemm_order(int order);
 
+/* K splits of the BF16 gathered contraction GEMM: 0 = automatic (split only
+ * grids of fewer than 512 workgroups), or 1..16 splits of every tile's row
+ * list, their partial sums added in split order.  Tuning knob; no
+ * counterpart in the reference. */
+int fricc_set_gemm_split(int nsplit);
+
 #ifdef __cplusplus
 }
 #endif
