@@ -637,7 +637,7 @@ def main():
     copy_gbs = 2.0 * COPY_BYTES * 10 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
     del ca, cb
     if lp.startswith("bf16"):
-        roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm_bf16)", "bound": "hbm",
+        roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm_bf16_pl)", "bound": "hbm",
                 "achieved": lad_bytes / (lad_ms * 1e-3) / 1e9 if lad_ms else None, "peak": HBM_GBS, "unit": "GB/s"}
     else:
         roof = {"kernel": "gathered ladder GEMM 2323 (k_ggemm)", "bound": "mfma",
