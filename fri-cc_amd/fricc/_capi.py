@@ -19,7 +19,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16x1": BF16X1}
 EXPORTS = (
     "fricc_abi_version", "fricc_last_error", "fricc_compress", "fricc_sample_systematic", "fricc_sparse_create", "fricc_sparse_destroy",
     "fricc_sparse_compress", "fricc_sparse_set", "fricc_sparse_info", "fricc_sparse_get", "fricc_sparse_device",
-    "fricc_contract", "fricc_contract_prepared", "fricc_contract_spec", "fricc_eri_build", "fricc_set_gemm_order", "fricc_set_gemm_split",
+    "fricc_contract", "fricc_contract_prepared", "fricc_contract_spec", "fricc_eri_build", "fricc_set_gemm_order", "fricc_set_gemm_split", "fricc_dense_zpair",
     "fricc_sparse_prepare", "fricc_split_bf16", "fricc_set_bf16_passes", "fricc_t2_finalize",
     "fricc_set_bf16_view_width", "fricc_stream_copy", "fricc_sparse_compress_async", "fricc_sparse_status",
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
@@ -101,6 +101,7 @@ def lib():
     L.fricc_eri_build.argtypes = [_vp, _i64, _i64, I64x8, I64x4, _vp, _vp]
     L.fricc_set_gemm_order.argtypes = [ctypes.c_int]
     L.fricc_set_gemm_split.argtypes = [ctypes.c_int]
+    L.fricc_dense_zpair.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6
     L.fricc_split_bf16.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_split_bf16_hi.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_split_bf16_lincomb.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i64, _i64, _vp,

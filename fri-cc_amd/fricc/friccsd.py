@@ -390,10 +390,19 @@ def _update_amps_device(cc, t1, t2, eris):
     # 2 sum_kc (kc|ai) t1[k,c] - sum_kc (ki|ac) t1[k,c]
     E("kcai,kc->ia", P["ovvo"], t1, out=t1new, alpha=2.0, beta=1.0)
     E("kiac,kc->ia", P["oovv"], t1, out=t1new, alpha=-1.0, beta=1.0)
-    # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] Z[(kcd),a]
-    E("ikcd,kcda->ia", t2c, P["Z"], out=t1new, beta=1.0)
-    # LvvT1[a,c] = sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac)
-    LvvT1 = C(E("kd,kcda->ac", t1, P["Z"]))
+    # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] Z[(kcd),a], and
+    # LvvT1[a,c] = sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac):
+    # both read the o*v^3 operand Z, so in FP32 one kernel streams it once
+    Z = P["Z"]
+    if cdt == torch.float32 and o <= 32 and v % 128 == 0:
+        LvvT1 = torch.empty((v, v), dtype=f64, device=t1.device)
+        t1f = t1.to(torch.float32).contiguous()
+        t2z = t2c.contiguous()
+        _capi.check(_capi.lib().fricc_dense_zpair(o, v, _vp(t2z), _vp(t1f), _vp(Z), _vp(t1new), _vp(LvvT1),
+                                                  _capi.stream_ptr()))
+    else:
+        E("ikcd,kcda->ia", t2c, Z, out=t1new, beta=1.0)
+        LvvT1 = C(E("kd,kcda->ac", t1, Z))
     E64("ic,ac->ia", t1, LvvT1, out=t1new, beta=1.0)
     # -2 lcki,klac + kcli,klac  ==  sum_klc Aoo[k,l,i,c] t2[k,l,a,c]
     E("klic,klac->ia", P["Aoo"], t2c, out=t1new, beta=1.0)
@@ -682,11 +691,12 @@ class GraphStep:
         with torch.cuda.graph(self.graph, stream=st):
             self.out = cc._device_step(self.t1, self.t2, eris)
         torch.cuda.synchronize()
-        # the graph holds the addresses of the arenas' slots: while it lives,
+        # the graph holds the addresses of the arena's slots: while it lives,
         # growing one of them (an eager call on a larger shape on this arena)
-        # is an error instead of freeing memory the graph uses
+        # is an error instead of freeing memory the graph uses.  The recorded
+        # step runs on one stream, so the side arena is not in the graph and
+        # stays free for eager steps.
         _pin_arena(cc.arena, self)
-        _pin_arena(side_arena(cc.arena), self)
 
     def issue(self):
         """Queue one iteration on the current stream (uniforms, replay); no wait."""

@@ -22,7 +22,7 @@ import math
 
 import numpy as np
 
-from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, _pin_arena, arena_stream, side_arena
+from .friccsd import FRICCSD, _check_m_keep, _draw_uniforms, _pin_arena, arena_stream
 from .uniforms import PhiloxUniforms
 
 
@@ -97,8 +97,8 @@ class ReplicaBatch:
                 self.t1[r].copy_(t1n)
                 self.t2[r].copy_(t2n)
         torch.cuda.synchronize()
-        _pin_arena(self.reps[0].arena, self)  # the graph holds that arena's slots
-        _pin_arena(side_arena(self.reps[0].arena), self)
+        # the graph holds that arena's slots (one stream: not the side arena's)
+        _pin_arena(self.reps[0].arena, self)
 
     def step(self):
         """One iteration of every replica; returns the R energies."""
