@@ -837,17 +837,28 @@ __device__ inline T lds_ld(const uint8_t* base, uint32_t off) {
   return *(const T*)(base + off);
 }
 
+// Row indices of the FP pipeline: wave w stages rows 4w..4w+3 of a stage
+// (16 bytes).  As in the BF16 kernel, inside the loop they reach the wave
+// through a two-slot LDS table, DMA'd by lane 0 one iteration before they are
+// used (in vmcnt order with the stage DMAs); the prologue reads them through
+// the scalar cache.
+__device__ inline uint4 fp_rows_scalar(cu32* rl, uint32_t k, int w) {
+  return uint4{rl[k + 4 * w], rl[k + 4 * w + 1], rl[k + 4 * w + 2], rl[k + 4 * w + 3]};
+}
+
+__device__ inline void fp_rows_dma(cu32* rl, uint32_t k, int w, int lane, uint4 (&tab)[2][4], int slot) {
+  if (lane == 0) dma16((const uint32_t*)rl + k + 4 * w, lds_addr(&tab[slot][w]));
+}
+
 template <typename T>
-__device__ inline void fp_issue(const FArgs& a, cu32* rl, int64_t aoff, uint32_t k, FpStage<T>& sm, int w,
+__device__ inline void fp_issue(const FArgs& a, uint4 rows, int64_t aoff, uint32_t k, FpStage<T>& sm, int w,
                                 int lane) {
   typedef FpGeom<T> G;
   const uint32_t la = lds_addr(&sm.A[0]), lb = lds_addr(&sm.B[0]);
   const T* Ab = (const T*)a.A;
+  const uint32_t rr[4] = {rows.x, rows.y, rows.z, rows.w};
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint32_t r = rl[k + 4 * w + i];
-    dma16(Ab + (int64_t)r * a.X + aoff, la + (uint32_t)((4 * w + i) * G::APITCH));
-  }
+  for (int i = 0; i < 4; i++) dma16(Ab + (int64_t)rr[i] * a.X + aoff, la + (uint32_t)((4 * w + i) * G::APITCH));
   const uint8_t* Bb = (const uint8_t*)a.B + (int64_t)k * G::BROW;
 #pragma unroll
   for (int j = 0; j < G::PERB; j++) {
@@ -880,7 +891,9 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_fp(FArgs a) {
   typedef FpGeom<T> G;
   typedef typename Acc<T>::type AT;
   constexpr int LW = 4 + G::PERB;  // DMA instructions per wave per stage
+  static_assert(S == 3, "the row-index prefetch assumes three stages");
   __shared__ FpStage<T> st[S];
+  __shared__ uint4 rtab[2][4];
   int64_t bx, ti;
   xcd_remap(a.nbx, a.ntiles, a.order, bx, ti);
   const int64_t t = a.tile_order[ti];
@@ -900,24 +913,36 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_fp(FArgs a) {
   for (int s = 0; s < G::MS; s++)
 #pragma unroll
     for (int q = 0; q < G::NQ; q++) acc[s][q] = AT{0, 0, 0, 0};
+  // Vector-memory order per iteration: the row-index DMA of the stage S
+  // ahead (clamped to a stage of this tile, so always issued), then the LW
+  // DMAs of the stage S-1 ahead; the wait at the top (all but the last LW
+  // operations) so also completes the indices DMA'd one iteration earlier.
   uint32_t k = k_beg;
   if (k < k_end) {
 #pragma unroll
-    for (int s = 0; s < S - 1; s++)
-      if (k + s * G::KR < k_end) fp_issue<T>(a, rl, aoff, k + s * G::KR, st[s], w, lane);
-    int cur = 0;
+    for (int s = 0; s < S - 1; s++) {
+      if (s == S - 2) fp_rows_dma(rl, k + (S - 1) * G::KR < k_end ? k + (S - 1) * G::KR : k, w, lane, rtab, (S - 1) & 1);
+      if (k + s * G::KR < k_end) fp_issue<T>(a, fp_rows_scalar(rl, k + s * G::KR, w), aoff, k + s * G::KR, st[s], w, lane);
+    }
+    int cur = 0, slot = (S - 1) & 1;  // slot of the indices of stage (k - k_beg)/KR + S - 1
     for (; k < k_end; k += G::KR) {
       if (k + G::KR < k_end)
         wait_vm<LW * (S - 2)>();
       else
         wait_vm<0>();
+      asm volatile("" ::: "memory");  // the DMA'd indices are read after the wait
+      const uint4 rows = rtab[slot][w];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
       __builtin_amdgcn_s_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
       const int nxt = cur == 0 ? S - 1 : cur - 1;
-      if (k + (S - 1) * G::KR < k_end) fp_issue<T>(a, rl, aoff, k + (S - 1) * G::KR, st[nxt], w, lane);
+      if (k + (S - 1) * G::KR < k_end) {
+        fp_rows_dma(rl, k + S * G::KR < k_end ? k + S * G::KR : k, w, lane, rtab, slot ^ 1);
+        fp_issue<T>(a, rows, aoff, k + (S - 1) * G::KR, st[nxt], w, lane);
+      }
       fp_compute<T>(st[cur], acc, wx, wy, lr, lk);
       cur = cur == S - 1 ? 0 : cur + 1;
+      slot ^= 1;
     }
   }
 #pragma unroll
