@@ -9,11 +9,15 @@ system's integrals and prepared operands.
 ``ReplicaBatch`` records one iteration of *every* replica -- each replica's
 whole device step (``friccsd._update_amps_device``) followed by the copy of its
 outputs into its inputs -- as a single graph, so one launch advances all R
-trajectories by one iteration.  Per iteration the host writes the R x k
-uniforms with one copy from pinned memory, replays, and reads every replica's
-energy and norm in one transfer.  Each replica's arithmetic is exactly that
-of the replica run alone (same kernels, same order), so a replica's
-trajectory in the batch is bit-identical to its trajectory alone.
+trajectories by one iteration.  The graph has K branches (streams): replica r
+runs on branch r mod K with scratch arena ``arenas[r mod K]``, so K replicas'
+kernels share the GPU at any time (one replica's grids at the small sizes
+replicas run at leave most CUs idle), and replicas on one branch run one
+after another.  Per iteration the host writes the R x k uniforms with one
+copy from pinned memory, replays, and reads every replica's energy and norm
+in one transfer.  Each replica's arithmetic is exactly that of the replica
+run alone (same kernels, same order, no kernel shared between replicas), so a
+replica's trajectory in the batch is bit-identical to its trajectory alone.
 
 The reference runs one trajectory per process and averages afterwards
 (examples/03_friccsd_unbiased.py; no batched form exists there).
@@ -46,18 +50,26 @@ class ReplicaBatch:
 
     settings: solver settings shared by all replicas (precision, compression,
     ...); ``m_list`` and ``seeds`` give each replica's m_keep and Philox seed;
-    ``solver``: FRICCSD (default) or FRILCCSD."""
+    ``solver``: FRICCSD (default) or FRILCCSD; ``arenas``: the library scratch
+    arenas (0..31) of the graph's branches, one branch per arena (default
+    0..min(R, 8)-1); no other solve may use them while the batch lives."""
 
-    def __init__(self, eris, settings, m_list, seeds, level_shift=0.0, solver=FRICCSD):
+    def __init__(self, eris, settings, m_list, seeds, level_shift=0.0, solver=FRICCSD, arenas=None):
         torch = _torch()
         if len(m_list) != len(seeds):
             raise ValueError("one m and one seed per replica")
         self.eris = eris
         self.R = len(seeds)
+        if arenas is None:
+            arenas = range(min(self.R, 8))
+        self.arenas = [int(a) for a in arenas]
+        if not self.arenas or len(set(self.arenas)) != len(self.arenas) or not all(0 <= a < 32 for a in self.arenas):
+            raise ValueError("arenas: distinct library arenas in 0..31")
         self.reps = []
-        for m, seed in zip(m_list, seeds):
+        for r, (m, seed) in enumerate(zip(m_list, seeds)):
             cc = solver(eris, dict(settings, m_keep=int(m)), uniforms=PhiloxUniforms(seed))
             cc.level_shift = level_shift
+            cc.arena = self.arenas[r % len(self.arenas)]
             if cc._iteration_dependent():
                 raise ValueError("replica batches need the unbiased averaging off")
             self.reps.append(cc)
@@ -75,30 +87,56 @@ class ReplicaBatch:
         self.energies = []  # per iteration: R energies
         self.graph = None
 
+    def _branches(self, st, body):
+        """body(r, cc) for every replica, replica r on branch r mod K; the
+        branches fork from and join back into stream st."""
+        torch = _torch()
+        streams = [arena_stream(a) for a in self.arenas]
+        for b in streams:
+            if b is not st:
+                b.wait_stream(st)
+        out = []
+        for r, cc in enumerate(self.reps):
+            with torch.cuda.stream(streams[r % len(streams)]):
+                out.append(body(r, cc))
+        for b in streams:
+            if b is not st:
+                st.wait_stream(b)
+        return out
+
     def _record(self):
         torch = _torch()
         for cc in self.reps:
-            cc._single_stream = True  # recorded on one stream (see friccsd._update_amps_device)
-        st = arena_stream(self.reps[0].arena)  # warm-up and capture on the arena's stream
+            cc._single_stream = True  # each step on its branch's one stream (see friccsd._update_amps_device)
+        st = arena_stream(self.arenas[0])  # warm-up and capture on the first branch's stream
         st.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(st):  # warm-up: every workspace, view and operand allocated
+        # warm-up: one replica alone first, which builds the operands every
+        # replica shares (prepared integrals, BF16 splits) before any other
+        # branch reads them; then every replica on its branch, which allocates
+        # each arena's workspaces and each replica's views
+        with torch.cuda.stream(st):
             self.u_dev.fill_(0.5)
-            for r, cc in enumerate(self.reps):
-                cc._device_step(self.t1[r], self.t2[r], self.eris)
+            self.reps[0]._device_step(self.t1[0], self.t2[0], self.eris)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(st):
+            self._branches(st, lambda r, cc: cc._device_step(self.t1[r], self.t2[r], self.eris))
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        self.out = []
+
+        def rec(r, cc):
+            t1n, t2n, red = cc._device_step(self.t1[r], self.t2[r], self.eris)
+            # the branch's stream orders the copies after the step that read t1[r], t2[r]
+            self.t1[r].copy_(t1n)
+            self.t2[r].copy_(t2n)
+            return (t1n, t2n, red)
+
         with torch.cuda.graph(self.graph, stream=st):
-            for r, cc in enumerate(self.reps):
-                t1n, t2n, red = cc._device_step(self.t1[r], self.t2[r], self.eris)
-                self.out.append((t1n, t2n, red))
+            self.out = self._branches(st, rec)
             self.red = torch.stack([o[2] for o in self.out])
-            for r, (t1n, t2n, _) in enumerate(self.out):
-                self.t1[r].copy_(t1n)
-                self.t2[r].copy_(t2n)
         torch.cuda.synchronize()
-        # the graph holds that arena's slots (one stream: not the side arena's)
-        _pin_arena(self.reps[0].arena, self)
+        # the graph holds the branches' arenas' slots (one stream each: no side arena)
+        for a in self.arenas:
+            _pin_arena(a, self)
 
     def step(self):
         """One iteration of every replica; returns the R energies."""

@@ -20,13 +20,15 @@ def env():
     return torch, FRICCSD, ReplicaBatch, run_alone, SyntheticSystem, rccsd, log_uniform_m
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_batch_equals_each_replica_alone(env, prec):
+@pytest.mark.parametrize("prec,arenas", [("fp64", None), ("mixed", None), ("mixed", (3, 5)), ("fp64", (7,))])
+def test_batch_equals_each_replica_alone(env, prec, arenas):
+    """One branch per replica (default), two replicas per branch on arenas 3
+    and 5, and all four on one branch: each replica bit for bit its run alone."""
     torch, FRICCSD, ReplicaBatch, run_alone, SyntheticSystem, rccsd, log_uniform_m = env
     eris = FRICCSD(SyntheticSystem(4, 14, 40, 8), {}).ao2mo()
     settings = {"compression": "fri", "sampling_method": "systematic", "precision": prec}
     m_list, seeds = [700, 1500, 2600, 1100], [11, 12, 13, 14]
-    batch = ReplicaBatch(eris, settings, m_list, seeds, level_shift=2.0)
+    batch = ReplicaBatch(eris, settings, m_list, seeds, level_shift=2.0, arenas=arenas)
     batch.run(4)
     for r in range(len(seeds)):
         en, t2 = run_alone(eris, settings, m_list[r], seeds[r], 4, level_shift=2.0)

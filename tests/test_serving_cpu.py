@@ -1,4 +1,4 @@
-"""Request generation and the admission memory model (CPU)."""
+"""Request generation, the admission memory model and the replica batch's argument checks (CPU)."""
 import numpy as np
 
 from fricc import serving as sv
@@ -231,3 +231,17 @@ def test_data_error_is_one_solves_failure(monkeypatch):
     monkeypatch.setattr(sv, "_release_arenas", lambda n: None)
     res = server.run([sv.Request(0, 0.0, 3, 10, m=50, max_cycle=2), sv.Request(1, 0.0, 3, 10, m=50, max_cycle=2)])
     assert len(res) == 2 and all("mocked status" in r.error for r in res)
+
+
+def test_replica_batch_rejects_bad_arenas():
+    """Branch arenas must be distinct library arenas 0..31 (checked before
+    any device work)."""
+    import pytest
+
+    from fricc.replicas import ReplicaBatch
+
+    for arenas in ([], [1, 1], [32], [-1]):
+        with pytest.raises(ValueError, match="arenas"):
+            ReplicaBatch(None, {}, [10], [0], arenas=arenas)
+    with pytest.raises(ValueError, match="one m and one seed"):
+        ReplicaBatch(None, {}, [10, 20], [0])
