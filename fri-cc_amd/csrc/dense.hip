@@ -72,6 +72,7 @@ struct DArgs {
   const void* A;
   const void* B;
   void* C;
+  const void* Csrc;  // the beta term's source: C itself, or a tensor of C's layout
   void* partial;
   int a64, b64, c64;
   int a_kfast, b_kfast;  // the tile load walks k fastest (else m / n fastest)
@@ -309,7 +310,8 @@ __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
         if (a.c64) {
           double2* p = (double2*)((double*)a.C + off);
           if (a.beta != 0.0) {
-            const double2 x = p[0], y = p[1];
+            const double2* q = (const double2*)((const double*)a.Csrc + off);
+            const double2 x = q[0], y = q[1];
             o[0] += a.beta * x.x;
             o[1] += a.beta * x.y;
             o[2] += a.beta * y.x;
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
         } else {
           float4* p = (float4*)((float*)a.C + off);
           if (a.beta != 0.0) {
-            const float4 x = *p;
+            const float4 x = *(const float4*)((const float*)a.Csrc + off);
             o[0] += a.beta * x.x;
             o[1] += a.beta * x.y;
             o[2] += a.beta * x.z;
@@ -333,7 +335,8 @@ __global__ __launch_bounds__(256) void k_dgemm(DArgs a) {
         for (int e = 0; e < 4; e++) {
           if (n0 + nl + e >= a.N) break;
           const int64_t off = rowoff + cnoff[nl + e];
-          if (a.beta != 0.0) o[e] += a.beta * (a.c64 ? ((const double*)a.C)[off] : (double)((const float*)a.C)[off]);
+          if (a.beta != 0.0)
+            o[e] += a.beta * (a.c64 ? ((const double*)a.Csrc)[off] : (double)((const float*)a.Csrc)[off]);
           if (a.c64)
             ((double*)a.C)[off] = o[e];
           else
@@ -370,7 +373,7 @@ __global__ void k_dgemm_reduce(DArgs a) {
     const int64_t n = e % a.N, m = (e / a.N) % a.M, b = e / (a.N * a.M);
     const int64_t off = lvl_off(a.Cb, b) + lvl_off(a.Cm, m) + lvl_off(a.Cn, n);
     double o = a.alpha * (double)s;
-    if (a.beta != 0.0) o += a.beta * (a.c64 ? ((const double*)a.C)[off] : (double)((const float*)a.C)[off]);
+    if (a.beta != 0.0) o += a.beta * (a.c64 ? ((const double*)a.Csrc)[off] : (double)((const float*)a.Csrc)[off]);
     if (a.c64)
       ((double*)a.C)[off] = o;
     else
@@ -491,7 +494,8 @@ static int pick_tile(int64_t n) { return n <= 32 ? 32 : n <= 64 ? 64 : 128; }
 int dense_contract(const char* spec, double alpha, const void* A, int a64, int an, const int64_t* ash,
                    const int64_t* ast, const void* B, int b64, int bn, const int64_t* bsh, const int64_t* bst,
                    double beta, void* C, int c64, int cn, const int64_t* csh, const int64_t* cst, int compute64,
-                   hipStream_t s) {
+                   hipStream_t s, const void* Csrc) {
+  if (!Csrc) Csrc = C;
   FRICC_CHECK(spec && A && B && C, "dense_contract: null argument");
   FRICC_CHECK(an >= 0 && an <= 8 && bn >= 0 && bn <= 8 && cn >= 0 && cn <= 8, "dense_contract: at most 8 axes");
   const char* comma = strchr(spec, ',');
@@ -573,6 +577,7 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
   a.A = A;
   a.B = B;
   a.C = C;
+  a.Csrc = Csrc;
   a.a64 = a64;
   a.b64 = b64;
   a.c64 = c64;
@@ -598,7 +603,7 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
   const int va = a.a_kfast ? (vec_ok(A, a.Ak, a.Am, a.Ab) ? 4 : 1) : (vec_ok(A, a.Am, a.Ak, a.Ab) ? 4 : 1);
   const int vb = a.b_kfast ? (vec_ok(B, a.Bk, a.Bn, a.Bb) ? 4 : 1) : (vec_ok(B, a.Bn, a.Bk, a.Bb) ? 4 : 1);
   a.c_vec = (a.Cn.s0 == 1 && a.Cn.L0 % 4 == 0 && mult4(a.Cn) && a.Cm.s0 % 4 == 0 && mult4(a.Cm) &&
-             a.Cb.s0 % 4 == 0 && mult4(a.Cb) && ((uintptr_t)C % 16) == 0)
+             a.Cb.s0 % 4 == 0 && mult4(a.Cb) && ((uintptr_t)C % 16) == 0 && ((uintptr_t)Csrc % 16) == 0)
                 ? 1
                 : 0;
   // split K when the output grid cannot fill the device (memory-bound
@@ -819,7 +824,25 @@ int fricc_dense_contract(const char* spec, double alpha, const void* A, int a_pr
               "dense_contract: precisions must be 0 (FP64) or 1 (FP32)");
   return fricc::dense_contract(spec, alpha, A, a_precision == 0, a_ndim, a_shape, a_strides, B, b_precision == 0,
                                b_ndim, b_shape, b_strides, beta, C, c_precision == 0, c_ndim, c_shape, c_strides,
-                               compute_precision == 0, (hipStream_t)stream);
+                               compute_precision == 0, (hipStream_t)stream, nullptr);
+}
+
+// C = alpha * einsum(spec, A, B) + beta * Csrc, Csrc of C's shape, strides and
+// element type (C itself, or a tensor that does not overlap it): an
+// accumulation onto a constant operand without copying it into C first.
+// No counterpart in the reference.
+int fricc_dense_contract_from(const char* spec, double alpha, const void* A, int a_precision, int a_ndim,
+                              const int64_t* a_shape, const int64_t* a_strides, const void* B, int b_precision,
+                              int b_ndim, const int64_t* b_shape, const int64_t* b_strides, double beta,
+                              const void* Csrc, void* C, int c_precision, int c_ndim, const int64_t* c_shape,
+                              const int64_t* c_strides, int compute_precision, void* stream) {
+  FRICC_CHECK((a_precision == 0 || a_precision == 1) && (b_precision == 0 || b_precision == 1) &&
+                  (c_precision == 0 || c_precision == 1) && (compute_precision == 0 || compute_precision == 1),
+              "dense_contract: precisions must be 0 (FP64) or 1 (FP32)");
+  FRICC_CHECK(Csrc, "dense_contract_from: null source");
+  return fricc::dense_contract(spec, alpha, A, a_precision == 0, a_ndim, a_shape, a_strides, B, b_precision == 0,
+                               b_ndim, b_shape, b_strides, beta, C, c_precision == 0, c_ndim, c_shape, c_strides,
+                               compute_precision == 0, (hipStream_t)stream, Csrc);
 }
 
 // out_ia[i,a] += sum_{k,c,d} t2[i,k,c,d] Z[k,c,d,a] and out_ac[a,c] =

@@ -25,7 +25,7 @@ EXPORTS = (
     "fricc_workspace_select", "fricc_workspace_release", "fricc_split_bf16_lincomb",
     "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
     "fricc_compress_noalloc", "fricc_compress_noalloc_max_n", "fricc_dense_contract",
-    "fricc_split_bf16_hi", "fricc_sample_pivotal", "fricc_parallel_sample",
+    "fricc_split_bf16_hi", "fricc_sample_pivotal", "fricc_parallel_sample", "fricc_dense_contract_from",
 )
 
 
@@ -98,6 +98,9 @@ def lib():
     L.fricc_dense_contract.argtypes = [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_int, _pi, _pi,
                                        _vp, ctypes.c_int, ctypes.c_int, _pi, _pi, ctypes.c_double, _vp, ctypes.c_int,
                                        ctypes.c_int, _pi, _pi, ctypes.c_int, _vp]
+    L.fricc_dense_contract_from.argtypes = [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_int, _pi,
+                                            _pi, _vp, ctypes.c_int, ctypes.c_int, _pi, _pi, ctypes.c_double, _vp, _vp,
+                                            ctypes.c_int, ctypes.c_int, _pi, _pi, ctypes.c_int, _vp]
     L.fricc_eri_build.argtypes = [_vp, _i64, _i64, I64x8, I64x4, _vp, _vp]
     L.fricc_set_gemm_order.argtypes = [ctypes.c_int]
     L.fricc_set_gemm_split.argtypes = [ctypes.c_int]

@@ -1,8 +1,10 @@
 """Dense two-operand contractions on the GPU's matrix cores
 (``fricc_dense_contract``, fri-cc_amd/csrc/dense.hip).
 
-``einsum(spec, A, B, out=None, alpha=1, beta=0, compute=None)`` computes
-``out = alpha * einsum(spec, A, B) + beta * out`` for torch CUDA tensors of
+``einsum(spec, A, B, out=None, alpha=1, beta=0, compute=None, c_src=None)``
+computes ``out = alpha * einsum(spec, A, B) + beta * C`` with C = ``c_src``
+(a tensor of out's shape, strides and dtype that does not overlap it) or
+``out`` itself, for torch CUDA tensors of
 any strides (permuted views and broadcasts need no copies), FP32 or FP64
 elements (converted to the compute type on load).  ``compute``: torch.float32
 (FP32 MFMA) or torch.float64 (FP64 MFMA); default FP64 when both inputs are
@@ -70,7 +72,7 @@ def _count(spec, A, B, out, beta, compute):
                      + out.numel() * out.element_size() * (2 if beta != 0.0 else 1))
 
 
-def einsum(spec, A, B, out=None, alpha=1.0, beta=0.0, compute=None, stream=None):
+def einsum(spec, A, B, out=None, alpha=1.0, beta=0.0, compute=None, stream=None, c_src=None):
     torch = _torch()
     if compute is None:
         compute = torch.float64 if (A.dtype == torch.float64 and B.dtype == torch.float64) else torch.float32
@@ -82,12 +84,18 @@ def einsum(spec, A, B, out=None, alpha=1.0, beta=0.0, compute=None, stream=None)
         if not t.is_cuda:
             raise ValueError("dense einsum operands must be CUDA tensors")
     cp = _capi.FP64 if compute == torch.float64 else _capi.FP32
-    _capi.check(_capi.lib().fricc_dense_contract(
-        spec.encode(), float(alpha),
-        ctypes.c_void_p(A.data_ptr()), _prec(A), A.dim(), _arr(A.shape), _arr(A.stride()),
-        ctypes.c_void_p(B.data_ptr()), _prec(B), B.dim(), _arr(B.shape), _arr(B.stride()),
-        float(beta), ctypes.c_void_p(out.data_ptr()), _prec(out), out.dim(), _arr(out.shape), _arr(out.stride()),
-        cp, _capi.stream_ptr(stream)))
+    a_args = (ctypes.c_void_p(A.data_ptr()), _prec(A), A.dim(), _arr(A.shape), _arr(A.stride()),
+              ctypes.c_void_p(B.data_ptr()), _prec(B), B.dim(), _arr(B.shape), _arr(B.stride()), float(beta))
+    c_args = (ctypes.c_void_p(out.data_ptr()), _prec(out), out.dim(), _arr(out.shape), _arr(out.stride()),
+              cp, _capi.stream_ptr(stream))
+    if c_src is None:
+        _capi.check(_capi.lib().fricc_dense_contract(spec.encode(), float(alpha), *a_args, *c_args))
+    else:
+        if (not c_src.is_cuda or c_src.shape != out.shape or c_src.stride() != out.stride()
+                or c_src.dtype != out.dtype):
+            raise ValueError("c_src must be a CUDA tensor of out's shape, strides and dtype")
+        _capi.check(_capi.lib().fricc_dense_contract_from(spec.encode(), float(alpha), *a_args,
+                                                          ctypes.c_void_p(c_src.data_ptr()), *c_args))
     if COUNTER is not None:
         _count(spec, A, B, out, beta, compute)
     return out

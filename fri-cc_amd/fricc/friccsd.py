@@ -356,8 +356,8 @@ def _update_amps_device(cc, t1, t2, eris):
     # are read in place through their strides, FP64 ones converted on load
     from .dense import einsum as _de
 
-    def E(spec, A, B, out=None, alpha=1.0, beta=0.0):
-        return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=cdt)
+    def E(spec, A, B, out=None, alpha=1.0, beta=0.0, c_src=None):
+        return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=cdt, c_src=c_src)
 
     def E64(spec, A, B, out=None, alpha=1.0, beta=0.0):
         return _de(spec, A, B, out=out, alpha=alpha, beta=beta, compute=f64)
@@ -434,10 +434,11 @@ def _update_amps_device(cc, t1, t2, eris):
     E("ic,kclj->klij", t1, U, out=Woooo, beta=1.0)  # sum_cd (kc|ld) t1[i,c] t1[j,d]
     Woooo += eris.oooo.permute(0, 2, 1, 3)
     # Wvoov as A_voov[k,c,i,a] = Wvoov[a,k,i,c];  Wvovo as A_vovo[k,c,i,a] = Wvovo[a,k,c,i]
-    A_voov = P["ovvo_p"].clone()
-    E("id,kcda->kcia", t1, P["P1"], out=A_voov, beta=1.0)
-    A_vovo = P["oovv_p"].clone()
-    E("id,kcda->kcia", t1, P["P2"], out=A_vovo, beta=1.0)
+    # (the integral part is read as the beta term's source: no copy of it first)
+    A_voov = torch.empty_like(P["ovvo_p"])
+    E("id,kcda->kcia", t1, P["P1"], out=A_voov, beta=1.0, c_src=P["ovvo_p"])
+    A_vovo = torch.empty_like(P["oovv_p"])
+    E("id,kcda->kcia", t1, P["P2"], out=A_vovo, beta=1.0, c_src=P["oovv_p"])
     # - sum_l (kcli + sum_d ldkc t1[i,d]) t1[l,a];  - sum_l (lcki + sum_d lckd t1[i,d]) t1[l,a]:
     # both read N1[p,c,q,i] = (pc|qi) + U[p,c,q,i], with (p,q) = (k,l) and (l,k)
     N1 = P["ovoo"] + U
@@ -535,9 +536,13 @@ def _update_amps_device(cc, t1, t2, eris):
             from .systems import bf16_planes_lincomb
 
             A12 = bf16_planes_lincomb(A_voov, A_vovo, 2.0, -1.0, ov, ov)
+            a12 = 1.0
         else:
-            A12 = _dyn_operand(2.0 * A_voov - A_vovo, ov, ov, sp)
-        _contract_into("akic,kjcb->ijab", 1.0, A12, (v, o, o, v), ts, S, sp)
+            # 2 Wvoov - Wvovo as 2 (Wvoov - Wvovo / 2): one pass instead of
+            # two, and scaling by 2 is exact, so the sum is bit for bit the same
+            A12 = _dyn_operand(torch.add(A_voov, A_vovo, alpha=-0.5), ov, ov, sp)
+            a12 = 2.0
+        _contract_into("akic,kjcb->ijab", a12, A12, (v, o, o, v), ts, S, sp)
         del A12
         Av = _dyn_operand(A_voov, ov, ov, sp)
         Avo = _dyn_operand(A_vovo, ov, ov, sp)

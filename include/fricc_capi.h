@@ -230,6 +230,18 @@ int fricc_dense_contract(const char* spec, double alpha, const void* A, int a_pr
                          int c_ndim, const int64_t* c_shape, const int64_t* c_strides, int compute_precision,
                          void* stream);
 
+/* C = alpha * einsum(spec, A, B) + beta * Csrc: fricc_dense_contract with the
+ * beta term read from Csrc, a tensor of C's shape, strides and element type
+ * (C itself, or one that does not overlap C), so an accumulation onto a
+ * constant operand needs no copy of it into C first (the reference's
+ * ``W = integrals + einsum(...)`` pattern, fricc/friccsd.py:194-207).  No
+ * counterpart in the reference. */
+int fricc_dense_contract_from(const char* spec, double alpha, const void* A, int a_precision, int a_ndim,
+                              const int64_t* a_shape, const int64_t* a_strides, const void* B, int b_precision,
+                              int b_ndim, const int64_t* b_shape, const int64_t* b_strides, double beta,
+                              const void* Csrc, void* C, int c_precision, int c_ndim, const int64_t* c_shape,
+                              const int64_t* c_strides, int compute_precision, void* stream);
+
 /* (pq|rs) = sum_Q L[Q,p,q] L[Q,r,s] (sequential fma chain over Q) for
  * p in [rng[0],rng[1]), q in [rng[2],rng[3]), r in [rng[4],rng[5]),
  * s in [rng[6],rng[7]); written at out[(p-rng[0])*st[0] + (q-rng[2])*st[1] +

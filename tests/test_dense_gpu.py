@@ -1,9 +1,10 @@
 """The dense MFMA contraction (fricc_dense_contract) against plain PyTorch
 einsum in FP64 on the CPU: permuted views, broadcasts (stride 0), ragged
 sizes, split-K shapes (tiny output, long sum), mixed FP32/FP64 operands,
-beta accumulation, every tile shape, and the products of the amplitude
-update at small sizes.  Tolerance per element: FP64 compute 1e-12 and FP32
-compute 2e-6 of sum_k |a||b| (the magnitude the sum's rounding scales with)."""
+beta accumulation (onto C or from a separate source), every tile shape, and
+the products of the amplitude update at small sizes.  Tolerance per element:
+FP64 compute 1e-12 and FP32 compute 2e-6 of sum_k |a||b| (the magnitude the
+sum's rounding scales with)."""
 import numpy as np
 import pytest
 
@@ -98,6 +99,34 @@ def test_dense_every_tile_shape_and_split(env):
                 A = torch.randn(m, k, generator=g, dtype=torch.float64).cuda()
                 B = torch.randn(k, n, generator=g, dtype=torch.float64).cuda()
                 _check(torch, dense, "mk,kn->mn", A, B, torch.float64)
+
+
+@pytest.mark.parametrize("compute", ["fp64", "fp32"])
+@pytest.mark.parametrize("spec,sa,sb,cshape", [
+    ("id,kcda->kcia", (6, 9), (5, 7, 9, 12), (5, 7, 6, 12)),   # the W builds' shape: 16-byte epilogue
+    ("mk,kn->mn", (5, 20000), (20000, 7), (5, 7)),            # split K: the reduce reads the source
+    ("mk,kn->mn", (130, 70), (70, 33), (130, 33)),            # ragged: the scalar epilogue
+])
+def test_dense_beta_term_from_a_source(env, compute, spec, sa, sb, cshape):
+    """out = alpha A.B + beta Csrc equals Csrc copied into out and then
+    accumulated (bit for bit), and the einsum of the CPU reference; Csrc is
+    left unchanged."""
+    torch, dense = env
+    cdt = torch.float64 if compute == "fp64" else torch.float32
+    g = torch.Generator().manual_seed(23)
+    A = torch.randn(*sa, generator=g, dtype=torch.float64).to(cdt).cuda()
+    B = torch.randn(*sb, generator=g, dtype=torch.float64).to(cdt).cuda()
+    src = torch.randn(*cshape, generator=g, dtype=torch.float64).to(cdt).cuda()
+    keep = src.clone()
+    via_copy = src.clone()
+    dense.einsum(spec, A, B, out=via_copy, alpha=0.75, beta=1.0, compute=cdt)
+    out = torch.full_like(src, float("nan"))
+    dense.einsum(spec, A, B, out=out, alpha=0.75, beta=1.0, compute=cdt, c_src=src)
+    assert torch.equal(out, via_copy)
+    assert torch.equal(src, keep)
+    _check(torch, dense, spec, A, B, cdt, beta=1.0, out=src.clone())
+    with pytest.raises(ValueError):
+        dense.einsum(spec, A, B, out=out, beta=1.0, compute=cdt, c_src=src.double() if compute == "fp32" else src.float())
 
 
 def test_dense_rejects_bad_specs(env):
