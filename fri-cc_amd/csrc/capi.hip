@@ -283,8 +283,23 @@ int fricc_sparse_compress_pair_async(fricc_sparse* sp, fricc_sparse* sp2, const 
               "sparse_compress_pair_async: two distinct tensors and two uniforms needed");
   for (int q = 0; q < 4; q++)
     FRICC_CHECK(sp->dims[q] == sp2->dims[q], "sparse_compress_pair_async: the tensors' dimensions differ");
-  FRICC_TRY(fricc_sparse_compress_async(sp, x, m, compression, u_dev, stream));
-  return resample_into(sp, sp2, x, m, compression, u_dev2, (hipStream_t)stream);
+  // one call for both samples: the selection once, both emissions in one pass
+  const int64_t n = sp->dims[0] * sp->dims[1] * sp->dims[2] * sp->dims[3];
+  for (fricc_sparse* t : {sp, sp2}) {
+    FRICC_TRY(sparse_reserve(t, m));
+    if (!t->dsc) FRICC_HIP(hipMalloc(&t->dsc, comp_scalars_size()));
+    t->gen++;
+    t->nnz = 0;
+  }
+  FRICC_TRY(compress_pair_launch(x, n, m, compression, u_dev, u_dev2, sp->idx, sp->val, sp->dsc, sp2->idx, sp2->val,
+                                 sp2->dsc, (hipStream_t)stream));
+  for (fricc_sparse* t : {sp, sp2}) {
+    t->nnz = m;
+    t->pending = true;
+    t->dup_host = 0;
+    t->dup_dev = comp_scalars_dup_flag(t->dsc);
+  }
+  return OK;
 }
 
 int fricc_sparse_compress_multi_async(fricc_sparse* const* sps, int count, const double* x, int64_t m,

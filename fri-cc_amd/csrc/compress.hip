@@ -547,26 +547,48 @@ __device__ inline void above_threshold(double a, u128& T, bool& tie) {
 // between threads: a thread's elements hold lattice points k+u exactly when
 // its whole range [P(first), P(last+1)) does (first_k_at_least of the rounded
 // prefix is monotone), so only those threads read their elements (one 128-byte
-// line each) and walk them.
-__global__ __launch_bounds__(NT) void k_samp_emit(const double* __restrict__ x, const double* __restrict__ pin,
-                                                  int64_t n, CompScalars* sc, const u128* __restrict__ tsum,
-                                                  const u128* __restrict__ thpre, uint32_t* __restrict__ out_idx,
-                                                  double* __restrict__ out_val) {
-  const CompScalars c = *sc;
+// line each) and walk them.  NS samples of one compression (uniforms in their
+// scalars, everything else shared) are emitted in one pass: the prefixes,
+// elements and probabilities are computed once, and each sample walks its own
+// lattice, so each output is the one its single emission would write.
+struct EmitOut {
+  CompScalars* sc;
+  uint32_t* idx;
+  double* val;
+};
+template <int NS>
+struct EmitOuts {
+  EmitOut o[NS];
+};
+
+template <int NS>
+__global__ __launch_bounds__(NT) void k_samp_emit_n(const double* __restrict__ x, const double* __restrict__ pin,
+                                                    int64_t n, const u128* __restrict__ tsum,
+                                                    const u128* __restrict__ thpre, EmitOuts<NS> eo) {
+  // every sample shares the first one's selection: n', K, remaining norm
+  const CompScalars c = *eo.o[0].sc;
   if (c.n_sampled == 0) return;
-  const double u = c.u;
+  double u[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) u[q] = q == 0 ? c.u : eo.o[q].sc->u;
   const double nsd = (double)c.n_sampled;
   const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
   u128 pre, end;
   thread_range(tsum, thpre, blockIdx.x, threadIdx.x, pre, end);
   const double Plo = u128_to_double(pre, -96), Phi = u128_to_double(end, -96);
   if (!(Phi > Plo)) return;
-  int64_t kc = first_k_at_least(Plo, u);
-  {
-    int64_t khi = first_k_at_least(Phi, u);
+  int64_t kc[NS];
+  bool live[NS];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    kc[q] = first_k_at_least(Plo, u[q]);
+    int64_t khi = first_k_at_least(Phi, u[q]);
     if (khi > ns) khi = ns;
-    if (kc >= khi) return;
+    live[q] = kc[q] < khi;
+    any |= live[q];
   }
+  if (!any) return;
   const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
   const double* src = pin ? pin : x;
   double vals[ITEMS];
@@ -578,9 +600,10 @@ __global__ __launch_bounds__(NT) void k_samp_emit(const double* __restrict__ x, 
   // first_k_at_least(P_j) .. first_k_at_least(P_{j+1}), by monotonicity).
   // fl(kc+u) < P_{j+1} is tested on the exact prefix against the threshold of
   // fl(kc+u) (above_threshold), so the prefix is rounded to a double only here
-  u128 T;
-  bool tie;
-  above_threshold((double)kc + u, T, tie);
+  u128 T[NS];
+  bool tie[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) above_threshold((double)kc[q] + u[q], T[q], tie[q]);
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
     const int64_t j = base + r;
@@ -589,17 +612,29 @@ __global__ __launch_bounds__(NT) void k_samp_emit(const double* __restrict__ x, 
     const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
     pre += f;
     if (f != 0) {
-      int ne = 0;
-      while (kc < ns && (pre > T || (pre == T && tie))) {
-        out_idx[K + kc] = (uint32_t)j;
-        if (out_val) out_val[K + kc] = v / p;
-        kc++;
-        ne++;
-        above_threshold((double)kc + u, T, tie);
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        if (!live[q]) continue;
+        int ne = 0;
+        while (kc[q] < ns && (pre > T[q] || (pre == T[q] && tie[q]))) {
+          eo.o[q].idx[K + kc[q]] = (uint32_t)j;
+          if (eo.o[q].val) eo.o[q].val[K + kc[q]] = v / p;
+          kc[q]++;
+          ne++;
+          above_threshold((double)kc[q] + u[q], T[q], tie[q]);
+        }
+        if (ne > 1) eo.o[q].sc->dup = 1u;  // j emitted more than once
       }
-      if (ne > 1) sc->dup = 1u;  // j emitted more than once
     }
   }
+}
+
+// One sample's emission (the single-sample case of k_samp_emit_n)
+static void samp_emit(const double* x, const double* pin, int64_t n, CompScalars* sc, const u128* tsum,
+                      const u128* thpre, uint32_t* out_idx, double* out_val, int64_t nt, hipStream_t s) {
+  EmitOuts<1> eo;
+  eo.o[0] = EmitOut{sc, out_idx, out_val};
+  k_samp_emit_n<1><<<nt, NT, 0, s>>>(x, pin, n, tsum, thpre, eo);
 }
 
 // Single thread: total interval end and the first not-found position.
@@ -1446,9 +1481,11 @@ static int check_compress_args(int64_t n, int64_t m, int method, double u_val, c
 // stays on the device and the kernels read it there; buffers are sized for
 // the worst case.  The call's scalars end in w.sc and, when export_sc is
 // given, in that device buffer.  u comes from u_dev when given, else u_val.
+// defer_emit (systematic FRI only): stop after the probability prefix; the
+// caller runs the emission (compress_pair_launch emits two samples at once).
 static int compress_pipeline(const double* x, int64_t n, int64_t m, int method, double u_val, const double* u_dev,
                              uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s,
-                             const CompScratch& w) {
+                             const CompScratch& w, bool defer_emit = false) {
   const int64_t nt = ceil_div(n, TILE);
   CompScalars* sc = w.sc;
   uint32_t* hist = w.hist;
@@ -1495,8 +1532,11 @@ static int compress_pipeline(const double* x, int64_t n, int64_t m, int method, 
     if (method == COMP_FRI_PIVOTAL) {
       FRICC_TRY(pivotal_passes(x, nullptr, nullptr, n, m, sc, ts, w.ths, w.tot, w.tmax, w.thmax, w.pst, out_idx,
                                out_val, s));
+    } else if (defer_emit) {
+      FRICC_HIP(hipGetLastError());
+      return OK;
     } else {
-      k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, ts, w.ths, out_idx, out_val);
+      samp_emit(x, nullptr, n, sc, ts, w.ths, out_idx, out_val, nt, s);
       k_samp_final<<<1, 1, 0, s>>>(sc, w.tot);
       k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
     }
@@ -1572,11 +1612,51 @@ int compress_resample(const double* x, int64_t n, int64_t m, int method, const d
     FRICC_TRY(pivotal_passes(x, nullptr, nullptr, n, m, sc, (const u128*)p_ts, (const u128*)p_ths,
                              (const u128*)p_tot, tmax, thmax, pst, out_idx, out_val, s));
   } else if (method != COMP_LARGEST) {
-    k_samp_emit<<<nt, NT, 0, s>>>(x, nullptr, n, sc, (const u128*)p_ts, (const u128*)p_ths, out_idx, out_val);
+    samp_emit(x, nullptr, n, sc, (const u128*)p_ts, (const u128*)p_ths, out_idx, out_val, nt, s);
     k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
     k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
   }
   if (export_sc) k_export_scalars<<<1, 64, 0, s>>>(sc, (CompScalars*)export_sc);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
+// Two samples of one compression (uniforms u_dev, u_dev2), bit for bit
+// compress_launch with u_dev followed by compress_resample with u_dev2.  For
+// systematic FRI the selection and the probability prefix run once and one
+// pass emits both samples (k_samp_emit_n<2>); the other methods take the two
+// calls.
+int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, const double* u_dev,
+                         const double* u_dev2, uint32_t* idx1, double* val1, void* export1, uint32_t* idx2,
+                         double* val2, void* export2, hipStream_t s) {
+  FRICC_CHECK(u_dev != nullptr && u_dev2 != nullptr, "compress: a pair needs two device uniforms");
+  if (method != COMP_FRI_SYSTEMATIC) {
+    void* sc = nullptr;
+    FRICC_TRY(compress_launch(x, n, m, method, 0.0, u_dev, idx1, val1, export1, s, &sc));
+    return compress_resample(x, n, m, method, u_dev2, idx1, val1, idx2, val2, export2, s);
+  }
+  FRICC_TRY(check_compress_args(n, m, method, 0.0, u_dev));
+  CompScratch w;
+  FRICC_TRY(comp_scratch_ws(n, &w));
+  FRICC_TRY(compress_pipeline(x, n, m, method, 0.0, u_dev, idx1, val1, nullptr, s, w, true));
+  void* p_sc2;
+  FRICC_TRY(ws_get(WS_SC2, sizeof(CompScalars), &p_sc2));
+  CompScalars* sc2 = (CompScalars*)p_sc2;
+  const int64_t nt = ceil_div(n, TILE);
+  // the second sample's scalars (taken before the first emission: it changes
+  // only what k_resample_scalars resets) and its kept part
+  k_resample_scalars<<<1, 64, 0, s>>>(w.sc, u_dev2, sc2);
+  k_copy_kept<<<(unsigned)std::min<int64_t>(ceil_div(m, NT), 2048), NT, 0, s>>>(idx1, val1, sc2, m, idx2, val2);
+  EmitOuts<2> eo;
+  eo.o[0] = EmitOut{w.sc, idx1, val1};
+  eo.o[1] = EmitOut{sc2, idx2, val2};
+  k_samp_emit_n<2><<<nt, NT, 0, s>>>(x, nullptr, n, w.ts, w.ths, eo);
+  k_samp_final<<<1, 1, 0, s>>>(w.sc, w.tot);
+  k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, w.sc, idx1, val1);
+  k_samp_final<<<1, 1, 0, s>>>(sc2, w.tot);
+  k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc2, idx2, val2);
+  if (export1) k_export_scalars<<<1, 64, 0, s>>>(w.sc, (CompScalars*)export1);
+  if (export2) k_export_scalars<<<1, 64, 0, s>>>(sc2, (CompScalars*)export2);
   FRICC_HIP(hipGetLastError());
   return OK;
 }
@@ -1671,7 +1751,7 @@ int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, ui
   k_init_scalars<<<1, NT, 0, s>>>(sc, 0, n_sample, u, nullptr, nullptr, 0);
   k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (u128*)p_ths);
   k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-  k_samp_emit<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (const u128*)p_ths, out_idx, nullptr);
+  samp_emit(nullptr, p, n, sc, ts, (const u128*)p_ths, out_idx, nullptr, nt, s);
   k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
   k_samp_notfound<<<ceil_div(n_sample, NT), NT, 0, s>>>(p, sc, out_idx, nullptr);
   FRICC_HIP(hipGetLastError());

@@ -31,6 +31,11 @@ int compress_launch(const double* x, int64_t n, int64_t m, int method, double u_
 // just compressed into idx/val (same x, n, m, method, stream and arena).
 int compress_resample(const double* x, int64_t n, int64_t m, int method, const double* u_dev, const uint32_t* idx,
                       const double* val, uint32_t* out_idx, double* out_val, void* export_sc, hipStream_t s);
+// Two samples (u_dev, u_dev2) of one compression in one call: bit for bit
+// compress_launch(u_dev) then compress_resample(u_dev2).
+int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, const double* u_dev,
+                         const double* u_dev2, uint32_t* idx1, double* val1, void* export1, uint32_t* idx2,
+                         double* val2, void* export2, hipStream_t s);
 // The same as compress(), with all scratch in a fixed device array (no
 // allocation); n <= compress_noalloc_max_n().
 int compress_noalloc(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,

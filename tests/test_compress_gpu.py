@@ -202,6 +202,37 @@ def test_pair_equals_two_compressions(mods, comp):
             assert np.array_equal(T.indices(), oi) and np.array_equal(T.values().view(np.int64), ov.view(np.int64))
 
 
+def test_pair_edge_cases(mods):
+    """The pair entry point when the kept part takes all m entries (no sample
+    to emit) and when the second uniform lies outside [0, 1] (that tensor's
+    status reports it; the first sample is unaffected)."""
+    py_rccsd, lo, torch = mods
+    rng = np.random.default_rng(4)
+    x = np.zeros(50_000)
+    x[rng.choice(x.size, 1000, replace=False)] = rng.standard_normal(1000)
+    xd = torch.from_numpy(x).cuda()
+    dims = (1, 1, 1, x.size)
+    ud = torch.tensor([0.3, 0.6], dtype=torch.float64, device="cuda")
+    A, B = py_rccsd.SparseTensor4d.empty(dims), py_rccsd.SparseTensor4d.empty(dims)
+    A.recompress_pair_async(B, xd, 1000, "fri", ud[0:1], ud[1:2])
+    for T, u in ((A, 0.3), (B, 0.6)):
+        info = T.sync_status()
+        assert info["n_sampled"] == 0 and info["n_D"] == 1000
+        oi, ov, _ = lo.compress(x, 1000, lo.FRI_SYSTEMATIC, u)
+        assert np.array_equal(T.indices(), oi) and np.array_equal(T.values().view(np.int64), ov.view(np.int64))
+    x = _vec(100_000, 6)
+    xd = torch.from_numpy(x).cuda()
+    dims = (1, 1, 1, x.size)
+    ud = torch.tensor([0.4, 1.5], dtype=torch.float64, device="cuda")
+    A, B = py_rccsd.SparseTensor4d.empty(dims), py_rccsd.SparseTensor4d.empty(dims)
+    A.recompress_pair_async(B, xd, 5000, "fri", ud[0:1], ud[1:2])
+    A.sync_status()
+    oi, ov, _ = lo.compress(x, 5000, lo.FRI_SYSTEMATIC, 0.4)
+    assert np.array_equal(A.indices(), oi) and np.array_equal(A.values().view(np.int64), ov.view(np.int64))
+    with pytest.raises((RuntimeError, ValueError)):
+        B.sync_status()
+
+
 def test_multi_equals_separate_compressions(mods):
     """compress_multi_async: six samples of one vector (uniforms in order,
     u = 1 among them), each bit for bit its own compression and the oracle's."""
