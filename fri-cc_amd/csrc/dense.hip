@@ -660,75 +660,90 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
 }
 
 // ---------------------------------------------------------------------------
-// Two products that read the same o*v^3 operand Z[k,c,d,a] (FP32), fused so
-// that Z is streamed once instead of twice:
-//   out_ia[i,a] += sum_{k,c,d} t2[i,k,c,d] Z[k,c,d,a]   (FP32 MFMA)
-//   out_ac[a,c]  = sum_{k,d}   t1[k,d]     Z[k,c,d,a]   (FP32 FMA)
-// A workgroup owns one c and 128 a's and walks K = (k, d) in steps of 32 d:
-// per step it stages t2[0..31, k, c, d0..d0+31], Z[k, c, d0..d0+31, a0..+127]
-// and t1[k, d0..d0+31] through three LDS stages (global->LDS DMA, two steps
-// in flight while one is computed).  The first product's i x a block goes to
-// a per-c partial (fixed-order FP64 sum over c in k_zpair_reduce); the second
+// Two products that read the same o*v^3 operand Z[k,c,d,a], fused so that Z
+// is streamed once instead of twice (T = float: FP32 operands and MFMA;
+// T = double: FP64):
+//   out_ia[i,a] += sum_{k,c,d} t2[i,k,c,d] Z[k,c,d,a]   (MFMA 16x16x4)
+//   out_ac[a,c]  = sum_{k,d}   t1[k,d]     Z[k,c,d,a]   (FMA)
+// A workgroup owns one c and 128 a's and walks K = (k, d) in steps of KD d
+// (KD * sizeof(T) = 128 bytes: 32 floats or 16 doubles): per step it stages
+// t2[0..31, k, c, d0..d0+KD), Z[k, c, d0..d0+KD, a0..+127] and
+// t1[k, d0..d0+KD) through three LDS stages (global->LDS DMA, two steps in
+// flight while one is computed).  The first product's i x a block goes to a
+// per-c partial (fixed-order FP64 sum over c in k_zpair_reduce); the second
 // is complete in the workgroup (its sum over k runs inside).  Needs o <= 32
 // and v % 128 == 0; rows i >= o read row o-1 and are not written.
-constexpr int ZP_KD = 32;   // d per step
 constexpr int ZP_BA = 128;  // a per workgroup
 constexpr int ZP_S = 3;     // LDS stages
 constexpr int ZP_LW = 6;    // DMA instructions per wave per step (A 1, B 4, t1 1)
 
-struct ZPStage {
-  float A[32][ZP_KD];     // t2 rows i
-  float B[ZP_KD][ZP_BA];  // Z rows d
-  float t1[ZP_KD];
+template <typename T>
+struct ZP {
+  static constexpr int KD = 128 / (int)sizeof(T);          // d per step
+  static constexpr int RPD = 1024 / (ZP_BA * (int)sizeof(T));  // Z rows per 1 KB DMA
+  static constexpr int LPR = 64 / RPD;                     // lanes per Z row
 };
 
+template <typename T>
+struct ZPStage {
+  T A[32][ZP<T>::KD];     // t2 rows i
+  T B[ZP<T>::KD][ZP_BA];  // Z rows d
+  T t1[ZP<T>::KD];
+};
+
+template <typename T>
 struct ZPArgs {
-  const float* t2;
-  const float* t1;
-  const float* Z;
-  float* part;  // [v][32][v]
+  const T* t2;
+  const T* t1;
+  const T* Z;
+  T* part;  // [v][32][v]
   double* out_ac;
   int o, v;
 };
 
-// DMAs of step q (k = q / (v/32), d0 = 32 (q % (v/32))) into one stage
-__device__ inline void zp_issue(const ZPArgs& a, int c, int a0, int q, ZPStage& st, int w, int lane) {
-  const int nd = a.v / ZP_KD;
-  const int k = q / nd, d0 = (q % nd) * ZP_KD;
+// DMAs of step q (k = q / (v/KD), d0 = KD (q % (v/KD))) into one stage
+template <typename T>
+__device__ inline void zp_issue(const ZPArgs<T>& a, int c, int a0, int q, ZPStage<T>& st, int w, int lane) {
+  constexpr int KD = ZP<T>::KD, RPD = ZP<T>::RPD, LPR = ZP<T>::LPR, E16 = 16 / (int)sizeof(T);
+  const int nd = a.v / KD;
+  const int k = q / nd, d0 = (q % nd) * KD;
   const int64_t v = a.v;
-  // A: rows 8w..8w+7, 8 lanes of 16 bytes per row
+  // A: rows 8w..8w+7, 8 lanes of 16 bytes per 128-byte row
   {
     const int i = min(8 * w + (lane >> 3), a.o - 1);
-    const float* g = a.t2 + (((int64_t)i * a.o + k) * v + c) * v + d0 + 4 * (lane & 7);
+    const T* g = a.t2 + (((int64_t)i * a.o + k) * v + c) * v + d0 + E16 * (lane & 7);
     dma16(g, lds_addr(&st.A[8 * w][0]));
   }
-  // B: rows 8w..8w+7, two rows (32 lanes x 16 bytes each) per DMA
+  // B: rows (KD/4)w .. +KD/4, RPD rows (LPR lanes x 16 bytes each) per DMA
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int r = 8 * w + 2 * j + (lane >> 5);
-    const float* g = a.Z + (((int64_t)k * v + c) * v + d0 + r) * v + a0 + 4 * (lane & 31);
-    dma16(g, lds_addr(&st.B[8 * w + 2 * j][0]));
+    const int r0 = (KD / 4) * w + RPD * j;
+    const T* g = a.Z + (((int64_t)k * v + c) * v + d0 + r0 + lane / LPR) * v + a0 + E16 * (lane % LPR);
+    dma16(g, lds_addr(&st.B[r0][0]));
   }
-  // t1[k, d0..d0+31]: lanes 0-7 of every wave write the same 128 bytes
-  if (lane < 8) dma16(a.t1 + (int64_t)k * v + d0 + 4 * lane, lds_addr(&st.t1[0]));
+  // t1[k, d0..d0+KD): lanes 0-7 of every wave write the same 128 bytes
+  if (lane < 8) dma16(a.t1 + (int64_t)k * v + d0 + E16 * lane, lds_addr(&st.t1[0]));
 }
 
-__global__ __launch_bounds__(256, 2) void k_zpair(ZPArgs a) {
-  __shared__ ZPStage st[ZP_S];
-  __shared__ float red[ZP_BA];
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_zpair(ZPArgs<T> a) {
+  constexpr int KD = ZP<T>::KD;
+  typedef typename Acc<T>::type V4;
+  __shared__ ZPStage<T> st[ZP_S];
+  __shared__ T red[ZP_BA];
   const int c = blockIdx.x / (a.v / ZP_BA);
   const int a0 = (blockIdx.x % (a.v / ZP_BA)) * ZP_BA;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int nq = a.o * (a.v / ZP_KD);
-  f4 acc[2][2];
+  const int nq = a.o * (a.v / KD);
+  V4 acc[2][2];
 #pragma unroll
   for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++) acc[rb][cb] = f4{0, 0, 0, 0};
-  float acc2 = 0.f;
-  const int ac = tid & (ZP_BA - 1), h = tid >> 7;  // the second product: column ac, rows 16h..16h+15
+    for (int cb = 0; cb < 2; cb++) acc[rb][cb] = V4{0, 0, 0, 0};
+  T acc2 = 0;
+  const int ac = tid & (ZP_BA - 1), h = tid >> 7;  // the second product: column ac, rows (KD/2)h ..
 #pragma unroll
   for (int s = 0; s < ZP_S - 1; s++)
     if (s < nq) zp_issue(a, c, a0, s, st[s], w, lane);
@@ -743,19 +758,19 @@ __global__ __launch_bounds__(256, 2) void k_zpair(ZPArgs a) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     const int nxt = cur == 0 ? ZP_S - 1 : cur - 1;  // (cur + S - 1) % S
     if (q + ZP_S - 1 < nq) zp_issue(a, c, a0, q + ZP_S - 1, st[nxt], w, lane);
-    const ZPStage& sm = st[cur];
+    const ZPStage<T>& sm = st[cur];
 #pragma unroll
-    for (int s = 0; s < ZP_KD / 4; s++) {
-      const float x0 = sm.A[lr][4 * s + lk], x1 = sm.A[16 + lr][4 * s + lk];
+    for (int s = 0; s < KD / 4; s++) {
+      const T x0 = sm.A[lr][4 * s + lk], x1 = sm.A[16 + lr][4 * s + lk];
 #pragma unroll
       for (int cb = 0; cb < 2; cb++) {
-        const float y = sm.B[4 * s + lk][w * 32 + cb * 16 + lr];
-        acc[0][cb] = Acc<float>::mfma(x0, y, acc[0][cb]);
-        acc[1][cb] = Acc<float>::mfma(x1, y, acc[1][cb]);
+        const T y = sm.B[4 * s + lk][w * 32 + cb * 16 + lr];
+        acc[0][cb] = Acc<T>::mfma(x0, y, acc[0][cb]);
+        acc[1][cb] = Acc<T>::mfma(x1, y, acc[1][cb]);
       }
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) acc2 = fmaf(sm.t1[16 * h + j], sm.B[16 * h + j][ac], acc2);
+    for (int j = 0; j < KD / 2; j++) acc2 = fma(sm.t1[(KD / 2) * h + j], sm.B[(KD / 2) * h + j][ac], acc2);
     cur = cur == ZP_S - 1 ? 0 : cur + 1;
   }
   // first product: this c's partial of rows i < o
@@ -765,7 +780,7 @@ __global__ __launch_bounds__(256, 2) void k_zpair(ZPArgs a) {
     for (int cb = 0; cb < 2; cb++)
 #pragma unroll
       for (int ii = 0; ii < 4; ii++) {
-        const int i = rb * 16 + Acc<float>::row(lk, ii);
+        const int i = rb * 16 + Acc<T>::row(lk, ii);
         if (i < a.o) a.part[((int64_t)c * 32 + i) * a.v + a0 + w * 32 + cb * 16 + lr] = acc[rb][cb][ii];
       }
   // second product: add the two row halves, write out_ac[a][c]
@@ -775,7 +790,8 @@ __global__ __launch_bounds__(256, 2) void k_zpair(ZPArgs a) {
 }
 
 // out_ia[i,a] += sum over c of the partials, in c order, in FP64
-__global__ void k_zpair_reduce(const float* __restrict__ part, int o, int v, double* __restrict__ out_ia) {
+template <typename T>
+__global__ void k_zpair_reduce(const T* __restrict__ part, int o, int v, double* __restrict__ out_ia) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)o * v) return;
   const int i = (int)(e / v), aa = (int)(e % v);
@@ -784,26 +800,34 @@ __global__ void k_zpair_reduce(const float* __restrict__ part, int o, int v, dou
   out_ia[e] += s;
 }
 
-int dense_zpair(int o, int v, const float* t2, const float* t1, const float* Z, double* out_ia, double* out_ac,
-                hipStream_t s) {
-  FRICC_CHECK(o > 0 && o <= 32 && v > 0 && v % ZP_BA == 0, "dense_zpair: needs 0 < o <= 32 and v a multiple of 128");
-  FRICC_CHECK(t2 && t1 && Z && out_ia && out_ac, "dense_zpair: null pointer");
-  FRICC_CHECK(((uintptr_t)t2 | (uintptr_t)t1 | (uintptr_t)Z) % 16 == 0, "dense_zpair: operands must be 16-byte aligned");
+template <typename T>
+static int zpair_launch(int o, int v, const T* t2, const T* t1, const T* Z, double* out_ia, double* out_ac,
+                        hipStream_t s) {
   void* p;
-  FRICC_TRY(ws_get(52, (size_t)v * 32 * v * 4, &p));
-  ZPArgs a;
+  FRICC_TRY(ws_get(52, (size_t)v * 32 * v * sizeof(T), &p));
+  ZPArgs<T> a;
   a.t2 = t2;
   a.t1 = t1;
   a.Z = Z;
-  a.part = (float*)p;
+  a.part = (T*)p;
   a.out_ac = out_ac;
   a.o = o;
   a.v = v;
-  k_zpair<<<v * (v / ZP_BA), 256, 0, s>>>(a);
+  k_zpair<T><<<v * (v / ZP_BA), 256, 0, s>>>(a);
   FRICC_HIP(hipGetLastError());
-  k_zpair_reduce<<<(unsigned)ceil_div((int64_t)o * v, 256), 256, 0, s>>>((const float*)p, o, v, out_ia);
+  k_zpair_reduce<T><<<(unsigned)ceil_div((int64_t)o * v, 256), 256, 0, s>>>((const T*)p, o, v, out_ia);
   FRICC_HIP(hipGetLastError());
   return OK;
+}
+
+int dense_zpair(int o, int v, int f64, const void* t2, const void* t1, const void* Z, double* out_ia,
+                double* out_ac, hipStream_t s) {
+  FRICC_CHECK(o > 0 && o <= 32 && v > 0 && v % ZP_BA == 0, "dense_zpair: needs 0 < o <= 32 and v a multiple of 128");
+  FRICC_CHECK(t2 && t1 && Z && out_ia && out_ac, "dense_zpair: null pointer");
+  FRICC_CHECK(((uintptr_t)t2 | (uintptr_t)t1 | (uintptr_t)Z) % 16 == 0, "dense_zpair: operands must be 16-byte aligned");
+  if (f64)
+    return zpair_launch<double>(o, v, (const double*)t2, (const double*)t1, (const double*)Z, out_ia, out_ac, s);
+  return zpair_launch<float>(o, v, (const float*)t2, (const float*)t1, (const float*)Z, out_ia, out_ac, s);
 }
 
 }  // namespace fricc
@@ -846,14 +870,16 @@ int fricc_dense_contract_from(const char* spec, double alpha, const void* A, int
 }
 
 // out_ia[i,a] += sum_{k,c,d} t2[i,k,c,d] Z[k,c,d,a] and out_ac[a,c] =
-// sum_{k,d} t1[k,d] Z[k,c,d,a] with Z read once (FP32 operands, FP64 outputs).
+// sum_{k,d} t1[k,d] Z[k,c,d,a] with Z read once (operands FP64 with precision
+// 0 or FP32 with precision 1, on the MFMA of that type; FP64 outputs).
 // With Z[k,c,d,a] = 2 (kd|ac) - (kc|ad) these are the t1 equation's ovvv*t2
 // terms (fricc/friccsd.py:165-166) and the ovvv*t1 part of Lvv
 // (imd.Lvv, fricc/friccsd.py:190; also the t1*t1 terms of :167-168), which
 // the reference evaluates as separate einsums; the fused entry point itself
 // has no counterpart in the reference.
-int fricc_dense_zpair(int o, int v, const float* t2, const float* t1, const float* Z, double* out_ia, double* out_ac,
-                      void* stream) {
-  return fricc::dense_zpair(o, v, t2, t1, Z, out_ia, out_ac, (hipStream_t)stream);
+int fricc_dense_zpair(int o, int v, int precision, const void* t2, const void* t1, const void* Z, double* out_ia,
+                      double* out_ac, void* stream) {
+  FRICC_CHECK(precision == 0 || precision == 1, "dense_zpair: precision must be 0 (FP64) or 1 (FP32)");
+  return fricc::dense_zpair(o, v, precision == 0, t2, t1, Z, out_ia, out_ac, (hipStream_t)stream);
 }
 }

@@ -104,7 +104,7 @@ def lib():
     L.fricc_eri_build.argtypes = [_vp, _i64, _i64, I64x8, I64x4, _vp, _vp]
     L.fricc_set_gemm_order.argtypes = [ctypes.c_int]
     L.fricc_set_gemm_split.argtypes = [ctypes.c_int]
-    L.fricc_dense_zpair.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6
+    L.fricc_dense_zpair.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6
     L.fricc_split_bf16.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_split_bf16_hi.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_split_bf16_lincomb.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i64, _i64, _vp,

@@ -392,14 +392,14 @@ def _update_amps_device(cc, t1, t2, eris):
     E("kiac,kc->ia", P["oovv"], t1, out=t1new, alpha=-1.0, beta=1.0)
     # O^2V^3: 2 kdac,ikcd - kcad,ikcd  ==  t2[i,(kcd)] Z[(kcd),a], and
     # LvvT1[a,c] = sum_kd Z[k,c,d,a] t1[k,d]  (== 2 kdac,kd - kcad,kd -> ac):
-    # both read the o*v^3 operand Z, so in FP32 one kernel streams it once
+    # both read the o*v^3 operand Z, so one kernel streams it once
     Z = P["Z"]
-    if cdt == torch.float32 and o <= 32 and v % 128 == 0:
+    if o <= 32 and v % 128 == 0:
         LvvT1 = torch.empty((v, v), dtype=f64, device=t1.device)
-        t1f = t1.to(torch.float32).contiguous()
+        t1f = t1.to(cdt).contiguous()
         t2z = t2c.contiguous()
-        _capi.check(_capi.lib().fricc_dense_zpair(o, v, _vp(t2z), _vp(t1f), _vp(Z), _vp(t1new), _vp(LvvT1),
-                                                  _capi.stream_ptr()))
+        _capi.check(_capi.lib().fricc_dense_zpair(o, v, _PREC_ID["fp64" if cdt == f64 else "fp32"], _vp(t2z),
+                                                  _vp(t1f), _vp(Z), _vp(t1new), _vp(LvvT1), _capi.stream_ptr()))
     else:
         E("ikcd,kcda->ia", t2c, Z, out=t1new, beta=1.0)
         LvvT1 = C(E("kd,kcda->ac", t1, Z))

@@ -297,16 +297,17 @@ int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void
                       const double* t1, const double* wsum, const double* eia, const double* t2old,
                       const double* lovov, const double* t1new, double* sums, void* stream);
 
-/* Two dense products that read the same o*v^3 FP32 operand Z[k][c][d][a],
- * with Z streamed once: out_ia[i][a] += sum_{k,c,d} t2[i][k][c][d] Z[k][c][d][a]
- * and out_ac[a][c] = sum_{k,d} t1[k][d] Z[k][c][d][a].  t2 [o][o][v][v], t1
- * [o][v], Z [o][v][v][v]: FP32, C order, 16-byte aligned; out_ia [o][v] and
- * out_ac [v][v]: FP64.  Needs o <= 32 and v a multiple of 128
- * (FRICC_EINVAL otherwise).  With Z = 2 (kd|ac) - (kc|ad) these are the t1
- * equation's ovvv*t2 terms and the t1 part of Lvv (fricc/friccsd.py:165-168,
- * 190), which the reference computes as separate einsums; the fused entry
- * point has no counterpart in the reference. */
-int fricc_dense_zpair(int o, int v, const float* t2, const float* t1, const float* Z, double* out_ia,
+/* Two dense products that read the same o*v^3 operand Z[k][c][d][a], with Z
+ * streamed once: out_ia[i][a] += sum_{k,c,d} t2[i][k][c][d] Z[k][c][d][a] and
+ * out_ac[a][c] = sum_{k,d} t1[k][d] Z[k][c][d][a].  t2 [o][o][v][v], t1
+ * [o][v], Z [o][v][v][v]: FP64 (precision FRICC_FP64, FP64 MFMA) or FP32
+ * (FRICC_FP32, FP32 MFMA), C order, 16-byte aligned; out_ia [o][v] and
+ * out_ac [v][v]: FP64.  Needs o <= 32 and v a multiple of 128 (FRICC_EINVAL
+ * otherwise).  With Z = 2 (kd|ac) - (kc|ad) these are the t1 equation's
+ * ovvv*t2 terms and the t1 part of Lvv (fricc/friccsd.py:165-168, 190),
+ * which the reference computes as separate einsums; the fused entry point
+ * has no counterpart in the reference. */
+int fricc_dense_zpair(int o, int v, int precision, const void* t2, const void* t1, const void* Z, double* out_ia,
                       double* out_ac, void* stream);
 
 /* Column-tile width of the gathered views of FRICC_BF16 contractions: 0 =

@@ -150,26 +150,30 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("o,v", [(32, 256), (20, 128), (1, 128), (8, 384), (7, 256)])
-def test_dense_zpair_matches_torch(env, o, v):
+def test_dense_zpair_matches_torch(env, o, v, prec):
     """fricc_dense_zpair (both products of the o*v^3 operand, Z read once)
     against FP64 einsums on the CPU: out_ia += t2.Z (accumulated onto a given
-    out_ia) and out_ac = t1.Z, FP32 operands; tolerance 2e-6 of sum |a||b|
-    per element, as for the FP32 dense contraction."""
+    out_ia) and out_ac = t1.Z, FP32 or FP64 operands; tolerance per element
+    2e-6 (FP32) or 1e-12 (FP64) of sum |a||b|, as for the dense contraction."""
     torch, _ = env
     from fricc import _capi
 
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    tol = 2e-6 if prec == "fp32" else 1e-12
     g = torch.Generator().manual_seed(o * 1000 + v)
-    t2 = torch.randn(o, o, v, v, generator=g, dtype=torch.float32) * 0.1
-    t1 = torch.randn(o, v, generator=g, dtype=torch.float32) * 0.1
-    Z = torch.randn(o, v, v, v, generator=g, dtype=torch.float32)
+    t2 = (torch.randn(o, o, v, v, generator=g, dtype=torch.float64) * 0.1).to(dt)
+    t1 = (torch.randn(o, v, generator=g, dtype=torch.float64) * 0.1).to(dt)
+    Z = torch.randn(o, v, v, v, generator=g, dtype=torch.float64).to(dt)
     out_ia0 = torch.randn(o, v, generator=g, dtype=torch.float64)
     # device copies held in names: a temporary freed before the launch could
     # be reused by the next allocation
     t2g, t1g, Zg = t2.cuda(), t1.cuda(), Z.cuda()
     out_ia = out_ia0.cuda()
     out_ac = torch.full((v, v), float("nan"), dtype=torch.float64, device="cuda")
-    _capi.check(_capi.lib().fricc_dense_zpair(o, v, _vp(t2g), _vp(t1g), _vp(Zg), _vp(out_ia), _vp(out_ac),
+    pid = _capi.FP32 if prec == "fp32" else _capi.FP64
+    _capi.check(_capi.lib().fricc_dense_zpair(o, v, pid, _vp(t2g), _vp(t1g), _vp(Zg), _vp(out_ia), _vp(out_ac),
                                               _capi.stream_ptr()))
     torch.cuda.synchronize()
     t2d, t1d, Zd = t2.double(), t1.double(), Z.double()
@@ -179,8 +183,8 @@ def test_dense_zpair_matches_torch(env, o, v):
     mag_ac = torch.einsum("kd,kcda->ac", t1d.abs(), Zd.abs())
     err_ia = (out_ia.cpu() - want_ia).abs()
     err_ac = (out_ac.cpu() - want_ac).abs()
-    assert torch.all(err_ia <= 2e-6 * mag_ia), float((err_ia / mag_ia).max())
-    assert torch.all(err_ac <= 2e-6 * mag_ac), float((err_ac / mag_ac).max())
+    assert torch.all(err_ia <= tol * mag_ia), float((err_ia / mag_ia).max())
+    assert torch.all(err_ac <= tol * mag_ac), float((err_ac / mag_ac).max())
 
 
 def test_dense_zpair_rejects_unsupported_shapes(env):
@@ -188,6 +192,7 @@ def test_dense_zpair_rejects_unsupported_shapes(env):
     from fricc import _capi
 
     buf = torch.zeros(1 << 16, dtype=torch.float64, device="cuda")
-    for o, v in ((33, 128), (4, 90), (4, 0), (0, 128)):
-        st = _capi.lib().fricc_dense_zpair(o, v, _vp(buf), _vp(buf), _vp(buf), _vp(buf), _vp(buf), _capi.stream_ptr())
-        assert st == _capi.FRICC_EINVAL, (o, v, st)
+    for o, v, p in ((33, 128, 1), (4, 90, 1), (4, 0, 0), (0, 128, 0), (4, 128, 2)):
+        st = _capi.lib().fricc_dense_zpair(o, v, p, _vp(buf), _vp(buf), _vp(buf), _vp(buf), _vp(buf),
+                                           _capi.stream_ptr())
+        assert st == _capi.FRICC_EINVAL, (o, v, p, st)
