@@ -470,8 +470,18 @@ def _update_amps_device(cc, t1, t2, eris):
     t0 = time.perf_counter()
     # the ladder's t1 dressing (Gx/Gy contractions and their t1 products) in
     # the dense-GEMM precision
-    X = torch.zeros((o, o, o, v), dtype=cdt, device=t2.device)  # sum_cd (kd|ac) T[ij,cd]
-    Y = torch.zeros((o, o, o, v), dtype=cdt, device=t2.device)  # sum_cd (kc|bd) T[ij,cd]
+    # X[ij,k,a] = sum_cd (kd|ac) T[ij,cd] and Y[ij,k,b] = sum_cd (kc|bd) T[ij,cd]
+    # live in halves of two stacked buffers whose other halves hold t1, so that
+    # acc = -(X.t1 + t1.Y) below is one batched product over ij with the sum
+    # running over (h, k):  XT[ij,0] = X, XT[ij,1] = t1;  YT[ij,0] = t1, YT[ij,1] = Y
+    XT = torch.empty((o, o, 2, o, v), dtype=cdt, device=t2.device)
+    YT = torch.empty((o, o, 2, o, v), dtype=cdt, device=t2.device)
+    X, Y = XT[:, :, 0], YT[:, :, 1]
+    X.zero_()
+    Y.zero_()
+    t1c = t1.to(cdt)
+    XT[:, :, 1].copy_(t1c.expand(o, o, o, v))
+    YT[:, :, 0].copy_(t1c.expand(o, o, o, v))
     if "O^2V^4" in ccs:
         _contract_into("kdac,ijcd->ijka", 1.0, eris.sparse_operand("Gx", sp), (o, v, v, v), ts, X, sp)
         _contract_into("kcbd,ijcd->ijkb", 1.0, eris.sparse_operand("Gy", sp), (o, v, v, v), ts, Y, sp)
@@ -483,8 +493,8 @@ def _update_amps_device(cc, t1, t2, eris):
         E("klij,ka->ijla", Woooo, t1, out=X, alpha=-1.0, beta=1.0)
     # acc[ij,a,b] = -sum_k X[ij,k,a] t1[k,b] - sum_k t1[k,a] Y[ij,k,b]
     acc = torch.empty((o, o, v, v), dtype=f64, device=t2.device)
-    E("ijka,kb->ijab", X, t1, out=acc, alpha=-1.0)
-    E("ka,ijkb->ijab", t1, Y, out=acc, alpha=-1.0, beta=1.0)
+    E("ijhka,ijhkb->ijab", XT, YT, out=acc, alpha=-1.0)
+    del XT, YT, X, Y
     if "O^4V^2" in ccs:
         _contract_into("klij,klab->ijab", 1.0, _dyn_operand(Woooo, o * o, o * o, sp), (o, o, o, o), ts, acc, sp)
     else:
