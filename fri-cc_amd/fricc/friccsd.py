@@ -527,12 +527,20 @@ def _update_amps_device(cc, t1, t2, eris):
     t0 = time.perf_counter()
     # tmp = (ia|cb) t1[j,c] - sum_k oovv[k,i,b,c] t1[j,c] t1[k,a]
     S = E("jc,icab->ijab", t1, P["Qv"])
-    Mt = E("kibc,jc->kijb", P["oovv"], t1)  # sum_c (ki|bc) t1[j,c]
-    E("ka,kijb->ijab", t1, Mt, out=S, alpha=-1.0, beta=1.0)
-    # - (sum_c (ia|kc) t1[j,c] + ovoo[i,a,j,k]) t1[k,b];  sum_c (ia|kc) t1[j,c] = U[i,a,k,j]
-    T2p = U + P["ovoo"].permute(0, 1, 3, 2)  # [i,a,k,j]
-    E("iakj,kb->ijab", T2p, t1, out=S, alpha=-1.0, beta=1.0)
-    del T2p
+    # - sum_k t1[k,a] Mt[k,i,j,b] with Mt[k,i,j,b] = sum_c (ki|bc) t1[j,c], and
+    # - sum_k T2p[i,a,k,j] t1[k,b] with T2p = sum_c (ia|kc) t1[j,c] + ovoo[i,a,j,k]
+    # (sum_c (ia|kc) t1[j,c] = U[i,a,k,j]): one batched product over ij with
+    # the sum over (h, k), from stacked buffers written in place:
+    # AS[ij,0,k,a] = t1[k,a], AS[ij,1,k,a] = T2p[i,a,k,j];
+    # BS[ij,0,k,b] = Mt[k,i,j,b], BS[ij,1,k,b] = t1[k,b]
+    AS = torch.empty((o, o, 2, o, v), dtype=cdt, device=t2.device)
+    BS = torch.empty((o, o, 2, o, v), dtype=cdt, device=t2.device)
+    AS[:, :, 0].copy_(t1c.expand(o, o, o, v))
+    BS[:, :, 1].copy_(t1c.expand(o, o, o, v))
+    E("kibc,jc->kijb", P["oovv"], t1, out=BS[:, :, 0].permute(2, 0, 1, 3))
+    torch.add(U, P["ovoo"].permute(0, 1, 3, 2), out=AS[:, :, 1].permute(0, 3, 2, 1))
+    E("ijhka,ijhkb->ijab", AS, BS, out=S, alpha=-1.0, beta=1.0)
+    del AS, BS
     # + Lvv[a,c] t2[i,j,c,b] - Loo[k,i] t2[k,j,a,b]
     E("ac,ijcb->ijab", Lvv, t2c, out=S, beta=1.0)
     E("ki,kjab->ijab", Loo, t2c, out=S, alpha=-1.0, beta=1.0)
