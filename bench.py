@@ -276,11 +276,19 @@ def replica_bench(args):
     seed 1): R replicas with m log-uniform in [2e5, 2e6] and Philox seeds
     0..R-1, one graph launch per iteration for all of them, against the same
     replicas stepped one at a time eagerly."""
+    import gc
+
     import torch
 
     from fricc.replicas import ReplicaBatch, log_uniform_m, run_alone
     from fricc.systems import CONFIGS, DeviceERIs, SyntheticSystem
 
+    # blocks cached by earlier legs (the serving leg keeps retired solves'
+    # buffers) go back to the device first: with the batch's 64 replicas and
+    # arenas resident, allocations of the eager replicas would otherwise hit
+    # the allocator's free-and-retry path
+    gc.collect()
+    torch.cuda.empty_cache()
     cfg = CONFIGS["c2"]
     o, v = cfg["nocc"], cfg["nvir"]
     sysm = SyntheticSystem.from_config("c2")
