@@ -152,7 +152,7 @@ def _prepare_views(cc, ts, ts_alt, o, v, sp, ccs):
     if "O^3V^3" in ccs:
         _prepare(ts, "akic,kjcb->ijab", (v, o, o, v), (o, o, v, v), sp)
         _prepare(ts, "akic,kjbc->ijab", (v, o, o, v), (o, o, v, v), sp)
-        _prepare(ts, "bkci,kjac->ijab", (v, o, v, o), (o, o, v, v), sp)
+        _prepare(ts, "bkci,kjac->jiba", (v, o, v, o), (o, o, v, v), sp)
 
 
 def _dyn_operand(A, R, X, precision):
@@ -565,7 +565,9 @@ def _update_amps_device(cc, t1, t2, eris):
         Av = _dyn_operand(A_voov, ov, ov, sp)
         Avo = _dyn_operand(A_vovo, ov, ov, sp)
         _contract_into("akic,kjbc->ijab", -1.0, Av, (v, o, o, v), ts, S, sp)    # 1303
-        _contract_into("bkci,kjac->ijab", -1.0, Avo, (v, o, v, o), ts, S, sp)   # 1203
+        # 1203, added transposed (S[j,i,b,a]): S enters only as S + S^T, and
+        # this way the gathered columns (j, a) run along S's unit-stride axis
+        _contract_into("bkci,kjac->jiba", -1.0, Avo, (v, o, v, o), ts, S, sp)
         tm.stop("O3V3")
     else:
         E("akic,kjcb->ijab", Wvoov, t2c, out=S, alpha=2.0, beta=1.0)
