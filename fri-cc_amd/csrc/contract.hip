@@ -24,6 +24,7 @@ struct OutDev {
   int of32;
   uint32_t Xl, Yl;
   int64_t oxh, oxl, oyh, oyl;
+  int vec;  // runs of 4 consecutive elements along y (1) or x (2) are contiguous and 16-byte aligned; 0: neither
 };
 
 // out[off] += alpha * v in the output's element type
@@ -643,6 +644,110 @@ __device__ inline uint32_t row_of(const RowIdx& r, int s) {
   }
 }
 
+// Epilogue through LDS for om.vec != 0: the workgroup's 256 x 128 block of
+// accumulators is staged in the stage buffers (idle once the K loop is done),
+// then read back along the output's unit-stride axis, so each lane reads and
+// writes 4 consecutive outputs with 16-byte (FP32) or 2 x 16-byte (FP64)
+// accesses and each wave covers whole runs (512 B of FP32 / 1 KB of FP64 along
+// y, 1 KB / 2 KB along x).  Same arithmetic as out_add_batch: out + (TO)(alpha
+// * acc).  E rows are padded by 4 floats: the writes from the MFMA layout
+// (lane groups 4 rows apart) stay off one bank, the 16-byte reads aligned.
+constexpr int EPY = 128 + 4;  // pitch (floats) of E[x][y] for y runs
+constexpr int EPX = 256 + 4;  // pitch of E[y][x] for x runs
+
+template <typename TO>
+__device__ inline void rmw4(TO* __restrict__ out, int64_t off, double alpha, const float4& v) {
+  if constexpr (sizeof(TO) == 4) {
+    float4* p = (float4*)(out + off);
+    float4 c = *p;
+    c.x = c.x + (float)(alpha * (double)v.x);
+    c.y = c.y + (float)(alpha * (double)v.y);
+    c.z = c.z + (float)(alpha * (double)v.z);
+    c.w = c.w + (float)(alpha * (double)v.w);
+    *p = c;
+  } else {
+    double2* p = (double2*)(out + off);
+    double2 c0 = p[0], c1 = p[1];
+    c0.x = c0.x + alpha * (double)v.x;
+    c0.y = c0.y + alpha * (double)v.y;
+    c1.x = c1.x + alpha * (double)v.z;
+    c1.y = c1.y + alpha * (double)v.w;
+    p[0] = c0;
+    p[1] = c1;
+  }
+}
+
+template <typename TO>
+__device__ inline void rmw1(TO* __restrict__ out, int64_t off, double alpha, float v) {
+  out[off] = out[off] + (TO)(alpha * (double)v);
+}
+
+__device__ inline int64_t out_yoff(const OutDev& om, int64_t y) {
+  const uint32_t yy = (uint32_t)y;
+  return (int64_t)(yy / om.Yl) * om.oyh + (int64_t)(yy % om.Yl) * om.oyl;
+}
+
+template <typename TO>
+__device__ inline void staged_store(const B16Args& a, const float* E, int64_t x0, int64_t y0, int tid) {
+  TO* out = (TO*)a.om.out;
+  if (a.om.vec == 1) {
+    // row r of x, 32 chunks of 4 y; a wave covers two rows
+    const int c = tid & 31;
+    for (int r = tid >> 5; r < 256; r += 8) {
+      const int64_t x = x0 + r;
+      if (x >= a.X) break;
+      const int64_t xo = out_xoff(a.om, x), y = y0 + 4 * c;
+      const float4 v = *(const float4*)&E[r * EPY + 4 * c];
+      if (y + 3 < a.Y) {
+        rmw4<TO>(out, xo + out_yoff(a.om, y), a.alpha, v);
+      } else {
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+        for (int j = 0; j < 4; j++)
+          if (y + j < a.Y) rmw1<TO>(out, xo + out_yoff(a.om, y + j), a.alpha, vv[j]);
+      }
+    }
+  } else {
+    // column y, 64 chunks of 4 x; a wave covers one column
+    const int c = tid & 63;
+    for (int yl = tid >> 6; yl < 128; yl += 4) {
+      const int64_t y = y0 + yl;
+      if (y >= a.Y) break;
+      const int64_t yo = out_yoff(a.om, y), x = x0 + 4 * c;
+      const float4 v = *(const float4*)&E[yl * EPX + 4 * c];
+      if (x + 3 < a.X) {
+        rmw4<TO>(out, out_xoff(a.om, x) + yo, a.alpha, v);
+      } else {
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+        for (int j = 0; j < 4; j++)
+          if (x + j < a.X) rmw1<TO>(out, out_xoff(a.om, x + j) + yo, a.alpha, vv[j]);
+      }
+    }
+  }
+}
+
+// The accumulators of wave (wx, wy) into E (layout by om.vec), then the
+// staged read-modify-write.  The caller has finished every LDS read of the
+// stage buffers E aliases and has no DMA in flight.
+__device__ inline void bf16_epilogue_staged(const B16Args& a, const f4 (&acc)[8][4], float* E, int64_t x0,
+                                            int64_t y0, int wx, int wy, int lr, int lk, int tid) {
+  __syncthreads();
+  const bool ymode = a.om.vec == 1;
+#pragma unroll
+  for (int s = 0; s < 8; s++)
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int xl = wx * 128 + 16 * s + 4 * lk + i, yl = wy * 64 + 16 * q + lr;
+        E[ymode ? xl * EPY + yl : yl * EPX + xl] = acc[s][q][i];
+      }
+  __syncthreads();
+  if (a.om.of32)
+    staged_store<float>(a, E, x0, y0, tid);
+  else
+    staged_store<double>(a, E, x0, y0, tid);
+}
+
 template <int NPASS, bool HILO>
 __global__ __launch_bounds__(256, 1) void k_ggemm_bf16_pl(B16Args a) {
   constexpr int S = 3;
@@ -768,6 +873,15 @@ __global__ __launch_bounds__(256, 1) void k_ggemm_bf16_pl(B16Args a) {
       cur = nb;
       slot ^= 1;
       k += KR;
+    }
+  }
+  // the staged epilogue needs the hi|lo stages' 148 KB (hi-only stages hold 100 KB)
+  if constexpr (HILO) {
+    if (a.nsplit == 1 && a.om.vec != 0) {
+      static_assert(sizeof(st) >= sizeof(float) * 256 * EPY && sizeof(st) >= sizeof(float) * 128 * EPX,
+                    "the staged epilogue fits in the stage buffers");
+      bf16_epilogue_staged(a, acc, reinterpret_cast<float*>(&st[0]), x0, y0, wx, wy, lr, lk, tid);
+      return;
     }
   }
   bf16_epilogue<4>(std::make_index_sequence<4>{}, a, acc, x0, y0, wx, wy, lr, lk, sp);
@@ -989,6 +1103,7 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
   om.oxl = m.oxl;
   om.oyh = m.oyh;
   om.oyl = m.oyl;
+  om.vec = 0;
   const size_t esz = adtype == DT_F64 ? 8 : 4;
   if (g_fp_lds && v->TY == GT_Y && X % (int64_t)(16 / esz) == 0 && ((uintptr_t)A & 15) == 0 && v->tile_order) {
     FArgs a;
@@ -1063,6 +1178,17 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
   a.om.oxl = m.oxl;
   a.om.oyh = m.oyh;
   a.om.oyl = m.oyl;
+  {
+    // staged epilogue: 4 consecutive y (or x) inside one level, unit stride,
+    // every other stride and the base 16-byte aligned
+    const bool al = ((uintptr_t)m.out % 16) == 0 && m.oxh % 4 == 0 && m.oyh % 4 == 0;
+    if (al && m.oyl == 1 && v->Yl % 4 == 0 && m.oxl % 4 == 0)
+      a.om.vec = 1;
+    else if (al && m.oxl == 1 && m.Xl % 4 == 0 && m.oyl % 4 == 0)
+      a.om.vec = 2;
+    else
+      a.om.vec = 0;
+  }
   a.A = A2;
   a.R = R;
   a.X = X;
