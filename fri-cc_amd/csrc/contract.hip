@@ -1000,6 +1000,70 @@ __device__ inline void fp_compute(const FpStage<T>& sm, typename Acc<T>::type (&
   }
 }
 
+// FP64 epilogue through LDS (om.vec != 0, FP64 output): the 128 x 64 block of
+// accumulators is staged in the stage buffers and read back along the
+// output's unit-stride axis, 2 consecutive outputs (16 bytes) per lane and
+// whole runs per wave (512 B along y, 1 KB along x); same arithmetic as
+// out_add_batch.  Rows padded by 2 doubles (16-byte aligned, off one bank).
+constexpr int FEPY = 64 + 2;   // pitch (doubles) of E[x][y] for y runs
+constexpr int FEPX = 128 + 2;  // pitch of E[y][x] for x runs
+
+__device__ inline void rmw2(double* __restrict__ out, int64_t off, double alpha, const double2& v) {
+  double2* p = (double2*)(out + off);
+  double2 c = *p;
+  c.x = c.x + alpha * v.x;
+  c.y = c.y + alpha * v.y;
+  *p = c;
+}
+
+__device__ inline void fp64_epilogue_staged(const FArgs& a, const d4 (&acc)[4][2], double* E, int64_t x0, int64_t y0,
+                                            int wx, int wy, int lr, int lk, int tid) {
+  __syncthreads();
+  const bool ymode = a.om.vec == 1;
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int xl = wx * 64 + 16 * s + Acc<double>::row(lk, i), yl = wy * 32 + 16 * q + lr;
+        E[ymode ? xl * FEPY + yl : yl * FEPX + xl] = acc[s][q][i];
+      }
+  __syncthreads();
+  double* out = (double*)a.om.out;
+  if (ymode) {
+    // row x, 32 chunks of 2 y; a wave covers two rows
+    const int c = tid & 31;
+    for (int r = tid >> 5; r < 128; r += 8) {
+      const int64_t x = x0 + r;
+      if (x >= a.X) break;
+      const int64_t xo = out_xoff(a.om, x), y = y0 + 2 * c;
+      const double2 v = *(const double2*)&E[r * FEPY + 2 * c];
+      if (y + 1 < a.Y) {
+        rmw2(out, xo + out_yoff(a.om, y), a.alpha, v);
+      } else if (y < a.Y) {
+        const int64_t off = xo + out_yoff(a.om, y);
+        out[off] = out[off] + a.alpha * v.x;
+      }
+    }
+  } else {
+    // column y, 64 chunks of 2 x; a wave covers one column
+    const int c = tid & 63;
+    for (int yl = tid >> 6; yl < 64; yl += 4) {
+      const int64_t y = y0 + yl;
+      if (y >= a.Y) break;
+      const int64_t yo = out_yoff(a.om, y), x = x0 + 2 * c;
+      const double2 v = *(const double2*)&E[yl * FEPX + 2 * c];
+      if (x + 1 < a.X) {
+        rmw2(out, out_xoff(a.om, x) + yo, a.alpha, v);
+      } else if (x < a.X) {
+        const int64_t off = out_xoff(a.om, x) + yo;
+        out[off] = out[off] + a.alpha * v.x;
+      }
+    }
+  }
+}
+
 template <typename T, int S>
 __global__ __launch_bounds__(256, 2) void k_ggemm_fp(FArgs a) {
   typedef FpGeom<T> G;
@@ -1059,6 +1123,15 @@ __global__ __launch_bounds__(256, 2) void k_ggemm_fp(FArgs a) {
       slot ^= 1;
     }
   }
+  if constexpr (sizeof(T) == 8) {
+    if (a.om.vec != 0 && !a.om.of32) {
+      static_assert(sizeof(st) >= sizeof(double) * 128 * FEPY && sizeof(st) >= sizeof(double) * 64 * FEPX,
+                    "the staged epilogue fits in the stage buffers");
+      static_assert(G::MS == 4 && G::NQ == 2, "FP64 wave tile of 64 x 32");
+      fp64_epilogue_staged(a, acc, reinterpret_cast<double*>(&st[0]), x0, t * GT_Y, wx, wy, lr, lk, tid);
+      return;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < G::NQ; q++) {
     const int64_t y = t * GT_Y + wy * G::WY + 16 * q + lr;
@@ -1103,7 +1176,17 @@ int gathered_gemm(const View* v, const void* A, int adtype, int64_t X, const Out
   om.oxl = m.oxl;
   om.oyh = m.oyh;
   om.oyl = m.oyl;
-  om.vec = 0;
+  {
+    // staged FP64 epilogue: 2 consecutive y (or x) inside one level, unit
+    // stride, every other stride even and the base 16-byte aligned
+    const bool al = ((uintptr_t)m.out % 16) == 0 && m.oxh % 2 == 0 && m.oyh % 2 == 0;
+    if (al && m.oyl == 1 && v->Yl % 2 == 0 && m.oxl % 2 == 0)
+      om.vec = 1;
+    else if (al && m.oxl == 1 && m.Xl % 2 == 0 && m.oyl % 2 == 0)
+      om.vec = 2;
+    else
+      om.vec = 0;
+  }
   const size_t esz = adtype == DT_F64 ? 8 : 4;
   if (g_fp_lds && v->TY == GT_Y && X % (int64_t)(16 / esz) == 0 && ((uintptr_t)A & 15) == 0 && v->tile_order) {
     FArgs a;
