@@ -479,6 +479,15 @@ int fricc_split_bf16_lincomb(const void* A, const void* B, int src_precision, do
   return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream, B, alpha, beta);
 }
 
+int fricc_split_bf16_lincomb3(const void* A, const void* B, int src_precision, double alpha, double beta, int64_t R,
+                              int64_t X, void* out, void* out_a, void* out_b, void* stream) {
+  FRICC_CHECK(A != nullptr && B != nullptr && out != nullptr, "split_bf16_lincomb3: null operand");
+  int dt;
+  FRICC_TRY(dtype_of(src_precision, &dt));
+  return split_bf16(A, dt, R, X, (uint16_t*)out, (hipStream_t)stream, B, alpha, beta, 1, (uint16_t*)out_a,
+                    (uint16_t*)out_b);
+}
+
 int fricc_set_bf16_passes(int npass) {
   FRICC_CHECK(npass == 1 || npass == 3, "set_bf16_passes: 1 or 3");
   g_npass = npass;

@@ -1337,34 +1337,50 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
 // With B: the split of alpha*A + beta*B, formed in the source precision (one
 // rounding of the sum of the two products; both exact for alpha, beta = 2, -1).
 // hilo 0: hi-only records [X/256][R][hi 256]
+// With B and out_a / out_b: the records of A and of B alone as well, from the
+// same reads (hi and lo, like out).
+__device__ inline void put_split(__hip_bfloat16* __restrict__ o, int64_t rec, int64_t rec_len, int64_t xi, int hilo,
+                                 float v) {
+  const __hip_bfloat16 h = __float2bfloat16(v);
+  o[rec * rec_len + xi] = h;
+  if (hilo) o[rec * rec_len + 256 + xi] = __float2bfloat16(v - __bfloat162float(h));
+}
+
 template <typename T>
 __global__ void k_split_bf16(const T* __restrict__ A, const T* __restrict__ B, T alpha, T beta, int64_t R,
-                             int64_t X, int64_t n, int hilo, __hip_bfloat16* __restrict__ out) {
+                             int64_t X, int64_t n, int hilo, __hip_bfloat16* __restrict__ out,
+                             __hip_bfloat16* __restrict__ out_a, __hip_bfloat16* __restrict__ out_b) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t rec_len = hilo ? 512 : 256;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
     // e = (xb * R + r) * 256 + xi
     const int64_t xi = e & 255, rec = e >> 8;
     const int64_t r = rec % R, x = (rec / R) * 256 + xi;
-    float v = 0.0f;
-    if (x < X) v = B ? (float)(alpha * A[r * X + x] + beta * B[r * X + x]) : (float)A[r * X + x];
-    const __hip_bfloat16 h = __float2bfloat16(v);
-    out[rec * rec_len + xi] = h;
-    if (hilo) out[rec * rec_len + 256 + xi] = __float2bfloat16(v - __bfloat162float(h));
+    T a = 0, b = 0;
+    if (x < X) {
+      a = A[r * X + x];
+      if (B) b = B[r * X + x];
+    }
+    const float v = B ? (float)(alpha * a + beta * b) : (float)a;
+    put_split(out, rec, rec_len, xi, hilo, v);
+    if (out_a) put_split(out_a, rec, rec_len, xi, hilo, (float)a);
+    if (out_b) put_split(out_b, rec, rec_len, xi, hilo, (float)b);
   }
 }
 
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s, const void* B,
-               double alpha, double beta, int hilo) {
+               double alpha, double beta, int hilo, uint16_t* A2a, uint16_t* A2b) {
   const int64_t n = R * bf16_xblocks(X) * 256;
   if (n == 0) return OK;
+  FRICC_CHECK(B || (!A2a && !A2b), "split_bf16: separate records need B");
   const int64_t grid = std::min<int64_t>(ceil_div(n, 256), 256 * 64);
   if (src_dtype == DT_F64)
     k_split_bf16<double><<<grid, 256, 0, s>>>((const double*)A, (const double*)B, alpha, beta, R, X, n, hilo,
-                                              (__hip_bfloat16*)A2);
+                                              (__hip_bfloat16*)A2, (__hip_bfloat16*)A2a, (__hip_bfloat16*)A2b);
   else if (src_dtype == DT_F32)
     k_split_bf16<float><<<grid, 256, 0, s>>>((const float*)A, (const float*)B, (float)alpha, (float)beta, R, X,
-                                             n, hilo, (__hip_bfloat16*)A2);
+                                             n, hilo, (__hip_bfloat16*)A2, (__hip_bfloat16*)A2a,
+                                             (__hip_bfloat16*)A2b);
   else
     FRICC_CHECK(false, "split_bf16: source must be FP64 or FP32");
   FRICC_HIP(hipGetLastError());

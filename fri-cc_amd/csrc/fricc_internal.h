@@ -117,10 +117,12 @@ int gathered_gemm_bf16(const View* v, const uint16_t* A2, int64_t R, int64_t X, 
 // tile width of BF16 views for an operand of R x X
 int bf16_view_width(int64_t R, int64_t X);
 // FP64/FP32 row-major A[R][X] -> [hi | lo] bf16 planes (layout above)
-// B (optional): split alpha*A + beta*B instead of A
+// B (optional): split alpha*A + beta*B instead of A; then A2a / A2b
+// (optional): the records of A and of B as well, from the same reads
 // hilo 0: hi-only records [bf16_xblocks(X)][R][hi 256]
 int split_bf16(const void* A, int src_dtype, int64_t R, int64_t X, uint16_t* A2, hipStream_t s,
-               const void* B = nullptr, double alpha = 1.0, double beta = 0.0, int hilo = 1);
+               const void* B = nullptr, double alpha = 1.0, double beta = 0.0, int hilo = 1, uint16_t* A2a = nullptr,
+               uint16_t* A2b = nullptr);
 // 256-column blocks of a BF16 operand (its size is R * blocks * 512 bf16)
 inline int64_t bf16_xblocks(int64_t X) { return (X + 255) / 256; }
 

@@ -26,6 +26,7 @@ EXPORTS = (
     "fricc_sparse_compress_pair_async", "fricc_sparse_compress_multi_async", "fricc_workspace_pin",
     "fricc_compress_noalloc", "fricc_compress_noalloc_max_n", "fricc_dense_contract",
     "fricc_split_bf16_hi", "fricc_sample_pivotal", "fricc_parallel_sample", "fricc_dense_contract_from",
+    "fricc_split_bf16_lincomb3",
 )
 
 
@@ -109,6 +110,8 @@ def lib():
     L.fricc_split_bf16_hi.argtypes = [_vp, ctypes.c_int, _i64, _i64, _vp, _vp]
     L.fricc_split_bf16_lincomb.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i64, _i64, _vp,
                                            _vp]
+    L.fricc_split_bf16_lincomb3.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _i64, _i64,
+                                            _vp, _vp, _vp, _vp]
     L.fricc_set_bf16_passes.argtypes = [ctypes.c_int]
     L.fricc_set_bf16_view_width.argtypes = [ctypes.c_int]
     L.fricc_stream_copy.argtypes = [_vp, _vp, _i64, _vp]

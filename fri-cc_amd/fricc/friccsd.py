@@ -551,19 +551,20 @@ def _update_amps_device(cc, t1, t2, eris):
         # 1302 and 1202 read the same view of T with operands in the same
         # [(k,c)][(i,a)] layout: one contraction with 2 Wvoov - Wvovo
         if sp == "bf16":
-            from .systems import bf16_planes_lincomb
+            from .systems import bf16_planes_lincomb3
 
-            A12 = bf16_planes_lincomb(A_voov, A_vovo, 2.0, -1.0, ov, ov)
+            # the three BF16 operands of this stage from one read of Wvoov and Wvovo
+            A12, Av, Avo = bf16_planes_lincomb3(A_voov, A_vovo, 2.0, -1.0, ov, ov)
             a12 = 1.0
         else:
             # 2 Wvoov - Wvovo as 2 (Wvoov - Wvovo / 2): one pass instead of
             # two, and scaling by 2 is exact, so the sum is bit for bit the same
             A12 = _dyn_operand(torch.add(A_voov, A_vovo, alpha=-0.5), ov, ov, sp)
             a12 = 2.0
+            Av = _dyn_operand(A_voov, ov, ov, sp)
+            Avo = _dyn_operand(A_vovo, ov, ov, sp)
         _contract_into("akic,kjcb->ijab", a12, A12, (v, o, o, v), ts, S, sp)
         del A12
-        Av = _dyn_operand(A_voov, ov, ov, sp)
-        Avo = _dyn_operand(A_vovo, ov, ov, sp)
         _contract_into("akic,kjbc->ijab", -1.0, Av, (v, o, o, v), ts, S, sp)    # 1303
         # 1203, added transposed (S[j,i,b,a]): S enters only as S + S^T, and
         # this way the gathered columns (j, a) run along S's unit-stride axis

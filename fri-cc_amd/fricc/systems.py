@@ -114,6 +114,23 @@ def bf16_planes_lincomb(A, B, alpha, beta, R, X, stream=None):
     return out
 
 
+def bf16_planes_lincomb3(A, B, alpha, beta, R, X, stream=None):
+    """(bf16_planes_lincomb(A, B, alpha, beta), bf16_planes(A), bf16_planes(B))
+    with A and B read once (fricc_split_bf16_lincomb3)."""
+    import torch
+
+    A, B = A.contiguous(), B.contiguous()
+    assert A.shape == B.shape and A.dtype == B.dtype and A.numel() == R * X
+    assert A.dtype in (torch.float64, torch.float32)
+    src = _capi.FP64 if A.dtype == torch.float64 else _capi.FP32
+    outs = [torch.empty(R * bf16_xblocks(X) * 512, dtype=torch.int16, device=A.device) for _ in range(3)]
+    _capi.check(_capi.lib().fricc_split_bf16_lincomb3(ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(B.data_ptr()),
+                                                      src, float(alpha), float(beta), int(R), int(X),
+                                                      *(ctypes.c_void_p(o.data_ptr()) for o in outs),
+                                                      _capi.stream_ptr(stream)))
+    return tuple(outs)
+
+
 class _LazyOperands(dict):
     """Operands of one dtype, built on first access.  Holds its integrals
     weakly: the integrals own this cache, and a reference cycle would keep

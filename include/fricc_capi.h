@@ -280,6 +280,13 @@ int fricc_split_bf16_hi(const void* A, int src_precision, int64_t R, int64_t X, 
 int fricc_split_bf16_lincomb(const void* A, const void* B, int src_precision, double alpha, double beta, int64_t R,
                              int64_t X, void* out, void* stream);
 
+/* fricc_split_bf16_lincomb into out and, from the same reads, fricc_split_bf16
+ * of A into out_a and of B into out_b (either may be NULL): the three
+ * operands one stage of the update reads (2 Wvoov - Wvovo, Wvoov, Wvovo),
+ * with A and B read once.  No counterpart in the reference. */
+int fricc_split_bf16_lincomb3(const void* A, const void* B, int src_precision, double alpha, double beta, int64_t R,
+                              int64_t X, void* out, void* out_a, void* out_b, void* stream);
+
 /* MFMA passes of FRICC_BF16 contractions: 3 = hi*hi + hi*lo + lo*hi (default,
  * about 16 significant bits per product), 1 = hi*hi only. */
 int fricc_set_bf16_passes(int npass);

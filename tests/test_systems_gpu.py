@@ -91,3 +91,23 @@ def test_split_of_a_combination_equals_split_of_the_sum(dtype):
     want = bf16_planes(2.0 * A - B, R, X)
     torch.cuda.synchronize()
     assert torch.equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_three_splits_from_one_read(dtype):
+    """fricc_split_bf16_lincomb3 gives, bit for bit, the planes of 2A - B, of
+    A and of B that three separate splits give (ragged column count)."""
+    import torch
+
+    from fricc.systems import bf16_planes, bf16_planes_lincomb, bf16_planes_lincomb3
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    dt = getattr(torch, dtype)
+    R, X = 41, 520
+    A = torch.randn(R, X, generator=g, device="cuda", dtype=dt)
+    B = torch.randn(R, X, generator=g, device="cuda", dtype=dt)
+    ab, a, b = bf16_planes_lincomb3(A, B, 2.0, -1.0, R, X)
+    torch.cuda.synchronize()
+    assert torch.equal(ab, bf16_planes_lincomb(A, B, 2.0, -1.0, R, X))
+    assert torch.equal(a, bf16_planes(A, R, X))
+    assert torch.equal(b, bf16_planes(B, R, X))
