@@ -590,7 +590,11 @@ int dense_contract(const char* spec, double alpha, const void* A, int a64, int a
   // coalescing: walk the unit-stride index of each tile
   a.a_kfast = (a.Ak.s0 == 1 && a.Am.s0 != 1) ? 1 : 0;
   a.b_kfast = (a.Bk.s0 == 1 && a.Bn.s0 != 1) ? 1 : 0;
-  const int bm = pick_tile(M), bnt = pick_tile(N);
+  int bm = pick_tile(M);
+  const int bnt = pick_tile(N);
+  // FP64 128 x 128 tiles hold 128 accumulator registers per lane and run one
+  // wave per SIMD; 64 x 128 tiles run two
+  if (compute64 && bm == 128 && bnt == 128) bm = 64;
   a.ntm = ceil_div(M, bm);
   a.ntn = ceil_div(N, bnt);
   // 16-byte loads along the unit-stride index of an operand when every group
