@@ -311,9 +311,32 @@ int fricc_sparse_compress_multi_async(fricc_sparse* const* sps, int count, const
     for (int q = 0; q < 4; q++)
       FRICC_CHECK(sps[k]->dims[q] == sps[0]->dims[q], "sparse_compress_multi_async: the tensors' dimensions differ");
   }
-  FRICC_TRY(fricc_sparse_compress_async(sps[0], x, m, compression, u_dev, stream));
-  for (int k = 1; k < count; k++)
-    FRICC_TRY(resample_into(sps[0], sps[k], x, m, compression, u_dev + k, (hipStream_t)stream));
+  // samples 0 and 1 from one selection and one emission pass, then the
+  // others two per emission pass (a last odd one alone)
+  if (count == 1) return fricc_sparse_compress_async(sps[0], x, m, compression, u_dev, stream);
+  FRICC_TRY(fricc_sparse_compress_pair_async(sps[0], sps[1], x, m, compression, u_dev, u_dev + 1, stream));
+  const hipStream_t s = (hipStream_t)stream;
+  const int64_t n = sps[0]->dims[0] * sps[0]->dims[1] * sps[0]->dims[2] * sps[0]->dims[3];
+  int k = 2;
+  for (; k + 1 < count; k += 2) {
+    fricc_sparse* a = sps[k];
+    fricc_sparse* b = sps[k + 1];
+    for (fricc_sparse* t : {a, b}) {
+      FRICC_TRY(sparse_reserve(t, m));
+      if (!t->dsc) FRICC_HIP(hipMalloc(&t->dsc, comp_scalars_size()));
+      t->gen++;
+      t->nnz = 0;
+    }
+    FRICC_TRY(compress_resample_pair(x, n, m, compression, u_dev + k, u_dev + k + 1, sps[0]->idx, sps[0]->val, a->idx,
+                                     a->val, a->dsc, b->idx, b->val, b->dsc, s));
+    for (fricc_sparse* t : {a, b}) {
+      t->nnz = m;
+      t->pending = true;
+      t->dup_host = 0;
+      t->dup_dev = comp_scalars_dup_flag(t->dsc);
+    }
+  }
+  if (k < count) FRICC_TRY(resample_into(sps[0], sps[k], x, m, compression, u_dev + k, s));
   return OK;
 }
 

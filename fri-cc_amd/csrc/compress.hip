@@ -1661,6 +1661,46 @@ int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, cons
   return OK;
 }
 
+// Samples a and b (uniforms u_a, u_b) of the vector compress_launch or
+// compress_pair_launch has just compressed (same x, n, m, method, stream and
+// arena, nothing in between): bit for bit two compress_resample calls.  For
+// systematic FRI one pass emits both (k_samp_emit_n<2>).  idx/val: the first
+// sample's output (its kept part is copied).
+int compress_resample_pair(const double* x, int64_t n, int64_t m, int method, const double* u_a, const double* u_b,
+                           const uint32_t* idx, const double* val, uint32_t* idx_a, double* val_a, void* export_a,
+                           uint32_t* idx_b, double* val_b, void* export_b, hipStream_t s) {
+  FRICC_CHECK(u_a != nullptr && u_b != nullptr, "compress: resampling needs device uniforms");
+  if (method != COMP_FRI_SYSTEMATIC) {
+    FRICC_TRY(compress_resample(x, n, m, method, u_a, idx, val, idx_a, val_a, export_a, s));
+    return compress_resample(x, n, m, method, u_b, idx, val, idx_b, val_b, export_b, s);
+  }
+  const int64_t nt = ceil_div(n, TILE);
+  void *p_sc, *p_sc2, *p_tot, *p_ts, *p_ths;
+  FRICC_TRY(ws_get(WS_SC, sizeof(CompScalars), &p_sc));
+  FRICC_TRY(ws_get(WS_SC2, 2 * sizeof(CompScalars), &p_sc2));
+  FRICC_TRY(ws_get(WS_TOT, 64, &p_tot));
+  FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
+  FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
+  CompScalars* sa = (CompScalars*)p_sc2;
+  CompScalars* sb = sa + 1;
+  const unsigned gk = (unsigned)std::min<int64_t>(ceil_div(m, NT), 2048);
+  k_resample_scalars<<<1, 64, 0, s>>>((const CompScalars*)p_sc, u_a, sa);
+  k_resample_scalars<<<1, 64, 0, s>>>((const CompScalars*)p_sc, u_b, sb);
+  k_copy_kept<<<gk, NT, 0, s>>>(idx, val, sa, m, idx_a, val_a);
+  k_copy_kept<<<gk, NT, 0, s>>>(idx, val, sb, m, idx_b, val_b);
+  EmitOuts<2> eo;
+  eo.o[0] = EmitOut{sa, idx_a, val_a};
+  eo.o[1] = EmitOut{sb, idx_b, val_b};
+  k_samp_emit_n<2><<<nt, NT, 0, s>>>(x, nullptr, n, (const u128*)p_ts, (const u128*)p_ths, eo);
+  for (CompScalars* sc : {sa, sb}) k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
+  k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sa, idx_a, val_a);
+  k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sb, idx_b, val_b);
+  if (export_a) k_export_scalars<<<1, 64, 0, s>>>(sa, (CompScalars*)export_a);
+  if (export_b) k_export_scalars<<<1, 64, 0, s>>>(sb, (CompScalars*)export_b);
+  FRICC_HIP(hipGetLastError());
+  return OK;
+}
+
 int compress(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx, double* out_val,
              CompressInfo* info, hipStream_t s, int* dup) {
   memset(info, 0, sizeof(*info));

@@ -36,6 +36,11 @@ int compress_resample(const double* x, int64_t n, int64_t m, int method, const d
 int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, const double* u_dev,
                          const double* u_dev2, uint32_t* idx1, double* val1, void* export1, uint32_t* idx2,
                          double* val2, void* export2, hipStream_t s);
+// Two more samples (u_a, u_b) after compress_launch / compress_pair_launch:
+// bit for bit two compress_resample calls.
+int compress_resample_pair(const double* x, int64_t n, int64_t m, int method, const double* u_a, const double* u_b,
+                           const uint32_t* idx, const double* val, uint32_t* idx_a, double* val_a, void* export_a,
+                           uint32_t* idx_b, double* val_b, void* export_b, hipStream_t s);
 // The same as compress(), with all scratch in a fixed device array (no
 // allocation); n <= compress_noalloc_max_n().
 int compress_noalloc(const double* x, int64_t n, int64_t m, int method, double u, uint32_t* out_idx,

@@ -233,15 +233,17 @@ def test_pair_edge_cases(mods):
         B.sync_status()
 
 
-def test_multi_equals_separate_compressions(mods):
-    """compress_multi_async: six samples of one vector (uniforms in order,
-    u = 1 among them), each bit for bit its own compression and the oracle's."""
+@pytest.mark.parametrize("count", [6, 5, 3, 1])
+def test_multi_equals_separate_compressions(mods, count):
+    """compress_multi_async: up to six samples of one vector (uniforms in
+    order, u = 1 among them; two samples per emission pass, an odd last one
+    alone), each bit for bit its own compression and the oracle's."""
     py_rccsd, lo, torch = mods
     x = _vec(200_000, 9)
     m = 15_000
     xd = torch.from_numpy(x).cuda()
     dims = (1, 1, 1, x.size)
-    us = [0.31, 1.0, 0.0, 0.77, 0.5, 0.123]
+    us = [0.31, 1.0, 0.0, 0.77, 0.5, 0.123][:count]
     ud = torch.tensor(us, dtype=torch.float64, device="cuda")
     ts = [py_rccsd.SparseTensor4d.empty(dims) for _ in us]
     py_rccsd.SparseTensor4d.compress_multi_async(ts, xd, m, "fri", ud)
