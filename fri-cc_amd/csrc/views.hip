@@ -66,30 +66,53 @@ __global__ void k_mark(const uint32_t* __restrict__ idx, int64_t nnz, VGeom g, u
   slot[(int64_t)(y / g.TY) * g.R + r] = 1u;
 }
 
-__global__ void k_tile_count(const uint32_t* __restrict__ slot, uint32_t R, int kstep, uint32_t* __restrict__ cnt) {
+// Tile t's rows are split into chunks of VCH rows, one block per (chunk,
+// tile), so that views with few tiles (8 for the ladder) still fill the GPU.
+constexpr uint32_t VCH = 2048;
+
+// Marked rows per (tile, chunk): cc[t * nch + c]
+__global__ void k_tile_count(const uint32_t* __restrict__ slot, uint32_t R, uint32_t nch, uint32_t* __restrict__ cc) {
   __shared__ uint32_t sm[VNT / WAVE + 1];
-  const uint32_t* row = slot + (int64_t)blockIdx.x * R;
-  uint32_t c = 0;
-  for (uint32_t r = threadIdx.x; r < R; r += VNT) c += row[r];
+  const uint32_t t = blockIdx.y, c = blockIdx.x;
+  const uint32_t* row = slot + (int64_t)t * R;
+  const uint32_t r1 = min(R, (c + 1) * VCH);
+  uint32_t n = 0;
+  for (uint32_t r = c * VCH + threadIdx.x; r < r1; r += VNT) n += row[r];
   uint32_t tot;
-  block_excl_scan_u32<VNT>(c, &tot, sm);
-  if (threadIdx.x == 0) cnt[blockIdx.x] = (tot + kstep - 1) / kstep * kstep;
+  block_excl_scan_u32<VNT>(n, &tot, sm);
+  if (threadIdx.x == 0) cc[(int64_t)t * nch + c] = tot;
 }
 
-__global__ void k_tile_slots(uint32_t* __restrict__ slot, uint32_t R, const uint32_t* __restrict__ tile_off,
-                             uint32_t* __restrict__ rowlist) {
+// Per tile: its rows padded to the K step
+__global__ void k_tile_total(const uint32_t* __restrict__ cc, uint32_t nch, int64_t ntiles, int kstep,
+                             uint32_t* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * VNT + threadIdx.x;
+  if (t >= ntiles) return;
+  uint32_t tot = 0;
+  for (uint32_t c = 0; c < nch; c++) tot += cc[t * nch + c];
+  cnt[t] = (tot + kstep - 1) / kstep * kstep;
+}
+
+// Compaction of one (tile, chunk): marked rows in increasing order, from the
+// tile's offset plus the earlier chunks' counts; the tile's last chunk also
+// writes the padding rows.  slot[r] becomes the row's position in the list.
+__global__ void k_tile_slots(uint32_t* __restrict__ slot, uint32_t R, uint32_t nch, const uint32_t* __restrict__ cc,
+                             const uint32_t* __restrict__ tile_off, uint32_t* __restrict__ rowlist) {
   __shared__ uint32_t sm[VNT / WAVE + 1];
-  uint32_t* row = slot + (int64_t)blockIdx.x * R;
-  const uint32_t base = tile_off[blockIdx.x], end = tile_off[blockIdx.x + 1];
-  uint32_t run = base;
-  constexpr int IT = 16;
-  for (uint32_t c0 = 0; c0 < R; c0 += VNT * IT) {
+  const uint32_t t = blockIdx.y, c = blockIdx.x;
+  uint32_t* row = slot + (int64_t)t * R;
+  uint32_t run = tile_off[t];
+  for (uint32_t q = 0; q < c; q++) run += cc[(int64_t)t * nch + q];
+  const uint32_t rbeg = c * VCH, rend = min(R, (c + 1) * VCH);
+  constexpr int IT = 8;  // VNT * IT == VCH
+  static_assert(VNT * IT == VCH, "one pass of the block covers a chunk");
+  {
     uint32_t f[IT];
     uint32_t s = 0;
-    const uint32_t r0 = c0 + threadIdx.x * IT;
+    const uint32_t r0 = rbeg + threadIdx.x * IT;
 #pragma unroll
     for (int q = 0; q < IT; q++) {
-      f[q] = (r0 + q < R) ? row[r0 + q] : 0u;
+      f[q] = (r0 + q < rend) ? row[r0 + q] : 0u;
       s += f[q];
     }
     uint32_t tot;
@@ -105,7 +128,8 @@ __global__ void k_tile_slots(uint32_t* __restrict__ slot, uint32_t R, const uint
     run += tot;
   }
   // padding rows (zero values) point at row 0
-  for (uint32_t k = run + threadIdx.x; k < end; k += VNT) rowlist[k] = 0u;
+  if (c == nch - 1)
+    for (uint32_t k = run + threadIdx.x; k < tile_off[t + 1]; k += VNT) rowlist[k] = 0u;
 }
 
 // Entries with distinct flat indices land on distinct elements of the zeroed
@@ -226,12 +250,15 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   g.TY = (uint32_t)v->TY;
   const bool grouped = v->dtype == DT_BF16 || v->dtype == DT_BF16X1;
   g.R = (uint32_t)v->R;
-  void *p_slot, *p_cnt;
+  void *p_slot, *p_cnt, *p_cc;
   const size_t nslot = (size_t)v->ntiles * g.R;
+  const uint32_t nch = (uint32_t)std::max<int64_t>(1, ceil_div((int64_t)g.R, (int64_t)VCH));
   FRICC_TRY(ws_get(20, nslot * 4, &p_slot));
   FRICC_TRY(ws_get(21, (size_t)(v->ntiles + 1) * 4, &p_cnt));
+  FRICC_TRY(ws_get(23, (size_t)v->ntiles * nch * 4, &p_cc));
   uint32_t* slot = (uint32_t*)p_slot;
   uint32_t* cnt = (uint32_t*)p_cnt;
+  uint32_t* cc = (uint32_t*)p_cc;
   if (!v->tile_off) FRICC_HIP(hipMalloc(&v->tile_off, (size_t)(v->ntiles + 1) * 4));
   if (!v->tile_order) FRICC_HIP(hipMalloc(&v->tile_order, (size_t)std::max<int64_t>(v->ntiles, 1) * 4));
   const int64_t nnz = sp->nnz;
@@ -253,9 +280,12 @@ static int build_view(Sparse* sp, View* v, hipStream_t s) {
   }
   k_zero_u32<<<(unsigned)std::min<size_t>(2048, (nslot + VNT - 1) / VNT), VNT, 0, s>>>(slot, (int64_t)nslot);
   if (nnz > 0) k_mark<<<ceil_div(nnz, VNT), VNT, 0, s>>>(sp->idx, nnz, g, slot);
-  k_tile_count<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->kstep, cnt);
+  FRICC_CHECK(v->ntiles < 65536, "view: too many tiles");
+  const dim3 gch(nch, (unsigned)v->ntiles);
+  k_tile_count<<<gch, VNT, 0, s>>>(slot, g.R, nch, cc);
+  k_tile_total<<<(unsigned)ceil_div(v->ntiles, (int64_t)VNT), VNT, 0, s>>>(cc, nch, v->ntiles, v->kstep, cnt);
   FRICC_TRY(scan_u32(cnt, v->tile_off, v->ntiles, v->tile_off + v->ntiles, s));
-  k_tile_slots<<<v->ntiles, VNT, 0, s>>>(slot, g.R, v->tile_off, v->rowlist);
+  k_tile_slots<<<gch, VNT, 0, s>>>(slot, g.R, nch, cc, v->tile_off, v->rowlist);
   k_tile_order<<<(unsigned)std::min<int64_t>(ceil_div(v->ntiles, VNT), 64), VNT, 0, s>>>(v->tile_off, v->ntiles,
                                                                                         v->tile_order);
   const uint32_t* total = v->tile_off + v->ntiles;  // gathered rows, on the device
