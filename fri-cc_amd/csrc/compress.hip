@@ -561,25 +561,22 @@ struct EmitOuts {
   EmitOut o[NS];
 };
 
+// Lattice points of the NS samples in the range of thread t of tile b:
+// kc[q] the first, live[q] whether any lies in the range.
 template <int NS>
-__global__ __launch_bounds__(NT) void k_samp_emit_n(const double* __restrict__ x, const double* __restrict__ pin,
-                                                    int64_t n, const u128* __restrict__ tsum,
-                                                    const u128* __restrict__ thpre, EmitOuts<NS> eo) {
-  // every sample shares the first one's selection: n', K, remaining norm
-  const CompScalars c = *eo.o[0].sc;
-  if (c.n_sampled == 0) return;
-  double u[NS];
-#pragma unroll
-  for (int q = 0; q < NS; q++) u[q] = q == 0 ? c.u : eo.o[q].sc->u;
-  const double nsd = (double)c.n_sampled;
-  const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
-  u128 pre, end;
-  thread_range(tsum, thpre, blockIdx.x, threadIdx.x, pre, end);
+__device__ inline bool range_points(const u128* __restrict__ tsum, const u128* __restrict__ thpre, int64_t b, int t,
+                                    const double (&u)[NS], int64_t ns, u128& pre, int64_t (&kc)[NS],
+                                    bool (&live)[NS]) {
+  u128 end;
+  thread_range(tsum, thpre, b, t, pre, end);
   const double Plo = u128_to_double(pre, -96), Phi = u128_to_double(end, -96);
-  if (!(Phi > Plo)) return;
-  int64_t kc[NS];
-  bool live[NS];
   bool any = false;
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    live[q] = false;
+    kc[q] = 0;
+  }
+  if (!(Phi > Plo)) return false;
 #pragma unroll
   for (int q = 0; q < NS; q++) {
     kc[q] = first_k_at_least(Plo, u[q]);
@@ -588,54 +585,136 @@ __global__ __launch_bounds__(NT) void k_samp_emit_n(const double* __restrict__ x
     live[q] = kc[q] < khi;
     any |= live[q];
   }
-  if (!any) return;
-  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)threadIdx.x * ITEMS;
+  return any;
+}
+
+template <int NS>
+__device__ inline void sample_uniforms(const EmitOuts<NS>& eo, const CompScalars& c, double (&u)[NS]) {
+#pragma unroll
+  for (int q = 0; q < NS; q++) u[q] = q == 0 ? c.u : eo.o[q].sc->u;
+}
+
+// Emission in two passes.  k_samp_live: every thread finds whether its range
+// holds lattice points of any sample and, if so, appends its id to a list
+// (one atomic per block; the list's order does not matter, each range writes
+// only its own output positions).  k_samp_walk: one thread per listed range
+// walks its elements.  The walk is the divergent, arithmetic-heavy part, and
+// about a third of the ranges hold points: listing them first keeps every lane
+// of a walking wave busy.
+template <int NS>
+__global__ __launch_bounds__(NT) void k_samp_live(const u128* __restrict__ tsum, const u128* __restrict__ thpre,
+                                                  EmitOuts<NS> eo, uint32_t* __restrict__ list,
+                                                  uint32_t* __restrict__ count) {
+  __shared__ uint32_t sm[NT / WAVE + 1];
+  __shared__ uint32_t base;
+  const CompScalars c = *eo.o[0].sc;
+  bool live_any = false;
+  if (c.n_sampled != 0) {
+    double u[NS];
+    sample_uniforms(eo, c, u);
+    u128 pre;
+    int64_t kc[NS];
+    bool live[NS];
+    live_any = range_points<NS>(tsum, thpre, blockIdx.x, threadIdx.x, u, (int64_t)c.n_sampled, pre, kc, live);
+  }
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan_u32<NT>(live_any ? 1u : 0u, &tot, sm);
+  if (threadIdx.x == 0) base = tot ? atomicAdd(count, tot) : 0u;
+  __syncthreads();
+  if (live_any) list[base + ex] = (uint32_t)(blockIdx.x * NT + threadIdx.x);
+}
+
+template <int NS>
+__global__ __launch_bounds__(NT) void k_samp_walk(const double* __restrict__ x, const double* __restrict__ pin,
+                                                  int64_t n, const u128* __restrict__ tsum,
+                                                  const u128* __restrict__ thpre, EmitOuts<NS> eo,
+                                                  const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ count) {
+  // every sample shares the first one's selection: n', K, remaining norm
+  const CompScalars c = *eo.o[0].sc;
+  if (c.n_sampled == 0) return;
+  double u[NS];
+  sample_uniforms(eo, c, u);
+  const double nsd = (double)c.n_sampled;
+  const int64_t ns = (int64_t)c.n_sampled, K = (int64_t)c.K;
+  const uint32_t cnt = *count;
   const double* src = pin ? pin : x;
-  double vals[ITEMS];
+  for (uint32_t i = blockIdx.x * NT + threadIdx.x; i < cnt; i += gridDim.x * NT) {
+    const uint32_t g = list[i];
+    const int64_t b = g / NT;
+    const int t = (int)(g % NT);
+    u128 pre;
+    int64_t kc[NS];
+    bool live[NS];
+    if (!range_points<NS>(tsum, thpre, b, t, u, ns, pre, kc, live)) continue;
+    const int64_t base = b * TILE + (int64_t)t * ITEMS;
+    double vals[ITEMS];
 #pragma unroll
-  for (int r = 0; r < ITEMS; r++) vals[r] = base + r < n ? src[base + r] : 0.0;
-  // Walk the lattice alongside the elements: kc = first k with fl(k+u) >= P
-  // of the running prefix.  Element j owns the k with P_j <= fl(k+u) <
-  // P_{j+1}, i.e. those kc passes while fl(kc+u) < P_{j+1} (the same set as
-  // first_k_at_least(P_j) .. first_k_at_least(P_{j+1}), by monotonicity).
-  // fl(kc+u) < P_{j+1} is tested on the exact prefix against the threshold of
-  // fl(kc+u) (above_threshold), so the prefix is rounded to a double only here
-  u128 T[NS];
-  bool tie[NS];
+    for (int r = 0; r < ITEMS; r++) vals[r] = base + r < n ? src[base + r] : 0.0;
+    // Walk the lattice alongside the elements: kc = first k with fl(k+u) >= P
+    // of the running prefix.  Element j owns the k with P_j <= fl(k+u) <
+    // P_{j+1}, i.e. those kc passes while fl(kc+u) < P_{j+1} (the same set as
+    // first_k_at_least(P_j) .. first_k_at_least(P_{j+1}), by monotonicity).
+    // fl(kc+u) < P_{j+1} is tested on the exact prefix against the threshold
+    // of fl(kc+u) (above_threshold), so the prefix is rounded to a double only
+    // here
+    u128 T[NS];
+    bool tie[NS];
 #pragma unroll
-  for (int q = 0; q < NS; q++) above_threshold((double)kc[q] + u[q], T[q], tie[q]);
+    for (int q = 0; q < NS; q++) above_threshold((double)kc[q] + u[q], T[q], tie[q]);
 #pragma unroll
-  for (int r = 0; r < ITEMS; r++) {
-    const int64_t j = base + r;
-    const double v = vals[r];
-    const double p = j < n ? (pin ? v : prob_of(v, (uint64_t)j, c, nsd)) : 0.0;
-    const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
-    pre += f;
-    if (f != 0) {
+    for (int r = 0; r < ITEMS; r++) {
+      const int64_t j = base + r;
+      const double v = vals[r];
+      const double p = j < n ? (pin ? v : prob_of(v, (uint64_t)j, c, nsd)) : 0.0;
+      const u128 f = fx_of(dbits(p <= 1.0 ? p : 0.0), 1023u);
+      pre += f;
+      if (f != 0) {
 #pragma unroll
-      for (int q = 0; q < NS; q++) {
-        if (!live[q]) continue;
-        int ne = 0;
-        while (kc[q] < ns && (pre > T[q] || (pre == T[q] && tie[q]))) {
-          eo.o[q].idx[K + kc[q]] = (uint32_t)j;
-          if (eo.o[q].val) eo.o[q].val[K + kc[q]] = v / p;
-          kc[q]++;
-          ne++;
-          above_threshold((double)kc[q] + u[q], T[q], tie[q]);
+        for (int q = 0; q < NS; q++) {
+          if (!live[q]) continue;
+          int ne = 0;
+          while (kc[q] < ns && (pre > T[q] || (pre == T[q] && tie[q]))) {
+            eo.o[q].idx[K + kc[q]] = (uint32_t)j;
+            if (eo.o[q].val) eo.o[q].val[K + kc[q]] = v / p;
+            kc[q]++;
+            ne++;
+            above_threshold((double)kc[q] + u[q], T[q], tie[q]);
+          }
+          if (ne > 1) eo.o[q].sc->dup = 1u;  // j emitted more than once
         }
-        if (ne > 1) eo.o[q].sc->dup = 1u;  // j emitted more than once
       }
     }
   }
 }
 
-// One sample's emission (the single-sample case of k_samp_emit_n)
+__global__ void k_zero_count(uint32_t* count) { *count = 0u; }
+
+// Scratch of the two-pass emission: a list of up to nt*NT range ids and its
+// count (callers pass buffers that are idle during the emission).
+struct EmitScratch {
+  uint32_t* list;
+  uint32_t* count;
+};
+
+// The emission of NS samples of one compression (or of given probabilities)
+template <int NS>
+static void samp_emit_n(const double* x, const double* pin, int64_t n, const EmitOuts<NS>& eo, const u128* tsum,
+                        const u128* thpre, const EmitScratch& es, int64_t nt, hipStream_t s) {
+  k_zero_count<<<1, 1, 0, s>>>(es.count);
+  k_samp_live<NS><<<nt, NT, 0, s>>>(tsum, thpre, eo, es.list, es.count);
+  const unsigned gw = (unsigned)std::min<int64_t>(nt, 4096);
+  k_samp_walk<NS><<<gw, NT, 0, s>>>(x, pin, n, tsum, thpre, eo, es.list, es.count);
+}
+
 static void samp_emit(const double* x, const double* pin, int64_t n, CompScalars* sc, const u128* tsum,
-                      const u128* thpre, uint32_t* out_idx, double* out_val, int64_t nt, hipStream_t s) {
+                      const u128* thpre, uint32_t* out_idx, double* out_val, const EmitScratch& es, int64_t nt,
+                      hipStream_t s) {
   EmitOuts<1> eo;
   eo.o[0] = EmitOut{sc, out_idx, out_val};
-  k_samp_emit_n<1><<<nt, NT, 0, s>>>(x, pin, n, tsum, thpre, eo);
+  samp_emit_n<1>(x, pin, n, eo, tsum, thpre, es, nt, s);
 }
+
 
 // Single thread: total interval end and the first not-found position.
 __global__ void k_samp_final(CompScalars* sc, const u128* __restrict__ total) {
@@ -1381,7 +1460,7 @@ static CompLayout comp_layout(int64_t n) {
                          (size_t)n * 8,
                          (size_t)n * 4,
                          (size_t)n * 8,
-                         (size_t)n * 4,
+                         (size_t)std::max<int64_t>(n, nt * NT) * 4,  // also the emission's range list
                          (size_t)nt * 256 * 4,
                          (size_t)(nt + 1) * sizeof(u128),
                          (size_t)nt * NT * sizeof(u128),
@@ -1448,7 +1527,7 @@ static int comp_scratch_ws(int64_t n, CompScratch* w) {
   FRICC_TRY(ws_get(WS_K0, (size_t)n * 8, &p_k0));
   FRICC_TRY(ws_get(WS_V0, (size_t)n * 4, &p_v0));
   FRICC_TRY(ws_get(WS_K1, (size_t)n * 8, &p_k1));
-  FRICC_TRY(ws_get(WS_V1, (size_t)n * 4, &p_v1));
+  FRICC_TRY(ws_get(WS_V1, (size_t)std::max<int64_t>(n, nt * NT) * 4, &p_v1));  // also the emission's range list
   FRICC_TRY(ws_get(WS_SH, (size_t)nt * 256 * 4, &p_sh));
   FRICC_TRY(ws_get(WS_TS, (size_t)(nt + 1) * sizeof(u128), &p_ts));
   FRICC_TRY(ws_get(WS_THS, (size_t)nt * NT * sizeof(u128), &p_ths));
@@ -1465,6 +1544,19 @@ static int comp_scratch_ws(int64_t n, CompScratch* w) {
   w->ts = (u128*)p_ts;
   w->ths = (u128*)p_ths;
   return piv_scratch_ws(n, &w->tmax, &w->thmax, &w->pst);
+}
+
+// Emission scratch from the workspace arena: the second radix-sort value
+// buffer and the first word of the histogram, both idle once the sort is done
+// (sized here for vectors of n elements; the compression sized them already).
+static int emit_scratch_ws(int64_t n, EmitScratch* es) {
+  const int64_t nt = ceil_div(n, TILE);
+  void *p_list, *p_cnt;
+  FRICC_TRY(ws_get(WS_V1, (size_t)std::max<int64_t>(n, nt * NT) * 4, &p_list));
+  FRICC_TRY(ws_get(WS_HIST, (4096 + 256) * 4, &p_cnt));
+  es->list = (uint32_t*)p_list;
+  es->count = (uint32_t*)p_cnt;
+  return OK;
 }
 
 static int check_compress_args(int64_t n, int64_t m, int method, double u_val, const double* u_dev) {
@@ -1536,7 +1628,7 @@ static int compress_pipeline(const double* x, int64_t n, int64_t m, int method, 
       FRICC_HIP(hipGetLastError());
       return OK;
     } else {
-      samp_emit(x, nullptr, n, sc, ts, w.ths, out_idx, out_val, nt, s);
+      samp_emit(x, nullptr, n, sc, ts, w.ths, out_idx, out_val, EmitScratch{w.vB, w.hist}, nt, s);
       k_samp_final<<<1, 1, 0, s>>>(sc, w.tot);
       k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
     }
@@ -1612,7 +1704,9 @@ int compress_resample(const double* x, int64_t n, int64_t m, int method, const d
     FRICC_TRY(pivotal_passes(x, nullptr, nullptr, n, m, sc, (const u128*)p_ts, (const u128*)p_ths,
                              (const u128*)p_tot, tmax, thmax, pst, out_idx, out_val, s));
   } else if (method != COMP_LARGEST) {
-    samp_emit(x, nullptr, n, sc, (const u128*)p_ts, (const u128*)p_ths, out_idx, out_val, nt, s);
+    EmitScratch es;
+    FRICC_TRY(emit_scratch_ws(n, &es));
+    samp_emit(x, nullptr, n, sc, (const u128*)p_ts, (const u128*)p_ths, out_idx, out_val, es, nt, s);
     k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
     k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sc, out_idx, out_val);
   }
@@ -1624,7 +1718,7 @@ int compress_resample(const double* x, int64_t n, int64_t m, int method, const d
 // Two samples of one compression (uniforms u_dev, u_dev2), bit for bit
 // compress_launch with u_dev followed by compress_resample with u_dev2.  For
 // systematic FRI the selection and the probability prefix run once and one
-// pass emits both samples (k_samp_emit_n<2>); the other methods take the two
+// pass emits both samples (samp_emit_n<2>); the other methods take the two
 // calls.
 int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, const double* u_dev,
                          const double* u_dev2, uint32_t* idx1, double* val1, void* export1, uint32_t* idx2,
@@ -1650,7 +1744,7 @@ int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, cons
   EmitOuts<2> eo;
   eo.o[0] = EmitOut{w.sc, idx1, val1};
   eo.o[1] = EmitOut{sc2, idx2, val2};
-  k_samp_emit_n<2><<<nt, NT, 0, s>>>(x, nullptr, n, w.ts, w.ths, eo);
+  samp_emit_n<2>(x, nullptr, n, eo, w.ts, w.ths, EmitScratch{w.vB, w.hist}, nt, s);
   k_samp_final<<<1, 1, 0, s>>>(w.sc, w.tot);
   k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, w.sc, idx1, val1);
   k_samp_final<<<1, 1, 0, s>>>(sc2, w.tot);
@@ -1664,7 +1758,7 @@ int compress_pair_launch(const double* x, int64_t n, int64_t m, int method, cons
 // Samples a and b (uniforms u_a, u_b) of the vector compress_launch or
 // compress_pair_launch has just compressed (same x, n, m, method, stream and
 // arena, nothing in between): bit for bit two compress_resample calls.  For
-// systematic FRI one pass emits both (k_samp_emit_n<2>).  idx/val: the first
+// systematic FRI one pass emits both (samp_emit_n<2>).  idx/val: the first
 // sample's output (its kept part is copied).
 int compress_resample_pair(const double* x, int64_t n, int64_t m, int method, const double* u_a, const double* u_b,
                            const uint32_t* idx, const double* val, uint32_t* idx_a, double* val_a, void* export_a,
@@ -1691,7 +1785,9 @@ int compress_resample_pair(const double* x, int64_t n, int64_t m, int method, co
   EmitOuts<2> eo;
   eo.o[0] = EmitOut{sa, idx_a, val_a};
   eo.o[1] = EmitOut{sb, idx_b, val_b};
-  k_samp_emit_n<2><<<nt, NT, 0, s>>>(x, nullptr, n, (const u128*)p_ts, (const u128*)p_ths, eo);
+  EmitScratch es;
+  FRICC_TRY(emit_scratch_ws(n, &es));
+  samp_emit_n<2>(x, nullptr, n, eo, (const u128*)p_ts, (const u128*)p_ths, es, nt, s);
   for (CompScalars* sc : {sa, sb}) k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
   k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sa, idx_a, val_a);
   k_samp_notfound<<<ceil_div(m, NT), NT, 0, s>>>(x, sb, idx_b, val_b);
@@ -1791,7 +1887,9 @@ int sample_systematic(const double* p, int64_t n, int64_t n_sample, double u, ui
   k_init_scalars<<<1, NT, 0, s>>>(sc, 0, n_sample, u, nullptr, nullptr, 0);
   k_samp_tilesums<<<nt, NT, 0, s>>>(nullptr, p, n, sc, ts, (u128*)p_ths);
   k_scan_u128_sums<<<1, 1024, 0, s>>>(ts, nt, (u128*)p_tot);
-  samp_emit(nullptr, p, n, sc, ts, (const u128*)p_ths, out_idx, nullptr, nt, s);
+  EmitScratch es;
+  FRICC_TRY(emit_scratch_ws(n, &es));
+  samp_emit(nullptr, p, n, sc, ts, (const u128*)p_ths, out_idx, nullptr, es, nt, s);
   k_samp_final<<<1, 1, 0, s>>>(sc, (u128*)p_tot);
   k_samp_notfound<<<ceil_div(n_sample, NT), NT, 0, s>>>(p, sc, out_idx, nullptr);
   FRICC_HIP(hipGetLastError());
