@@ -15,60 +15,146 @@
 //   E2    = sum_ijab (t2new[ijab] + t1new[i,a] t1new[j,b]) * Lovov[ijab]
 //   dnorm = sum over ia >= jb of (t2new[ijab] - t2old[ijab])^2
 // (the packed lower triangle of t2 as an (ia, jb) matrix,
-// ccsd.amplitudes_to_vector).  Partial sums are written per block and added
-// in a fixed order by a second kernel, so the result is deterministic.
+// ccsd.amplitudes_to_vector).  Lovov[ijab] = 2 ovovT[ijab] - ovovT[ijba] is
+// either read or, when the caller passes no Lovov, formed in the kernel from
+// the ovovT tiles it already holds.  Partial sums are written per block and
+// added in a fixed order by a second kernel, so the result is deterministic.
 #include "fricc_internal.h"
 
 namespace fricc {
 
-constexpr int FT = 64;  // (a, b) tile edge
-constexpr int FR = 4;   // rows of a tile per pass of the 256 threads (64 columns each)
+constexpr int QE = 32;  // edge of one (a, b) tile of a quad
 
-template <typename TS>
-__global__ __launch_bounds__(256) void k_t2_finalize(int o, int v, const double* __restrict__ ovovT,
-                                                      double* __restrict__ acc, const TS* __restrict__ S,
-                                                      const double* __restrict__ t1, const double* __restrict__ wsum,
-                                                      const double* __restrict__ eia,
-                                                      const double* __restrict__ t2old,
-                                                      const double* __restrict__ lovov,
-                                                      const double* __restrict__ t1new,
-                                                      double* __restrict__ partial) {
-  __shared__ double st[FT][FT + 1];
-  __shared__ double red[2][256 / WAVE];
-  // one block per (i, j, 64-row band of a)
-  const int nat = (v + FT - 1) / FT;
-  const int pair = blockIdx.x / nat, a0 = (blockIdx.x % nat) * FT;
-  const int i = pair / o, j = pair % o;
-  const int64_t vv = (int64_t)v * v;
-  const int64_t pij = (int64_t)(i * o + j) * vv, pji = (int64_t)(j * o + i) * vv;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
-  double e2 = 0.0, dn = 0.0;
-  for (int b0 = 0; b0 < v; b0 += FT) {
-    // S[j,i,b,a] for b in [b0,b0+64), a in [a0,a0+64): a is contiguous
-#pragma unroll 4
-    for (int r = ty; r < FT; r += FR) {
-      const int b = b0 + r, a = a0 + tx;
-      st[r][tx] = (b < v && a < v) ? (double)S[pji + (int64_t)b * v + a] : 0.0;
+// k -> (lo, hi) with lo <= hi < n, k = lo*n - lo*(lo-1)/2 + (hi - lo)
+__device__ __forceinline__ void tri_decode(int k, int n, int& lo, int& hi) {
+  int a = 0;
+  while (k >= n - a) {
+    k -= n - a;
+    a++;
+  }
+  lo = a;
+  hi = a + k;
+}
+
+template <int VEC, typename T>
+__device__ __forceinline__ void ldv(const T* __restrict__ p, T (&d)[VEC]) {
+  if constexpr (VEC == 2) {
+    if constexpr (sizeof(T) == 8) {
+      const double2 t = *reinterpret_cast<const double2*>(p);
+      d[0] = t.x;
+      d[1] = t.y;
+    } else {
+      const float2 t = *reinterpret_cast<const float2*>(p);
+      d[0] = t.x;
+      d[1] = t.y;
     }
-    __syncthreads();
-    // 16 rows per thread, unrolled so that their loads are in flight together
-#pragma unroll 4
-    for (int r = ty; r < FT; r += FR) {
-      const int a = a0 + r, b = b0 + tx;
-      if (a < v && b < v) {
-        const int64_t e = pij + (int64_t)a * v + b;
-        double val = ovovT[e] + acc[e] + (double)S[e] + st[tx][r];
-        if (wsum) val += t1[i * v + a] * t1[j * v + b] * wsum[a * v + b];
-        const double t2n = val / (eia[i * v + a] + eia[j * v + b]);
-        acc[e] = t2n;
-        e2 += (t2n + t1new[i * v + a] * t1new[j * v + b]) * lovov[e];
-        if (i * v + a >= j * v + b) {
-          const double d = t2n - t2old[e];
+  } else {
+    d[0] = p[0];
+  }
+}
+
+// One block per quad: an unordered occupied pair {i <= j} and an unordered
+// pair of 32-wide virtual tiles {A <= B}.  Tile q of the quad is the pair
+// (i,j) for q < 2 and (j,i) otherwise, with rows in A and columns in B for
+// even q and the reverse for odd q.  Every element's partners then lie in the
+// same quad: the exchange partner ovovT[P,Q,s,f] of (P,Q,f,s) in tile q^1 and
+// the symmetrisation partner S[Q,P,s,f] in tile 3-q, both at the transposed
+// position, so each input is read from HBM once, row by row, and the
+// transposes go through LDS.  With lovov NULL (READL false) the exchange
+// integral 2 ovovT[ijab] - ovovT[ijba] is formed from the two ovovT tiles
+// instead of being read.  On the diagonal (i == j or A == B) some tiles
+// coincide; only the distinct ones are read, written and summed.
+template <typename TS, int VEC, bool READL>
+__global__ __launch_bounds__(256) void k_t2_finalize_quad(int o, int v, const double* __restrict__ ovovT,
+                                                           double* __restrict__ acc, const TS* __restrict__ S,
+                                                           const double* __restrict__ t1,
+                                                           const double* __restrict__ wsum,
+                                                           const double* __restrict__ eia,
+                                                           const double* __restrict__ t2old,
+                                                           const double* __restrict__ lovov,
+                                                           const double* __restrict__ t1new,
+                                                           double* __restrict__ partial) {
+  __shared__ double xs[4][QE][QE + 1];
+  __shared__ TS ss[4][QE][QE + 1];
+  __shared__ double red[2][256 / WAVE];
+  constexpr int LPR = QE / VEC, RPP = 256 / LPR, NP = QE / RPP;  // lanes per row, rows per pass, passes
+  int i, j, ta, tb;
+  tri_decode(blockIdx.x, o, i, j);
+  tri_decode(blockIdx.y, (v + QE - 1) / QE, ta, tb);
+  const int A0 = ta * QE, B0 = tb * QE;
+  // tile q duplicates tile q & dmask
+  int dmask = 3;
+  if (i == j) dmask &= 1;
+  if (ta == tb) dmask &= 2;
+  const int64_t vv = (int64_t)v * v;
+  const int cl = (threadIdx.x % LPR) * VEC, rl = threadIdx.x / LPR;
+  double x[4][NP][VEC], ac[4][NP][VEC], to[4][NP][VEC], lv[4][NP][VEC];
+  TS sv[4][NP][VEC];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if ((q & dmask) != q) continue;
+    const int P = (q >> 1) ? j : i, Q = (q >> 1) ? i : j;
+    const int R0 = (q & 1) ? B0 : A0, C0 = (q & 1) ? A0 : B0;
+    const int64_t base = (int64_t)(P * o + Q) * vv;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      const int r = rl + p * RPP, f = R0 + r, s = C0 + cl;
+      if (f < v && s < v) {
+        const int64_t e = base + (int64_t)f * v + s;
+        ldv<VEC>(ovovT + e, x[q][p]);
+        ldv<VEC>(S + e, sv[q][p]);
+        ldv<VEC>(acc + e, ac[q][p]);
+        ldv<VEC>(t2old + e, to[q][p]);
+        if constexpr (READL) ldv<VEC>(lovov + e, lv[q][p]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+          x[q][p][k] = 0.0;
+          sv[q][p][k] = TS(0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < VEC; k++) {
+        xs[q][r][cl + k] = x[q][p][k];
+        ss[q][r][cl + k] = sv[q][p][k];
+      }
+    }
+  }
+  __syncthreads();
+  double e2 = 0.0, dn = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if ((q & dmask) != q) continue;
+    const int P = (q >> 1) ? j : i, Q = (q >> 1) ? i : j;
+    const int R0 = (q & 1) ? B0 : A0, C0 = (q & 1) ? A0 : B0;
+    const int qx = (q ^ 1) & dmask, qs = (3 - q) & dmask;
+    const int64_t base = (int64_t)(P * o + Q) * vv;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      const int r = rl + p * RPP, f = R0 + r, s0 = C0 + cl;
+      if (f >= v || s0 >= v) continue;
+      const double ef = eia[P * v + f], tf = t1[P * v + f], nf = t1new[P * v + f];
+      double out[VEC];
+#pragma unroll
+      for (int k = 0; k < VEC; k++) {
+        const int c = cl + k, s = s0 + k;
+        double val = x[q][p][k] + ac[q][p][k] + (double)sv[q][p][k] + (double)ss[qs][c][r];
+        if (wsum) val += tf * t1[Q * v + s] * wsum[(int64_t)f * v + s];
+        const double t2n = val / (ef + eia[Q * v + s]);
+        out[k] = t2n;
+        const double L = READL ? lv[q][p][k] : 2.0 * x[q][p][k] - xs[qx][c][r];
+        e2 += (t2n + nf * t1new[Q * v + s]) * L;
+        if (P * v + f >= Q * v + s) {
+          const double d = t2n - to[q][p][k];
           dn += d * d;
         }
       }
+      double* dst = acc + base + (int64_t)f * v + s0;
+      if constexpr (VEC == 2)
+        *reinterpret_cast<double2*>(dst) = make_double2(out[0], out[1]);
+      else
+        dst[0] = out[0];
     }
-    __syncthreads();
   }
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   for (int d = WAVE / 2; d > 0; d >>= 1) {
@@ -86,26 +172,38 @@ __global__ __launch_bounds__(256) void k_t2_finalize(int o, int v, const double*
       a += red[0][k];
       b += red[1][k];
     }
-    partial[2 * blockIdx.x] = a;
-    partial[2 * blockIdx.x + 1] = b;
+    partial[2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x)] = a;
+    partial[2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) + 1] = b;
   }
 }
 
-// One block of 256 threads: thread t adds the partials t, t+256, ... in order,
-// then a fixed-shape tree adds the 256 results, so the sum does not depend on
-// timing.
-__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partial, int n,
+// One block of SPT threads: thread t adds the partials t, t+SPT, ... in
+// order (eight loads in flight at a time, added in that same order), then a
+// fixed-shape tree adds the SPT results, so the sum does not depend on timing.
+constexpr int SPT = 1024;
+__global__ __launch_bounds__(SPT) void k_sum_partials(const double* __restrict__ partial, int n,
                                                       double* __restrict__ out) {
-  __shared__ double red[2][256];
+  __shared__ double red[2][SPT];
   double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < n; k += 256) {
-    a += partial[2 * k];
-    b += partial[2 * k + 1];
+  for (int k0 = threadIdx.x; k0 < n; k0 += 8 * SPT) {
+    double2 u[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = k0 + q * SPT;
+      u[q] = k < n ? *reinterpret_cast<const double2*>(partial + 2 * k) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      if (k0 + q * SPT < n) {
+        a += u[q].x;
+        b += u[q].y;
+      }
+    }
   }
   red[0][threadIdx.x] = a;
   red[1][threadIdx.x] = b;
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = SPT / 2; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h) {
       red[0][threadIdx.x] += red[0][threadIdx.x + h];
       red[1][threadIdx.x] += red[1][threadIdx.x + h];
@@ -118,21 +216,47 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
   }
 }
 
+template <typename TS, int VEC>
+static void finalize_launch(dim3 grid, int o, int v, const double* ovovT, double* acc, const void* S,
+                            const double* t1, const double* wsum, const double* eia, const double* t2old,
+                            const double* lovov, const double* t1new, double* partial, hipStream_t s) {
+  if (lovov)
+    k_t2_finalize_quad<TS, VEC, true><<<grid, 256, 0, s>>>(o, v, ovovT, acc, (const TS*)S, t1, wsum, eia, t2old,
+                                                           lovov, t1new, partial);
+  else
+    k_t2_finalize_quad<TS, VEC, false><<<grid, 256, 0, s>>>(o, v, ovovT, acc, (const TS*)S, t1, wsum, eia, t2old,
+                                                            lovov, t1new, partial);
+}
+
 int t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_f32, const double* t1,
                 const double* wsum, const double* eia, const double* t2old, const double* lovov, const double* t1new,
                 double* sums, hipStream_t s) {
   FRICC_CHECK(o > 0 && v > 0, "t2_finalize: bad dimensions");
+  const int nt = (v + QE - 1) / QE;
+  const int64_t npair = (int64_t)o * (o + 1) / 2, ntile = (int64_t)nt * (nt + 1) / 2;
+  FRICC_CHECK(npair < (1LL << 31) && ntile <= 65535 && npair * ntile < (1LL << 31),
+              "t2_finalize: dimensions too large");
+  const int64_t nb = npair * ntile;
   void* p;
-  const int nb = o * o * ((v + FT - 1) / FT);
   FRICC_TRY(ws_get(40, (size_t)nb * 2 * 8, &p));
   double* partial = (double*)p;
-  if (s_f32)
-    k_t2_finalize<float><<<nb, 256, 0, s>>>(o, v, ovovT, acc, (const float*)S, t1, wsum, eia, t2old, lovov, t1new,
-                                            partial);
-  else
-    k_t2_finalize<double><<<nb, 256, 0, s>>>(o, v, ovovT, acc, (const double*)S, t1, wsum, eia, t2old, lovov, t1new,
-                                             partial);
-  k_sum_partials<<<1, 256, 0, s>>>(partial, nb, sums);
+  // 16-byte accesses need an even v and 16-byte aligned operands (8 for an FP32 S)
+  auto al = [](const void* q, uintptr_t n) { return q == nullptr || ((uintptr_t)q % n) == 0; };
+  const bool vec = v % 2 == 0 && al(ovovT, 16) && al(acc, 16) && al(t2old, 16) && al(lovov, 16) &&
+                   al(S, s_f32 ? 8 : 16);
+  const dim3 grid((unsigned)npair, (unsigned)ntile);
+  if (s_f32) {
+    if (vec)
+      finalize_launch<float, 2>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+    else
+      finalize_launch<float, 1>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+  } else {
+    if (vec)
+      finalize_launch<double, 2>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+    else
+      finalize_launch<double, 1>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+  }
+  k_sum_partials<<<1, SPT, 0, s>>>(partial, (int)nb, sums);
   FRICC_HIP(hipGetLastError());
   return OK;
 }
@@ -142,8 +266,9 @@ int t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, i
 extern "C" {
 // Fused end of the amplitude update (see the top of this file).  All arrays
 // are device C-order: ovovT, acc (in/out: direct terms in, t2new out), t2old,
-// lovov [o][o][v][v] FP64; S [o][o][v][v] FP64 or FP32 (s_precision);
-// t1, eia, t1new [o][v]; wsum [v][v] or NULL.  sums (device, 2 doubles)
+// lovov [o][o][v][v] FP64 or NULL (then 2 ovovT[ijab] - ovovT[ijba] is used);
+// S [o][o][v][v] FP64 or FP32 (s_precision); t1, eia, t1new [o][v]; wsum
+// [v][v] or NULL.  sums (device, 2 doubles)
 // receives {E2, dnorm}.  No counterpart in the reference.
 int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_precision,
                       const double* t1, const double* wsum, const double* eia, const double* t2old,

@@ -583,7 +583,7 @@ def _update_amps_device(cc, t1, t2, eris):
     red = torch.empty(4, dtype=f64, device=t2.device)
     _capi.check(_capi.lib().fricc_t2_finalize(
         o, v, _vp(P64["ovovT"]), _vp(acc), _vp(S), _PREC_ID["fp32" if S.dtype == torch.float32 else "fp64"],
-        _vp(t1), _vp(wsum) if wsum is not None else None, _vp(eia), _vp(t2), _vp(P64["Lovov"]), _vp(t1new),
+        _vp(t1), _vp(wsum) if wsum is not None else None, _vp(eia), _vp(t2), None, _vp(t1new),
         _vp(red), _capi.stream_ptr()))
     red[2].copy_(2 * (fov * t1new).sum())
     red[3].copy_(((t1new - t1) ** 2).sum())

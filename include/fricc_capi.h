@@ -298,6 +298,8 @@ int fricc_set_bf16_passes(int npass);
  * packed lower triangle of (t2new - t2old)^2 (device, FP64, block-ordered).
  * ovovT, acc, t2old, lovov: [o][o][v][v] FP64; S: same shape, FP64
  * (s_precision 0) or FP32 (1); t1, eia, t1new: [o][v]; wsum [v][v] or NULL.
+ * lovov NULL means the exchange integral 2 ovovT[ijab] - ovovT[ijba] (the
+ * reference's Lovov), formed in the kernel instead of read.
  * Replaces the symmetrisation, denominator, energy and norm passes of
  * fricc/friccsd.py:239-284 and :418-427. */
 int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_precision,
