@@ -73,6 +73,7 @@ __global__ __launch_bounds__(256) void k_t2_finalize_quad(int o, int v, const do
                                                            const double* __restrict__ t2old,
                                                            const double* __restrict__ lovov,
                                                            const double* __restrict__ t1new,
+                                                           float* __restrict__ out32,
                                                            double* __restrict__ partial) {
   __shared__ double xs[4][QE][QE + 1];
   __shared__ TS ss[4][QE][QE + 1];
@@ -154,6 +155,13 @@ __global__ __launch_bounds__(256) void k_t2_finalize_quad(int o, int v, const do
         *reinterpret_cast<double2*>(dst) = make_double2(out[0], out[1]);
       else
         dst[0] = out[0];
+      if (out32) {
+        float* d32 = out32 + base + (int64_t)f * v + s0;
+        if constexpr (VEC == 2)
+          *reinterpret_cast<float2*>(d32) = make_float2((float)out[0], (float)out[1]);
+        else
+          d32[0] = (float)out[0];
+      }
     }
   }
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
@@ -219,18 +227,19 @@ __global__ __launch_bounds__(SPT) void k_sum_partials(const double* __restrict__
 template <typename TS, int VEC>
 static void finalize_launch(dim3 grid, int o, int v, const double* ovovT, double* acc, const void* S,
                             const double* t1, const double* wsum, const double* eia, const double* t2old,
-                            const double* lovov, const double* t1new, double* partial, hipStream_t s) {
+                            const double* lovov, const double* t1new, float* out32, double* partial,
+                            hipStream_t s) {
   if (lovov)
     k_t2_finalize_quad<TS, VEC, true><<<grid, 256, 0, s>>>(o, v, ovovT, acc, (const TS*)S, t1, wsum, eia, t2old,
-                                                           lovov, t1new, partial);
+                                                           lovov, t1new, out32, partial);
   else
     k_t2_finalize_quad<TS, VEC, false><<<grid, 256, 0, s>>>(o, v, ovovT, acc, (const TS*)S, t1, wsum, eia, t2old,
-                                                            lovov, t1new, partial);
+                                                            lovov, t1new, out32, partial);
 }
 
 int t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_f32, const double* t1,
                 const double* wsum, const double* eia, const double* t2old, const double* lovov, const double* t1new,
-                double* sums, hipStream_t s) {
+                float* out32, double* sums, hipStream_t s) {
   FRICC_CHECK(o > 0 && v > 0, "t2_finalize: bad dimensions");
   const int nt = (v + QE - 1) / QE;
   const int64_t npair = (int64_t)o * (o + 1) / 2, ntile = (int64_t)nt * (nt + 1) / 2;
@@ -243,18 +252,18 @@ int t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, i
   // 16-byte accesses need an even v and 16-byte aligned operands (8 for an FP32 S)
   auto al = [](const void* q, uintptr_t n) { return q == nullptr || ((uintptr_t)q % n) == 0; };
   const bool vec = v % 2 == 0 && al(ovovT, 16) && al(acc, 16) && al(t2old, 16) && al(lovov, 16) &&
-                   al(S, s_f32 ? 8 : 16);
+                   al(S, s_f32 ? 8 : 16) && al(out32, 8);
   const dim3 grid((unsigned)npair, (unsigned)ntile);
   if (s_f32) {
     if (vec)
-      finalize_launch<float, 2>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+      finalize_launch<float, 2>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, out32, partial, s);
     else
-      finalize_launch<float, 1>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+      finalize_launch<float, 1>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, out32, partial, s);
   } else {
     if (vec)
-      finalize_launch<double, 2>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+      finalize_launch<double, 2>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, out32, partial, s);
     else
-      finalize_launch<double, 1>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, partial, s);
+      finalize_launch<double, 1>(grid, o, v, ovovT, acc, S, t1, wsum, eia, t2old, lovov, t1new, out32, partial, s);
   }
   k_sum_partials<<<1, SPT, 0, s>>>(partial, (int)nb, sums);
   FRICC_HIP(hipGetLastError());
@@ -268,13 +277,14 @@ extern "C" {
 // are device C-order: ovovT, acc (in/out: direct terms in, t2new out), t2old,
 // lovov [o][o][v][v] FP64 or NULL (then 2 ovovT[ijab] - ovovT[ijba] is used);
 // S [o][o][v][v] FP64 or FP32 (s_precision); t1, eia, t1new [o][v]; wsum
-// [v][v] or NULL.  sums (device, 2 doubles)
+// [v][v] or NULL; t2new_f32 [o][o][v][v] FP32 or NULL (also receives t2new,
+// rounded to FP32).  sums (device, 2 doubles)
 // receives {E2, dnorm}.  No counterpart in the reference.
 int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_precision,
                       const double* t1, const double* wsum, const double* eia, const double* t2old,
-                      const double* lovov, const double* t1new, double* sums, void* stream) {
+                      const double* lovov, const double* t1new, float* t2new_f32, double* sums, void* stream) {
   FRICC_CHECK(s_precision == 0 || s_precision == 1, "t2_finalize: S must be FP64 or FP32");
-  return fricc::t2_finalize(o, v, ovovT, acc, S, s_precision, t1, wsum, eia, t2old, lovov, t1new, sums,
+  return fricc::t2_finalize(o, v, ovovT, acc, S, s_precision, t1, wsum, eia, t2old, lovov, t1new, t2new_f32, sums,
                             (hipStream_t)stream);
 }
 }

@@ -122,7 +122,7 @@ def lib():
     L.fricc_workspace_select.argtypes = [ctypes.c_int]
     L.fricc_workspace_release.argtypes = [ctypes.c_int]
     L.fricc_workspace_pin.argtypes = [ctypes.c_int, ctypes.c_int]
-    L.fricc_t2_finalize.argtypes = [ctypes.c_int, ctypes.c_int] + [_vp] * 3 + [ctypes.c_int] + [_vp] * 7 + [_vp]
+    L.fricc_t2_finalize.argtypes = [ctypes.c_int, ctypes.c_int] + [_vp] * 3 + [ctypes.c_int] + [_vp] * 8 + [_vp]
     L.fricc_sparse_prepare.argtypes = [_vp, ctypes.c_char_p, I64x4, I64x4, ctypes.c_int, _vp]
     for name in EXPORTS:
         getattr(L, name)  # every declared symbol must resolve

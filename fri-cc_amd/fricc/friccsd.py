@@ -33,6 +33,7 @@ import copy
 import logging
 import math
 import time
+import weakref
 
 import numpy as np
 
@@ -204,6 +205,18 @@ def _contract_into(spec, alpha, A, wshape, T, out, precision):
             _capi.stream_ptr()))
 
 
+def _t2_reduced(cc, t2, cdt):
+    """t2 in the products' reduced precision: the copy the previous step's
+    finalize wrote when t2 is that step's output, unmodified (same tensor,
+    same version counter) and no graph is being recorded; else a conversion."""
+    torch = _torch()
+    c = getattr(cc, "_t2_low", None)
+    if (c is not None and c[2].dtype == cdt and c[0]() is t2 and t2._version == c[1]
+            and not torch.cuda.is_current_stream_capturing()):
+        return c[2]
+    return t2.to(cdt)
+
+
 def _check_m_keep(cc, t2):
     m_keep = cc.fri_settings["m_keep"]
     if m_keep > t2.numel() or m_keep == 0:
@@ -292,7 +305,7 @@ def _update_amps_device(cc, t1, t2, eris):
     fvv = fock[o:, o:]
     t1 = t1.contiguous()
     t2 = t2.contiguous()
-    t2c = t2.to(cdt) if cdt != f64 else t2
+    t2c = _t2_reduced(cc, t2, cdt) if cdt != f64 else t2
     C = lambda a: a.to(f64) if cdt != f64 else a  # noqa: E731
 
     # ---- compression (fricc/friccsd.py:99-138)
@@ -581,10 +594,16 @@ def _update_amps_device(cc, t1, t2, eris):
     eia = (mo_e_o[:, None] - mo_e_v).contiguous()
     t1new = (t1new / eia).contiguous()
     red = torch.empty(4, dtype=f64, device=t2.device)
+    # the next step's FP32 copy of T2 comes out of the finalize (not in a
+    # recorded graph, whose replays convert their own input)
+    t2low = None
+    if cdt == torch.float32 and not torch.cuda.is_current_stream_capturing():
+        t2low = torch.empty(acc.shape, dtype=torch.float32, device=acc.device)
     _capi.check(_capi.lib().fricc_t2_finalize(
         o, v, _vp(P64["ovovT"]), _vp(acc), _vp(S), _PREC_ID["fp32" if S.dtype == torch.float32 else "fp64"],
         _vp(t1), _vp(wsum) if wsum is not None else None, _vp(eia), _vp(t2), None, _vp(t1new),
-        _vp(red), _capi.stream_ptr()))
+        _vp(t2low) if t2low is not None else None, _vp(red), _capi.stream_ptr()))
+    cc._t2_low = (weakref.ref(acc), acc._version, t2low) if t2low is not None else None
     red[2].copy_(2 * (fov * t1new).sum())
     red[3].copy_(((t1new - t1) ** 2).sum())
     return t1new, acc, red
@@ -676,8 +695,6 @@ def _unpin(arena, device):
 def _pin_arena(arena, owner):
     """Pin the library arena ``arena`` on the current device for as long as
     ``owner`` (a recorded graph's holder) lives."""
-    import weakref
-
     _capi.check(_capi.lib().fricc_workspace_pin(int(arena), 1))
     weakref.finalize(owner, _unpin, int(arena), _torch().cuda.current_device())
 

@@ -177,7 +177,7 @@ def _update_amps_lccsd_device(cc, t1, t2, eris):
     zero = torch.zeros_like(t1new)  # the linearised energy has no t1*t1 part
     _capi.check(_capi.lib().fricc_t2_finalize(
         o, v, _vp(P64["ovovT"]), _vp(acc), _vp(S), _PREC_ID["fp32" if S.dtype == torch.float32 else "fp64"],
-        _vp(t1), None, _vp(eia), _vp(t2), None, _vp(zero), _vp(red), _capi.stream_ptr()))
+        _vp(t1), None, _vp(eia), _vp(t2), None, _vp(zero), None, _vp(red), _capi.stream_ptr()))
     red[2].copy_(2 * (fov * t1new).sum())
     red[3].copy_(((t1new - t1_in) ** 2).sum())
     return t1new, acc, red

@@ -299,12 +299,14 @@ int fricc_set_bf16_passes(int npass);
  * ovovT, acc, t2old, lovov: [o][o][v][v] FP64; S: same shape, FP64
  * (s_precision 0) or FP32 (1); t1, eia, t1new: [o][v]; wsum [v][v] or NULL.
  * lovov NULL means the exchange integral 2 ovovT[ijab] - ovovT[ijba] (the
- * reference's Lovov), formed in the kernel instead of read.
+ * reference's Lovov), formed in the kernel instead of read.  t2new_f32
+ * [o][o][v][v] FP32 or NULL: also receives t2new rounded to FP32 (the
+ * reduced-precision copy the next step's products read).
  * Replaces the symmetrisation, denominator, energy and norm passes of
  * fricc/friccsd.py:239-284 and :418-427. */
 int fricc_t2_finalize(int o, int v, const double* ovovT, double* acc, const void* S, int s_precision,
                       const double* t1, const double* wsum, const double* eia, const double* t2old,
-                      const double* lovov, const double* t1new, double* sums, void* stream);
+                      const double* lovov, const double* t1new, float* t2new_f32, double* sums, void* stream);
 
 /* Two dense products that read the same o*v^3 operand Z[k][c][d][a], with Z
  * streamed once: out_ia[i][a] += sum_{k,c,d} t2[i][k][c][d] Z[k][c][d][a] and

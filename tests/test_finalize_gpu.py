@@ -7,7 +7,8 @@ random inputs, against the same formula restated in FP64 on the CPU:
 
 with Lovov passed in and with Lovov NULL (formed in the kernel), FP32 and FP64
 S, with and without wsum, at sizes whose 32-wide tiles are ragged (v = 70, 33),
-odd v (the 8-byte path) and a single occupied orbital.  The whole update is
+odd v (the 8-byte path) and a single occupied orbital, with and without the
+FP32 copy of t2new (which must equal t2new rounded to FP32).  The whole update is
 checked against the oracle in the update tests; this one pins the kernel's
 quad and diagonal bookkeeping element by element.  Tolerance: 1e-12 relative
 per element, 1e-11 relative on the sums (FP64 throughout; an FP32 S enters
@@ -37,7 +38,8 @@ def _p(t):
 @pytest.mark.parametrize("s_f32", [True, False])
 @pytest.mark.parametrize("read_lovov", [True, False])
 @pytest.mark.parametrize("with_wsum", [True, False])
-def test_finalize_matches_formula(env, o, v, s_f32, read_lovov, with_wsum):
+@pytest.mark.parametrize("f32_copy", [True, False])
+def test_finalize_matches_formula(env, o, v, s_f32, read_lovov, with_wsum, f32_copy):
     torch, _capi = env
     g = torch.Generator().manual_seed(o * 1000 + v)
     f64 = torch.float64
@@ -68,12 +70,15 @@ def test_finalize_matches_formula(env, o, v, s_f32, read_lovov, with_wsum):
     Wd = W.to(dev) if with_wsum else None
     t1d, t1nd, eiad = t1.to(dev), t1n.to(dev), eia.to(dev)
     sums = torch.full((2,), float("nan"), dtype=f64, device=dev)
+    o32 = torch.full((o, o, v, v), float("nan"), dtype=torch.float32, device=dev) if f32_copy else None
     _capi.check(_capi.lib().fricc_t2_finalize(
         o, v, _p(Xd), _p(accd), _p(Sd), 1 if s_f32 else 0, _p(t1d), _p(Wd), _p(eiad), _p(t2od), _p(Ld),
-        _p(t1nd), _p(sums), _capi.stream_ptr()))
+        _p(t1nd), _p(o32), _p(sums), _capi.stream_ptr()))
     torch.cuda.synchronize()
     got = accd.cpu()
     assert torch.all((got - ref).abs() <= 1e-12 * (ref.abs() + 1.0)), float((got - ref).abs().max())
+    if f32_copy:  # the FP32 copy is the FP64 result rounded once
+        assert torch.equal(o32.cpu(), got.float())
     s = sums.cpu()
     assert abs(float(s[0]) - float(e2)) <= 1e-11 * max(1.0, abs(float(e2))), (float(s[0]), float(e2))
     assert abs(float(s[1]) - float(dn)) <= 1e-11 * max(1.0, float(dn)), (float(s[1]), float(dn))
@@ -94,8 +99,8 @@ def test_finalize_lovov_null_is_bitwise_equal(env):
         a = acc.clone()
         sums = torch.empty(2, dtype=torch.float64, device="cuda")
         _capi.check(_capi.lib().fricc_t2_finalize(
-            o, v, _p(X), _p(a), _p(S), 0, _p(t1), None, _p(eia), _p(t2o), _p(lov), _p(t1n), _p(sums),
-            _capi.stream_ptr()))
+            o, v, _p(X), _p(a), _p(S), 0, _p(t1), None, _p(eia), _p(t2o), _p(lov), _p(t1n), None,
+            _p(sums), _capi.stream_ptr()))
         outs.append((a, sums))
     torch.cuda.synchronize()
     assert torch.equal(outs[0][0], outs[1][0])
