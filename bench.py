@@ -152,7 +152,7 @@ def iteration_roofline(o, v, m, mode, dense_rec, ms_measured):
     streaming passes of x and the COO output per sample; contractions:
     2 m (v^2 + 2 o^2 + 8 o v) flops in the mode's precision, the ladder's
     operand bytes v^4 s_W plus the FP64 output read and written; the dense
-    products as counted)."""
+    products as counted; the finalize's o^2 v^2 arrays once each)."""
     N = o * o * v * v
     sp, lp = MODE_PREC[mode]
     peak = lambda p: 1e12 * PEAK_TFLOPS[p]  # noqa: E731
@@ -170,7 +170,11 @@ def iteration_roofline(o, v, m, mode, dense_rec, ms_measured):
     for rec in dense_rec.values():
         tdense += max(rec["flops"] / peak("fp64" if rec["fp64"] else "fp32"), rec["bytes"] / bw)
     parts["dense remainder"] = tdense
-    parts["finalize"] = (6 * 8 * N) / bw  # five FP64 o^2 v^2 inputs read, t2new written
+    # ovovT, t2old and the direct terms read, t2new written (FP64), S read in
+    # the mode's precision; the reduced modes also write the FP32 copy of t2new
+    # their next step reads (Lovov is formed from ovovT, not read)
+    reduced = mode != "fp64"
+    parts["finalize"] = ((4 * 8 + (4 if reduced else 8) + (4 if reduced else 0)) * N) / bw
     t_roof = sum(parts.values())
     return {"T_roof_ms": 1e3 * t_roof, "T_measured_ms": ms_measured, "frac": 1e3 * t_roof / ms_measured,
             "parts_ms": {k: 1e3 * x for k, x in parts.items()},
