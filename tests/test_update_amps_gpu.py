@@ -251,3 +251,41 @@ def test_graph_mode_kernel_loop(env):
         cc.kernel()
         res[graph] = list(cc.energies)
     assert res[True] == res[False] and len(res[True]) == 4
+
+
+def test_mixed_step_reuses_the_finalize_fp32_copy(env):
+    """In the reduced-precision modes the finalize also writes t2new rounded to
+    FP32, and the next step reads that copy when it is handed back the same,
+    unmodified tensor.  A second step from the returned T2 (copy reused) must
+    match a second step from a clone of it (converted afresh), and a T2
+    changed in place after the first step must not be served the stale copy:
+    it must match a step from an equal, freshly built tensor.  The tolerance
+    (1e-6 relative) is far below the change a stale copy would make (about
+    1e-2 relative here)."""
+    from fricc import friccsd
+
+    torch = env[0]
+    o, v, naux = 6, 24, 40
+    us = [0.42, 0.58, 0.31, 0.77]
+    base = {"m_keep": 6000, "compression": "fri", "sampling_method": "systematic", "precision": "mixed"}
+    runs = [_pair(env, o, v, naux, 3, base, us) for _ in range(4)]
+    t1, t2 = runs[0][0].get_init_guess(runs[0][1])
+    t1 = t1 + 0.01 * torch.randn_like(t1)
+    firsts = [cc.update_amps(t1, t2, eris) for cc, eris, _, _ in runs]
+    for cc, (_, t2n) in zip(runs, firsts):
+        assert friccsd._t2_reduced(cc[0], t2n, torch.float32) is cc[0]._t2_low[2]
+        assert torch.equal(cc[0]._t2_low[2], t2n.float())
+    g = torch.Generator(device=t2.device).manual_seed(5)
+    bump = 0.01 * torch.randn(t2.shape, generator=g, dtype=t2.dtype, device=t2.device) * t2.abs().max()
+    (c0, e0, _, _), (c1, e1, _, _), (c2, e2, _, _), (c3, e3, _, _) = runs
+    reused = c0.update_amps(firsts[0][0], firsts[0][1], e0)[1]
+    fresh = c1.update_amps(firsts[1][0], firsts[1][1].clone(), e1)[1]
+    t2x = firsts[2][1]
+    t2x.add_(bump)  # in place: the version counter moves, the FP32 copy is stale
+    assert friccsd._t2_reduced(c2, t2x, torch.float32) is not c2._t2_low[2]
+    changed = c2.update_amps(firsts[2][0], t2x, e2)[1]
+    rebuilt = c3.update_amps(firsts[3][0], firsts[3][1].clone() + bump, e3)[1]
+    scale = reused.abs().max().item()
+    assert (reused - fresh).abs().max().item() <= 1e-6 * scale
+    assert (changed - rebuilt).abs().max().item() <= 1e-6 * scale
+    assert (changed - reused).abs().max().item() > 1e-4 * scale  # the bump is visible
